@@ -1,0 +1,211 @@
+"""Benchmark of the diffusion-decay step on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload precision_400|drop_in_200|lod_50]
+
+Default workload (the configuration the headline metric is quoted on):
+``precision_400`` — 400^3 voxels x 8 substrates (dx=10 um, dt=0.01 min), D
+log-uniform in [1e2,1e5], lambda log-uniform in [1e-3,1], 1e6 agents in a
+Gaussian cluster (sigma=600 um); each step = agent secretion/uptake + LOD
+diffusion-decay.  The FP32 store is timed, then the FP64 store on the same
+inputs; the FP32 result is compared with the FP64 one after the same steps.
+
+Prints ONE JSON line.  ``value`` is voxel-substrate updates/s of the FP32
+full step (nx*ny*nz*S*K / wall seconds of K steps, inputs resident in HBM).
+``roofline`` prices the dominant kernel's algorithmic HBM bytes per launch
+against the 8 TB/s HBM3E peak, using its average launch time from HIP events
+on the library's stream.  ``cpu_baseline`` times BioFVM itself (compiled from
+source, oracle/_ref) on a slab of the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(case, w):
+    """Per-step algorithmic HBM bytes: 3 sweeps x (1R+1W) of the store, bulk
+    fields A/B read once, agents (voxel key + temp1/temp2), Dirichlet nodes
+    (index + values) before each sweep."""
+    nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    S = case["S"]
+    ph = case["phases"]
+    b = nvs * 6 * w
+    if ph & 1 and case.get("bulk_supply") is not None:
+        b += nvs * 2 * w  # read A and B (the store pass is shared with the x-sweep when fused)
+    if ph & 2 and case.get("agent_volume") is not None:
+        b += len(case["agent_volume"]) * (4 + 2 * S * w)
+    if ph & 4 and case.get("dir_idx") is not None:
+        b += 3 * len(case["dir_idx"]) * (4 + S * w)
+    return b
+
+
+def kernel_bytes(case, kernel, w):
+    """Algorithmic bytes of one launch of `kernel`."""
+    nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    if kernel.startswith("sweep"):
+        return 2 * nvs * w
+    if kernel == "bulk":
+        return 4 * nvs * w
+    if kernel == "agents":
+        return len(case["agent_volume"]) * (4 + 2 * case["S"] * w)
+    if kernel == "dirichlet":
+        return len(case["dir_idx"]) * (4 + case["S"] * w)
+    return 0
+
+
+def time_precision(case, precision, steps, warmup):
+    from biofvm_amd import setup_case
+    m = setup_case(case, precision)
+    dt, ph = case["dt"], case["phases"]
+    m.run(dt, warmup, ph)
+    m.sync()
+    t0 = time.perf_counter()
+    ev_ms, _ = m.timed_run(dt, steps, ph)
+    m.sync()
+    wall = time.perf_counter() - t0
+    # a short second pass with an event pair around every launch
+    prof_steps = max(1, min(steps, 10))
+    _, per = m.timed_run(dt, prof_steps, ph, per_kernel=True)
+    m.sync()
+    total_steps = warmup + steps + prof_steps
+    return m, wall, ev_ms, per, total_steps
+
+
+def cpu_baseline(case, workload):
+    """BioFVM itself (oracle/_ref) on a z-slab of the workload, all host threads."""
+    import oracle
+    from biofvm_amd import workloads as W
+    nz_s = 16 if case["nx"] >= 200 else case["nz"]
+    nx, ny, S, dx = case["nx"], case["ny"], case["S"], case["dx"]
+    slab = W.make_case(nx, ny, nz_s, S, dx, case["D"], case["lam"], case["dt"], 1, case["phases"],
+                       case["rho0"][: nx * ny * nz_s * S])
+    desc = f"{nx}x{ny}x{nz_s} slab of {workload}"
+    if case.get("agent_pos") is not None:
+        pos = case["agent_pos"].reshape(-1, 3)
+        zc = case["nz"] * dx / 2
+        keep = np.flatnonzero(np.abs(pos[:, 2] - zc) < nz_s * dx / 2)
+        p = pos[keep].copy()
+        p[:, 2] -= zc - nz_s * dx / 2
+        S_ = S
+        slab.update(agent_pos=p.reshape(-1), agent_volume=case["agent_volume"][keep],
+                    agent_secretion=case["agent_secretion"].reshape(-1, S_)[keep].reshape(-1),
+                    agent_saturation=case["agent_saturation"].reshape(-1, S_)[keep].reshape(-1),
+                    agent_uptake=case["agent_uptake"].reshape(-1, S_)[keep].reshape(-1))
+        desc += f", {keep.size} agents"
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    nthreads = min(nthreads, os.cpu_count() or nthreads)
+    nvs = nx * ny * nz_s * S
+    if oracle.have_ref():
+        slab["steps"] = 2
+        _, secs = oracle.ref_run(slab, nthreads=nthreads, return_seconds=True)
+        steps = int(max(2, min(200, 10.0 / max(secs / 2, 1e-6))))
+        slab["steps"] = steps
+        _, secs = oracle.ref_run(slab, nthreads=nthreads, return_seconds=True)
+        kind, cores = "reference", nthreads
+    else:
+        slab["steps"] = 2
+        t0 = time.perf_counter()
+        oracle.oracle_run(slab)
+        secs = time.perf_counter() - t0
+        steps = 2
+        kind, cores = "port", 1
+    return {"value": nvs * steps / secs, "unit": "voxel-substrate updates/s", "cores": cores,
+            "kind": kind, "sample": f"{desc}, {steps} steps, {secs:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="precision_400",
+                    choices=["precision_400", "drop_in_200", "lod_50"])
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the z-slab decomposition is a design, not code). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from biofvm_amd import FP32, FP64, _capi
+    from biofvm_amd import workloads as W
+    _capi.load()
+
+    case = getattr(W, args.workload)()
+    nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    steps, warmup = args.steps, args.warmup
+    # the CPU-runnable and drop-in workloads are FP64 by definition
+    main_prec = FP32 if args.workload == "precision_400" else FP64
+    w_main = 4 if main_prec == FP32 else 8
+
+    m32, wall, ev_ms, per, n_done = time_precision(case, main_prec, steps, warmup)
+    value = nvs * steps / wall
+    step_ms = 1e3 * wall / steps
+    dom = max(per, key=lambda k: per[k][0])
+    dom_ms, dom_n = per[dom]
+    dom_avg = dom_ms / dom_n
+    dom_bytes = kernel_bytes(case, dom, w_main)
+    achieved = dom_bytes / (dom_avg * 1e-3) / 1e9
+    step_bytes = algorithmic_bytes(case, w_main)
+
+    out_extra = {}
+    if main_prec == FP32 and not args.no_fp64:
+        m64, wall64, ev64, per64, n64 = time_precision(case, FP64, steps, warmup)
+        assert n64 == n_done
+        a = m32.download()
+        b = m64.download()
+        m64.close()
+        diff = np.abs(a - b)
+        scale = np.abs(b)
+        big = scale > 1e-3 * scale.max()
+        out_extra["fp64"] = {"value": nvs * steps / wall64, "ms_per_step": 1e3 * wall64 / steps,
+                             "hbm_frac_step": algorithmic_bytes(case, 8) / (wall64 / steps) / 1e9 / HBM_PEAK_GBS}
+        out_extra["fp32_speedup_vs_fp64"] = wall64 / wall
+        out_extra["error_vs_fp64"] = {
+            "steps": n_done, "max_rel_normwise": float(diff.max() / scale.max()),
+            "max_rel_elementwise": float((diff[big] / scale[big]).max()),
+            "l2_rel": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
+        del a, b
+    m32.close()
+
+    line = {
+        "metric": "voxel-substrate updates/s (full step: agents + LOD diffusion-decay)",
+        "value": value,
+        "unit": "voxel-substrate updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": step_ms,
+        "events_ms_per_step": ev_ms / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if main_prec == FP32 else "f64",
+        "data": "synthetic (seeded numpy default_rng; see biofvm_amd/workloads.py)",
+        "config": {"workload": f"{args.workload}: {case['nx']}x{case['ny']}x{case['nz']} voxels x "
+                               f"{case['S']} substrates, dx={case['dx']}, dt={case['dt']}, "
+                               f"{0 if case.get('agent_volume') is None else len(case['agent_volume'])} agents, "
+                               f"phases={case['phases']}"},
+        "hbm_frac_step": step_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg},
+        "kernels_ms_per_launch": {k: (v[0] / v[1] if v[1] else None) for k, v in per.items()},
+    }
+    line.update(out_extra)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(case, args.workload)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,74 @@
+"""ctypes binding of the C ABI declared in include/biofvm_amd.h.
+
+The shared library is built in-tree (``python -m biofvm_amd.build``).  There
+is no fallback: if ``libbiofvm_amd.so`` is missing or fails to load, every
+entry point raises.
+"""
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbiofvm_amd.so")
+
+FP64, FP32 = 0, 1
+BULK, CELLS, DIFFUSION, ALL = 1, 2, 4, 7
+K_DIRICHLET, K_BULK, K_AGENTS, K_SWEEP_X, K_SWEEP_Y, K_SWEEP_Z, KERNEL_KINDS = range(7)
+KERNEL_NAMES = ["dirichlet", "bulk", "agents", "sweep_x", "sweep_y", "sweep_z"]
+
+# Every exported symbol with its ctypes signature: (restype, argtypes).
+_h = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+_dp = ctypes.POINTER(ctypes.c_double)
+_up = ctypes.POINTER(ctypes.c_ubyte)
+_sp = ctypes.POINTER(ctypes.c_size_t)
+
+SIGNATURES = {
+    "bfvm_create": (_i, [_i, _i, _i, _i, _d, _i, ctypes.POINTER(_h)]),
+    "bfvm_create_mesh": (_i, [_i, _i, _i, _i, _d, _d, _d, _i, ctypes.POINTER(_h)]),
+    "bfvm_destroy": (_i, [_h]),
+    "bfvm_set_coefficients": (_i, [_h, _dp, _dp]),
+    "bfvm_upload_density": (_i, [_h, _dp]),
+    "bfvm_download_density": (_i, [_h, _dp]),
+    "bfvm_set_dirichlet": (_i, [_h, _i, _ip, _dp, _up]),
+    "bfvm_set_bulk": (_i, [_h, _dp, _dp, _dp]),
+    "bfvm_set_agents": (_i, [_h, _i, _ip, _dp, _dp]),
+    "bfvm_set_agent_rates": (_i, [_h, _i, _dp, _dp, _dp, _dp, _dp, _d, _ip]),
+    "bfvm_step": (_i, [_h, _d, _i]),
+    "bfvm_run": (_i, [_h, _d, _i, _i]),
+    "bfvm_sync": (_i, [_h]),
+    "bfvm_timed_run": (_i, [_h, _d, _i, _i, _dp, _dp, _ip]),
+    "bfvm_get_info": (_i, [_h, _ip, _ip, _ip, _ip, _ip, _sp]),
+    "bfvm_density_device_ptr": (_i, [_h, ctypes.POINTER(_vp)]),
+    "bfvm_last_error": (ctypes.c_char_p, [_h]),
+}
+
+_lib = None
+
+
+class BfvmError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"biofvm_amd error {code}: {message}")
+        self.code = code
+
+
+def load():
+    """Loads libbiofvm_amd.so once and binds every signature."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m biofvm_amd.build`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc, handle=None):
+    if rc != 0:
+        msg = load().bfvm_last_error(handle)
+        raise BfvmError(rc, msg.decode() if msg else "")

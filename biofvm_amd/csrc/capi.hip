@@ -1,0 +1,598 @@
+// C ABI of biofvm_amd: handle lifetime, host<->device transfers, the FP64
+// Thomas factorisation, agent grouping and the per-step launch sequence.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "bfvm_internal.h"
+
+using bfvm::Env;
+
+namespace {
+
+thread_local std::string g_thread_err;
+
+int fail(Env* e, int code, const std::string& msg)
+{
+    if (e) e->err = msg;
+    g_thread_err = msg;
+    return code;
+}
+
+int hip_fail(Env* e, hipError_t st, const char* what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(st);
+    return fail(e, BFVM_E_DEVICE, m);
+}
+
+#define HIP_OR_FAIL(e, call)                               \
+    do {                                                   \
+        hipError_t st_ = (call);                           \
+        if (st_ != hipSuccess) return hip_fail(e, st_, #call); \
+    } while (0)
+
+void dfree(void*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <typename P>
+void dfree_t(P*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// Host values -> device buffer in the handle's precision.
+hipError_t upload_values(Env& e, void* dst, const double* src, size_t n)
+{
+    if (e.precision == BFVM_FP64) return hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice);
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (float)src[i];
+    return hipMemcpy(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice);
+}
+
+hipError_t alloc_values(Env& e, void** p, size_t n)
+{
+    return hipMalloc(p, std::max<size_t>(n, 1) * e.elem);
+}
+
+#pragma clang fp contract(off)
+// Thomas factorisation of one axis in FP64, in the reference's operation
+// order (BioFVM_solvers.cpp:104-142).  denom/c are (n, S), element [i*S+s].
+void thomas_axis(int n, int S, double dx, double dt, const std::vector<double>& D,
+                 const std::vector<double>& lambda, std::vector<double>& denom,
+                 std::vector<double>& c, std::vector<double>& c1v)
+{
+    denom.assign((size_t)n * S, 0.0);
+    c.assign((size_t)n * S, 0.0);
+    c1v.assign(S, 0.0);
+    for (int s = 0; s < S; ++s) {
+        double c1 = D[s];
+        c1 *= dt;
+        c1 /= dx;
+        c1 /= dx;
+        double c1a = c1 * -1.0;
+        double c2 = lambda[s];
+        c2 *= dt;
+        c2 /= 3.0;
+        double c3 = 1.0;
+        c3 += c1;
+        c3 += c1;
+        c3 += c2;
+        double c3a = 1.0;
+        c3a += c1;
+        c3a += c2;
+        c1v[s] = c1;
+        for (int i = 0; i < n; ++i) {
+            c[(size_t)i * S + s] = c1a;
+            denom[(size_t)i * S + s] = c3;
+        }
+        denom[s] = c3a;
+        denom[(size_t)(n - 1) * S + s] = c3a;
+        if (n == 1) {
+            denom[s] = 1.0;
+            denom[s] += c2;
+        }
+        c[s] /= denom[s];
+        for (int i = 1; i <= n - 1; ++i) {
+            double prod = c1 * c[(size_t)(i - 1) * S + s];
+            denom[(size_t)i * S + s] += prod;
+            c[(size_t)i * S + s] /= denom[(size_t)i * S + s];
+        }
+    }
+}
+
+// temp1 = ((0 + S)*T)*c, temp2 = (1 + c*S) + c*U with c = dt*V/V_voxel
+// (BioFVM_basic_agent.cpp:122-147).
+void agent_constants(int S, double dt, double voxel_volume, double volume, const double* sec,
+                     const double* sat, const double* upt, double* t1, double* t2)
+{
+    double c = dt * volume;
+    c = c / voxel_volume;
+    for (int s = 0; s < S; ++s) {
+        double a = 0.0;
+        a += sec[s];
+        a *= sat[s];
+        a *= c;
+        double b = 1.0;
+        double cs = c * sec[s];
+        b += cs;
+        double cu = c * upt[s];
+        b += cu;
+        t1[s] = a;
+        t2[s] = b;
+    }
+}
+#pragma clang fp contract(on)
+
+int ensure_coefficients(Env& e, double dt)
+{
+    if (e.coef_valid && e.coef_dt == dt) return BFVM_OK;
+    const int n_axis[3] = {e.nx, e.ny, e.nz};
+    std::vector<double> denom, c, c1v;
+    for (int a = 0; a < 3; ++a) {
+        thomas_axis(n_axis[a], e.S, e.dx, dt, e.D, e.lambda, denom, c, c1v);
+        bfvm::AxisCoef& ax = e.ax[a];
+        size_t m = (size_t)n_axis[a] * e.S;
+        std::vector<double> inv(m);
+        for (size_t q = 0; q < m; ++q) inv[q] = 1.0 / denom[q];
+        HIP_OR_FAIL(&e, upload_values(e, ax.denom, denom.data(), m));
+        HIP_OR_FAIL(&e, upload_values(e, ax.inv_denom, inv.data(), m));
+        HIP_OR_FAIL(&e, upload_values(e, ax.c, c.data(), m));
+        // First index from which (denom, c) hold their interior fixed point
+        // exactly up to the last node (which uses c3a and differs).
+        int n = n_axis[a];
+        int from = std::max(n - 1, 0);
+        if (n >= 3) {
+            int last = n - 2;
+            from = last;
+            while (from > 0) {
+                bool same = true;
+                for (int s = 0; s < e.S && same; ++s)
+                    same = denom[(size_t)(from - 1) * e.S + s] == denom[(size_t)last * e.S + s] &&
+                           c[(size_t)(from - 1) * e.S + s] == c[(size_t)last * e.S + s];
+                if (!same) break;
+                --from;
+            }
+        }
+        ax.converged_from = from;
+    }
+    HIP_OR_FAIL(&e, upload_values(e, e.c1, c1v.data(), e.S));
+    e.coef_valid = true;
+    e.coef_dt = dt;
+    return BFVM_OK;
+}
+
+// Launch wrapper: optional per-kernel HIP events for bfvm_timed_run.
+template <typename F>
+int run_kernel(Env& e, int kind, F&& launch)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (e.kernel_timing) {
+        while (e.ev_pool.size() < e.ev_next + 2) {
+            hipEvent_t ev;
+            HIP_OR_FAIL(&e, hipEventCreate(&ev));
+            e.ev_pool.push_back(ev);
+        }
+        a = e.ev_pool[e.ev_next++];
+        b = e.ev_pool[e.ev_next++];
+        HIP_OR_FAIL(&e, hipEventRecord(a, e.stream));
+    }
+    hipError_t st = launch();
+    if (st != hipSuccess) return hip_fail(&e, st, "kernel launch");
+    if (e.kernel_timing) {
+        HIP_OR_FAIL(&e, hipEventRecord(b, e.stream));
+        e.ev_marks.push_back({kind, {a, b}});
+    }
+    return BFVM_OK;
+}
+
+int step_impl(Env& e, double dt, int flags)
+{
+    int rc;
+    if ((flags & BFVM_BULK) && e.have_bulk) {
+        if (!e.bulk_valid || e.bulk_dt != dt) {
+            HIP_OR_FAIL(&e, bfvm::launch_bulk_prepare(e, dt));
+            e.bulk_valid = true;
+            e.bulk_dt = dt;
+        }
+        if ((rc = run_kernel(e, BFVM_K_BULK, [&] { return bfvm::launch_bulk(e); }))) return rc;
+    }
+    if ((flags & BFVM_CELLS) && e.n_agents > 0) {
+        if ((rc = run_kernel(e, BFVM_K_AGENTS, [&] { return bfvm::launch_agents(e); }))) return rc;
+    }
+    if (flags & BFVM_DIFFUSION) {
+        if (!e.uniform)
+            return fail(&e, BFVM_E_MESH,
+                        "This algorithm is written for uniform Cartesian meshes (dx == dy == dz).");
+        if (e.D.empty()) return fail(&e, BFVM_E_STATE, "coefficients not set (bfvm_set_coefficients)");
+        if ((rc = ensure_coefficients(e, dt))) return rc;
+        for (int axis = 0; axis < 3; ++axis) {
+            if (e.n_dir > 0)
+                if ((rc = run_kernel(e, BFVM_K_DIRICHLET, [&] { return bfvm::launch_dirichlet(e); })))
+                    return rc;
+            if ((rc = run_kernel(e, BFVM_K_SWEEP_X + axis, [&] { return bfvm::launch_sweep(e, axis); })))
+                return rc;
+        }
+    }
+    return BFVM_OK;
+}
+
+void free_agents(Env& e)
+{
+    dfree_t(e.agent_seg_voxel);
+    dfree_t(e.agent_seg_start);
+    dfree(e.agent_t1);
+    dfree(e.agent_t2);
+    e.n_agents = 0;
+    e.n_agent_voxels = 0;
+}
+
+int set_agents_impl(Env& e, int n, const int* voxel, const double* t1, const double* t2)
+{
+    free_agents(e);
+    if (n == 0) return BFVM_OK;
+    const int S = e.S;
+    for (int a = 0; a < n; ++a)
+        if (voxel[a] < 0 || (size_t)voxel[a] >= e.nvox)
+            return fail(&e, BFVM_E_ARG, "agent voxel index out of range");
+    // Group by voxel, keeping list order inside a voxel (stable sort).
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return voxel[x] < voxel[y]; });
+    std::vector<int> gvox, gstart;
+    std::vector<double> s1((size_t)n * S), s2((size_t)n * S);
+    for (int q = 0; q < n; ++q) {
+        int a = order[q];
+        if (q == 0 || voxel[a] != voxel[order[q - 1]]) {
+            gvox.push_back(voxel[a]);
+            gstart.push_back(q);
+        }
+        std::memcpy(&s1[(size_t)q * S], t1 + (size_t)a * S, sizeof(double) * S);
+        std::memcpy(&s2[(size_t)q * S], t2 + (size_t)a * S, sizeof(double) * S);
+    }
+    gstart.push_back(n);
+    int ng = (int)gvox.size();
+    HIP_OR_FAIL(&e, hipMalloc((void**)&e.agent_seg_voxel, sizeof(int) * ng));
+    HIP_OR_FAIL(&e, hipMalloc((void**)&e.agent_seg_start, sizeof(int) * (ng + 1)));
+    HIP_OR_FAIL(&e, alloc_values(e, &e.agent_t1, (size_t)n * S));
+    HIP_OR_FAIL(&e, alloc_values(e, &e.agent_t2, (size_t)n * S));
+    HIP_OR_FAIL(&e, hipMemcpy(e.agent_seg_voxel, gvox.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(&e, hipMemcpy(e.agent_seg_start, gstart.data(), sizeof(int) * (ng + 1),
+                              hipMemcpyHostToDevice));
+    HIP_OR_FAIL(&e, upload_values(e, e.agent_t1, s1.data(), (size_t)n * S));
+    HIP_OR_FAIL(&e, upload_values(e, e.agent_t2, s2.data(), (size_t)n * S));
+    e.n_agents = n;
+    e.n_agent_voxels = ng;
+    return BFVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bfvm_create_mesh(int nx, int ny, int nz, int n_substrates, double dx, double dy, double dz,
+                     int precision, bfvm_handle* out)
+{
+    if (!out) return fail(nullptr, BFVM_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (nx < 1 || ny < 1 || nz < 1 || n_substrates < 1)
+        return fail(nullptr, BFVM_E_ARG, "grid sizes and substrate count must be >= 1");
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0)) return fail(nullptr, BFVM_E_ARG, "voxel edges must be > 0");
+    if (precision != BFVM_FP64 && precision != BFVM_FP32)
+        return fail(nullptr, BFVM_E_ARG, "precision must be BFVM_FP64 or BFVM_FP32");
+    size_t nvox = (size_t)nx * ny * nz;
+    if (nvox * (size_t)n_substrates >= ((size_t)1 << 31))
+        return fail(nullptr, BFVM_E_ARG, "nx*ny*nz*S must stay below 2^31");
+    bfvm_env* e = new bfvm_env;
+    e->nx = nx; e->ny = ny; e->nz = nz; e->S = n_substrates;
+    e->dx = dx; e->dy = dy; e->dz = dz;
+    const double tol = 1e-16;  // BioFVM_mesh.cpp:811-815
+    e->uniform = !(std::fabs(dx - dy) > tol || std::fabs(dy - dz) > tol || std::fabs(dx - dz) > tol);
+    e->precision = precision;
+    e->elem = precision == BFVM_FP64 ? 8 : 4;
+    e->nvox = nvox;
+    e->nvs = nvox * n_substrates;
+    e->D.clear();
+    e->lambda.clear();
+    hipError_t st;
+    if ((st = hipGetDevice(&e->device)) != hipSuccess ||
+        (st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (st = hipMalloc(&e->rho, e->nvs * e->elem)) != hipSuccess ||
+        (st = hipMemset(e->rho, 0, e->nvs * e->elem)) != hipSuccess ||
+        (st = hipMalloc(&e->c1, e->S * e->elem)) != hipSuccess ||
+        (st = hipMalloc((void**)&e->dir_mask, e->S)) != hipSuccess) {
+        int rc = hip_fail(nullptr, st, "bfvm_create");
+        bfvm_destroy(e);
+        return rc;
+    }
+    const int n_axis[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a) {
+        size_t m = (size_t)n_axis[a] * e->S;
+        e->ax[a].n = n_axis[a];
+        if ((st = alloc_values(*e, &e->ax[a].denom, m)) != hipSuccess ||
+            (st = alloc_values(*e, &e->ax[a].inv_denom, m)) != hipSuccess ||
+            (st = alloc_values(*e, &e->ax[a].c, m)) != hipSuccess) {
+            int rc = hip_fail(nullptr, st, "bfvm_create");
+            bfvm_destroy(e);
+            return rc;
+        }
+    }
+    *out = e;
+    return BFVM_OK;
+}
+
+int bfvm_create(int nx, int ny, int nz, int n_substrates, double dx, int precision, bfvm_handle* out)
+{
+    return bfvm_create_mesh(nx, ny, nz, n_substrates, dx, dx, dx, precision, out);
+}
+
+int bfvm_destroy(bfvm_handle h)
+{
+    if (!h) return BFVM_OK;
+    bfvm_env* e = h;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    dfree(e->rho);
+    dfree(e->c1);
+    for (auto& a : e->ax) {
+        dfree(a.denom);
+        dfree(a.inv_denom);
+        dfree(a.c);
+    }
+    dfree_t(e->dir_idx);
+    dfree(e->dir_val);
+    dfree_t(e->dir_mask);
+    dfree_t(e->bulk_supply);
+    dfree_t(e->bulk_target);
+    dfree_t(e->bulk_uptake);
+    dfree(e->bulk_A);
+    dfree(e->bulk_B);
+    free_agents(*e);
+    for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return BFVM_OK;
+}
+
+int bfvm_set_coefficients(bfvm_handle h, const double* D, const double* lambda)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (!D || !lambda) return fail(h, BFVM_E_ARG, "D and lambda must be non-NULL");
+    h->D.assign(D, D + h->S);
+    h->lambda.assign(lambda, lambda + h->S);
+    h->coef_valid = false;
+    return BFVM_OK;
+}
+
+int bfvm_upload_density(bfvm_handle h, const double* host)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (!host) return fail(h, BFVM_E_ARG, "host is NULL");
+    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    HIP_OR_FAIL(h, upload_values(*h, h->rho, host, h->nvs));
+    return BFVM_OK;
+}
+
+int bfvm_download_density(bfvm_handle h, double* host)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (!host) return fail(h, BFVM_E_ARG, "host is NULL");
+    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    if (h->precision == BFVM_FP64) {
+        HIP_OR_FAIL(h, hipMemcpy(host, h->rho, h->nvs * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(h->nvs);
+        HIP_OR_FAIL(h, hipMemcpy(tmp.data(), h->rho, h->nvs * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < h->nvs; ++i) host[i] = tmp[i];
+    }
+    return BFVM_OK;
+}
+
+int bfvm_set_dirichlet(bfvm_handle h, int n, const int* voxel_index, const double* values,
+                       const unsigned char* mask)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (n < 0) return fail(h, BFVM_E_ARG, "n < 0");
+    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    dfree_t(h->dir_idx);
+    dfree(h->dir_val);
+    h->n_dir = 0;
+    if (n == 0) return BFVM_OK;
+    if (!voxel_index || !values) return fail(h, BFVM_E_ARG, "voxel_index and values must be non-NULL");
+    for (int d = 0; d < n; ++d)
+        if (voxel_index[d] < 0 || (size_t)voxel_index[d] >= h->nvox)
+            return fail(h, BFVM_E_ARG, "Dirichlet voxel index out of range");
+    std::vector<unsigned char> m(h->S, 1);
+    if (mask)
+        for (int s = 0; s < h->S; ++s) m[s] = mask[s] ? 1 : 0;
+    HIP_OR_FAIL(h, hipMalloc((void**)&h->dir_idx, sizeof(int) * n));
+    HIP_OR_FAIL(h, alloc_values(*h, &h->dir_val, (size_t)n * h->S));
+    HIP_OR_FAIL(h, hipMemcpy(h->dir_idx, voxel_index, sizeof(int) * n, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(h, upload_values(*h, h->dir_val, values, (size_t)n * h->S));
+    HIP_OR_FAIL(h, hipMemcpy(h->dir_mask, m.data(), h->S, hipMemcpyHostToDevice));
+    h->n_dir = n;
+    return BFVM_OK;
+}
+
+int bfvm_set_bulk(bfvm_handle h, const double* supply_rate, const double* supply_target,
+                  const double* uptake_rate)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    bool none = !supply_rate && !supply_target && !uptake_rate;
+    bool all = supply_rate && supply_target && uptake_rate;
+    if (!none && !all) return fail(h, BFVM_E_ARG, "give all three bulk fields or none");
+    h->have_bulk = false;
+    h->bulk_valid = false;
+    if (none) {
+        dfree_t(h->bulk_supply);
+        dfree_t(h->bulk_target);
+        dfree_t(h->bulk_uptake);
+        dfree(h->bulk_A);
+        dfree(h->bulk_B);
+        return BFVM_OK;
+    }
+    size_t bytes = h->nvs * sizeof(double);
+    if (!h->bulk_supply) {
+        HIP_OR_FAIL(h, hipMalloc((void**)&h->bulk_supply, bytes));
+        HIP_OR_FAIL(h, hipMalloc((void**)&h->bulk_target, bytes));
+        HIP_OR_FAIL(h, hipMalloc((void**)&h->bulk_uptake, bytes));
+        HIP_OR_FAIL(h, alloc_values(*h, &h->bulk_A, h->nvs));
+        HIP_OR_FAIL(h, alloc_values(*h, &h->bulk_B, h->nvs));
+    }
+    HIP_OR_FAIL(h, hipMemcpy(h->bulk_supply, supply_rate, bytes, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(h, hipMemcpy(h->bulk_target, supply_target, bytes, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(h, hipMemcpy(h->bulk_uptake, uptake_rate, bytes, hipMemcpyHostToDevice));
+    h->have_bulk = true;
+    return BFVM_OK;
+}
+
+int bfvm_set_agents(bfvm_handle h, int n, const int* voxel_index, const double* temp1,
+                    const double* temp2)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (n < 0) return fail(h, BFVM_E_ARG, "n < 0");
+    if (n > 0 && (!voxel_index || !temp1 || !temp2))
+        return fail(h, BFVM_E_ARG, "voxel_index, temp1 and temp2 must be non-NULL");
+    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    return set_agents_impl(*h, n, voxel_index, temp1, temp2);
+}
+
+int bfvm_set_agent_rates(bfvm_handle h, int n, const double* position, const double* volume,
+                         const double* secretion, const double* saturation, const double* uptake,
+                         double dt, int* out_voxel)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (n < 0) return fail(h, BFVM_E_ARG, "n < 0");
+    if (n > 0 && (!position || !volume || !secretion || !saturation || !uptake))
+        return fail(h, BFVM_E_ARG, "agent arrays must be non-NULL");
+    const int S = h->S;
+    std::vector<int> vox(n);
+    std::vector<double> t1((size_t)n * S), t2((size_t)n * S);
+    const double ext[3] = {h->nx * h->dx, h->ny * h->dy, h->nz * h->dz};
+    const double edge[3] = {h->dx, h->dy, h->dz};
+    const int cnt[3] = {h->nx, h->ny, h->nz};
+    const double dV = h->dx * h->dy * h->dz;  // BioFVM_mesh.cpp:839
+    for (int a = 0; a < n; ++a) {
+        int ijk[3];
+        for (int d = 0; d < 3; ++d) {
+            double p = position[3 * a + d];
+            if (!(p >= 0.0 && p <= ext[d]))  // BioFVM_mesh.cpp:208-217
+                return fail(h, BFVM_E_ARG, "agent " + std::to_string(a) + " lies outside the domain");
+            int q = (int)std::floor(p / edge[d]);  // BioFVM_mesh.cpp:932-946
+            ijk[d] = std::min(std::max(q, 0), cnt[d] - 1);
+        }
+        vox[a] = (ijk[2] * h->ny + ijk[1]) * h->nx + ijk[0];
+        agent_constants(S, dt, dV, volume[a], secretion + (size_t)a * S, saturation + (size_t)a * S,
+                        uptake + (size_t)a * S, &t1[(size_t)a * S], &t2[(size_t)a * S]);
+    }
+    if (out_voxel) std::copy(vox.begin(), vox.end(), out_voxel);
+    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    return set_agents_impl(*h, n, vox.data(), t1.data(), t2.data());
+}
+
+int bfvm_step(bfvm_handle h, double dt, int flags)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    return step_impl(*h, dt, flags);
+}
+
+int bfvm_run(bfvm_handle h, double dt, int steps, int flags)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (steps < 0) return fail(h, BFVM_E_ARG, "steps < 0");
+    for (int n = 0; n < steps; ++n) {
+        int rc = step_impl(*h, dt, flags);
+        if (rc) return rc;
+    }
+    return BFVM_OK;
+}
+
+int bfvm_sync(bfvm_handle h)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    return BFVM_OK;
+}
+
+int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_total,
+                   double* per_kernel_ms, int* launches)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (steps < 0) return fail(h, BFVM_E_ARG, "steps < 0");
+    Env& e = *h;
+    // Factorisation and bulk terms are prepared outside the timed region.
+    int rc;
+    if ((flags & BFVM_DIFFUSION) && e.uniform && !e.D.empty())
+        if ((rc = ensure_coefficients(e, dt))) return rc;
+    if ((flags & BFVM_BULK) && e.have_bulk && (!e.bulk_valid || e.bulk_dt != dt)) {
+        HIP_OR_FAIL(&e, bfvm::launch_bulk_prepare(e, dt));
+        e.bulk_valid = true;
+        e.bulk_dt = dt;
+    }
+    hipEvent_t t0, t1;
+    HIP_OR_FAIL(&e, hipEventCreate(&t0));
+    HIP_OR_FAIL(&e, hipEventCreate(&t1));
+    e.kernel_timing = per_kernel_ms != nullptr;
+    e.ev_next = 0;
+    e.ev_marks.clear();
+    HIP_OR_FAIL(&e, hipStreamSynchronize(e.stream));
+    HIP_OR_FAIL(&e, hipEventRecord(t0, e.stream));
+    rc = BFVM_OK;
+    for (int n = 0; n < steps && rc == BFVM_OK; ++n) rc = step_impl(e, dt, flags);
+    HIP_OR_FAIL(&e, hipEventRecord(t1, e.stream));
+    HIP_OR_FAIL(&e, hipEventSynchronize(t1));
+    e.kernel_timing = false;
+    if (rc) return rc;
+    float ms = 0;
+    HIP_OR_FAIL(&e, hipEventElapsedTime(&ms, t0, t1));
+    if (ms_total) *ms_total = ms;
+    if (per_kernel_ms) {
+        for (int k = 0; k < BFVM_KERNEL_KINDS; ++k) {
+            per_kernel_ms[k] = 0;
+            if (launches) launches[k] = 0;
+        }
+        for (auto& m : e.ev_marks) {
+            float km = 0;
+            HIP_OR_FAIL(&e, hipEventElapsedTime(&km, m.second.first, m.second.second));
+            per_kernel_ms[m.first] += km;
+            if (launches) launches[m.first] += 1;
+        }
+    }
+    e.ev_marks.clear();
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return BFVM_OK;
+}
+
+int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, int* precision,
+                  size_t* density_bytes)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (nx) *nx = h->nx;
+    if (ny) *ny = h->ny;
+    if (nz) *nz = h->nz;
+    if (n_substrates) *n_substrates = h->S;
+    if (precision) *precision = h->precision;
+    if (density_bytes) *density_bytes = h->nvs * h->elem;
+    return BFVM_OK;
+}
+
+int bfvm_density_device_ptr(bfvm_handle h, void** ptr)
+{
+    if (!h || !ptr) return fail(h, BFVM_E_ARG, "null argument");
+    *ptr = h->rho;
+    return BFVM_OK;
+}
+
+const char* bfvm_last_error(bfvm_handle h)
+{
+    return h ? h->err.c_str() : g_thread_err.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+"""Python handle over the C ABI: one device-resident microenvironment.
+
+``DeviceMicroenvironment`` owns a ``bfvm_handle``; ``run_case`` drives one
+synthetic case (see ``biofvm_amd.workloads``) end to end through the C ABI.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import ALL, BULK, CELLS, DIFFUSION, FP32, FP64, check
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DeviceMicroenvironment:
+    """A uniform nx*ny*nz grid with S substrates, densities resident in HBM."""
+
+    def __init__(self, nx, ny, nz, S, dx, precision=FP64, dy=None, dz=None):
+        self.lib = _capi.load()
+        self.h = ctypes.c_void_p()
+        dy = dx if dy is None else dy
+        dz = dx if dz is None else dz
+        rc = self.lib.bfvm_create_mesh(nx, ny, nz, S, dx, dy, dz, precision, ctypes.byref(self.h))
+        check(rc, None)
+        self.nx, self.ny, self.nz, self.S, self.dx = nx, ny, nz, S, dx
+        self.precision = precision
+        self.nvox = nx * ny * nz
+
+    def close(self):
+        if self.h:
+            self.lib.bfvm_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        check(rc, self.h)
+
+    def set_coefficients(self, D, lam):
+        D, lam = _f64(D), _f64(lam)
+        assert D.size == self.S and lam.size == self.S
+        self._check(self.lib.bfvm_set_coefficients(self.h, _dptr(D), _dptr(lam)))
+
+    def upload(self, rho):
+        rho = _f64(rho).reshape(-1)
+        assert rho.size == self.nvox * self.S
+        self._check(self.lib.bfvm_upload_density(self.h, _dptr(rho)))
+
+    def download(self):
+        out = np.empty(self.nvox * self.S)
+        self._check(self.lib.bfvm_download_density(self.h, _dptr(out)))
+        return out
+
+    def set_dirichlet(self, idx, values, mask=None):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        values = _f64(values).reshape(-1)
+        assert values.size == idx.size * self.S
+        mp = None
+        if mask is not None:
+            m = np.ascontiguousarray(mask, dtype=np.uint8)
+            mp = m.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+        self._check(self.lib.bfvm_set_dirichlet(self.h, int(idx.size), _iptr(idx), _dptr(values), mp))
+
+    def clear_dirichlet(self):
+        self._check(self.lib.bfvm_set_dirichlet(self.h, 0, None, None, None))
+
+    def set_bulk(self, supply, target, uptake):
+        if supply is None:
+            self._check(self.lib.bfvm_set_bulk(self.h, None, None, None))
+            return
+        f = [_f64(x).reshape(-1) for x in (supply, target, uptake)]
+        assert all(x.size == self.nvox * self.S for x in f)
+        self._check(self.lib.bfvm_set_bulk(self.h, _dptr(f[0]), _dptr(f[1]), _dptr(f[2])))
+
+    def set_agents(self, voxel, temp1, temp2):
+        voxel = np.ascontiguousarray(voxel, dtype=np.int32)
+        t1, t2 = _f64(temp1).reshape(-1), _f64(temp2).reshape(-1)
+        self._check(self.lib.bfvm_set_agents(self.h, int(voxel.size), _iptr(voxel), _dptr(t1), _dptr(t2)))
+
+    def set_agent_rates(self, pos, volume, secretion, saturation, uptake, dt):
+        volume = _f64(volume).reshape(-1)
+        n = int(volume.size)
+        pos = _f64(pos).reshape(-1)
+        sec, sat, upt = (_f64(x).reshape(-1) for x in (secretion, saturation, uptake))
+        assert pos.size == 3 * n and sec.size == n * self.S
+        vox = np.zeros(max(n, 1), dtype=np.int32)
+        self._check(self.lib.bfvm_set_agent_rates(self.h, n, _dptr(pos), _dptr(volume), _dptr(sec),
+                                                  _dptr(sat), _dptr(upt), float(dt), _iptr(vox)))
+        return vox[:n]
+
+    def step(self, dt, flags=ALL):
+        self._check(self.lib.bfvm_step(self.h, float(dt), int(flags)))
+
+    def run(self, dt, steps, flags=ALL):
+        self._check(self.lib.bfvm_run(self.h, float(dt), int(steps), int(flags)))
+
+    def sync(self):
+        self._check(self.lib.bfvm_sync(self.h))
+
+    def timed_run(self, dt, steps, flags=ALL, per_kernel=False):
+        """Returns (ms_total, {kernel: (ms_sum, launches)} or None)."""
+        ms = ctypes.c_double(0)
+        pk = np.zeros(_capi.KERNEL_KINDS)
+        nl = np.zeros(_capi.KERNEL_KINDS, dtype=np.int32)
+        self._check(self.lib.bfvm_timed_run(self.h, float(dt), int(steps), int(flags), ctypes.byref(ms),
+                                            _dptr(pk) if per_kernel else None,
+                                            _iptr(nl) if per_kernel else None))
+        if not per_kernel:
+            return ms.value, None
+        return ms.value, {n: (float(pk[k]), int(nl[k])) for k, n in enumerate(_capi.KERNEL_NAMES)}
+
+
+def setup_case(case, precision=FP64):
+    """Creates a DeviceMicroenvironment holding every input of `case`."""
+    m = DeviceMicroenvironment(case["nx"], case["ny"], case["nz"], case["S"], case["dx"], precision)
+    m.set_coefficients(case["D"], case["lam"])
+    m.upload(case["rho0"])
+    if case.get("dir_idx") is not None and len(case["dir_idx"]) > 0:
+        m.set_dirichlet(case["dir_idx"], case["dir_val"], case.get("dir_mask"))
+    if case.get("bulk_supply") is not None:
+        m.set_bulk(case["bulk_supply"], case["bulk_target"], case["bulk_uptake"])
+    if case.get("agent_pos") is not None and len(case["agent_volume"]) > 0:
+        m.set_agent_rates(case["agent_pos"], case["agent_volume"], case["agent_secretion"],
+                          case["agent_saturation"], case["agent_uptake"], case["dt"])
+    return m
+
+
+def run_case(case, precision=FP64):
+    """Runs `case` for its step count on the device; returns final densities (float64)."""
+    with setup_case(case, precision) as m:
+        m.run(case["dt"], case["steps"], case["phases"])
+        return m.download()

@@ -1,0 +1,177 @@
+/*
+ * biofvm_amd — MI355X (gfx950) diffusion-decay engine, C ABI.
+ *
+ * Drop-in for BioFVM 1.0.4's hot path: the LOD diffusion-decay solver plugged
+ * into Microenvironment::diffusion_decay_solver, plus the bulk and per-agent
+ * source/sink steps that run beside it each time step.  Densities live in HBM
+ * as one array rho[((k*ny + j)*nx + i)*S + s] (voxel-major, substrate
+ * innermost: the reference's vector<vector<double>> store flattened).
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a negative BFVM_E* code; the
+ *     message is available from bfvm_last_error(handle) (or
+ *     bfvm_last_error(NULL) for failures that have no handle yet).
+ *   - Work is enqueued on the handle's own HIP stream and is asynchronous;
+ *     functions that read results back to host memory synchronise first.
+ *   - One host thread per handle; handles are independent.
+ *   - Host arrays are float64, voxel-major / substrate-innermost, whatever
+ *     the device precision.
+ */
+#ifndef BIOFVM_AMD_H
+#define BIOFVM_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bfvm_env* bfvm_handle;
+
+/* Device arithmetic of the density store and the sweeps. */
+enum bfvm_precision {
+    BFVM_FP64 = 0, /* bit-for-bit reference semantics up to FMA rounding */
+    BFVM_FP32 = 1  /* FP32 state; Thomas factorisation computed in FP64 and rounded */
+};
+
+/* Phases of one time step, in the order they run
+ * (examples/convergence_test5.cpp:236-238). */
+enum bfvm_phase {
+    BFVM_BULK = 1,      /* Microenvironment::simulate_bulk_sources_and_sinks */
+    BFVM_CELLS = 2,     /* Microenvironment::simulate_cell_sources_and_sinks */
+    BFVM_DIFFUSION = 4, /* Microenvironment::simulate_diffusion_decay (LOD_3D) */
+    BFVM_ALL = 7
+};
+
+enum bfvm_status {
+    BFVM_OK = 0,
+    BFVM_E_ARG = -1,     /* invalid argument (sizes, null pointers, ranges) */
+    BFVM_E_DEVICE = -2,  /* HIP runtime error (allocation, launch, copy) */
+    BFVM_E_STATE = -3,   /* call not valid in the handle's current state */
+    BFVM_E_MESH = -4     /* non-uniform mesh: the LOD solver refuses it */
+};
+
+/*
+ * Creates a uniform Cartesian grid of nx*ny*nz voxels (edge dx) carrying
+ * n_substrates densities, all zero.  Replaces Microenvironment's
+ * set_density/add_density + resize_space_uniform
+ * (BioFVM_microenvironment.cpp:287-290, 325-458; mesh BioFVM_mesh.cpp:796-922).
+ */
+int bfvm_create(int nx, int ny, int nz, int n_substrates, double dx, int precision,
+                bfvm_handle* out);
+int bfvm_destroy(bfvm_handle h);
+
+/*
+ * Same, for a mesh with separate dx, dy, dz.  The LOD solver only accepts a
+ * uniform mesh (|dx-dy|, |dy-dz|, |dx-dz| <= 1e-16, BioFVM_mesh.cpp:811-815);
+ * otherwise the handle is created but every BFVM_DIFFUSION step returns
+ * BFVM_E_MESH and leaves the densities untouched, as the reference prints an
+ * error and returns (BioFVM_solvers.cpp:79-83).
+ */
+int bfvm_create_mesh(int nx, int ny, int nz, int n_substrates, double dx, double dy, double dz,
+                     int precision, bfvm_handle* out);
+
+/* Diffusion coefficients D[s] and decay rates lambda[s]
+ * (Microenvironment::diffusion_coefficients / decay_rates, microenvironment.h:118-119).
+ * The Thomas factorisation (BioFVM_solvers.cpp:87-173) is recomputed in FP64
+ * whenever D, lambda or dt change. */
+int bfvm_set_coefficients(bfvm_handle h, const double* D, const double* lambda);
+
+/* Whole density store, host <-> device (density_vector(n), microenvironment.h:186-191). */
+int bfvm_upload_density(bfvm_handle h, const double* host);
+int bfvm_download_density(bfvm_handle h, double* host);
+
+/*
+ * Dirichlet nodes: n voxel indices and n*S values.  Replaces the whole node
+ * list built by add_dirichlet_node (BioFVM_microenvironment.cpp:146-151) and
+ * applied by apply_dirichlet_conditions (:192-197) before each of the x, y
+ * and z sweeps.  mask[S] (optional) limits the overwrite to substrates with
+ * mask[s] != 0; NULL = whole vector, the reference's behaviour.  n = 0 clears.
+ */
+int bfvm_set_dirichlet(bfvm_handle h, int n, const int* voxel_index, const double* values,
+                       const unsigned char* mask);
+
+/*
+ * Bulk source/sink fields, nvox*S each: supply rate S, supply target T and
+ * uptake rate U — the values the three bulk callbacks write
+ * (microenvironment.h:130-132, used at BioFVM_microenvironment.cpp:595-610).
+ * All NULL removes bulk sources.
+ */
+int bfvm_set_bulk(bfvm_handle h, const double* supply_rate, const double* supply_target,
+                  const double* uptake_rate);
+
+/*
+ * Agents in list order (the order of all_basic_agents, BioFVM_basic_agent.cpp:35):
+ * voxel index and the implicit constants temp1/temp2 (n*S each) that
+ * Basic_Agent::set_internal_uptake_constants computes
+ * (BioFVM_basic_agent.cpp:114-148).  Each step applies, per agent in list
+ * order, rho[v] = (rho[v] + temp1) / temp2 (:229-238) — on the device the
+ * agents are grouped by voxel, keeping list order inside a voxel, so the
+ * result is deterministic.  n = 0 clears.
+ */
+int bfvm_set_agents(bfvm_handle h, int n, const int* voxel_index, const double* temp1,
+                    const double* temp2);
+
+/*
+ * Agents from rates: positions (3n), volumes (n), secretion, saturation and
+ * uptake rates (n*S each).  Voxels come from Cartesian_Mesh::nearest_voxel_index
+ * (BioFVM_mesh.cpp:930-948) on a mesh spanning [0, n*dx] per axis, and the
+ * constants from set_internal_uptake_constants(dt) (BioFVM_basic_agent.cpp:114-148).
+ * A position outside the box is rejected with BFVM_E_ARG (the reference's
+ * assign_position refuses it, BioFVM_basic_agent.cpp:66-71).
+ * out_voxel (optional, n ints) receives the voxel of each agent.
+ */
+int bfvm_set_agent_rates(bfvm_handle h, int n, const double* position, const double* volume,
+                         const double* secretion, const double* saturation,
+                         const double* uptake, double dt, int* out_voxel);
+
+/*
+ * One time step: the phases in `flags`, in the order bulk -> cells ->
+ * diffusion.  BFVM_DIFFUSION replaces
+ * diffusion_decay_solver__constant_coefficients_LOD_3D
+ * (BioFVM_solvers.cpp:77-276) called through simulate_diffusion_decay
+ * (BioFVM_microenvironment.cpp:502-513).
+ */
+int bfvm_step(bfvm_handle h, double dt, int flags);
+
+/* `steps` consecutive calls of bfvm_step, enqueued back to back. */
+int bfvm_run(bfvm_handle h, double dt, int steps, int flags);
+
+/* Waits for all work enqueued on the handle. */
+int bfvm_sync(bfvm_handle h);
+
+/*
+ * Runs `steps` steps and times them on the handle's stream with HIP events.
+ * ms_total receives the elapsed time of the whole run.  If per_kernel_ms is
+ * non-NULL, each kernel launch is bracketed by its own events and
+ * per_kernel_ms[kind] (kind in enum bfvm_kernel_kind, array of
+ * BFVM_KERNEL_KINDS doubles) receives the summed time of that kind, and
+ * launches[kind] (optional) the number of launches.
+ */
+enum bfvm_kernel_kind {
+    BFVM_K_DIRICHLET = 0,
+    BFVM_K_BULK = 1,
+    BFVM_K_AGENTS = 2,
+    BFVM_K_SWEEP_X = 3,
+    BFVM_K_SWEEP_Y = 4,
+    BFVM_K_SWEEP_Z = 5,
+    BFVM_KERNEL_KINDS = 6
+};
+int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_total,
+                   double* per_kernel_ms, int* launches);
+
+/* Grid and store description. */
+int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, int* precision,
+                  size_t* density_bytes);
+
+/* Device address of the density store (for zero-copy interop). */
+int bfvm_density_device_ptr(bfvm_handle h, void** ptr);
+
+/* Message for the last failure on `h` (or of the calling thread when h is NULL). */
+const char* bfvm_last_error(bfvm_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BIOFVM_AMD_H */

@@ -1,0 +1,159 @@
+/*
+ * TEST INFRASTRUCTURE ONLY — not part of the product.
+ *
+ * A C-ABI driver around the compiled BioFVM 1.0.4 reference (built from the
+ * unmodified sources under the reference tree by oracle/Makefile into
+ * oracle/_ref/).  It sets up one Microenvironment exactly as the reference's
+ * example programs do, feeds it caller-supplied arrays, runs the reference's
+ * own step methods and copies the densities back out.  It is linked only
+ * into oracle/_ref/libbfvm_ref.so and is loaded only by tests/, by the
+ * fixture generator under tests/golden/ and by bench.py's CPU-baseline leg.
+ *
+ * Reference call sequence followed (file:line in the reference tree):
+ *   Microenvironment ctor + set_density/add_density    BioFVM_microenvironment.cpp:88-139, 325-458
+ *   resize_space_uniform                                BioFVM_microenvironment.cpp:287-290
+ *   bulk callbacks (S, T, U)                            BioFVM_microenvironment.h:130-132
+ *   add_dirichlet_node                                  BioFVM_microenvironment.cpp:146-151
+ *   create_basic_agent / register / assign_position     BioFVM_basic_agent.cpp:61-81, 172-181
+ *   set_internal_uptake_constants                       BioFVM_basic_agent.cpp:114-148
+ *   step order bulk -> cells -> diffusion               examples/convergence_test5.cpp:236-238
+ */
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+#include "BioFVM.h"
+
+namespace BioFVM {
+void set_default_microenvironment(Microenvironment* M);
+}
+
+using namespace BioFVM;
+
+namespace {
+
+// Per-voxel source/sink fields handed to the reference's bulk callbacks.
+const double* g_bulk_supply = nullptr;
+const double* g_bulk_target = nullptr;
+const double* g_bulk_uptake = nullptr;
+int g_S = 0;
+
+void copy_field(const double* field, int voxel, std::vector<double>* dst)
+{
+    for (int s = 0; s < g_S; ++s) (*dst)[s] = field[(size_t)voxel * g_S + s];
+}
+void supply_cb(Microenvironment*, int voxel, std::vector<double>* dst) { copy_field(g_bulk_supply, voxel, dst); }
+void target_cb(Microenvironment*, int voxel, std::vector<double>* dst) { copy_field(g_bulk_target, voxel, dst); }
+void uptake_cb(Microenvironment*, int voxel, std::vector<double>* dst) { copy_field(g_bulk_uptake, voxel, dst); }
+
+}  // namespace
+
+extern "C" {
+
+/* Flags for ref_run's `phases`: which of the three step methods run each step. */
+enum { REF_BULK = 1, REF_CELLS = 2, REF_DIFFUSION = 4 };
+
+/*
+ * Runs `steps` steps of the reference on an nx*ny*nz grid of S substrates.
+ *   rho        in/out, nx*ny*nz*S doubles, voxel-major / substrate-innermost
+ *   dir_*      n_dir Dirichlet nodes (voxel index, S values each)
+ *   bulk_*     per-voxel supply rate S, target T, uptake U (nvox*S each), or NULL
+ *   agent_*    n_agents agents: position (3 each), volume, and S rates each
+ *   out_voxel  optional, n_agents ints: the voxel each agent was assigned
+ *   seconds    optional: wall-clock seconds of the step loop only
+ * Returns 0 on success, a negative code if the reference rejected the input.
+ */
+int ref_run(int nx, int ny, int nz, int S, double dx,
+            const double* D, const double* lambda, double dt, int steps, int phases,
+            double* rho,
+            int n_dir, const int* dir_idx, const double* dir_val,
+            const double* bulk_supply, const double* bulk_target, const double* bulk_uptake,
+            int n_agents, const double* agent_pos, const double* agent_volume,
+            const double* agent_secretion, const double* agent_saturation, const double* agent_uptake,
+            int* out_voxel, int nthreads, double* seconds)
+{
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+
+    Microenvironment* M = new Microenvironment;
+    set_default_microenvironment(M);
+    M->set_density(0, "s0", "dimensionless");
+    for (int s = 1; s < S; ++s) M->add_density("s", "dimensionless");
+    M->resize_space_uniform(0.0, nx * dx, 0.0, ny * dx, 0.0, nz * dx, dx);
+    if ((int)M->mesh.x_coordinates.size() != nx || (int)M->mesh.y_coordinates.size() != ny ||
+        (int)M->mesh.z_coordinates.size() != nz || M->number_of_densities() != S) {
+        delete M;
+        return -1;
+    }
+    M->diffusion_decay_solver = diffusion_decay_solver__constant_coefficients_LOD_3D;
+    for (int s = 0; s < S; ++s) {
+        M->diffusion_coefficients[s] = D[s];
+        M->decay_rates[s] = lambda[s];
+    }
+    const int nvox = nx * ny * nz;
+    for (int v = 0; v < nvox; ++v) {
+        std::vector<double>& d = M->density_vector(v);
+        for (int s = 0; s < S; ++s) d[s] = rho[(size_t)v * S + s];
+    }
+    for (int d = 0; d < n_dir; ++d) {
+        std::vector<double> value(dir_val + (size_t)d * S, dir_val + (size_t)(d + 1) * S);
+        M->add_dirichlet_node(dir_idx[d], value);
+    }
+    g_S = S;
+    g_bulk_supply = bulk_supply;
+    g_bulk_target = bulk_target;
+    g_bulk_uptake = bulk_uptake;
+    if (bulk_supply && bulk_target && bulk_uptake) {
+        M->bulk_supply_rate_function = supply_cb;
+        M->bulk_supply_target_densities_function = target_cb;
+        M->bulk_uptake_rate_function = uptake_cb;
+    }
+
+    int rc = 0;
+    std::vector<Basic_Agent*> agents;
+    agents.reserve(n_agents);
+    for (int a = 0; a < n_agents; ++a) {
+        Basic_Agent* p = create_basic_agent();
+        p->register_microenvironment(M);
+        if (!p->assign_position(agent_pos[3 * a], agent_pos[3 * a + 1], agent_pos[3 * a + 2])) {
+            rc = -2;  // the reference would leave this agent's voxel uninitialised
+            break;
+        }
+        p->volume = agent_volume[a];
+        for (int s = 0; s < S; ++s) {
+            (*p->secretion_rates)[s] = agent_secretion[(size_t)a * S + s];
+            (*p->saturation_densities)[s] = agent_saturation[(size_t)a * S + s];
+            (*p->uptake_rates)[s] = agent_uptake[(size_t)a * S + s];
+        }
+        p->set_internal_uptake_constants(dt);
+        if (out_voxel) out_voxel[a] = p->nearest_voxel_index();
+        agents.push_back(p);
+    }
+
+    if (rc == 0) {
+        auto t0 = std::chrono::steady_clock::now();
+        for (int n = 0; n < steps; ++n) {
+            if (phases & REF_BULK) M->simulate_bulk_sources_and_sinks(dt);
+            if (phases & REF_CELLS) M->simulate_cell_sources_and_sinks(agents, dt);
+            if (phases & REF_DIFFUSION) M->simulate_diffusion_decay(dt);
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+        for (int v = 0; v < nvox; ++v) {
+            std::vector<double>& d = M->density_vector(v);
+            for (int s = 0; s < S; ++s) rho[(size_t)v * S + s] = d[s];
+        }
+    }
+
+    // Tear down: the agents of this run must not leak into the next one
+    // (the reference keeps them in the global all_basic_agents list).
+    for (Basic_Agent* p : agents) delete p;
+    all_basic_agents.clear();
+    set_default_microenvironment(nullptr);
+    delete M;
+    g_bulk_supply = g_bulk_target = g_bulk_uptake = nullptr;
+    return rc;
+}
+
+}  // extern "C"

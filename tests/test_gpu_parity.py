@@ -1,0 +1,157 @@
+"""GPU parity: the HIP path through the C ABI against the golden fixtures and the oracle.
+
+Tolerances (stated per the precision of the device store):
+  FP64  max|gpu - ref| <= 1e-12 * max|ref|  (FMA and reciprocal rounding only)
+  FP32  max|gpu - ref| <= 2e-6  * max|ref|  over the few steps of a fixture
+"""
+import numpy as np
+import pytest
+
+import gen_golden
+import oracle
+from biofvm_amd import FP32, FP64, BfvmError, DeviceMicroenvironment, run_case, setup_case
+from biofvm_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = {FP64: 1e-12, FP32: 2e-6}
+FIXTURES = gen_golden.names()
+
+
+def rel_err(got, ref):
+    scale = max(np.max(np.abs(ref)), 1e-300)
+    return float(np.max(np.abs(got - ref)) / scale)
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_fixture_parity(name, precision):
+    case, expected, voxels = gen_golden.load(name)
+    if voxels is not None:
+        with DeviceMicroenvironment(case["nx"], case["ny"], case["nz"], case["S"], case["dx"],
+                                    precision) as m:
+            got_vox = m.set_agent_rates(case["agent_pos"], case["agent_volume"],
+                                        case["agent_secretion"], case["agent_saturation"],
+                                        case["agent_uptake"], case["dt"])
+        np.testing.assert_array_equal(got_vox, voxels)
+    got = run_case(case, precision)
+    err = rel_err(got, expected)
+    assert err <= TOL[precision], f"{name}: rel err {err:.3e}"
+
+
+def _random_case(rng, sources):
+    nx, ny, nz = (int(v) for v in rng.integers(1, 40, size=3))
+    S = int(rng.integers(1, 9))
+    D = 10.0 ** rng.uniform(2, 5, S)
+    lam = 10.0 ** rng.uniform(-3, 0, S)
+    case = W.make_case(nx, ny, nz, S, 10.0, D, lam, 0.01, int(rng.integers(1, 5)),
+                       W.ALL if sources else W.DIFFUSION, rng.uniform(0, 1, nx * ny * nz * S))
+    if sources:
+        case.update(W.random_bulk(rng, nx * ny * nz, S))
+        case.update(W.random_agents(rng, int(rng.integers(0, 300)), S, nx, ny, nz, 10.0,
+                                    share_voxel=int(rng.integers(0, 20))))
+        idx = W.outer_face_voxels(nx, ny, nz)
+        case.update(dir_idx=idx, dir_val=rng.uniform(0, 2, idx.size * S))
+        if rng.uniform() < 0.5:
+            case["dir_mask"] = (rng.uniform(size=S) < 0.5).astype(np.uint8)
+    return case
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_shapes_against_oracle(seed):
+    rng = np.random.default_rng(1000 + seed)
+    case = _random_case(rng, sources=seed % 2 == 1)
+    ref = oracle.oracle_run(case)
+    for precision in (FP64, FP32):
+        err = rel_err(run_case(case, precision), ref)
+        assert err <= TOL[precision], (seed, precision, err)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 127, 200, 401])
+def test_axis_lengths_each_axis(n):
+    # a long line along each axis in turn, with a short cross-section
+    rng = np.random.default_rng(n)
+    for dims in [(n, 3, 2), (3, n, 2), (2, 3, n)]:
+        case = W.make_case(*dims, 5, 10.0, [1e5, 1e2, 3e4, 7e3, 1e3], [0.1, 1.0, 0.01, 0.3, 0.001],
+                           0.01, 2, W.DIFFUSION, rng.uniform(0, 1, int(np.prod(dims)) * 5))
+        ref = oracle.oracle_run(case)
+        for precision in (FP64, FP32):
+            err = rel_err(run_case(case, precision), ref)
+            assert err <= TOL[precision], (dims, precision, err)
+
+
+def test_dirichlet_mask_parity():
+    rng = np.random.default_rng(77)
+    nx, ny, nz, S = 20, 17, 12, 4
+    idx = W.outer_face_voxels(nx, ny, nz)
+    val = np.zeros((idx.size, S))
+    val[:, 0] = 1.0
+    case = W.make_case(nx, ny, nz, S, 10.0, [1e5, 1e3, 5e4, 1e4], [0.1, 0.01, 1, 0.5], 0.01, 5,
+                       W.DIFFUSION, rng.uniform(0, 1, nx * ny * nz * S), dir_idx=idx,
+                       dir_val=val.reshape(-1), dir_mask=np.array([1, 0, 0, 0], np.uint8))
+    ref = oracle.oracle_run(case)
+    assert rel_err(run_case(case, FP64), ref) <= TOL[FP64]
+
+
+def test_agents_given_as_constants_match_rates():
+    case, expected, voxels = gen_golden.load("g4_agents")
+    dV = case["dx"] ** 3
+    t1, t2 = oracle.agent_constants(case["S"], case["dt"], dV, case["agent_volume"],
+                                    case["agent_secretion"], case["agent_saturation"],
+                                    case["agent_uptake"])
+    with setup_case({k: v for k, v in case.items() if not k.startswith("agent_")}, FP64) as m:
+        m.set_agents(voxels, t1, t2)
+        m.run(case["dt"], case["steps"], case["phases"])
+        got = m.download()
+    assert rel_err(got, expected) <= TOL[FP64]
+
+
+def test_non_uniform_mesh_is_refused_and_state_kept():
+    rng = np.random.default_rng(5)
+    rho = rng.uniform(0, 1, 6 * 5 * 4)
+    with DeviceMicroenvironment(6, 5, 4, 1, 10.0, FP64, dy=12.0) as m:
+        m.set_coefficients([1e3], [0.1])
+        m.upload(rho)
+        with pytest.raises(BfvmError) as ei:
+            m.step(0.01, W.DIFFUSION)
+        assert ei.value.code == -4
+        np.testing.assert_array_equal(m.download(), rho)
+
+
+def test_argument_errors():
+    with DeviceMicroenvironment(4, 4, 4, 2, 10.0, FP64) as m:
+        with pytest.raises(BfvmError):
+            m.step(0.01, W.DIFFUSION)  # coefficients not set
+        with pytest.raises(BfvmError):
+            m.set_dirichlet([64], [1.0, 1.0])
+        with pytest.raises(BfvmError):
+            m.set_agent_rates([1.0, 1.0, 40.5], [1.0], [1, 1], [1, 1], [1, 1], 0.01)
+        with pytest.raises(BfvmError):
+            m.set_agents([-1], [0.0, 0.0], [1.0, 1.0])
+
+
+def test_clearing_sources():
+    rng = np.random.default_rng(8)
+    case = W.make_case(6, 6, 6, 2, 10.0, [1e3, 1e4], [0.1, 0.2], 0.01, 2, W.ALL,
+                       rng.uniform(0, 1, 6 ** 3 * 2))
+    ref = oracle.oracle_run(case)
+    full = dict(case, **W.random_bulk(rng, 6 ** 3, 2), **W.random_agents(rng, 30, 2, 6, 6, 6, 10.0))
+    with setup_case(full, FP64) as m:
+        m.set_bulk(None, None, None)
+        m.set_agents([], [], [])
+        m.run(case["dt"], case["steps"], case["phases"])
+        assert rel_err(m.download(), ref) <= TOL[FP64]
+
+
+def test_step_by_step_equals_run_and_timed_run():
+    case, expected, _ = gen_golden.load("g8_shell")
+    with setup_case(case, FP64) as m:
+        for _ in range(case["steps"]):
+            m.step(case["dt"], case["phases"])
+        a = m.download()
+    with setup_case(case, FP64) as m:
+        ms, per = m.timed_run(case["dt"], case["steps"], case["phases"], per_kernel=True)
+        b = m.download()
+    assert ms > 0 and per["sweep_x"][1] == case["steps"] and per["dirichlet"][1] == 3 * case["steps"]
+    np.testing.assert_array_equal(a, b)
+    assert rel_err(a, expected) <= TOL[FP64]
