@@ -31,17 +31,17 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(case, w):
-    """Per-step algorithmic HBM bytes: 3 sweeps x (1R+1W) of the store, bulk
-    fields A/B read once, agents (voxel key + temp1/temp2), Dirichlet nodes
-    (index + values) before each sweep."""
+    """Per-step algorithmic HBM bytes: 3 sweeps x (1R+1W) of the store (w bytes
+    per value), bulk terms A/B read once (FP64), agents (voxel key + FP64
+    temp1/temp2), Dirichlet nodes (index + values) before each sweep."""
     nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
     S = case["S"]
     ph = case["phases"]
     b = nvs * 6 * w
     if ph & 1 and case.get("bulk_supply") is not None:
-        b += nvs * 2 * w  # read A and B (the store pass is shared with the x-sweep when fused)
+        b += nvs * 2 * 8  # read A and B (FP64); the store pass is shared with the x-sweep when fused
     if ph & 2 and case.get("agent_volume") is not None:
-        b += len(case["agent_volume"]) * (4 + 2 * S * w)
+        b += len(case["agent_volume"]) * (4 + 2 * S * 8)  # voxel key + FP64 temp1/temp2
     if ph & 4 and case.get("dir_idx") is not None:
         b += 3 * len(case["dir_idx"]) * (4 + S * w)
     return b
@@ -53,9 +53,9 @@ def kernel_bytes(case, kernel, w):
     if kernel.startswith("sweep"):
         return 2 * nvs * w
     if kernel == "bulk":
-        return 4 * nvs * w
+        return nvs * (2 * w + 2 * 8)
     if kernel == "agents":
-        return len(case["agent_volume"]) * (4 + 2 * case["S"] * w)
+        return len(case["agent_volume"]) * (4 + 2 * case["S"] * 8)
     if kernel == "dirichlet":
         return len(case["dir_idx"]) * (4 + case["S"] * w)
     return 0

@@ -11,15 +11,22 @@
 
 namespace bfvm {
 
-// Thomas factorisation of one axis for all substrates, in device precision.
-// Element (i, s) of an axis table is stored at [i*S + s].
+// Thomas factorisation of one axis for all substrates, FP64 on the device
+// for either store precision.  Element (i, s) of a table is at [i*S + s].
+//
+// The segmented sweeps use the Thomas recurrences in multiplicative form,
+//   forward  y_i = x_i*r_i + a_i*y_{i-1},   r_i = 1/denom_i, a_i = c1/denom_i
+//   backward z_i = y_i + b_i*z_{i+1},       b_i = -c_i
+// with r, a, b computed in FP64 on the host.
 struct AxisCoef {
     int n = 0;
-    void* denom = nullptr;      // denom_i                  (BioFVM_solvers.cpp:130-141)
-    void* inv_denom = nullptr;  // 1/denom_i, rounded once from FP64
-    void* c = nullptr;          // c_i = -c1/denom_i        (BioFVM_solvers.cpp:137-141)
-    int converged_from = 0;     // first i with (denom_i, c_i) equal to their interior
-                                // fixed point for every i in [converged_from, n-1)
+    void* denom = nullptr;  // denom_i                  (BioFVM_solvers.cpp:130-141)
+    void* c = nullptr;      // c_i = -c1/denom_i        (BioFVM_solvers.cpp:137-141)
+    void* r = nullptr;      // 1/denom_i
+    void* a = nullptr;      // c1/denom_i
+    void* b = nullptr;      // -c_i
+    int k0 = 0;             // rows [k0, n-1) of (r, a, b) all equal row k0 (the
+                            // converged interior); rows < k0 and row n-1 differ
 };
 
 struct Env {
@@ -39,7 +46,7 @@ struct Env {
     std::vector<double> D, lambda;
     bool coef_valid = false;
     double coef_dt = 0;
-    void* c1 = nullptr;  // dt*D/dx^2 per substrate (device precision)
+    void* c1 = nullptr;  // dt*D/dx^2 per substrate (FP64)
     AxisCoef ax[3];
 
     // Dirichlet nodes.
@@ -48,8 +55,8 @@ struct Env {
     void* dir_val = nullptr;          // n_dir*S, device precision
     unsigned char* dir_mask = nullptr;  // S bytes
 
-    // Bulk fields (FP64 on the device) and the implicit-step terms built
-    // from them for one dt: A = dt*(T*S), B = (U+S)*dt + 1 (device precision).
+    // Bulk fields and the implicit-step terms built from them for one dt:
+    // A = dt*(T*S), B = (U+S)*dt + 1 (all FP64 on the device).
     bool have_bulk = false;
     double* bulk_supply = nullptr;
     double* bulk_target = nullptr;
@@ -64,7 +71,7 @@ struct Env {
     int n_agent_voxels = 0;
     int* agent_seg_voxel = nullptr;  // n_agent_voxels
     int* agent_seg_start = nullptr;  // n_agent_voxels + 1
-    void* agent_t1 = nullptr;        // n_agents*S, grouped order
+    void* agent_t1 = nullptr;        // n_agents*S, grouped order, FP64
     void* agent_t2 = nullptr;
 
     // Per-launch timing (bfvm_timed_run with per-kernel events).

@@ -61,6 +61,18 @@ hipError_t alloc_values(Env& e, void** p, size_t n)
     return hipMalloc(p, std::max<size_t>(n, 1) * e.elem);
 }
 
+// FP64 tables (coefficients, source terms) are kept in FP64 whatever the
+// store precision.
+hipError_t upload_f64(void* dst, const double* src, size_t n)
+{
+    return hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice);
+}
+
+hipError_t alloc_f64(void** p, size_t n)
+{
+    return hipMalloc(p, std::max<size_t>(n, 1) * sizeof(double));
+}
+
 #pragma clang fp contract(off)
 // Thomas factorisation of one axis in FP64, in the reference's operation
 // order (BioFVM_solvers.cpp:104-142).  denom/c are (n, S), element [i*S+s].
@@ -134,35 +146,42 @@ int ensure_coefficients(Env& e, double dt)
 {
     if (e.coef_valid && e.coef_dt == dt) return BFVM_OK;
     const int n_axis[3] = {e.nx, e.ny, e.nz};
+    const int S = e.S;
     std::vector<double> denom, c, c1v;
     for (int a = 0; a < 3; ++a) {
-        thomas_axis(n_axis[a], e.S, e.dx, dt, e.D, e.lambda, denom, c, c1v);
+        thomas_axis(n_axis[a], S, e.dx, dt, e.D, e.lambda, denom, c, c1v);
         bfvm::AxisCoef& ax = e.ax[a];
-        size_t m = (size_t)n_axis[a] * e.S;
-        std::vector<double> inv(m);
-        for (size_t q = 0; q < m; ++q) inv[q] = 1.0 / denom[q];
-        HIP_OR_FAIL(&e, upload_values(e, ax.denom, denom.data(), m));
-        HIP_OR_FAIL(&e, upload_values(e, ax.inv_denom, inv.data(), m));
-        HIP_OR_FAIL(&e, upload_values(e, ax.c, c.data(), m));
-        // First index from which (denom, c) hold their interior fixed point
-        // exactly up to the last node (which uses c3a and differs).
-        int n = n_axis[a];
-        int from = std::max(n - 1, 0);
-        if (n >= 3) {
-            int last = n - 2;
-            from = last;
-            while (from > 0) {
-                bool same = true;
-                for (int s = 0; s < e.S && same; ++s)
-                    same = denom[(size_t)(from - 1) * e.S + s] == denom[(size_t)last * e.S + s] &&
-                           c[(size_t)(from - 1) * e.S + s] == c[(size_t)last * e.S + s];
-                if (!same) break;
-                --from;
-            }
+        const int n = n_axis[a];
+        size_t m = (size_t)n * S;
+        std::vector<double> r(m), av(m), b(m);
+        for (size_t q = 0; q < m; ++q) {
+            r[q] = 1.0 / denom[q];
+            av[q] = c1v[q % S] / denom[q];
+            b[q] = -c[q];
         }
-        ax.converged_from = from;
+        HIP_OR_FAIL(&e, upload_f64(ax.denom, denom.data(), m));
+        HIP_OR_FAIL(&e, upload_f64(ax.c, c.data(), m));
+        HIP_OR_FAIL(&e, upload_f64(ax.r, r.data(), m));
+        HIP_OR_FAIL(&e, upload_f64(ax.a, av.data(), m));
+        HIP_OR_FAIL(&e, upload_f64(ax.b, b.data(), m));
+        // k0: first row from which (r, a, b) equal their value at row n-2 for
+        // every row up to n-2 (the converged interior of the factorisation;
+        // the last row uses 1+c1+c2 and always differs).
+        auto row_equal = [&](int p, int q) {
+            for (int s = 0; s < S; ++s) {
+                size_t i = (size_t)p * S + s, j = (size_t)q * S + s;
+                if (r[i] != r[j] || av[i] != av[j] || b[i] != b[j]) return false;
+            }
+            return true;
+        };
+        int k0 = std::max(n - 1, 0);
+        if (n >= 3) {
+            k0 = n - 2;
+            while (k0 > 0 && row_equal(k0 - 1, n - 2)) --k0;
+        }
+        ax.k0 = k0;
     }
-    HIP_OR_FAIL(&e, upload_values(e, e.c1, c1v.data(), e.S));
+    HIP_OR_FAIL(&e, upload_f64(e.c1, c1v.data(), S));
     e.coef_valid = true;
     e.coef_dt = dt;
     return BFVM_OK;
@@ -260,13 +279,13 @@ int set_agents_impl(Env& e, int n, const int* voxel, const double* t1, const dou
     int ng = (int)gvox.size();
     HIP_OR_FAIL(&e, hipMalloc((void**)&e.agent_seg_voxel, sizeof(int) * ng));
     HIP_OR_FAIL(&e, hipMalloc((void**)&e.agent_seg_start, sizeof(int) * (ng + 1)));
-    HIP_OR_FAIL(&e, alloc_values(e, &e.agent_t1, (size_t)n * S));
-    HIP_OR_FAIL(&e, alloc_values(e, &e.agent_t2, (size_t)n * S));
+    HIP_OR_FAIL(&e, alloc_f64(&e.agent_t1, (size_t)n * S));
+    HIP_OR_FAIL(&e, alloc_f64(&e.agent_t2, (size_t)n * S));
     HIP_OR_FAIL(&e, hipMemcpy(e.agent_seg_voxel, gvox.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
     HIP_OR_FAIL(&e, hipMemcpy(e.agent_seg_start, gstart.data(), sizeof(int) * (ng + 1),
                               hipMemcpyHostToDevice));
-    HIP_OR_FAIL(&e, upload_values(e, e.agent_t1, s1.data(), (size_t)n * S));
-    HIP_OR_FAIL(&e, upload_values(e, e.agent_t2, s2.data(), (size_t)n * S));
+    HIP_OR_FAIL(&e, upload_f64(e.agent_t1, s1.data(), (size_t)n * S));
+    HIP_OR_FAIL(&e, upload_f64(e.agent_t2, s2.data(), (size_t)n * S));
     e.n_agents = n;
     e.n_agent_voxels = ng;
     return BFVM_OK;
@@ -305,7 +324,7 @@ int bfvm_create_mesh(int nx, int ny, int nz, int n_substrates, double dx, double
         (st = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess ||
         (st = hipMalloc(&e->rho, e->nvs * e->elem)) != hipSuccess ||
         (st = hipMemset(e->rho, 0, e->nvs * e->elem)) != hipSuccess ||
-        (st = hipMalloc(&e->c1, e->S * e->elem)) != hipSuccess ||
+        (st = hipMalloc(&e->c1, e->S * sizeof(double))) != hipSuccess ||
         (st = hipMalloc((void**)&e->dir_mask, e->S)) != hipSuccess) {
         int rc = hip_fail(nullptr, st, "bfvm_create");
         bfvm_destroy(e);
@@ -315,9 +334,11 @@ int bfvm_create_mesh(int nx, int ny, int nz, int n_substrates, double dx, double
     for (int a = 0; a < 3; ++a) {
         size_t m = (size_t)n_axis[a] * e->S;
         e->ax[a].n = n_axis[a];
-        if ((st = alloc_values(*e, &e->ax[a].denom, m)) != hipSuccess ||
-            (st = alloc_values(*e, &e->ax[a].inv_denom, m)) != hipSuccess ||
-            (st = alloc_values(*e, &e->ax[a].c, m)) != hipSuccess) {
+        if ((st = alloc_f64(&e->ax[a].denom, m)) != hipSuccess ||
+            (st = alloc_f64(&e->ax[a].c, m)) != hipSuccess ||
+            (st = alloc_f64(&e->ax[a].r, m)) != hipSuccess ||
+            (st = alloc_f64(&e->ax[a].a, m)) != hipSuccess ||
+            (st = alloc_f64(&e->ax[a].b, m)) != hipSuccess) {
             int rc = hip_fail(nullptr, st, "bfvm_create");
             bfvm_destroy(e);
             return rc;
@@ -341,8 +362,10 @@ int bfvm_destroy(bfvm_handle h)
     dfree(e->c1);
     for (auto& a : e->ax) {
         dfree(a.denom);
-        dfree(a.inv_denom);
         dfree(a.c);
+        dfree(a.r);
+        dfree(a.a);
+        dfree(a.b);
     }
     dfree_t(e->dir_idx);
     dfree(e->dir_val);
@@ -442,8 +465,8 @@ int bfvm_set_bulk(bfvm_handle h, const double* supply_rate, const double* supply
         HIP_OR_FAIL(h, hipMalloc((void**)&h->bulk_supply, bytes));
         HIP_OR_FAIL(h, hipMalloc((void**)&h->bulk_target, bytes));
         HIP_OR_FAIL(h, hipMalloc((void**)&h->bulk_uptake, bytes));
-        HIP_OR_FAIL(h, alloc_values(*h, &h->bulk_A, h->nvs));
-        HIP_OR_FAIL(h, alloc_values(*h, &h->bulk_B, h->nvs));
+        HIP_OR_FAIL(h, alloc_f64(&h->bulk_A, h->nvs));
+        HIP_OR_FAIL(h, alloc_f64(&h->bulk_B, h->nvs));
     }
     HIP_OR_FAIL(h, hipMemcpy(h->bulk_supply, supply_rate, bytes, hipMemcpyHostToDevice));
     HIP_OR_FAIL(h, hipMemcpy(h->bulk_target, supply_target, bytes, hipMemcpyHostToDevice));

@@ -29,55 +29,57 @@ __global__ void k_dirichlet(T* __restrict__ rho, int S, int n_dir, const int* __
 // in FP64 with explicit round-to-nearest operations so that no FMA
 // contraction changes the reference's rounding:
 //   A = dt*(T*S),  B = ((U+S)*dt) + 1
-template <typename T>
+// A and B stay FP64 for either store precision: rounding them to FP32 would
+// bias every step's source term the same way.
 __global__ void k_bulk_prepare(size_t nvs, double dt, const double* __restrict__ sup,
                                const double* __restrict__ tgt, const double* __restrict__ upt,
-                               T* __restrict__ A, T* __restrict__ B)
+                               double* __restrict__ A, double* __restrict__ B)
 {
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nvs) return;
     double st = __dmul_rn(tgt[e], sup[e]);
-    A[e] = (T)__dmul_rn(dt, st);
+    A[e] = __dmul_rn(dt, st);
     double us = __dadd_rn(upt[e], sup[e]);
-    B[e] = (T)__dadd_rn(__dmul_rn(us, dt), 1.0);
+    B[e] = __dadd_rn(__dmul_rn(us, dt), 1.0);
 }
 
-// rho = (rho + A) / B  (BioFVM_microenvironment.cpp:604,609).  Four values
-// per thread when the store is 16-byte aligned (always, from hipMalloc).
+// rho = (rho + A) / B  (BioFVM_microenvironment.cpp:604,609), evaluated in
+// FP64 and rounded once to the store.  Four values per thread.
 template <typename T>
-__global__ void k_bulk(T* __restrict__ rho, const T* __restrict__ A, const T* __restrict__ B,
-                       size_t nvs)
+__global__ void k_bulk(T* __restrict__ rho, const double* __restrict__ A,
+                       const double* __restrict__ B, size_t nvs)
 {
     size_t e = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (e + 4 <= nvs) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) rho[e + q] = (rho[e + q] + A[e + q]) / B[e + q];
+        for (int q = 0; q < 4; ++q) rho[e + q] = (T)(((double)rho[e + q] + A[e + q]) / B[e + q]);
     } else {
-        for (; e < nvs; ++e) rho[e] = (rho[e] + A[e]) / B[e];
+        for (; e < nvs; ++e) rho[e] = (T)(((double)rho[e] + A[e]) / B[e]);
     }
 }
 
 // Basic_Agent::simulate_secretion_and_uptake (BioFVM_basic_agent.cpp:229-238)
 // for every agent, grouped by voxel: one thread per (voxel group, substrate)
 // applies the group's agents in list order, rho = (rho + t1)/t2, so the
-// result is the reference's single-threaded result, with no atomics.
+// result is the reference's single-threaded result, with no atomics.  The
+// constants are FP64 and the chain runs in FP64 for either store precision.
 template <typename T>
 __global__ void k_agents(T* __restrict__ rho, int S, int n_groups, const int* __restrict__ gvox,
-                         const int* __restrict__ gstart, const T* __restrict__ t1,
-                         const T* __restrict__ t2)
+                         const int* __restrict__ gstart, const double* __restrict__ t1,
+                         const double* __restrict__ t2)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)n_groups * S) return;
     int g = (int)(t / S);
     int s = (int)(t - (size_t)g * S);
     T* r = rho + (size_t)gvox[g] * S + s;
-    T v = *r;
+    double v = (double)*r;
     int a1 = gstart[g + 1];
     for (int a = gstart[g]; a < a1; ++a) {
         size_t q = (size_t)a * S + s;
         v = (v + t1[q]) / t2[q];
     }
-    *r = v;
+    *r = (T)v;
 }
 
 }  // namespace
@@ -97,14 +99,9 @@ hipError_t launch_dirichlet(Env& e)
 
 hipError_t launch_bulk_prepare(Env& e, double dt)
 {
-    if (e.precision == BFVM_FP64)
-        k_bulk_prepare<double><<<grid_for(e.nvs), kBlock, 0, e.stream>>>(
-            e.nvs, dt, e.bulk_supply, e.bulk_target, e.bulk_uptake, (double*)e.bulk_A,
-            (double*)e.bulk_B);
-    else
-        k_bulk_prepare<float><<<grid_for(e.nvs), kBlock, 0, e.stream>>>(
-            e.nvs, dt, e.bulk_supply, e.bulk_target, e.bulk_uptake, (float*)e.bulk_A,
-            (float*)e.bulk_B);
+    k_bulk_prepare<<<grid_for(e.nvs), kBlock, 0, e.stream>>>(e.nvs, dt, e.bulk_supply, e.bulk_target,
+                                                          e.bulk_uptake, (double*)e.bulk_A,
+                                                          (double*)e.bulk_B);
     return hipGetLastError();
 }
 
@@ -115,8 +112,8 @@ hipError_t launch_bulk(Env& e)
         k_bulk<double><<<grid, kBlock, 0, e.stream>>>((double*)e.rho, (const double*)e.bulk_A,
                                                       (const double*)e.bulk_B, e.nvs);
     else
-        k_bulk<float><<<grid, kBlock, 0, e.stream>>>((float*)e.rho, (const float*)e.bulk_A,
-                                                     (const float*)e.bulk_B, e.nvs);
+        k_bulk<float><<<grid, kBlock, 0, e.stream>>>((float*)e.rho, (const double*)e.bulk_A,
+                                                     (const double*)e.bulk_B, e.nvs);
     return hipGetLastError();
 }
 
@@ -131,7 +128,7 @@ hipError_t launch_agents(Env& e)
     else
         k_agents<float><<<grid_for(n), kBlock, 0, e.stream>>>(
             (float*)e.rho, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
-            (const float*)e.agent_t1, (const float*)e.agent_t2);
+            (const double*)e.agent_t1, (const double*)e.agent_t2);
     return hipGetLastError();
 }
 

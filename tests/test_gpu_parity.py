@@ -2,7 +2,7 @@
 
 Tolerances (stated per the precision of the device store):
   FP64  max|gpu - ref| <= 1e-12 * max|ref|  (FMA and reciprocal rounding only)
-  FP32  max|gpu - ref| <= 2e-6  * max|ref|  over the few steps of a fixture
+  FP32  max|gpu - ref| <= 1e-5  * max|ref|  (FP32 store rounded once per sweep; FP64 arithmetic)
 """
 import numpy as np
 import pytest
@@ -14,7 +14,7 @@ from biofvm_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-TOL = {FP64: 1e-12, FP32: 2e-6}
+TOL = {FP64: 1e-12, FP32: 1e-5}
 FIXTURES = gen_golden.names()
 
 
