@@ -65,6 +65,7 @@ def time_precision(case, precision, steps, warmup):
     from biofvm_amd import setup_case
     m = setup_case(case, precision)
     dt, ph = case["dt"], case["phases"]
+    m.autotune(dt)  # launch shapes, picked on a scratch copy outside the timed region
     m.run(dt, warmup, ph)
     m.sync()
     t0 = time.perf_counter()
@@ -148,6 +149,7 @@ def main():
     w_main = 4 if main_prec == FP32 else 8
 
     m32, wall, ev_ms, per, n_done = time_precision(case, main_prec, steps, warmup)
+    plans = m32.sweep_plans()
     value = nvs * steps / wall
     step_ms = 1e3 * wall / steps
     dom = max(per, key=lambda k: per[k][0])
@@ -200,6 +202,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg},
         "kernels_ms_per_launch": {k: (v[0] / v[1] if v[1] else None) for k, v in per.items()},
+        "sweep_plans_lines_x_nodes": {"x": plans[0], "y": plans[1], "z": plans[2]},
     }
     line.update(out_extra)
     if not args.no_cpu_baseline:

@@ -29,6 +29,12 @@ struct AxisCoef {
                             // converged interior); rows < k0 and row n-1 differ
 };
 
+// Launch shape of the segmented sweep of one axis (see lod.hip).
+struct SweepPlan {
+    int cols = 0;  // lines per block (16, 32 or 64); 0 = default
+    int L = 0;     // nodes per segment
+};
+
 struct Env {
     int nx = 0, ny = 0, nz = 0, S = 0;
     double dx = 0, dy = 0, dz = 0;
@@ -48,6 +54,7 @@ struct Env {
     double coef_dt = 0;
     void* c1 = nullptr;  // dt*D/dx^2 per substrate (FP64)
     AxisCoef ax[3];
+    SweepPlan plan[3];  // per-axis launch shape chosen by the autotuner
 
     // Dirichlet nodes.
     int n_dir = 0;
@@ -90,6 +97,10 @@ hipError_t launch_bulk_prepare(Env& e, double dt);
 hipError_t launch_bulk(Env& e);
 hipError_t launch_agents(Env& e);
 hipError_t launch_sweep(Env& e, int axis);
+// Valid segmented shapes for a line of n nodes, for the autotuner.
+std::vector<SweepPlan> sweep_candidates(int n);
+// The launch shape a sweep of `axis` uses now (autotuned or default).
+SweepPlan effective_plan(const Env& e, int axis);
 
 }  // namespace bfvm
 

@@ -157,7 +157,9 @@ int ensure_coefficients(Env& e, double dt)
         for (size_t q = 0; q < m; ++q) {
             r[q] = 1.0 / denom[q];
             av[q] = c1v[q % S] / denom[q];
-            b[q] = -c[q];
+            // c at the last node is never used by the back substitution
+            // (BioFVM_solvers.cpp:141): b there is 0, so z_{n-1} = y_{n-1}
+            b[q] = q >= (size_t)(n - 1) * S ? 0.0 : -c[q];
         }
         HIP_OR_FAIL(&e, upload_f64(ax.denom, denom.data(), m));
         HIP_OR_FAIL(&e, upload_f64(ax.c, c.data(), m));
@@ -590,6 +592,65 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
     e.ev_marks.clear();
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    return BFVM_OK;
+}
+
+int bfvm_autotune(bfvm_handle h, double dt)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    Env& e = *h;
+    if (e.D.empty()) return fail(&e, BFVM_E_STATE, "coefficients not set (bfvm_set_coefficients)");
+    if (!e.uniform) return fail(&e, BFVM_E_MESH, "non-uniform mesh: the LOD solver is not used");
+    int rc = ensure_coefficients(e, dt);
+    if (rc) return rc;
+    void* scratch = nullptr;
+    HIP_OR_FAIL(&e, hipMalloc(&scratch, e.nvs * e.elem));
+    HIP_OR_FAIL(&e, hipMemcpyAsync(scratch, e.rho, e.nvs * e.elem, hipMemcpyDeviceToDevice, e.stream));
+    void* keep = e.rho;
+    e.rho = scratch;
+    hipEvent_t t0, t1;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    hipError_t st = hipSuccess;
+    const int n_axis[3] = {e.nx, e.ny, e.nz};
+    for (int axis = 0; axis < 3 && st == hipSuccess; ++axis) {
+        bfvm::SweepPlan best{};
+        float best_ms = 1e30f;
+        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(n_axis[axis])) {
+            e.plan[axis] = cand;
+            if ((st = bfvm::launch_sweep(e, axis)) != hipSuccess) break;  // warm-up
+            (void)hipEventRecord(t0, e.stream);
+            for (int rep = 0; rep < 3 && st == hipSuccess; ++rep) st = bfvm::launch_sweep(e, axis);
+            (void)hipEventRecord(t1, e.stream);
+            if (st != hipSuccess || (st = hipEventSynchronize(t1)) != hipSuccess) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = cand;
+            }
+        }
+        e.plan[axis] = best;
+    }
+    (void)hipStreamSynchronize(e.stream);
+    e.rho = keep;
+    (void)hipFree(scratch);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (st != hipSuccess) {
+        for (auto& p : e.plan) p = bfvm::SweepPlan{};
+        return hip_fail(&e, st, "bfvm_autotune");
+    }
+    return BFVM_OK;
+}
+
+int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
+    bfvm::SweepPlan p = bfvm::effective_plan(*h, axis);
+    if (lines_per_block) *lines_per_block = p.cols;
+    if (nodes_per_segment) *nodes_per_segment = p.L;
     return BFVM_OK;
 }
 

@@ -16,16 +16,15 @@
 // reference's thomas_denom / thomas_c (:130-141) computed in FP64.
 //
 // Coefficients: rows [k0, n-1) of the tables are identical (the factorisation
-// converges to its interior fixed point), so a thread keeps that row in
-// registers and reads the table only at rows < k0 and at row n-1.
+// converges to its interior fixed point), so a segment inside that range
+// keeps one row in registers; a segment holding a row < k0 or the last row
+// reads the tables.
 //
 // Kernels
-//   k_sweep_cols  y and z sweeps: lanes run across 64 adjacent columns (i,s)
-//                 of a row (coalesced), the block's waves are the segments.
-//   k_sweep_rows  x sweep: a tile of whole x-rows is staged through LDS with
-//                 coalesced loads/stores; threads are (row, segment, substrate).
+//   k_sweep       all three sweeps: 32 lines per block, up to 32 segments per
+//                 line, one thread per (line, segment); see its comment.
 //   k_line_sweep  one thread per whole line, for lines longer than the
-//                 segmented kernels hold (n > 16*64).
+//                 segmented kernel holds (n > 32*32).
 #include <algorithm>
 
 #include "bfvm_internal.h"
@@ -34,53 +33,61 @@ namespace bfvm {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxSeg = 16;  // segments per line in the segmented kernels
+constexpr int kMaxThreads = 1024;  // block = COLS columns x nseg segments
 
-// Coefficient row p for substrate s: the converged interior from registers,
-// else the table.  Tables and arithmetic are FP64 for either store
-// precision; an FP32 store is rounded once per sweep, on the way out.
-struct Coef {
-    const double* __restrict__ r;
-    const double* __restrict__ a;
-    const double* __restrict__ b;
-    double r_in, a_in, b_in;
-    int k0, n, S, s;
-
-    __device__ __forceinline__ bool special(int p) const { return p < k0 || p == n - 1; }
+// One thread's segment: nodes [p0, p0+cnt) of one (line, substrate).
+// Tables are FP64 for either store precision; an FP32 store is rounded once
+// per sweep, on the way out.
+struct SegCtx {
+    const double* __restrict__ tr;  // r_i = 1/denom_i           [i*S + s]
+    const double* __restrict__ ta;  // a_i = c1/denom_i
+    const double* __restrict__ tb;  // b_i = -c_i (0 at the last node)
+    double r_in, a_in, b_in;        // the converged interior row
+    int p0, cnt, S, s;
+    bool special;                   // segment holds a row < k0 or the last row
 };
 
-__device__ __forceinline__ Coef make_coef(const double* r, const double* a, const double* b, int k0,
-                                          int n, int S, int s)
+__device__ __forceinline__ SegCtx make_ctx(const double* tr, const double* ta, const double* tb,
+                                           int k0, int n, int S, int s, int p0, int cnt)
 {
-    Coef c;
-    c.r = r; c.a = a; c.b = b; c.k0 = k0; c.n = n; c.S = S; c.s = s;
-    int row = k0 < n - 1 ? k0 : n - 1;
-    c.r_in = r[row * S + s];
-    c.a_in = a[row * S + s];
-    c.b_in = b[row * S + s];
+    SegCtx c;
+    c.tr = tr; c.ta = ta; c.tb = tb;
+    c.S = S; c.s = s; c.p0 = p0; c.cnt = cnt;
+    const int row = k0 < n - 1 ? k0 : n - 1;
+    c.r_in = tr[row * S + s];
+    c.a_in = ta[row * S + s];
+    c.b_in = tb[row * S + s];
+    c.special = cnt > 0 && (p0 < k0 || p0 + cnt >= n);
     return c;
 }
 
-// Forward elimination over v[0..cnt) (positions p0..) with zero carry-in.
-// Returns yhat at the segment end and the product of a over the segment.
+// Forward elimination over v[0..cnt) with zero carry-in: yhat, its value at
+// the segment end and the product of a over the segment.  Interior segments
+// use the register constants; the others read the tables row by row
+// (branch-free, so the loads pipeline ahead of the recurrence).
 template <int LMAX>
-__device__ __forceinline__ void seg_forward(double (&v)[LMAX], int cnt, int p0, const Coef& c,
-                                            double& yend, double& pa)
+__device__ __forceinline__ void seg_forward(double (&v)[LMAX], const SegCtx& c, double& yend,
+                                            double& pa)
 {
     double y = 0.0, P = 1.0;
+    if (!c.special) {
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t) {
-        if (t < cnt) {
-            int p = p0 + t;
-            double r = c.r_in, a = c.a_in;
-            if (c.special(p)) {
-                r = c.r[p * c.S + c.s];
-                a = c.a[p * c.S + c.s];
+        for (int t = 0; t < LMAX; ++t)
+            if (t < c.cnt) {
+                y = fma(c.a_in, y, v[t] * c.r_in);
+                v[t] = y;
+                P *= c.a_in;
             }
-            y = v[t] * r + a * y;
-            v[t] = y;
-            P *= a;
-        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < LMAX; ++t)
+            if (t < c.cnt) {
+                const int q = (c.p0 + t) * c.S + c.s;
+                const double a = c.ta[q];
+                y = fma(a, y, v[t] * c.tr[q]);
+                v[t] = y;
+                P *= a;
+            }
     }
     yend = y;
     pa = P;
@@ -88,38 +95,50 @@ __device__ __forceinline__ void seg_forward(double (&v)[LMAX], int cnt, int p0, 
 
 // y_i += (prod_{m=p0..i} a_m) * Y
 template <int LMAX>
-__device__ __forceinline__ void seg_fix_forward(double (&v)[LMAX], int cnt, int p0, const Coef& c,
-                                                double Y)
+__device__ __forceinline__ void seg_fix_forward(double (&v)[LMAX], const SegCtx& c, double Y)
 {
     double P = 1.0;
+    if (!c.special) {
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t) {
-        if (t < cnt) {
-            int p = p0 + t;
-            double a = c.special(p) ? c.a[p * c.S + c.s] : c.a_in;
-            P *= a;
-            v[t] += P * Y;
-        }
+        for (int t = 0; t < LMAX; ++t)
+            if (t < c.cnt) {
+                P *= c.a_in;
+                v[t] = fma(P, Y, v[t]);
+            }
+    } else {
+#pragma unroll
+        for (int t = 0; t < LMAX; ++t)
+            if (t < c.cnt) {
+                P *= c.ta[(c.p0 + t) * c.S + c.s];
+                v[t] = fma(P, Y, v[t]);
+            }
     }
 }
 
-// Back substitution over v[0..cnt) with zero carry-in from above.  Returns
-// zhat at the segment start and the product of b over the segment.
+// Back substitution over v[0..cnt) with zero carry-in from above: zhat, its
+// value at the segment start and the product of b over the segment.
 template <int LMAX>
-__device__ __forceinline__ void seg_backward(double (&v)[LMAX], int cnt, int p0, const Coef& c,
-                                             double& zstart, double& pb)
+__device__ __forceinline__ void seg_backward(double (&v)[LMAX], const SegCtx& c, double& zstart,
+                                             double& pb)
 {
     double z = 0.0, Q = 1.0;
+    if (!c.special) {
 #pragma unroll
-    for (int t = LMAX - 1; t >= 0; --t) {
-        if (t < cnt) {
-            int p = p0 + t;
-            double b = c.special(p) ? c.b[p * c.S + c.s] : c.b_in;
-            if (p == c.n - 1) b = 0.0;  // z_{n-1} = y_{n-1}
-            z = v[t] + b * z;
-            v[t] = z;
-            Q *= b;
-        }
+        for (int t = LMAX - 1; t >= 0; --t)
+            if (t < c.cnt) {
+                z = fma(c.b_in, z, v[t]);
+                v[t] = z;
+                Q *= c.b_in;
+            }
+    } else {
+#pragma unroll
+        for (int t = LMAX - 1; t >= 0; --t)
+            if (t < c.cnt) {
+                const double b = c.tb[(c.p0 + t) * c.S + c.s];
+                z = fma(b, z, v[t]);
+                v[t] = z;
+                Q *= b;
+            }
     }
     zstart = z;
     pb = Q;
@@ -127,41 +146,58 @@ __device__ __forceinline__ void seg_backward(double (&v)[LMAX], int cnt, int p0,
 
 // z_i += (prod_{m=i..q-1} b_m) * Z
 template <int LMAX>
-__device__ __forceinline__ void seg_fix_backward(double (&v)[LMAX], int cnt, int p0, const Coef& c,
-                                                 double Z)
+__device__ __forceinline__ void seg_fix_backward(double (&v)[LMAX], const SegCtx& c, double Z)
 {
     double Q = 1.0;
+    if (!c.special) {
 #pragma unroll
-    for (int t = LMAX - 1; t >= 0; --t) {
-        if (t < cnt) {
-            int p = p0 + t;
-            double b = c.special(p) ? c.b[p * c.S + c.s] : c.b_in;
-            Q *= b;
-            v[t] += Q * Z;
-        }
+        for (int t = LMAX - 1; t >= 0; --t)
+            if (t < c.cnt) {
+                Q *= c.b_in;
+                v[t] = fma(Q, Z, v[t]);
+            }
+    } else {
+#pragma unroll
+        for (int t = LMAX - 1; t >= 0; --t)
+            if (t < c.cnt) {
+                Q *= c.tb[(c.p0 + t) * c.S + c.s];
+                v[t] = fma(Q, Z, v[t]);
+            }
     }
 }
 
-// y and z sweeps.  Block (64, nseg): lane = column within a 64-column tile of
-// one plane, threadIdx.y = segment.  Column col of plane pl starts at
-// rho[pl*plane_stride + col]; node p of its line is line_stride further per p.
-template <typename T, int LMAX>
-__global__ void __launch_bounds__(64 * kMaxSeg)
-k_sweep_cols(T* __restrict__ rho, int n, int L, int nseg, size_t line_stride, int ncols,
-             size_t plane_stride, int S, const double* __restrict__ tr, const double* __restrict__ ta,
-             const double* __restrict__ tb, int k0)
+// One kernel for the three sweeps.  The lines of a sweep are "columns": a
+// column is one (voxel of a plane, substrate) pair, col = q*S + s, whose line
+// starts at rho[plane*plane_stride + q*vstride + s] and advances line_stride
+// per node.
+//   x: plane = k, q = j, vstride = nx*S, line_stride = S
+//   y: plane = k, q = i, vstride = S,    line_stride = nx*S
+//   z: one plane, q = j*nx + i, vstride = S, line_stride = nx*ny*S
+// Block = COLS columns x nseg segments; thread tid -> column tid % COLS,
+// segment tid / COLS, so a wave holds COLS columns of 64/COLS consecutive
+// segments.  For y and z the COLS columns of a row are contiguous
+// (coalesced); for x they are row pieces of S contiguous values, and
+// consecutive nodes of a segment reuse the same cache lines.  Two blocks of
+// 1024 threads fit a CU (8 waves per SIMD) up to 16 nodes per segment.
+template <typename T, int LMAX, int COLS>
+__global__ void __launch_bounds__(kMaxThreads, LMAX <= 16 ? 8 : 4)
+k_sweep(T* __restrict__ rho, int n, int L, int nseg, size_t line_stride, int cols_per_plane,
+        size_t vstride, size_t plane_stride, int S, const double* __restrict__ tr,
+        const double* __restrict__ ta, const double* __restrict__ tb, int k0)
 {
-    __shared__ double xa[kMaxSeg][64];
-    __shared__ double xb[kMaxSeg][64];
-    const int lane = threadIdx.x;
-    const int g = threadIdx.y;
-    const int col = blockIdx.x * 64 + lane;
-    const bool act = col < ncols;
-    const int s = (act ? col : 0) % S;
-    T* base = rho + (size_t)blockIdx.y * plane_stride + (act ? col : 0);
+    __shared__ double xa[kMaxThreads];
+    __shared__ double xb[kMaxThreads];
+    const int tid = threadIdx.x;
+    const int lane = tid % COLS;
+    const int g = tid / COLS;
+    const int col = blockIdx.x * COLS + lane;
+    const bool act = col < cols_per_plane && g < nseg;
+    const int cc = act ? col : 0;
+    const int s = cc % S;
+    T* base = rho + (size_t)blockIdx.y * plane_stride + (size_t)(cc / S) * vstride + s;
     const int p0 = g * L;
     const int cnt = act ? max(0, min(L, n - p0)) : 0;
-    const Coef c = make_coef(tr, ta, tb, k0, n, S, s);
+    const SegCtx c = make_ctx(tr, ta, tb, k0, n, S, s, p0, cnt);
 
     double v[LMAX];
 #pragma unroll
@@ -169,97 +205,29 @@ k_sweep_cols(T* __restrict__ rho, int n, int L, int nseg, size_t line_stride, in
         if (t < cnt) v[t] = (double)base[(size_t)(p0 + t) * line_stride];
 
     double yend = 0.0, pa = 1.0;
-    seg_forward<LMAX>(v, cnt, p0, c, yend, pa);
-    xa[g][lane] = yend;
-    xb[g][lane] = pa;
+    seg_forward<LMAX>(v, c, yend, pa);
+    xa[tid] = yend;
+    xb[tid] = pa;
     __syncthreads();
     double Y = 0.0;
-    for (int u = 0; u < g; ++u) Y = xa[u][lane] + xb[u][lane] * Y;
-    seg_fix_forward<LMAX>(v, cnt, p0, c, Y);
+#pragma unroll 4
+    for (int u = 0; u < g; ++u) Y = fma(xb[u * COLS + lane], Y, xa[u * COLS + lane]);
+    seg_fix_forward<LMAX>(v, c, Y);
 
     double zs = 0.0, pb = 1.0;
-    seg_backward<LMAX>(v, cnt, p0, c, zs, pb);
-    __syncthreads();  // all carries Y read
-    xa[g][lane] = zs;
-    xb[g][lane] = pb;
+    seg_backward<LMAX>(v, c, zs, pb);
+    __syncthreads();  // every forward carry has been read
+    xa[tid] = zs;
+    xb[tid] = pb;
     __syncthreads();
     double Z = 0.0;
-    for (int u = nseg - 1; u > g; --u) Z = xa[u][lane] + xb[u][lane] * Z;
-    seg_fix_backward<LMAX>(v, cnt, p0, c, Z);
+#pragma unroll 4
+    for (int u = nseg - 1; u > g; --u) Z = fma(xb[u * COLS + lane], Z, xa[u * COLS + lane]);
+    seg_fix_backward<LMAX>(v, c, Z);
 
 #pragma unroll
     for (int t = 0; t < LMAX; ++t)
         if (t < cnt) base[(size_t)(p0 + t) * line_stride] = (T)v[t];
-}
-
-// x sweep.  A block owns `rows` consecutive x-rows (contiguous in memory:
-// rows*nx*S values), staged in LDS.  Thread tid -> substrate s = tid % S,
-// segment g = (tid / S) % nseg, row rr = tid / (S*nseg).
-template <typename T, int LMAX>
-__global__ void __launch_bounds__(1024)
-k_sweep_rows(T* __restrict__ rho, int nx, int L, int nseg, int S, int nrows, int rows_per_block,
-             const double* __restrict__ tr, const double* __restrict__ ta,
-             const double* __restrict__ tb, int k0)
-{
-    extern __shared__ double smem_raw[];
-    double* xa = smem_raw;                                   // rows_per_block*nseg*S
-    double* xb = xa + (size_t)rows_per_block * nseg * S;     // rows_per_block*nseg*S
-    T* tile = reinterpret_cast<T*>(xb + (size_t)rows_per_block * nseg * S);
-    const int row0 = blockIdx.x * rows_per_block;
-    const int rows = min(rows_per_block, nrows - row0);
-    const size_t row_len = (size_t)nx * S;
-    const size_t tile_len = (size_t)rows * row_len;
-    T* src = rho + (size_t)row0 * row_len;
-
-    for (size_t e = threadIdx.x; e < tile_len; e += blockDim.x) tile[e] = src[e];
-    __syncthreads();
-
-    const int tid = threadIdx.x;
-    const int s = tid % S;
-    const int g = (tid / S) % nseg;
-    const int rr = tid / (S * nseg);
-    const bool act = rr < rows;
-    const int p0 = g * L;
-    const int cnt = act ? max(0, min(L, nx - p0)) : 0;
-    T* line = tile + (size_t)(act ? rr : 0) * row_len + s;
-    const int xi = ((act ? rr : 0) * nseg + g) * S + s;
-    const Coef c = make_coef(tr, ta, tb, k0, nx, S, s);
-
-    double v[LMAX];
-#pragma unroll
-    for (int t = 0; t < LMAX; ++t)
-        if (t < cnt) v[t] = (double)line[(size_t)(p0 + t) * S];
-
-    double yend = 0.0, pa = 1.0;
-    seg_forward<LMAX>(v, cnt, p0, c, yend, pa);
-    if (act) {
-        xa[xi] = yend;
-        xb[xi] = pa;
-    }
-    __syncthreads();
-    double Y = 0.0;
-    if (act)
-        for (int u = 0; u < g; ++u) Y = xa[xi - (g - u) * S] + xb[xi - (g - u) * S] * Y;
-    seg_fix_forward<LMAX>(v, cnt, p0, c, Y);
-
-    double zs = 0.0, pb = 1.0;
-    seg_backward<LMAX>(v, cnt, p0, c, zs, pb);
-    __syncthreads();
-    if (act) {
-        xa[xi] = zs;
-        xb[xi] = pb;
-    }
-    __syncthreads();
-    double Z = 0.0;
-    if (act)
-        for (int u = nseg - 1; u > g; --u) Z = xa[xi + (u - g) * S] + xb[xi + (u - g) * S] * Z;
-    seg_fix_backward<LMAX>(v, cnt, p0, c, Z);
-
-#pragma unroll
-    for (int t = 0; t < LMAX; ++t)
-        if (t < cnt) line[(size_t)(p0 + t) * S] = (T)v[t];
-    __syncthreads();
-    for (size_t e = threadIdx.x; e < tile_len; e += blockDim.x) src[e] = tile[e];
 }
 
 // One thread owns one whole (line, substrate) pair and runs the reference's
@@ -293,109 +261,124 @@ __global__ void k_line_sweep(T* __restrict__ rho, int n, size_t line_stride, siz
     }
 }
 
-// Segment shape for a line of n nodes: L nodes per segment (<= LMAX), nseg
-// segments (<= kMaxSeg).  Returns the LMAX instance to use, 0 if none fits.
-int pick_lmax(int n, int& L, int& nseg)
-{
-    static const int lmax_set[] = {8, 16, 32, 64};
-    L = std::max(4, (n + kMaxSeg - 1) / kMaxSeg);
-    for (int lm : lmax_set) {
-        if (L <= lm) {
-            nseg = (n + L - 1) / L;
-            return lm;
-        }
-    }
-    return 0;
-}
-
 struct Geometry {
     int n, ncols, nplanes;
-    size_t line_stride, plane_stride;
+    size_t line_stride, plane_stride, vstride;
 };
 
+// Lines of one sweep as columns of planes (see k_sweep).
 Geometry geometry(const Env& e, int axis)
 {
     const size_t S = e.S, nx = e.nx, ny = e.ny;
-    if (axis == 0) return {e.nx, e.S, e.ny * e.nz, S, nx * S};
-    if (axis == 1) return {e.ny, (int)(nx * S), e.nz, nx * S, nx * ny * S};
-    return {e.nz, (int)(nx * ny * S), 1, nx * ny * S, 0};
+    if (axis == 0) return {e.nx, (int)(ny * S), e.nz, S, nx * ny * S, nx * S};
+    if (axis == 1) return {e.ny, (int)(nx * S), e.nz, nx * S, nx * ny * S, S};
+    return {e.nz, (int)(nx * ny * S), 1, nx * ny * S, 0, S};
+}
+
+// Launch shape of one sweep: COLS lines per block and L nodes per segment
+// (nseg = ceil(n/L) segments, COLS*nseg <= 1024, L <= 32).  cols == 0 means
+// the line is too long for the segmented kernel.  A plan set by the
+// autotuner (bfvm_autotune) wins over the default.
+SweepPlan default_plan(int n)
+{
+    SweepPlan p;
+    p.cols = 32;
+    p.L = std::max(4, (n + 31) / 32);  // up to 32 segments of 32 lines
+    if (p.L > 32) {
+        p.cols = 16;  // up to 64 segments of 16 lines
+        p.L = std::max(4, (n + 63) / 64);
+    }
+    if (p.L > 32) p.cols = 0;
+    return p;
+}
+
+SweepPlan plan_for(const Env& e, int axis)
+{
+    const SweepPlan& t = e.plan[axis];
+    if (t.cols > 0) return t;
+    return default_plan(axis == 0 ? e.nx : axis == 1 ? e.ny : e.nz);
 }
 
 template <typename T>
 hipError_t sweep_naive(Env& e, int axis)
 {
-    Geometry gm = geometry(e, axis);
-    size_t threads = (size_t)gm.ncols * gm.nplanes;
+    // one thread per whole line: lines as (plane, column) with the naive
+    // kernel's plane/column convention
+    const size_t S = e.S, nx = e.nx, ny = e.ny;
+    int n, ncols, nplanes;
+    size_t line_stride, plane_stride;
+    if (axis == 0) {
+        n = e.nx; ncols = e.S; nplanes = e.ny * e.nz; line_stride = S; plane_stride = nx * S;
+    } else if (axis == 1) {
+        n = e.ny; ncols = (int)(nx * S); nplanes = e.nz; line_stride = nx * S; plane_stride = nx * ny * S;
+    } else {
+        n = e.nz; ncols = (int)(nx * ny * S); nplanes = 1; line_stride = nx * ny * S; plane_stride = 0;
+    }
+    size_t threads = (size_t)ncols * nplanes;
     unsigned grid = (unsigned)((threads + kBlock - 1) / kBlock);
     const AxisCoef& a = e.ax[axis];
-    k_line_sweep<T><<<grid, kBlock, 0, e.stream>>>((T*)e.rho, gm.n, gm.line_stride, gm.plane_stride,
-                                                  gm.ncols, gm.nplanes, e.S, (const double*)e.c1,
+    k_line_sweep<T><<<grid, kBlock, 0, e.stream>>>((T*)e.rho, n, line_stride, plane_stride, ncols,
+                                                  nplanes, e.S, (const double*)e.c1,
                                                   (const double*)a.denom, (const double*)a.c);
     return hipGetLastError();
 }
 
-template <typename T, int LMAX>
-hipError_t launch_cols(Env& e, int axis, const Geometry& gm, int L, int nseg)
+template <typename T, int LMAX, int COLS>
+hipError_t launch_seg(Env& e, int axis, const Geometry& gm, int L, int nseg)
 {
     const AxisCoef& a = e.ax[axis];
-    dim3 block(64, nseg);
-    dim3 grid((gm.ncols + 63) / 64, gm.nplanes);
-    k_sweep_cols<T, LMAX><<<grid, block, 0, e.stream>>>(
-        (T*)e.rho, gm.n, L, nseg, gm.line_stride, gm.ncols, gm.plane_stride, e.S,
+    dim3 grid((gm.ncols + COLS - 1) / COLS, gm.nplanes);
+    int threads = (COLS * nseg + 63) / 64 * 64;
+    k_sweep<T, LMAX, COLS><<<grid, threads, 0, e.stream>>>(
+        (T*)e.rho, gm.n, L, nseg, gm.line_stride, gm.ncols, gm.vstride, gm.plane_stride, e.S,
         (const double*)a.r, (const double*)a.a, (const double*)a.b, a.k0);
     return hipGetLastError();
 }
 
-constexpr size_t kRowTileBytes = 48 * 1024;  // LDS budget of one x-row tile
-
-template <typename T, int LMAX>
-hipError_t launch_rows(Env& e, int L, int nseg, int rows_per_block)
+template <typename T, int COLS>
+hipError_t launch_cols(Env& e, int axis, const Geometry& gm, int L, int nseg)
 {
-    const AxisCoef& a = e.ax[0];
-    const int nrows = e.ny * e.nz;
-    int threads = rows_per_block * nseg * e.S;
-    threads = (threads + 63) / 64 * 64;
-    size_t lds = sizeof(T) * (size_t)rows_per_block * e.nx * e.S +
-                 sizeof(double) * 2 * (size_t)rows_per_block * nseg * e.S;
-    unsigned grid = (unsigned)((nrows + rows_per_block - 1) / rows_per_block);
-    k_sweep_rows<T, LMAX><<<grid, threads, lds, e.stream>>>(
-        (T*)e.rho, e.nx, L, nseg, e.S, nrows, rows_per_block, (const double*)a.r,
-        (const double*)a.a, (const double*)a.b, a.k0);
-    return hipGetLastError();
+    if (L <= 8) return launch_seg<T, 8, COLS>(e, axis, gm, L, nseg);
+    if (L <= 16) return launch_seg<T, 16, COLS>(e, axis, gm, L, nseg);
+    return launch_seg<T, 32, COLS>(e, axis, gm, L, nseg);
 }
 
 template <typename T>
 hipError_t sweep(Env& e, int axis)
 {
     Geometry gm = geometry(e, axis);
-    int L = 0, nseg = 0;
-    int lm = pick_lmax(gm.n, L, nseg);
-    if (axis == 0) {
-        // rows per block: keep the tile within the LDS budget and the block
-        // within 1024 threads
-        size_t row_bytes = sizeof(T) * (size_t)e.nx * e.S;
-        int per_row_threads = nseg * e.S;
-        int rpb = (int)std::min<size_t>(kRowTileBytes / std::max<size_t>(row_bytes, 1),
-                                        (size_t)(1024 / std::max(per_row_threads, 1)));
-        rpb = std::min(rpb, 8);
-        if (lm == 0 || rpb < 1) return sweep_naive<T>(e, axis);
-        switch (lm) {
-        case 8: return launch_rows<T, 8>(e, L, nseg, rpb);
-        case 16: return launch_rows<T, 16>(e, L, nseg, rpb);
-        case 32: return launch_rows<T, 32>(e, L, nseg, rpb);
-        default: return launch_rows<T, 64>(e, L, nseg, rpb);
-        }
-    }
-    switch (lm) {
-    case 8: return launch_cols<T, 8>(e, axis, gm, L, nseg);
-    case 16: return launch_cols<T, 16>(e, axis, gm, L, nseg);
-    case 32: return launch_cols<T, 32>(e, axis, gm, L, nseg);
-    case 64: return launch_cols<T, 64>(e, axis, gm, L, nseg);
-    default: return sweep_naive<T>(e, axis);
+    SweepPlan p = plan_for(e, axis);
+    if (p.cols == 0) return sweep_naive<T>(e, axis);
+    int nseg = (gm.n + p.L - 1) / p.L;
+    switch (p.cols) {
+    case 16: return launch_cols<T, 16>(e, axis, gm, p.L, nseg);
+    case 32: return launch_cols<T, 32>(e, axis, gm, p.L, nseg);
+    default: return launch_cols<T, 64>(e, axis, gm, p.L, nseg);
     }
 }
 
 }  // namespace
+
+SweepPlan effective_plan(const Env& e, int axis)
+{
+    return plan_for(e, axis);
+}
+
+std::vector<SweepPlan> sweep_candidates(int n)
+{
+    std::vector<SweepPlan> out;
+    for (int cols : {16, 32, 64}) {
+        int lmin = std::max(4, (n + kMaxThreads / cols - 1) / (kMaxThreads / cols));
+        for (int L : {lmin, 8, 12, 16, 24, 32}) {
+            if (L < lmin || L > 32) continue;
+            if (L > n && L != lmin) continue;
+            bool dup = false;
+            for (const SweepPlan& q : out) dup |= q.cols == cols && q.L == L;
+            if (!dup) out.push_back(SweepPlan{cols, L});
+        }
+    }
+    return out;
+}
 
 hipError_t launch_sweep(Env& e, int axis)
 {

@@ -118,6 +118,19 @@ class DeviceMicroenvironment:
     def sync(self):
         self._check(self.lib.bfvm_sync(self.h))
 
+    def autotune(self, dt):
+        """Picks the fastest launch shape of each sweep (on a scratch copy)."""
+        self._check(self.lib.bfvm_autotune(self.h, float(dt)))
+
+    def sweep_plans(self):
+        """[(lines_per_block, nodes_per_segment)] for the x, y and z sweeps."""
+        out = []
+        for axis in range(3):
+            a, b = ctypes.c_int(), ctypes.c_int()
+            self._check(self.lib.bfvm_get_sweep_plan(self.h, axis, ctypes.byref(a), ctypes.byref(b)))
+            out.append((a.value, b.value))
+        return out
+
     def timed_run(self, dt, steps, flags=ALL, per_kernel=False):
         """Returns (ms_total, {kernel: (ms_sum, launches)} or None)."""
         ms = ctypes.c_double(0)

@@ -31,7 +31,8 @@ typedef struct bfvm_env* bfvm_handle;
 /* Device arithmetic of the density store and the sweeps. */
 enum bfvm_precision {
     BFVM_FP64 = 0, /* bit-for-bit reference semantics up to FMA rounding */
-    BFVM_FP32 = 1  /* FP32 state; Thomas factorisation computed in FP64 and rounded */
+    BFVM_FP32 = 1  /* FP32 store (half the HBM bytes); sweep and source arithmetic and
+                      all coefficient tables stay FP64, one rounding per store write */
 };
 
 /* Phases of one time step, in the order they run
@@ -159,6 +160,20 @@ enum bfvm_kernel_kind {
 };
 int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_total,
                    double* per_kernel_ms, int* launches);
+
+/*
+ * Autotuner: times every valid launch shape of the three segmented sweeps
+ * (lines per block x nodes per segment) for this grid, substrate count and
+ * precision on a scratch copy of the store, and keeps the fastest per axis.
+ * The densities are left unchanged.  No counterpart in the reference (its
+ * OpenMP loops have no launch shape).
+ */
+int bfvm_autotune(bfvm_handle h, double dt);
+
+/* The launch shape in use for one axis (0 = x, 1 = y, 2 = z):
+ * lines per block and nodes per segment; *lines_per_block = 0 means the
+ * one-thread-per-line fallback. */
+int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment);
 
 /* Grid and store description. */
 int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, int* precision,

@@ -155,3 +155,16 @@ def test_step_by_step_equals_run_and_timed_run():
     assert ms > 0 and per["sweep_x"][1] == case["steps"] and per["dirichlet"][1] == 3 * case["steps"]
     np.testing.assert_array_equal(a, b)
     assert rel_err(a, expected) <= TOL[FP64]
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_autotune_keeps_state_and_parity(precision):
+    case, expected, _ = gen_golden.load("g6_ensemble")
+    with setup_case(case, precision) as m:
+        before = m.download()
+        m.autotune(case["dt"])
+        np.testing.assert_array_equal(m.download(), before)
+        for cols, L in m.sweep_plans():
+            assert cols in (16, 32, 64) and 4 <= L <= 32
+        m.run(case["dt"], case["steps"], case["phases"])
+        assert rel_err(m.download(), expected) <= TOL[precision]
