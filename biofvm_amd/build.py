@@ -1,4 +1,5 @@
-"""Builds libbiofvm_amd.so (hand-written gfx950 HIP kernels + the C ABI) in-tree.
+"""Builds libbiofvm_amd.so (hand-written gfx950 HIP kernels + the C ABI) and
+libbiofvm_facade.so (the BioFVM-compatible C++ host façade over it) in-tree.
 
     python -m biofvm_amd.build          # or biofvm_amd.build.build()
 
@@ -48,7 +49,25 @@ def build(verbose=False):
         print(" ".join(shlex.quote(c) for c in cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_facade(verbose)
     return LIB
+
+
+FACADE_DIR = os.path.join(HERE, "facade")
+FACADE_LIB = os.path.join(HERE, "libbiofvm_facade.so")
+
+
+def build_facade(verbose=False):
+    """The BioFVM-compatible C++ host façade (plain g++, links the C ABI library)."""
+    tmp = FACADE_LIB + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
+           f"-I{INCLUDE}", f"-I{FACADE_DIR}", os.path.join(FACADE_DIR, "BioFVM_gpu.cpp"),
+           f"-L{HERE}", "-lbiofvm_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(shlex.quote(c) for c in cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, FACADE_LIB)
+    return FACADE_LIB
 
 
 if __name__ == "__main__":

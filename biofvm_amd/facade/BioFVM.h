@@ -1,0 +1,234 @@
+/*
+ * BioFVM-compatible host façade over the biofvm_amd C ABI.
+ *
+ * The classes and functions below keep the names, signatures and argument
+ * meaning of BioFVM 1.0.4's hot-path API, so a program written against the
+ * reference's Microenvironment / Basic_Agent (e.g. its examples/ drivers)
+ * compiles against this header instead and runs its time loop on the GPU:
+ *
+ *   Microenvironment                 BioFVM_microenvironment.h:45-228
+ *     diffusion_decay_solver         function pointer, h:129 (default: the GPU LOD_3D solver)
+ *     bulk_*_function callbacks      h:130-132
+ *     simulate_diffusion_decay       h:194, cpp:502-513
+ *     simulate_bulk_sources_and_sinks h:197, cpp:584-613
+ *     simulate_cell_sources_and_sinks h:200,202, cpp:615-633
+ *     add/update/remove_dirichlet_node, apply_dirichlet_conditions  h:206-209
+ *     density_vector / operator()    h:170-191
+ *   Cartesian_Mesh (geometry + indexing)  BioFVM_mesh.h:146-188
+ *   Basic_Agent, all_basic_agents, create/delete_basic_agent  BioFVM_basic_agent.h:36-94
+ *   diffusion_decay_solver__constant_coefficients_LOD_3D       BioFVM_solvers.h:38
+ *
+ * Storage: the densities live on the GPU (substrate-innermost HBM array);
+ * the std::vector<double> returned by density_vector() is a host mirror
+ * synchronised lazily — any density access first copies the device state
+ * down if it is newer, and marks the mirror as the newer copy, so the next
+ * simulate_* call uploads it.  Only the hot path is here; I/O (MATLAB,
+ * MultiCellDS), gradients and the general mesh are not part of this façade.
+ */
+#ifndef BIOFVM_AMD_FACADE_H
+#define BIOFVM_AMD_FACADE_H
+
+#include <iostream>
+#include <string>
+#include <vector>
+
+namespace BioFVM {
+
+class Microenvironment;
+class Basic_Agent;
+
+class Voxel {
+ public:
+    Voxel();
+    int mesh_index;
+    double volume;
+    std::vector<double> center;
+    bool is_Dirichlet;
+};
+
+/* Uniform Cartesian mesh: node coordinates, bounding box, voxel volumes and
+ * the index maps of BioFVM_mesh.cpp:641-661, 663-928, 930-973. */
+class Cartesian_Mesh {
+ public:
+    Cartesian_Mesh();
+    std::vector<double> x_coordinates, y_coordinates, z_coordinates;
+    std::vector<double> bounding_box;  // [xmin ymin zmin xmax ymax zmax]
+    std::vector<Voxel> voxels;
+    double dx, dy, dz, dV, dS, dS_xy, dS_yz, dS_xz;
+    bool Cartesian_mesh, uniform_mesh, regular_mesh, use_voxel_faces;
+    std::string units;
+
+    int voxel_index(int i, int j, int k);
+    std::vector<int> cartesian_indices(int n);
+    void resize(int x_nodes, int y_nodes, int z_nodes);
+    void resize(double x_start, double x_end, double y_start, double y_end, double z_start,
+                double z_end, int x_nodes, int y_nodes, int z_nodes);
+    void resize(double x_start, double x_end, double y_start, double y_end, double z_start,
+                double z_end, double dx, double dy, double dz);
+    void resize_uniform(double x_start, double x_end, double y_start, double y_end, double z_start,
+                        double z_end, double dx);
+    int nearest_voxel_index(std::vector<double>& position);
+    std::vector<int> nearest_cartesian_indices(std::vector<double>& position);
+    Voxel& nearest_voxel(std::vector<double>& position);
+    bool is_position_valid(double x, double y, double z);
+    void display_information(std::ostream& os);
+
+ private:
+    void fill_voxels();
+};
+
+struct DeviceState;  // the biofvm_amd handle and its synchronisation flags
+
+class Microenvironment {
+ public:
+    Microenvironment();
+    explicit Microenvironment(std::string name);
+    ~Microenvironment();
+    Microenvironment(const Microenvironment&) = delete;
+    Microenvironment& operator=(const Microenvironment&) = delete;
+
+    Cartesian_Mesh mesh;
+    std::string spatial_units, time_units, name;
+    std::vector<std::string> density_names, density_units;
+    std::vector<double> diffusion_coefficients, decay_rates;
+
+    void (*diffusion_decay_solver)(Microenvironment&, double);
+    void (*bulk_supply_rate_function)(Microenvironment* pMicroenvironment, int voxel_index,
+                                      std::vector<double>* write_destination);
+    void (*bulk_supply_target_densities_function)(Microenvironment* pMicroenvironment,
+                                                  int voxel_index,
+                                                  std::vector<double>* write_destination);
+    void (*bulk_uptake_rate_function)(Microenvironment* pMicroenvironment, int voxel_index,
+                                      std::vector<double>* write_destination);
+
+    int number_of_densities(void);
+    int number_of_voxels(void);
+
+    void resize_space(int x_nodes, int y_nodes, int z_nodes);
+    void resize_space(double x_start, double x_end, double y_start, double y_end, double z_start,
+                      double z_end, int x_nodes, int y_nodes, int z_nodes);
+    void resize_space(double x_start, double x_end, double y_start, double y_end, double z_start,
+                      double z_end, double dx_new, double dy_new, double dz_new);
+    void resize_space_uniform(double x_start, double x_end, double y_start, double y_end,
+                              double z_start, double z_end, double dx_new);
+    void resize_densities(int new_size);
+    void add_density(void);
+    void add_density(std::string name, std::string units);
+    void add_density(std::string name, std::string units, double diffusion_constant,
+                     double decay_rate);
+    void set_density(int index, std::string name, std::string units);
+    void set_density(int index, std::string name, std::string units, double diffusion_constant,
+                     double decay_rate);
+
+    int voxel_index(int i, int j, int k);
+    std::vector<int> cartesian_indices(int n);
+    int nearest_voxel_index(std::vector<double>& position);
+    Voxel& voxels(int voxel_index);
+    Voxel& nearest_voxel(std::vector<double>& position);
+    std::vector<double>& nearest_density_vector(std::vector<double>& position);
+    std::vector<double>& nearest_density_vector(int voxel_index);
+    std::vector<double>& operator()(int i, int j, int k);
+    std::vector<double>& operator()(int i, int j);
+    std::vector<double>& operator()(int n);
+    std::vector<double>& density_vector(int i, int j, int k);
+    std::vector<double>& density_vector(int i, int j);
+    std::vector<double>& density_vector(int n);
+
+    void simulate_diffusion_decay(double dt);
+    void simulate_bulk_sources_and_sinks(double dt);
+    void simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>& basic_agent_list, double dt);
+    void simulate_cell_sources_and_sinks(double dt);
+
+    void add_dirichlet_node(int voxel_index, std::vector<double>& value);
+    void update_dirichlet_node(int voxel_index, std::vector<double>& new_value);
+    void remove_dirichlet_node(int voxel_index);
+    void apply_dirichlet_conditions(void);
+
+    void display_information(std::ostream& os);
+
+    /* Extensions (no counterpart in the reference). */
+    // Restrict Dirichlet overwrites to substrates with mask[s] != 0
+    // (empty = every substrate, the reference's behaviour).
+    std::vector<unsigned char> dirichlet_substrate_mask;
+    // Store precision of the device copy: 0 = FP64 (default), 1 = FP32 store
+    // with FP64 arithmetic.  Takes effect at the next resize.
+    int device_precision;
+    // Copy the device state into the host mirror now.
+    void synchronize_host(void);
+
+    // internal: the device handle (created on first use)
+    DeviceState* device(void);
+    friend void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, double dt);
+
+ private:
+    std::vector<std::vector<double> > densities;  // host mirror
+    std::vector<int> dirichlet_indices;
+    std::vector<std::vector<double> > dirichlet_value_vectors;
+    std::vector<std::vector<double> > bulk_temp_S, bulk_temp_T, bulk_temp_U;
+    std::vector<double> zero, one;
+    DeviceState* dev;
+    void reset_space(void);
+    void host_view(void);           // mirror current + mark it the newer copy
+    int push_to_device(void);       // upload whatever the device lacks
+};
+
+class Basic_Agent {
+ private:
+    Microenvironment* microenvironment;
+    std::vector<double> cell_source_sink_solver_temp1;
+    std::vector<double> cell_source_sink_solver_temp2;
+
+ protected:
+    int current_voxel_index;
+
+ public:
+    std::vector<double>* secretion_rates;
+    std::vector<double>* saturation_densities;
+    std::vector<double>* uptake_rates;
+    void set_internal_uptake_constants(double dt);
+    void register_microenvironment(Microenvironment*);
+    Microenvironment* get_microenvironment(void);
+
+    int ID;
+    int index;
+    int type;
+    double volume;
+    bool assign_position(double x, double y, double z);
+    bool assign_position(std::vector<double> new_position);
+    std::vector<double> position;
+    std::vector<double> velocity;
+    void update_position(double dt);
+
+    Basic_Agent();
+    ~Basic_Agent();
+    void simulate_secretion_and_uptake(Microenvironment* M, double dt);
+    int nearest_voxel_index(void);
+    std::vector<double>& nearest_density_vector(void);
+
+    // internal: monotone counter bumped whenever this agent's voxel or
+    // implicit constants change, so the device copy knows to refresh
+    unsigned long long revision;
+    const std::vector<double>& temp1(void) const { return cell_source_sink_solver_temp1; }
+    const std::vector<double>& temp2(void) const { return cell_source_sink_solver_temp2; }
+};
+
+extern std::vector<Basic_Agent*> all_basic_agents;
+Basic_Agent* create_basic_agent(void);
+void delete_basic_agent(int);
+void delete_basic_agent(Basic_Agent*);
+
+void set_default_microenvironment(Microenvironment* M);
+Microenvironment* get_default_microenvironment(void);
+
+/* The GPU LOD solver: same signature as BioFVM_solvers.h:38, assignable to
+ * Microenvironment::diffusion_decay_solver. */
+void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, double dt);
+
+void zero_function(Microenvironment* pMicroenvironment, int voxel_index,
+                   std::vector<double>* write_destination);
+void one_function(Microenvironment* pMicroenvironment, int voxel_index,
+                  std::vector<double>* write_destination);
+
+}  // namespace BioFVM
+
+#endif

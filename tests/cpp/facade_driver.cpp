@@ -1,0 +1,133 @@
+// Drives the BioFVM-compatible façade (biofvm_amd/facade/BioFVM.h) the way
+// the reference's example programs drive BioFVM (examples/convergence_test5.cpp:
+// 152-241): build the microenvironment through its API, add Dirichlet nodes,
+// bulk callbacks and agents, then loop bulk -> cells -> diffusion.
+//
+//   facade_driver <case.bin> <out.bin>
+//
+// case.bin (little endian): int32 nx ny nz S steps phases n_dir n_agents
+// has_bulk precision; float64 dx dt; D[S] lambda[S]; rho0[nvox*S];
+// int32 dir_idx[n_dir]; float64 dir_val[n_dir*S]; if has_bulk: supply,
+// target, uptake [nvox*S each]; agents: pos[3n] volume[n] secretion[nS]
+// saturation[nS] uptake[nS].  out.bin: float64 rho[nvox*S].
+// Halfway through, every density is read through density_vector() (which
+// synchronises the host mirror) and written back unchanged, so the result
+// also covers the mirror's download/upload path.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "BioFVM.h"
+
+using namespace BioFVM;
+
+namespace {
+int S_ = 0;
+std::vector<double> g_sup, g_tgt, g_upt;
+void sup_fn(Microenvironment*, int v, std::vector<double>* d) { for (int s = 0; s < S_; ++s) (*d)[s] = g_sup[(size_t)v * S_ + s]; }
+void tgt_fn(Microenvironment*, int v, std::vector<double>* d) { for (int s = 0; s < S_; ++s) (*d)[s] = g_tgt[(size_t)v * S_ + s]; }
+void upt_fn(Microenvironment*, int v, std::vector<double>* d) { for (int s = 0; s < S_; ++s) (*d)[s] = g_upt[(size_t)v * S_ + s]; }
+
+template <typename T>
+void rd(FILE* f, T* p, size_t n)
+{
+    if (n && fread(p, sizeof(T), n, f) != n) {
+        fprintf(stderr, "short read\n");
+        exit(2);
+    }
+}
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc != 3) {
+        fprintf(stderr, "usage: facade_driver case.bin out.bin\n");
+        return 2;
+    }
+    FILE* f = fopen(argv[1], "rb");
+    if (!f) return 2;
+    int hdr[10];
+    rd(f, hdr, 10);
+    const int nx = hdr[0], ny = hdr[1], nz = hdr[2], S = hdr[3], steps = hdr[4], phases = hdr[5],
+              n_dir = hdr[6], n_agents = hdr[7], has_bulk = hdr[8], precision = hdr[9];
+    double dd[2];
+    rd(f, dd, 2);
+    const double dx = dd[0], dt = dd[1];
+    const size_t nvox = (size_t)nx * ny * nz;
+    std::vector<double> D(S), lam(S), rho((size_t)nvox * S), dval((size_t)n_dir * S);
+    std::vector<int> didx(n_dir);
+    rd(f, D.data(), S);
+    rd(f, lam.data(), S);
+    rd(f, rho.data(), rho.size());
+    rd(f, didx.data(), n_dir);
+    rd(f, dval.data(), dval.size());
+    S_ = S;
+    if (has_bulk) {
+        g_sup.resize(nvox * S); g_tgt.resize(nvox * S); g_upt.resize(nvox * S);
+        rd(f, g_sup.data(), g_sup.size());
+        rd(f, g_tgt.data(), g_tgt.size());
+        rd(f, g_upt.data(), g_upt.size());
+    }
+    std::vector<double> pos(3 * (size_t)n_agents), vol(n_agents), sec((size_t)n_agents * S),
+        sat((size_t)n_agents * S), upt((size_t)n_agents * S);
+    rd(f, pos.data(), pos.size());
+    rd(f, vol.data(), vol.size());
+    rd(f, sec.data(), sec.size());
+    rd(f, sat.data(), sat.size());
+    rd(f, upt.data(), upt.size());
+    fclose(f);
+
+    Microenvironment M;
+    M.device_precision = precision;
+    M.set_density(0, "substrate0", "dimensionless");
+    for (int s = 1; s < S; ++s) M.add_density("substrate", "dimensionless");
+    M.resize_space_uniform(0.0, nx * dx, 0.0, ny * dx, 0.0, nz * dx, dx);
+    M.diffusion_decay_solver = diffusion_decay_solver__constant_coefficients_LOD_3D;
+    for (int s = 0; s < S; ++s) {
+        M.diffusion_coefficients[s] = D[s];
+        M.decay_rates[s] = lam[s];
+    }
+    for (size_t v = 0; v < nvox; ++v)
+        for (int s = 0; s < S; ++s) M.density_vector((int)v)[s] = rho[v * S + s];
+    for (int d = 0; d < n_dir; ++d) {
+        std::vector<double> value(dval.begin() + (size_t)d * S, dval.begin() + (size_t)(d + 1) * S);
+        M.add_dirichlet_node(didx[d], value);
+    }
+    if (has_bulk) {
+        M.bulk_supply_rate_function = sup_fn;
+        M.bulk_supply_target_densities_function = tgt_fn;
+        M.bulk_uptake_rate_function = upt_fn;
+    }
+    for (int a = 0; a < n_agents; ++a) {
+        Basic_Agent* p = create_basic_agent();
+        p->register_microenvironment(&M);
+        if (!p->assign_position(pos[3 * a], pos[3 * a + 1], pos[3 * a + 2])) {
+            fprintf(stderr, "agent %d outside the domain\n", a);
+            return 3;
+        }
+        p->volume = vol[a];
+        for (int s = 0; s < S; ++s) {
+            (*p->secretion_rates)[s] = sec[(size_t)a * S + s];
+            (*p->saturation_densities)[s] = sat[(size_t)a * S + s];
+            (*p->uptake_rates)[s] = upt[(size_t)a * S + s];
+        }
+        p->set_internal_uptake_constants(dt);
+    }
+    for (int n = 0; n < steps; ++n) {
+        if (n == steps / 2) {
+            for (size_t v = 0; v < nvox; ++v) {
+                std::vector<double>& d = M.density_vector((int)v);
+                for (int s = 0; s < S; ++s) d[s] = d[s] + 0.0;
+            }
+        }
+        if ((phases & 1) && has_bulk) M.simulate_bulk_sources_and_sinks(dt);
+        if (phases & 2) M.simulate_cell_sources_and_sinks(dt);
+        if (phases & 4) M.simulate_diffusion_decay(dt);
+    }
+    FILE* o = fopen(argv[2], "wb");
+    if (!o) return 2;
+    for (size_t v = 0; v < nvox; ++v) fwrite(M.density_vector((int)v).data(), sizeof(double), S, o);
+    fclose(o);
+    while (!all_basic_agents.empty()) delete_basic_agent((int)all_basic_agents.size() - 1);
+    return 0;
+}

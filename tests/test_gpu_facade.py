@@ -1,0 +1,80 @@
+"""The BioFVM-compatible C++ façade, driven like the reference's examples.
+
+tests/cpp/facade_driver builds a Microenvironment through the reference's
+API (set_density/add_density, resize_space_uniform, density_vector,
+add_dirichlet_node, bulk callbacks, create_basic_agent/assign_position/
+set_internal_uptake_constants) and loops bulk -> cells -> diffusion; its
+output must match the compiled reference's fixtures (FP64 <= 1e-12) and the
+oracle (FP32 store <= 1e-5).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gen_golden
+import oracle
+from biofvm_amd import FP32, FP64
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "facade_driver")
+TOL = {FP64: 1e-12, FP32: 1e-5}
+
+
+def write_case(path, case, precision):
+    S = case["S"]
+    nvox = case["nx"] * case["ny"] * case["nz"]
+    n_dir = 0 if case.get("dir_idx") is None else len(case["dir_idx"])
+    n_ag = 0 if case.get("agent_volume") is None else len(case["agent_volume"])
+    has_bulk = case.get("bulk_supply") is not None
+    with open(path, "wb") as f:
+        np.array([case["nx"], case["ny"], case["nz"], S, case["steps"], case["phases"], n_dir, n_ag,
+                  int(has_bulk), precision], dtype="<i4").tofile(f)
+        np.array([case["dx"], case["dt"]], dtype="<f8").tofile(f)
+        for k in ("D", "lam", "rho0"):
+            np.asarray(case[k], dtype="<f8").tofile(f)
+        if n_dir:
+            np.asarray(case["dir_idx"], dtype="<i4").tofile(f)
+            np.asarray(case["dir_val"], dtype="<f8").tofile(f)
+        if has_bulk:
+            for k in ("bulk_supply", "bulk_target", "bulk_uptake"):
+                np.asarray(case[k], dtype="<f8").tofile(f)
+        if n_ag:
+            for k in ("agent_pos", "agent_volume", "agent_secretion", "agent_saturation",
+                      "agent_uptake"):
+                np.asarray(case[k], dtype="<f8").tofile(f)
+    return nvox * S
+
+
+def run_driver(tmp_path, case, precision):
+    cin, cout = tmp_path / "case.bin", tmp_path / "out.bin"
+    n = write_case(cin, case, precision)
+    r = subprocess.run([DRIVER, str(cin), str(cout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.fromfile(cout, dtype="<f8")
+    assert out.size == n
+    return out
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["g1_lod", "g2_dirichlet", "g3_bulk", "g4_agents", "g8_shell",
+                                  "g5_degenerate_1x6x5"])
+def test_facade_matches_reference_fixture(tmp_path, name, precision):
+    case, expected, _ = gen_golden.load(name)
+    got = run_driver(tmp_path, case, precision)
+    err = float(np.max(np.abs(got - expected)) / np.max(np.abs(expected)))
+    assert err <= TOL[precision], err
+
+
+def test_facade_many_agents_per_voxel(tmp_path):
+    from biofvm_amd import workloads as W
+    rng = np.random.default_rng(21)
+    n = 12
+    case = W.make_case(n, n, n, 2, 10.0, [1e4, 1e2], [0.1, 0.01], 0.01, 6, W.ALL,
+                       rng.uniform(0, 1, n ** 3 * 2))
+    case.update(W.random_agents(rng, 500, 2, n, n, n, 10.0, cluster_sigma=8.0, share_voxel=40))
+    ref = oracle.oracle_run(case)
+    got = run_driver(tmp_path, case, FP64)
+    assert float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))) <= 1e-12
