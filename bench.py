@@ -122,6 +122,25 @@ def cpu_baseline(case, workload):
             "kind": kind, "sample": f"{desc}, {steps} steps, {secs:.2f} s wall"}
 
 
+def calibrate_copy(mib):
+    """Copies `mib` MiB device to device with hipMemcpy (the runtime's copy
+    kernel), three times, so rocprofv3 counters of a kernel with known bytes
+    (read mib, write mib) can scale those of the sweeps."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_mib": mib, "bytes_read": n, "bytes_written": n}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,7 +150,13 @@ def main():
                     choices=["precision_400", "drop_in_200", "lod_50"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device hipMemcpy of MIB MiB (a plain copy "
+                         "kernel of known size, to scale HBM counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the z-slab decomposition is a design, not code). Run with --gpus 1.", file=sys.stderr)

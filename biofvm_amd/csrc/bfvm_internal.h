@@ -11,22 +11,30 @@
 
 namespace bfvm {
 
-// Thomas factorisation of one axis for all substrates, FP64 on the device
-// for either store precision.  Element (i, s) of a table is at [i*S + s].
+// Tridiagonal solve of one axis, all substrates, FP64 on the device for
+// either store precision.  Element (i, s) of an (n, S) table is at [i*S + s].
 //
-// The segmented sweeps use the Thomas recurrences in multiplicative form,
-//   forward  y_i = x_i*r_i + a_i*y_{i-1},   r_i = 1/denom_i, a_i = c1/denom_i
-//   backward z_i = y_i + b_i*z_{i+1},       b_i = -c_i
-// with r, a, b computed in FP64 on the host.
+// The reference factorises A = tridiag(-c1, 1+2c1+c2, -c1) with end
+// diagonals 1+c1+c2 (BioFVM_solvers.cpp:104-142); its Thomas coefficients
+// denom_i converge to a fixed point d away from the ends.  The segmented
+// sweep inverts instead the constant-coefficient product A_inf = L U with
+// L = bidiag(-c1, d), U = bidiag(1, -c1/d), which differs from A only in
+// the two corner diagonals, and restores A^-1 exactly with a rank-2
+// (Woodbury) correction:
+//   w = A_inf^-1 x,   z = A^-1 x = w + mu0*g0 + mu1*g1,
+//   (mu0, mu1) = K (w_0, w_{n-1}),  g0 = A_inf^-1 (delta0 e_0),
+//   g1 = A_inf^-1 (delta1 e_{n-1}),  K = (I - [[g0_0, g1_0], [g0_{n-1}, g1_{n-1}]])^-1
+// with delta0 = d - (1+c1+c2), delta1 = d + c1^2/d - (1+c1+c2).  g0 and g1
+// decay geometrically from their end, so only their first kg0 / last kg1
+// rows (above 1e-18 of their peak) are stored.
 struct AxisCoef {
     int n = 0;
-    void* denom = nullptr;  // denom_i                  (BioFVM_solvers.cpp:130-141)
-    void* c = nullptr;      // c_i = -c1/denom_i        (BioFVM_solvers.cpp:137-141)
-    void* r = nullptr;      // 1/denom_i
-    void* a = nullptr;      // c1/denom_i
-    void* b = nullptr;      // -c_i
-    int k0 = 0;             // rows [k0, n-1) of (r, a, b) all equal row k0 (the
-                            // converged interior); rows < k0 and row n-1 differ
+    void* denom = nullptr;  // denom_i (BioFVM_solvers.cpp:130-141), for the per-line kernel
+    void* c = nullptr;      // c_i = -c1/denom_i (:137-141)
+    void* wb = nullptr;     // per substrate, 8 values: 1/d, c1/d, K00, K01, K10, K11, 0, 0
+    void* g0 = nullptr;     // (kg0, S): g0 rows 0..kg0-1
+    void* g1 = nullptr;     // (kg1, S): g1 rows n-1, n-2, ..., n-kg1
+    int kg0 = 0, kg1 = 0;
 };
 
 // Launch shape of the segmented sweep of one axis (see lod.hip).

@@ -142,6 +142,83 @@ void agent_constants(int S, double dt, double voxel_volume, double volume, const
 }
 #pragma clang fp contract(on)
 
+// Constant-coefficient solve A_inf^-1 x (see AxisCoef), in FP64.
+void solve_const(int n, double r, double a, const std::vector<double>& x, std::vector<double>& z)
+{
+    std::vector<double> y(n);
+    double prev = 0.0;
+    for (int i = 0; i < n; ++i) y[i] = prev = a * prev + x[i] * r;
+    z.assign(n, 0.0);
+    double next = 0.0;
+    for (int i = n - 1; i >= 0; --i) z[i] = next = y[i] + (i < n - 1 ? a * next : 0.0);
+}
+
+// Per-axis tables of the segmented sweep (see AxisCoef).
+int build_woodbury(Env& e, bfvm::AxisCoef& ax, int n, double dt)
+{
+    const int S = e.S;
+    std::vector<double> wb((size_t)S * 8, 0.0), g0all((size_t)n * S), g1all((size_t)n * S);
+    int kg0 = 1, kg1 = 1;
+    for (int s = 0; s < S; ++s) {
+        double c1 = e.D[s];
+        c1 *= dt;
+        c1 /= e.dx;
+        c1 /= e.dx;
+        double c2 = e.lambda[s];
+        c2 *= dt;
+        c2 /= 3.0;
+        const double c3 = 1.0 + c1 + c1 + c2, c3a = 1.0 + c1 + c2;
+        // fixed point of the reference's recurrence denom_i = c3 + c1*c_{i-1},
+        // c_i = -c1/denom_i, iterated from its first row
+        double d = c3a;
+        for (int it = 0; it < 100000; ++it) {
+            double dn = c3 + c1 * (-c1 / d);
+            if (dn == d) break;
+            d = dn;
+        }
+        const double r = 1.0 / d, a = c1 / d;
+        const double delta0 = d - c3a, delta1 = (d + c1 * (c1 / d)) - c3a;
+        std::vector<double> unit(n, 0.0), g0, g1;
+        unit[0] = delta0;
+        solve_const(n, r, a, unit, g0);
+        unit.assign(n, 0.0);
+        unit[n - 1] = delta1;
+        solve_const(n, r, a, unit, g1);
+        const double m00 = g0[0], m01 = g1[0], m10 = g0[n - 1], m11 = g1[n - 1];
+        const double det = (1.0 - m00) * (1.0 - m11) - m01 * m10;
+        double* w = &wb[(size_t)s * 8];
+        w[0] = r;
+        w[1] = a;
+        w[2] = (1.0 - m11) / det;
+        w[3] = m01 / det;
+        w[4] = m10 / det;
+        w[5] = (1.0 - m00) / det;
+        double p0 = 0, p1 = 0;
+        for (int i = 0; i < n; ++i) {
+            p0 = std::max(p0, std::fabs(g0[i]));
+            p1 = std::max(p1, std::fabs(g1[i]));
+        }
+        for (int i = 0; i < n; ++i) {
+            g0all[(size_t)i * S + s] = g0[i];
+            g1all[(size_t)i * S + s] = g1[n - 1 - i];
+            if (std::fabs(g0[i]) > 1e-18 * p0) kg0 = std::max(kg0, i + 1);
+            if (std::fabs(g1[n - 1 - i]) > 1e-18 * p1) kg1 = std::max(kg1, i + 1);
+        }
+    }
+    dfree(ax.wb);
+    dfree(ax.g0);
+    dfree(ax.g1);
+    HIP_OR_FAIL(&e, alloc_f64(&ax.wb, wb.size()));
+    HIP_OR_FAIL(&e, alloc_f64(&ax.g0, (size_t)kg0 * S));
+    HIP_OR_FAIL(&e, alloc_f64(&ax.g1, (size_t)kg1 * S));
+    HIP_OR_FAIL(&e, upload_f64(ax.wb, wb.data(), wb.size()));
+    HIP_OR_FAIL(&e, upload_f64(ax.g0, g0all.data(), (size_t)kg0 * S));
+    HIP_OR_FAIL(&e, upload_f64(ax.g1, g1all.data(), (size_t)kg1 * S));
+    ax.kg0 = kg0;
+    ax.kg1 = kg1;
+    return BFVM_OK;
+}
+
 int ensure_coefficients(Env& e, double dt)
 {
     if (e.coef_valid && e.coef_dt == dt) return BFVM_OK;
@@ -151,37 +228,11 @@ int ensure_coefficients(Env& e, double dt)
     for (int a = 0; a < 3; ++a) {
         thomas_axis(n_axis[a], S, e.dx, dt, e.D, e.lambda, denom, c, c1v);
         bfvm::AxisCoef& ax = e.ax[a];
-        const int n = n_axis[a];
-        size_t m = (size_t)n * S;
-        std::vector<double> r(m), av(m), b(m);
-        for (size_t q = 0; q < m; ++q) {
-            r[q] = 1.0 / denom[q];
-            av[q] = c1v[q % S] / denom[q];
-            // c at the last node is never used by the back substitution
-            // (BioFVM_solvers.cpp:141): b there is 0, so z_{n-1} = y_{n-1}
-            b[q] = q >= (size_t)(n - 1) * S ? 0.0 : -c[q];
-        }
+        const size_t m = (size_t)n_axis[a] * S;
         HIP_OR_FAIL(&e, upload_f64(ax.denom, denom.data(), m));
         HIP_OR_FAIL(&e, upload_f64(ax.c, c.data(), m));
-        HIP_OR_FAIL(&e, upload_f64(ax.r, r.data(), m));
-        HIP_OR_FAIL(&e, upload_f64(ax.a, av.data(), m));
-        HIP_OR_FAIL(&e, upload_f64(ax.b, b.data(), m));
-        // k0: first row from which (r, a, b) equal their value at row n-2 for
-        // every row up to n-2 (the converged interior of the factorisation;
-        // the last row uses 1+c1+c2 and always differs).
-        auto row_equal = [&](int p, int q) {
-            for (int s = 0; s < S; ++s) {
-                size_t i = (size_t)p * S + s, j = (size_t)q * S + s;
-                if (r[i] != r[j] || av[i] != av[j] || b[i] != b[j]) return false;
-            }
-            return true;
-        };
-        int k0 = std::max(n - 1, 0);
-        if (n >= 3) {
-            k0 = n - 2;
-            while (k0 > 0 && row_equal(k0 - 1, n - 2)) --k0;
-        }
-        ax.k0 = k0;
+        int rc = build_woodbury(e, ax, n_axis[a], dt);
+        if (rc) return rc;
     }
     HIP_OR_FAIL(&e, upload_f64(e.c1, c1v.data(), S));
     e.coef_valid = true;
@@ -337,10 +388,7 @@ int bfvm_create_mesh(int nx, int ny, int nz, int n_substrates, double dx, double
         size_t m = (size_t)n_axis[a] * e->S;
         e->ax[a].n = n_axis[a];
         if ((st = alloc_f64(&e->ax[a].denom, m)) != hipSuccess ||
-            (st = alloc_f64(&e->ax[a].c, m)) != hipSuccess ||
-            (st = alloc_f64(&e->ax[a].r, m)) != hipSuccess ||
-            (st = alloc_f64(&e->ax[a].a, m)) != hipSuccess ||
-            (st = alloc_f64(&e->ax[a].b, m)) != hipSuccess) {
+            (st = alloc_f64(&e->ax[a].c, m)) != hipSuccess) {
             int rc = hip_fail(nullptr, st, "bfvm_create");
             bfvm_destroy(e);
             return rc;
@@ -365,9 +413,9 @@ int bfvm_destroy(bfvm_handle h)
     for (auto& a : e->ax) {
         dfree(a.denom);
         dfree(a.c);
-        dfree(a.r);
-        dfree(a.a);
-        dfree(a.b);
+        dfree(a.wb);
+        dfree(a.g0);
+        dfree(a.g1);
     }
     dfree_t(e->dir_idx);
     dfree(e->dir_val);
@@ -652,6 +700,23 @@ int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* node
     if (lines_per_block) *lines_per_block = p.cols;
     if (nodes_per_segment) *nodes_per_segment = p.L;
     return BFVM_OK;
+}
+
+int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
+    if (lines_per_block == 0) {
+        h->plan[axis] = bfvm::SweepPlan{};
+        return BFVM_OK;
+    }
+    const int n = axis == 0 ? h->nx : axis == 1 ? h->ny : h->nz;
+    for (const bfvm::SweepPlan& c : bfvm::sweep_candidates(n))
+        if (c.cols == lines_per_block && c.L == nodes_per_segment) {
+            h->plan[axis] = c;
+            return BFVM_OK;
+        }
+    return fail(h, BFVM_E_ARG, "launch shape not valid for this axis length");
 }
 
 int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, int* precision,

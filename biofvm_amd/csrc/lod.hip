@@ -1,30 +1,26 @@
 // LOD Thomas sweeps (diffusion_decay_solver__constant_coefficients_LOD_3D,
 // BioFVM_solvers.cpp:175-273) on the substrate-innermost store.
 //
-// Segmented Thomas.  Every line of n nodes is cut into nseg segments of at
-// most L nodes; one thread holds one segment of one (line, substrate) in
-// registers, so each sweep reads and writes the store once.  With
-//   forward  y_i = x_i*r_i + a_i*y_{i-1}      backward  z_i = y_i + b_i*z_{i+1}
-// a segment [p, q) is first solved with zero carries (y_{p-1} = 0, z_q = 0);
-// the true values follow from the carries by linearity,
-//   y_i = yhat_i + (prod_{m=p..i} a_m) * Y,   Y = y_{p-1}
-//   z_i = zhat_i + (prod_{m=i..q-1} b_m) * Z,  Z = z_q
-// and the carries chain across segments through LDS:
-//   Y_w = yhat_end(w) + (prod_w a) * Y_{w-1},  Z_w = zhat_start(w) + (prod_w b) * Z_{w+1}.
-// This is the reference's forward elimination / back substitution
-// (BioFVM_solvers.cpp:186-201) reassociated; the coefficient tables are the
-// reference's thomas_denom / thomas_c (:130-141) computed in FP64.
-//
-// Coefficients: rows [k0, n-1) of the tables are identical (the factorisation
-// converges to its interior fixed point), so a segment inside that range
-// keeps one row in registers; a segment holding a row < k0 or the last row
-// reads the tables.
+// Each sweep solves A z = x on every line, A the reference's tridiagonal
+// matrix (BioFVM_solvers.cpp:104-142).  The solve is reorganised for the GPU:
+//   1. A = A_inf - (rank 2 at the corners), A_inf = L U with constant
+//      bidiagonal factors (see AxisCoef), so z = A_inf^-1 x plus a rank-2
+//      Woodbury correction confined to the first kg0 / last kg1 nodes;
+//   2. each line is cut into segments of L nodes held in registers, one
+//      thread per (line, segment): the constant-coefficient recurrences are
+//      run with zero carries, the carries are chained across segments
+//      through LDS, and added back by linearity,
+//        y_i = yhat_i + a^(i-p+1) Y,   z_i = zhat_i + a^(q-i) Z.
+// Each sweep reads and writes the store once.  The arithmetic is FP64 for
+// either store precision.  The result equals the reference's forward
+// elimination / back substitution (:186-201) up to rounding.
 //
 // Kernels
-//   k_sweep       all three sweeps: 32 lines per block, up to 32 segments per
-//                 line, one thread per (line, segment); see its comment.
-//   k_line_sweep  one thread per whole line, for lines longer than the
-//                 segmented kernel holds (n > 32*32).
+//   k_sweep       all three sweeps: 16-64 lines per block, up to 1024/lines
+//                 segments of 4-32 nodes per line; see its comment.
+//   k_line_sweep  one thread per whole line with the reference's own
+//                 recurrences, for lines of one node or longer than the
+//                 segmented kernel holds (n > 64*32).
 #include <algorithm>
 
 #include "bfvm_internal.h"
@@ -35,135 +31,31 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxThreads = 1024;  // block = COLS columns x nseg segments
 
-// One thread's segment: nodes [p0, p0+cnt) of one (line, substrate).
-// Tables are FP64 for either store precision; an FP32 store is rounded once
-// per sweep, on the way out.
-struct SegCtx {
-    const double* __restrict__ tr;  // r_i = 1/denom_i           [i*S + s]
-    const double* __restrict__ ta;  // a_i = c1/denom_i
-    const double* __restrict__ tb;  // b_i = -c_i (0 at the last node)
-    double r_in, a_in, b_in;        // the converged interior row
-    int p0, cnt, S, s;
-    bool special;                   // segment holds a row < k0 or the last row
-};
-
-__device__ __forceinline__ SegCtx make_ctx(const double* tr, const double* ta, const double* tb,
-                                           int k0, int n, int S, int s, int p0, int cnt)
+// Carry chain of one column across its nseg segments, run by one thread:
+// slot u holds (alpha_u, beta_u) on entry and the carry into segment u on
+// exit, for C' = alpha_u + beta_u * C walked from segment 0 up (forward) or
+// from the top down (backward).  Returns the final C.
+template <int COLS>
+__device__ __forceinline__ double chain(double* xa, const double* xb, int lane, int nseg,
+                                        bool forward)
 {
-    SegCtx c;
-    c.tr = tr; c.ta = ta; c.tb = tb;
-    c.S = S; c.s = s; c.p0 = p0; c.cnt = cnt;
-    const int row = k0 < n - 1 ? k0 : n - 1;
-    c.r_in = tr[row * S + s];
-    c.a_in = ta[row * S + s];
-    c.b_in = tb[row * S + s];
-    c.special = cnt > 0 && (p0 < k0 || p0 + cnt >= n);
-    return c;
-}
-
-// Forward elimination over v[0..cnt) with zero carry-in: yhat, its value at
-// the segment end and the product of a over the segment.  Interior segments
-// use the register constants; the others read the tables row by row
-// (branch-free, so the loads pipeline ahead of the recurrence).
-template <int LMAX>
-__device__ __forceinline__ void seg_forward(double (&v)[LMAX], const SegCtx& c, double& yend,
-                                            double& pa)
-{
-    double y = 0.0, P = 1.0;
-    if (!c.special) {
-#pragma unroll
-        for (int t = 0; t < LMAX; ++t)
-            if (t < c.cnt) {
-                y = fma(c.a_in, y, v[t] * c.r_in);
-                v[t] = y;
-                P *= c.a_in;
-            }
+    double C = 0.0;
+    if (forward) {
+#pragma unroll 4
+        for (int u = 0; u < nseg; ++u) {
+            const double a = xa[u * COLS + lane], b = xb[u * COLS + lane];
+            xa[u * COLS + lane] = C;
+            C = fma(b, C, a);
+        }
     } else {
-#pragma unroll
-        for (int t = 0; t < LMAX; ++t)
-            if (t < c.cnt) {
-                const int q = (c.p0 + t) * c.S + c.s;
-                const double a = c.ta[q];
-                y = fma(a, y, v[t] * c.tr[q]);
-                v[t] = y;
-                P *= a;
-            }
+#pragma unroll 4
+        for (int u = nseg - 1; u >= 0; --u) {
+            const double a = xa[u * COLS + lane], b = xb[u * COLS + lane];
+            xa[u * COLS + lane] = C;
+            C = fma(b, C, a);
+        }
     }
-    yend = y;
-    pa = P;
-}
-
-// y_i += (prod_{m=p0..i} a_m) * Y
-template <int LMAX>
-__device__ __forceinline__ void seg_fix_forward(double (&v)[LMAX], const SegCtx& c, double Y)
-{
-    double P = 1.0;
-    if (!c.special) {
-#pragma unroll
-        for (int t = 0; t < LMAX; ++t)
-            if (t < c.cnt) {
-                P *= c.a_in;
-                v[t] = fma(P, Y, v[t]);
-            }
-    } else {
-#pragma unroll
-        for (int t = 0; t < LMAX; ++t)
-            if (t < c.cnt) {
-                P *= c.ta[(c.p0 + t) * c.S + c.s];
-                v[t] = fma(P, Y, v[t]);
-            }
-    }
-}
-
-// Back substitution over v[0..cnt) with zero carry-in from above: zhat, its
-// value at the segment start and the product of b over the segment.
-template <int LMAX>
-__device__ __forceinline__ void seg_backward(double (&v)[LMAX], const SegCtx& c, double& zstart,
-                                             double& pb)
-{
-    double z = 0.0, Q = 1.0;
-    if (!c.special) {
-#pragma unroll
-        for (int t = LMAX - 1; t >= 0; --t)
-            if (t < c.cnt) {
-                z = fma(c.b_in, z, v[t]);
-                v[t] = z;
-                Q *= c.b_in;
-            }
-    } else {
-#pragma unroll
-        for (int t = LMAX - 1; t >= 0; --t)
-            if (t < c.cnt) {
-                const double b = c.tb[(c.p0 + t) * c.S + c.s];
-                z = fma(b, z, v[t]);
-                v[t] = z;
-                Q *= b;
-            }
-    }
-    zstart = z;
-    pb = Q;
-}
-
-// z_i += (prod_{m=i..q-1} b_m) * Z
-template <int LMAX>
-__device__ __forceinline__ void seg_fix_backward(double (&v)[LMAX], const SegCtx& c, double Z)
-{
-    double Q = 1.0;
-    if (!c.special) {
-#pragma unroll
-        for (int t = LMAX - 1; t >= 0; --t)
-            if (t < c.cnt) {
-                Q *= c.b_in;
-                v[t] = fma(Q, Z, v[t]);
-            }
-    } else {
-#pragma unroll
-        for (int t = LMAX - 1; t >= 0; --t)
-            if (t < c.cnt) {
-                Q *= c.tb[(c.p0 + t) * c.S + c.s];
-                v[t] = fma(Q, Z, v[t]);
-            }
-    }
+    return C;
 }
 
 // One kernel for the three sweeps.  The lines of a sweep are "columns": a
@@ -173,20 +65,31 @@ __device__ __forceinline__ void seg_fix_backward(double (&v)[LMAX], const SegCtx
 //   x: plane = k, q = j, vstride = nx*S, line_stride = S
 //   y: plane = k, q = i, vstride = S,    line_stride = nx*S
 //   z: one plane, q = j*nx + i, vstride = S, line_stride = nx*ny*S
-// Block = COLS columns x nseg segments; thread tid -> column tid % COLS,
-// segment tid / COLS, so a wave holds COLS columns of 64/COLS consecutive
-// segments.  For y and z the COLS columns of a row are contiguous
-// (coalesced); for x they are row pieces of S contiguous values, and
-// consecutive nodes of a segment reuse the same cache lines.  Two blocks of
-// 1024 threads fit a CU (8 waves per SIMD) up to 16 nodes per segment.
-template <typename T, int LMAX, int COLS>
-__global__ void __launch_bounds__(kMaxThreads, LMAX <= 16 ? 8 : 4)
-k_sweep(T* __restrict__ rho, int n, int L, int nseg, size_t line_stride, int cols_per_plane,
-        size_t vstride, size_t plane_stride, int S, const double* __restrict__ tr,
-        const double* __restrict__ ta, const double* __restrict__ tb, int k0)
+// Block = COLS columns x nseg segments of L nodes; thread tid -> column
+// tid % COLS, segment tid / COLS, so a wave holds COLS columns of 64/COLS
+// consecutive segments.  For y and z the COLS columns of a row are
+// contiguous (coalesced); for x they are row pieces of S contiguous values,
+// and consecutive nodes of a segment reuse the same cache lines.
+//
+// Each thread solves A_inf on its segment with constant coefficients (see
+// AxisCoef): forward y_i = a*y_{i-1} + x_i*r and backward z_i = y_i + a*z_{i+1}
+// (b = a except at the last node, where b = 0), first with zero carries;
+// the segment-end values go through LDS, one thread per column walks the
+// carry chain, and each segment adds its carry times the running product
+// of a.  Segments within kg0 / kg1 of the line ends then add the rank-2
+// correction.  Only the segment holding node n-1 can be cut short: its nodes
+// past n-1 are clamped reads, are never stored, and cannot reach node n-1
+// because b is 0 there.  No loop reads a table or needs a mask.
+template <typename T, int L, int COLS>
+__global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4)
+k_sweep(T* __restrict__ rho, int n, int nseg, size_t line_stride, int cols_per_plane,
+        size_t vstride, size_t plane_stride, int S, const double* __restrict__ wb,
+        const double* __restrict__ g0, const double* __restrict__ g1, int kg0, int kg1)
 {
     __shared__ double xa[kMaxThreads];
     __shared__ double xb[kMaxThreads];
+    __shared__ double wfirst[COLS];  // w at node 0 of each column
+    __shared__ double wlast[COLS];   // w at node n-1 of each column
     const int tid = threadIdx.x;
     const int lane = tid % COLS;
     const int g = tid / COLS;
@@ -195,38 +98,87 @@ k_sweep(T* __restrict__ rho, int n, int L, int nseg, size_t line_stride, int col
     const int cc = act ? col : 0;
     const int s = cc % S;
     T* base = rho + (size_t)blockIdx.y * plane_stride + (size_t)(cc / S) * vstride + s;
-    const int p0 = g * L;
-    const int cnt = act ? max(0, min(L, n - p0)) : 0;
-    const SegCtx c = make_ctx(tr, ta, tb, k0, n, S, s, p0, cnt);
+    const int p0 = act ? g * L : 0;
+    const int cnt = act ? min(L, n - p0) : 0;
+    const bool has_last = p0 + L >= n;  // holds node n-1 (or nodes past it)
+    const double r = wb[s * 8 + 0], a = wb[s * 8 + 1];
 
-    double v[LMAX];
+    double v[L];
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t)
-        if (t < cnt) v[t] = (double)base[(size_t)(p0 + t) * line_stride];
+    for (int t = 0; t < L; ++t) v[t] = (double)base[(size_t)min(p0 + t, n - 1) * line_stride];
 
-    double yend = 0.0, pa = 1.0;
-    seg_forward<LMAX>(v, c, yend, pa);
-    xa[tid] = yend;
-    xb[tid] = pa;
+    // forward, zero carry-in
+    double y = 0.0;
+#pragma unroll
+    for (int t = 0; t < L; ++t) v[t] = y = fma(a, y, v[t] * r);
+    double aL = a;
+#pragma unroll
+    for (int t = 1; t < L; ++t) aL *= a;
+    xa[tid] = y;
+    xb[tid] = aL;
     __syncthreads();
-    double Y = 0.0;
-#pragma unroll 4
-    for (int u = 0; u < g; ++u) Y = fma(xb[u * COLS + lane], Y, xa[u * COLS + lane]);
-    seg_fix_forward<LMAX>(v, c, Y);
+    if (tid < COLS) chain<COLS>(xa, xb, lane, nseg, true);
+    __syncthreads();
+    {
+        double PY = xa[tid];
+#pragma unroll
+        for (int t = 0; t < L; ++t) {
+            PY *= a;
+            v[t] += PY;
+        }
+    }
+    if (has_last && act) {  // z_{n-1} = y_{n-1}
+        double wl = 0.0;
+#pragma unroll
+        for (int t = 0; t < L; ++t)
+            if (p0 + t == n - 1) wl = v[t];
+        wlast[lane] = wl;
+    }
 
-    double zs = 0.0, pb = 1.0;
-    seg_backward<LMAX>(v, c, zs, pb);
-    __syncthreads();  // every forward carry has been read
-    xa[tid] = zs;
-    xb[tid] = pb;
+    // backward, zero carry-in from above
+    double z = 0.0, bL;
+    if (!has_last) {
+#pragma unroll
+        for (int t = L - 1; t >= 0; --t) v[t] = z = fma(a, z, v[t]);
+        bL = aL;
+    } else {
+#pragma unroll
+        for (int t = L - 1; t >= 0; --t) v[t] = z = fma(p0 + t < n - 1 ? a : 0.0, z, v[t]);
+        bL = 0.0;  // nothing lies above the last segment
+    }
+    xa[tid] = z;  // each thread overwrites only the slot it alone has read
+    xb[tid] = bL;
     __syncthreads();
-    double Z = 0.0;
-#pragma unroll 4
-    for (int u = nseg - 1; u > g; --u) Z = fma(xb[u * COLS + lane], Z, xa[u * COLS + lane]);
-    seg_fix_backward<LMAX>(v, c, Z);
+    if (tid < COLS) wfirst[lane] = chain<COLS>(xa, xb, lane, nseg, false);
+    __syncthreads();
+    {
+        double QZ = xa[tid];
+        if (!has_last) {
+#pragma unroll
+            for (int t = L - 1; t >= 0; --t) {
+                QZ *= a;
+                v[t] += QZ;
+            }
+        }  // the last segment's carry from above is 0
+    }
+
+    // rank-2 correction near the ends
+    if (p0 < kg0 || p0 + L > n - kg1) {
+        const double w0 = wfirst[lane], wn = wlast[lane];
+        const double mu0 = wb[s * 8 + 2] * w0 + wb[s * 8 + 3] * wn;
+        const double mu1 = wb[s * 8 + 4] * w0 + wb[s * 8 + 5] * wn;
+#pragma unroll
+        for (int t = 0; t < L; ++t) {
+            const int p = p0 + t;
+            const int j = n - 1 - p;
+            const double e0 = g0[min(p, kg0 - 1) * S + s];
+            const double e1 = g1[min(max(j, 0), kg1 - 1) * S + s];
+            v[t] += (p < kg0 ? mu0 * e0 : 0.0) + (j >= 0 && j < kg1 ? mu1 * e1 : 0.0);
+        }
+    }
 
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t)
+    for (int t = 0; t < L; ++t)
         if (t < cnt) base[(size_t)(p0 + t) * line_stride] = (T)v[t];
 }
 
@@ -275,21 +227,24 @@ Geometry geometry(const Env& e, int axis)
     return {e.nz, (int)(nx * ny * S), 1, nx * ny * S, 0, S};
 }
 
-// Launch shape of one sweep: COLS lines per block and L nodes per segment
-// (nseg = ceil(n/L) segments, COLS*nseg <= 1024, L <= 32).  cols == 0 means
-// the line is too long for the segmented kernel.  A plan set by the
-// autotuner (bfvm_autotune) wins over the default.
+// Launch shape of one sweep: COLS lines per block and L nodes per segment,
+// L one of kNodes, nseg = ceil(n/L) segments with COLS*nseg <= 1024.
+// cols == 0 means the line is too long for the segmented kernel.  A plan set
+// by the autotuner (bfvm_autotune) wins over the default.
+constexpr int kNodes[] = {4, 8, 12, 16, 24, 32};
+constexpr int kColsSet[] = {16, 32, 64};
+
+bool plan_valid(int n, int cols, int L)
+{
+    return cols * ((n + L - 1) / L) <= kMaxThreads;
+}
+
 SweepPlan default_plan(int n)
 {
-    SweepPlan p;
-    p.cols = 32;
-    p.L = std::max(4, (n + 31) / 32);  // up to 32 segments of 32 lines
-    if (p.L > 32) {
-        p.cols = 16;  // up to 64 segments of 16 lines
-        p.L = std::max(4, (n + 63) / 64);
-    }
-    if (p.L > 32) p.cols = 0;
-    return p;
+    for (int cols : {32, 16})
+        for (int L : kNodes)
+            if (plan_valid(n, cols, L)) return SweepPlan{cols, L};
+    return SweepPlan{0, 0};
 }
 
 SweepPlan plan_for(const Env& e, int axis)
@@ -323,24 +278,29 @@ hipError_t sweep_naive(Env& e, int axis)
     return hipGetLastError();
 }
 
-template <typename T, int LMAX, int COLS>
-hipError_t launch_seg(Env& e, int axis, const Geometry& gm, int L, int nseg)
+template <typename T, int L, int COLS>
+hipError_t launch_seg(Env& e, int axis, const Geometry& gm, int nseg)
 {
     const AxisCoef& a = e.ax[axis];
     dim3 grid((gm.ncols + COLS - 1) / COLS, gm.nplanes);
     int threads = (COLS * nseg + 63) / 64 * 64;
-    k_sweep<T, LMAX, COLS><<<grid, threads, 0, e.stream>>>(
-        (T*)e.rho, gm.n, L, nseg, gm.line_stride, gm.ncols, gm.vstride, gm.plane_stride, e.S,
-        (const double*)a.r, (const double*)a.a, (const double*)a.b, a.k0);
+    k_sweep<T, L, COLS><<<grid, threads, 0, e.stream>>>(
+        (T*)e.rho, gm.n, nseg, gm.line_stride, gm.ncols, gm.vstride, gm.plane_stride, e.S,
+        (const double*)a.wb, (const double*)a.g0, (const double*)a.g1, a.kg0, a.kg1);
     return hipGetLastError();
 }
 
 template <typename T, int COLS>
 hipError_t launch_cols(Env& e, int axis, const Geometry& gm, int L, int nseg)
 {
-    if (L <= 8) return launch_seg<T, 8, COLS>(e, axis, gm, L, nseg);
-    if (L <= 16) return launch_seg<T, 16, COLS>(e, axis, gm, L, nseg);
-    return launch_seg<T, 32, COLS>(e, axis, gm, L, nseg);
+    switch (L) {
+    case 4: return launch_seg<T, 4, COLS>(e, axis, gm, nseg);
+    case 8: return launch_seg<T, 8, COLS>(e, axis, gm, nseg);
+    case 12: return launch_seg<T, 12, COLS>(e, axis, gm, nseg);
+    case 16: return launch_seg<T, 16, COLS>(e, axis, gm, nseg);
+    case 24: return launch_seg<T, 24, COLS>(e, axis, gm, nseg);
+    default: return launch_seg<T, 32, COLS>(e, axis, gm, nseg);
+    }
 }
 
 template <typename T>
@@ -348,7 +308,7 @@ hipError_t sweep(Env& e, int axis)
 {
     Geometry gm = geometry(e, axis);
     SweepPlan p = plan_for(e, axis);
-    if (p.cols == 0) return sweep_naive<T>(e, axis);
+    if (gm.n < 2 || p.cols == 0 || !plan_valid(gm.n, p.cols, p.L)) return sweep_naive<T>(e, axis);
     int nseg = (gm.n + p.L - 1) / p.L;
     switch (p.cols) {
     case 16: return launch_cols<T, 16>(e, axis, gm, p.L, nseg);
@@ -367,16 +327,9 @@ SweepPlan effective_plan(const Env& e, int axis)
 std::vector<SweepPlan> sweep_candidates(int n)
 {
     std::vector<SweepPlan> out;
-    for (int cols : {16, 32, 64}) {
-        int lmin = std::max(4, (n + kMaxThreads / cols - 1) / (kMaxThreads / cols));
-        for (int L : {lmin, 8, 12, 16, 24, 32}) {
-            if (L < lmin || L > 32) continue;
-            if (L > n && L != lmin) continue;
-            bool dup = false;
-            for (const SweepPlan& q : out) dup |= q.cols == cols && q.L == L;
-            if (!dup) out.push_back(SweepPlan{cols, L});
-        }
-    }
+    for (int cols : kColsSet)
+        for (int L : kNodes)
+            if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n)) out.push_back(SweepPlan{cols, L});
     return out;
 }
 

@@ -122,6 +122,10 @@ class DeviceMicroenvironment:
         """Picks the fastest launch shape of each sweep (on a scratch copy)."""
         self._check(self.lib.bfvm_autotune(self.h, float(dt)))
 
+    def set_sweep_plan(self, axis, lines_per_block, nodes_per_segment):
+        self._check(self.lib.bfvm_set_sweep_plan(self.h, int(axis), int(lines_per_block),
+                                                 int(nodes_per_segment)))
+
     def sweep_plans(self):
         """[(lines_per_block, nodes_per_segment)] for the x, y and z sweeps."""
         out = []

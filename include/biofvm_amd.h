@@ -175,6 +175,10 @@ int bfvm_autotune(bfvm_handle h, double dt);
  * one-thread-per-line fallback. */
 int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment);
 
+/* Sets the launch shape of one axis by hand (lines_per_block = 0 restores
+ * the default).  Returns BFVM_E_ARG for a shape the kernel cannot run. */
+int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment);
+
 /* Grid and store description. */
 int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, int* precision,
                   size_t* density_bytes);
