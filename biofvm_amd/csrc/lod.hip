@@ -31,6 +31,38 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxThreads = 1024;  // block = COLS columns x nseg segments
 
+// Raw buffer access: a per-lane byte offset plus a uniform (scalar) one, no
+// per-element address arithmetic.  Reads past the resource's end return 0
+// and writes there are dropped.  The descriptor's last word is the raw
+// 32-bit-element format used for gfx9-class buffers.
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, size_t bytes)
+{
+    const int nrec = bytes >= 0xffffffffull ? -1 : (int)(unsigned)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nrec, kRsrcWord3);
+}
+
+__device__ __forceinline__ double buf_load(float*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+__device__ __forceinline__ double buf_load(double*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ void buf_store(float*, __amdgpu_buffer_rsrc_t r, double x, unsigned vo,
+                                          unsigned so)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)x), r, vo, so, 0);
+}
+__device__ __forceinline__ void buf_store(double*, __amdgpu_buffer_rsrc_t r, double x, unsigned vo,
+                                          unsigned so)
+{
+    using u64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, x), r, vo, so, 0);
+}
+
 // Carry chain of one column across its nseg segments, run by one thread:
 // slot u holds (alpha_u, beta_u) on entry and the carry into segment u on
 // exit, for C' = alpha_u + beta_u * C walked from segment 0 up (forward) or
@@ -82,8 +114,8 @@ __device__ __forceinline__ double chain(double* xa, const double* xb, int lane, 
 // because b is 0 there.  No loop reads a table or needs a mask.
 template <typename T, int L, int COLS>
 __global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4)
-k_sweep(T* __restrict__ rho, int n, int nseg, size_t line_stride, int cols_per_plane,
-        size_t vstride, size_t plane_stride, int S, const double* __restrict__ wb,
+k_sweep(T* __restrict__ rho, size_t nvs, int n, int nseg, unsigned line_stride, int cols_per_plane,
+        unsigned vstride, size_t plane_stride, int S, const double* __restrict__ wb,
         const double* __restrict__ g0, const double* __restrict__ g1, int kg0, int kg1)
 {
     __shared__ double xa[kMaxThreads];
@@ -97,15 +129,30 @@ k_sweep(T* __restrict__ rho, int n, int nseg, size_t line_stride, int cols_per_p
     const bool act = col < cols_per_plane && g < nseg;
     const int cc = act ? col : 0;
     const int s = cc % S;
-    T* base = rho + (size_t)blockIdx.y * plane_stride + (size_t)(cc / S) * vstride + s;
+    const size_t plane0 = (size_t)blockIdx.y * plane_stride;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(rho + plane0, (nvs - plane0) * sizeof(T));
     const int p0 = act ? g * L : 0;
     const int cnt = act ? min(L, n - p0) : 0;
     const bool has_last = p0 + L >= n;  // holds node n-1 (or nodes past it)
     const double r = wb[s * 8 + 0], a = wb[s * 8 + 1];
+    // byte offset of node p0 of this lane's line; node p0+t adds t*step
+    const unsigned vo = ((unsigned)(cc / S) * vstride + (unsigned)s + (unsigned)p0 * line_stride) *
+                        (unsigned)sizeof(T);
+    const unsigned step = line_stride * (unsigned)sizeof(T);
 
+    // The last segment clamps its reads to node n-1 (every address stays in
+    // the store); its nodes past n-1 are never stored and cannot reach node
+    // n-1 because b is 0 there.
     double v[L];
+    if (!has_last) {
 #pragma unroll
-    for (int t = 0; t < L; ++t) v[t] = (double)base[(size_t)min(p0 + t, n - 1) * line_stride];
+        for (int t = 0; t < L; ++t) v[t] = buf_load((T*)nullptr, rs, vo, t * step);
+    } else {
+        const unsigned vl = vo + (unsigned)(n - 1 - p0) * step;  // node n-1
+#pragma unroll
+        for (int t = 0; t < L; ++t)
+            v[t] = buf_load((T*)nullptr, rs, p0 + t < n ? vo + t * step : vl, 0);
+    }
 
     // forward, zero carry-in
     double y = 0.0;
@@ -177,9 +224,15 @@ k_sweep(T* __restrict__ rho, int n, int nseg, size_t line_stride, int cols_per_p
         }
     }
 
+    if (!act) return;  // padding threads alias column 0 and must not store
+    if (!has_last) {
 #pragma unroll
-    for (int t = 0; t < L; ++t)
-        if (t < cnt) base[(size_t)(p0 + t) * line_stride] = (T)v[t];
+        for (int t = 0; t < L; ++t) buf_store((T*)nullptr, rs, v[t], vo, t * step);
+    } else {
+#pragma unroll
+        for (int t = 0; t < L; ++t)
+            if (t < cnt) buf_store((T*)nullptr, rs, v[t], vo, t * step);
+    }
 }
 
 // One thread owns one whole (line, substrate) pair and runs the reference's
@@ -285,8 +338,9 @@ hipError_t launch_seg(Env& e, int axis, const Geometry& gm, int nseg)
     dim3 grid((gm.ncols + COLS - 1) / COLS, gm.nplanes);
     int threads = (COLS * nseg + 63) / 64 * 64;
     k_sweep<T, L, COLS><<<grid, threads, 0, e.stream>>>(
-        (T*)e.rho, gm.n, nseg, gm.line_stride, gm.ncols, gm.vstride, gm.plane_stride, e.S,
-        (const double*)a.wb, (const double*)a.g0, (const double*)a.g1, a.kg0, a.kg1);
+        (T*)e.rho, e.nvs, gm.n, nseg, (unsigned)gm.line_stride, gm.ncols, (unsigned)gm.vstride,
+        gm.plane_stride, e.S, (const double*)a.wb, (const double*)a.g0, (const double*)a.g1, a.kg0,
+        a.kg1);
     return hipGetLastError();
 }
 
@@ -308,7 +362,11 @@ hipError_t sweep(Env& e, int axis)
 {
     Geometry gm = geometry(e, axis);
     SweepPlan p = plan_for(e, axis);
-    if (gm.n < 2 || p.cols == 0 || !plan_valid(gm.n, p.cols, p.L)) return sweep_naive<T>(e, axis);
+    // buffer offsets are 32-bit: the segmented kernel addresses at most 4 GiB
+    // from a plane's start
+    const size_t span = (axis == 2 ? e.nvs : (size_t)e.nx * e.ny * e.S) * sizeof(T);
+    if (gm.n < 2 || p.cols == 0 || !plan_valid(gm.n, p.cols, p.L) || span >= 0xffffffffull)
+        return sweep_naive<T>(e, axis);
     int nseg = (gm.n + p.L - 1) / p.L;
     switch (p.cols) {
     case 16: return launch_cols<T, 16>(e, axis, gm, p.L, nseg);
