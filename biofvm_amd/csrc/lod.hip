@@ -30,6 +30,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxThreads = 1024;  // block = COLS columns x nseg segments
+constexpr size_t kMaxCorrectionLds = 48 * 1024;  // g0/g1 rows staged per block
+constexpr int kPad = 32;  // zero rows around the correction tables (>= the largest L)
 
 // Raw buffer access: a per-lane byte offset plus a uniform (scalar) one, no
 // per-element address arithmetic.  Reads past the resource's end return 0
@@ -43,9 +45,9 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, si
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nrec, kRsrcWord3);
 }
 
-__device__ __forceinline__ double buf_load(float*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
+__device__ __forceinline__ float buf_load(float*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
 {
-    return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
 }
 __device__ __forceinline__ double buf_load(double*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
 {
@@ -122,6 +124,11 @@ k_sweep(T* __restrict__ rho, size_t nvs, int n, int nseg, unsigned line_stride, 
     __shared__ double xb[kMaxThreads];
     __shared__ double wfirst[COLS];  // w at node 0 of each column
     __shared__ double wlast[COLS];   // w at node n-1 of each column
+    // correction tables with zero padding so that no index needs a clamp:
+    //   G0 rows 0..kg0-1 = g0, then kPad zero rows
+    //   G1 kPad zero rows, rows kPad+j = g1 at node n-1-j (j < kg1), kPad zero rows
+    //   then wb (8 values per substrate)
+    extern __shared__ double gtab[];
     const int tid = threadIdx.x;
     const int lane = tid % COLS;
     const int g = tid / COLS;
@@ -143,21 +150,39 @@ k_sweep(T* __restrict__ rho, size_t nvs, int n, int nseg, unsigned line_stride, 
     // The last segment clamps its reads to node n-1 (every address stays in
     // the store); its nodes past n-1 are never stored and cannot reach node
     // n-1 because b is 0 there.
-    double v[L];
+    // raw store values, widened to FP64 one by one in the forward pass
+    T raw[L];
     if (!has_last) {
 #pragma unroll
-        for (int t = 0; t < L; ++t) v[t] = buf_load((T*)nullptr, rs, vo, t * step);
+        for (int t = 0; t < L; ++t) raw[t] = buf_load((T*)nullptr, rs, vo, t * step);
     } else {
         const unsigned vl = vo + (unsigned)(n - 1 - p0) * step;  // node n-1
 #pragma unroll
         for (int t = 0; t < L; ++t)
-            v[t] = buf_load((T*)nullptr, rs, p0 + t < n ? vo + t * step : vl, 0);
+            raw[t] = buf_load((T*)nullptr, rs, p0 + t < n ? vo + t * step : vl, 0);
+    }
+
+    // the correction tables, staged in LDS while the loads are in flight
+    // (read after the first barrier below)
+    double* G0 = gtab;
+    double* G1 = gtab + (kg0 + kPad) * S;
+    double* K = G1 + (kg1 + 2 * kPad) * S;
+    const int ng = (kg0 + kg1 + 3 * kPad) * S;
+    for (int i = tid; i < ng + 8 * S; i += blockDim.x) {
+        const int row = i / S, col = i - row * S;
+        double val = 0.0;
+        if (row < kg0) val = g0[i];
+        else if (row >= kg0 + 2 * kPad && row < kg0 + kg1 + 2 * kPad)
+            val = g1[(row - kg0 - 2 * kPad) * S + col];
+        else if (i >= ng) val = wb[i - ng];
+        gtab[i] = val;
     }
 
     // forward, zero carry-in
+    double v[L];
     double y = 0.0;
 #pragma unroll
-    for (int t = 0; t < L; ++t) v[t] = y = fma(a, y, v[t] * r);
+    for (int t = 0; t < L; ++t) v[t] = y = fma(a, y, (double)raw[t] * r);
     double aL = a;
 #pragma unroll
     for (int t = 1; t < L; ++t) aL *= a;
@@ -198,40 +223,31 @@ k_sweep(T* __restrict__ rho, size_t nvs, int n, int nseg, unsigned line_stride, 
     __syncthreads();
     if (tid < COLS) wfirst[lane] = chain<COLS>(xa, xb, lane, nseg, false);
     __syncthreads();
-    {
-        double QZ = xa[tid];
-        if (!has_last) {
-#pragma unroll
-            for (int t = L - 1; t >= 0; --t) {
-                QZ *= a;
-                v[t] += QZ;
-            }
-        }  // the last segment's carry from above is 0
-    }
 
-    // rank-2 correction near the ends
-    if (p0 < kg0 || p0 + L > n - kg1) {
+    // Last pass, from the top of the segment down, each node finished and
+    // stored as soon as it is final: the carry from above (a^(q-i) Z), the
+    // rank-2 correction near the line ends (mu0 g0_i + mu1 g1_i), store.
+    const double QZ0 = has_last ? 0.0 : xa[tid];  // the last segment has nothing above
+    const bool head = p0 < kg0, tail = p0 + L > n - kg1;
+    double mu0 = 0.0, mu1 = 0.0;
+    if (head || tail) {
         const double w0 = wfirst[lane], wn = wlast[lane];
-        const double mu0 = wb[s * 8 + 2] * w0 + wb[s * 8 + 3] * wn;
-        const double mu1 = wb[s * 8 + 4] * w0 + wb[s * 8 + 5] * wn;
-#pragma unroll
-        for (int t = 0; t < L; ++t) {
-            const int p = p0 + t;
-            const int j = n - 1 - p;
-            const double e0 = g0[min(p, kg0 - 1) * S + s];
-            const double e1 = g1[min(max(j, 0), kg1 - 1) * S + s];
-            v[t] += (p < kg0 ? mu0 * e0 : 0.0) + (j >= 0 && j < kg1 ? mu1 * e1 : 0.0);
-        }
+        const double* k = K + s * 8 + 2;  // K00, K01, K10, K11
+        mu0 = head ? k[0] * w0 + k[1] * wn : 0.0;
+        mu1 = tail ? k[2] * w0 + k[3] * wn : 0.0;
     }
-
-    if (!act) return;  // padding threads alias column 0 and must not store
-    if (!has_last) {
+    // g0 rows from node p0 up; g1 rows from node p0 down (zero padded)
+    const double* g0r = G0 + (head ? p0 : 0) * S + s;
+    const double* g1r = G1 + (tail ? kPad + n - 1 - p0 : 0) * S + s;
+    const bool store_all = act && !has_last;
+    double QZ = QZ0;
 #pragma unroll
-        for (int t = 0; t < L; ++t) buf_store((T*)nullptr, rs, v[t], vo, t * step);
-    } else {
-#pragma unroll
-        for (int t = 0; t < L; ++t)
-            if (t < cnt) buf_store((T*)nullptr, rs, v[t], vo, t * step);
+    for (int t = L - 1; t >= 0; --t) {
+        QZ *= a;
+        double z = v[t] + QZ;
+        if (head) z = fma(mu0, g0r[t * S], z);
+        if (tail) z = fma(mu1, g1r[-t * S], z);
+        if (store_all || (act && t < cnt)) buf_store((T*)nullptr, rs, z, vo, t * step);
     }
 }
 
@@ -337,7 +353,8 @@ hipError_t launch_seg(Env& e, int axis, const Geometry& gm, int nseg)
     const AxisCoef& a = e.ax[axis];
     dim3 grid((gm.ncols + COLS - 1) / COLS, gm.nplanes);
     int threads = (COLS * nseg + 63) / 64 * 64;
-    k_sweep<T, L, COLS><<<grid, threads, 0, e.stream>>>(
+    const size_t lds = sizeof(double) * (size_t)(a.kg0 + a.kg1 + 3 * kPad + 8) * e.S;
+    k_sweep<T, L, COLS><<<grid, threads, lds, e.stream>>>(
         (T*)e.rho, e.nvs, gm.n, nseg, (unsigned)gm.line_stride, gm.ncols, (unsigned)gm.vstride,
         gm.plane_stride, e.S, (const double*)a.wb, (const double*)a.g0, (const double*)a.g1, a.kg0,
         a.kg1);
@@ -365,7 +382,11 @@ hipError_t sweep(Env& e, int axis)
     // buffer offsets are 32-bit: the segmented kernel addresses at most 4 GiB
     // from a plane's start
     const size_t span = (axis == 2 ? e.nvs : (size_t)e.nx * e.ny * e.S) * sizeof(T);
-    if (gm.n < 2 || p.cols == 0 || !plan_valid(gm.n, p.cols, p.L) || span >= 0xffffffffull)
+    // the end-correction tables live in LDS beside the 16 KiB carry arrays
+    const size_t gbytes =
+        sizeof(double) * (size_t)(e.ax[axis].kg0 + e.ax[axis].kg1 + 3 * kPad + 8) * e.S;
+    if (gm.n < 2 || p.cols == 0 || !plan_valid(gm.n, p.cols, p.L) || span >= 0xffffffffull ||
+        gbytes > kMaxCorrectionLds)
         return sweep_naive<T>(e, axis);
     int nseg = (gm.n + p.L - 1) / p.L;
     switch (p.cols) {
