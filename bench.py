@@ -61,6 +61,17 @@ def kernel_bytes(case, kernel, w):
     return 0
 
 
+def measured_traffic(workload, kernel_name):
+    """HBM bytes per launch of `kernel_name` from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_sweep_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t["traffic_bytes_per_launch"] if t.get("kernel") == kernel_name else None
+
+
 def time_precision(case, precision, steps, warmup):
     from biofvm_amd import setup_case
     m = setup_case(case, precision)
@@ -182,6 +193,11 @@ def main():
     dom_avg = dom_ms / dom_n
     dom_bytes = kernel_bytes(case, dom, w_main)
     achieved = dom_bytes / (dom_avg * 1e-3) / 1e9
+    traffic = None
+    if dom.startswith("sweep"):
+        cols, L = plans["xyz".index(dom[-1])]
+        traffic = measured_traffic(args.workload,
+                                   f"k_sweep<{'float' if main_prec == FP32 else 'double'}, {L}, {cols}>")
     step_bytes = algorithmic_bytes(case, w_main)
 
     out_extra = {}
@@ -224,7 +240,7 @@ def main():
                                f"phases={case['phases']}"},
         "hbm_frac_step": step_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg},
         "kernels_ms_per_launch": {k: (v[0] / v[1] if v[1] else None) for k, v in per.items()},
         "sweep_plans_lines_x_nodes": {"x": plans[0], "y": plans[1], "z": plans[2]},
