@@ -1,6 +1,6 @@
 """Benchmark of the diffusion-decay step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload precision_400|drop_in_200|lod_50]
+    python bench.py [--steps K] [--warmup W] [--workload precision_400|drop_in_200|lod_50|ensemble_64]
 
 Default workload (the configuration the headline metric is quoted on):
 ``precision_400`` — 400^3 voxels x 8 substrates (dx=10 um, dt=0.01 min), D
@@ -15,6 +15,11 @@ full step (nx*ny*nz*S*K / wall seconds of K steps, inputs resident in HBM).
 against the 8 TB/s HBM3E peak, using its average launch time from HIP events
 on the library's stream.  ``cpu_baseline`` times BioFVM itself (compiled from
 source, oracle/_ref) on a slab of the same workload on this host's cores.
+
+``--workload ensemble_64`` times the batching form instead: 64 independent
+uneven grids (FP32 stores) advanced together, one step = one HIP graph replay
+of batched agent and sweep launches; its roofline prices the three batched
+sweeps of a diffusion-only step.
 """
 import argparse
 import json
@@ -152,13 +157,83 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_mib": mib, "bytes_read": n, "bytes_written": n}), flush=True)
 
 
+def ensemble_bench(args):
+    from biofvm_amd import FP32, FP64, Ensemble, setup_case
+    from biofvm_amd import workloads as W
+    cases = W.ensemble_64()
+    dt, ph = cases[0]["dt"], cases[0]["phases"]
+    nvs = sum(c["nx"] * c["ny"] * c["nz"] * c["S"] for c in cases)
+    n_agents = sum(len(c["agent_volume"]) for c in cases)
+    steps, warmup = args.steps, args.warmup
+
+    def run(precision):
+        members = [setup_case(c, precision) for c in cases]
+        ens = Ensemble(members)
+        L = ens.autotune(dt)
+        ens.run(dt, warmup, ph, graph=True)
+        ens.sync()
+        t0 = time.perf_counter()
+        ev_ms = ens.timed_run(dt, steps, ph, graph=True)
+        wall = time.perf_counter() - t0
+        eager_ms = ens.timed_run(dt, max(1, steps // 4), ph, graph=False) / max(1, steps // 4)
+        return ens, members, L, wall, ev_ms, eager_ms
+
+    ens, members, L, wall, ev_ms, eager_ms = run(FP32)
+    out = [m.download() for m in members]
+    # diffusion-only steps (after the result was taken) price the three batched sweeps
+    sweep_steps = max(3, min(steps, 20))
+    sweep_ms = ens.timed_run(dt, sweep_steps, 4, graph=True) / sweep_steps
+    ens.close()
+    for m in members:
+        m.close()
+    extra = {}
+    if not args.no_fp64:
+        ens64, mem64, L64, wall64, ev64, eager64 = run(FP64)
+        err_n, err_l2n, err_l2d = 0.0, 0.0, 0.0
+        for m, a in zip(mem64, out):
+            b = m.download()
+            err_n = max(err_n, float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)))
+            err_l2n += float(np.sum((a - b) ** 2))
+            err_l2d += float(np.sum(b ** 2))
+        ens64.close()
+        for m in mem64:
+            m.close()
+        extra = {"fp64": {"value": nvs * steps / wall64, "ms_per_step": 1e3 * wall64 / steps,
+                          "segment_nodes": L64},
+                 "error_vs_fp64": {"steps": warmup + steps + max(1, steps // 4),
+                                   "max_rel_normwise_per_member": err_n,
+                                   "l2_rel": (err_l2n / err_l2d) ** 0.5}}
+    sweep_bytes = 3 * 2 * nvs * 4
+    achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9  # three launches' bytes over their summed time
+    line = {
+        "metric": "voxel-substrate updates/s (ensemble step: agents + LOD diffusion-decay, all members)",
+        "value": nvs * steps / wall,
+        "unit": "voxel-substrate updates/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * wall / steps,
+        "events_ms_per_step": ev_ms / steps,
+        "eager_ms_per_step": eager_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded numpy default_rng; see biofvm_amd/workloads.py)",
+        "config": {"workload": f"ensemble_64: {len(cases)} grids, {nvs} voxel-substrate values, "
+                               f"{n_agents} agents, dt={dt}, phases={ph}, one HIP graph per step"},
+        "segment_nodes": L,
+        "roofline": {"bound": "hbm", "kernel": "k_sweep_batch (3 per step, average)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": sweep_bytes / 3, "avg_launch_ms": sweep_ms / 3},
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="precision_400",
-                    choices=["precision_400", "drop_in_200", "lod_50"])
+                    choices=["precision_400", "drop_in_200", "lod_50", "ensemble_64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -176,6 +251,9 @@ def main():
     from biofvm_amd import FP32, FP64, _capi
     from biofvm_amd import workloads as W
     _capi.load()
+    if args.workload == "ensemble_64":
+        ensemble_bench(args)
+        return
 
     case = getattr(W, args.workload)()
     nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]

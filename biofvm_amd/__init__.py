@@ -6,7 +6,7 @@ the C ABI in ``include/biofvm_amd.h``.  This package is a thin ctypes layer
 over that ABI (``device``) plus seeded synthetic inputs (``workloads``).
 """
 from ._capi import ALL, BULK, CELLS, DIFFUSION, FP32, FP64, BfvmError, LIB_PATH, load
-from .device import DeviceMicroenvironment, run_case, setup_case
+from .device import DeviceMicroenvironment, Ensemble, run_case, setup_case
 
 __all__ = ["ALL", "BULK", "CELLS", "DIFFUSION", "FP32", "FP64", "BfvmError", "LIB_PATH", "load",
-           "DeviceMicroenvironment", "run_case", "setup_case"]
+           "DeviceMicroenvironment", "Ensemble", "run_case", "setup_case"]

@@ -45,6 +45,15 @@ SIGNATURES = {
     "bfvm_get_info": (_i, [_h, _ip, _ip, _ip, _ip, _ip, _sp]),
     "bfvm_density_device_ptr": (_i, [_h, ctypes.POINTER(_vp)]),
     "bfvm_last_error": (ctypes.c_char_p, [_h]),
+    "bfvm_ensemble_create": (_i, [_i, ctypes.POINTER(_h), ctypes.POINTER(_vp)]),
+    "bfvm_ensemble_destroy": (_i, [_vp]),
+    "bfvm_ensemble_run": (_i, [_vp, _d, _i, _i, _i]),
+    "bfvm_ensemble_sync": (_i, [_vp]),
+    "bfvm_ensemble_timed_run": (_i, [_vp, _d, _i, _i, _i, _dp]),
+    "bfvm_ensemble_autotune": (_i, [_vp, _d]),
+    "bfvm_ensemble_get_segment": (_i, [_vp, _ip]),
+    "bfvm_ensemble_set_segment": (_i, [_vp, _i]),
+    "bfvm_ensemble_last_error": (ctypes.c_char_p, [_vp]),
 }
 
 _lib = None
@@ -74,4 +83,10 @@ def load():
 def check(rc, handle=None):
     if rc != 0:
         msg = load().bfvm_last_error(handle)
+        raise BfvmError(rc, msg.decode() if msg else "")
+
+
+def check_ensemble(rc, ens):
+    if rc != 0:
+        msg = load().bfvm_ensemble_last_error(ens)
         raise BfvmError(rc, msg.decode() if msg else "")

@@ -37,6 +37,23 @@ struct AxisCoef {
     int kg0 = 0, kg1 = 0;
 };
 
+// Arguments of the segmented sweep kernel for one axis of one grid (see
+// lod.hip); also the per-member table of the ensemble launch.
+struct SweepArgs {
+    void* rho;               // density store (float or double)
+    size_t nvs;              // values in the store
+    size_t plane_stride;     // values between planes
+    const double* wb;        // per substrate: 1/d, c1/d, K00, K01, K10, K11, 0, 0
+    const double* g0;        // (kg0, S)
+    const double* g1;        // (kg1, S)
+    int n, nseg;             // nodes per line, segments per line
+    int cols_per_plane, S, kg0, kg1;
+    unsigned line_stride;    // values between nodes of a line
+    unsigned vstride;        // values between voxels of a plane's column set
+    int cols;                // lines per block (a power of two)
+    int tiles, planes;       // column tiles per plane, planes
+};
+
 // Launch shape of the segmented sweep of one axis (see lod.hip).
 struct SweepPlan {
     int cols = 0;  // lines per block (16, 32 or 64); 0 = default
@@ -51,6 +68,7 @@ struct Env {
     size_t nvox = 0, nvs = 0;  // voxels, voxel-substrate values
     size_t elem = 8;           // bytes per device value
     hipStream_t stream = nullptr;
+    hipStream_t ensemble_stream = nullptr;  // stream of the ensemble holding this handle, if any
     int device = 0;
 
     void* rho = nullptr;  // density store, nvs values
@@ -60,6 +78,7 @@ struct Env {
     std::vector<double> D, lambda;
     bool coef_valid = false;
     double coef_dt = 0;
+    unsigned coef_epoch = 0;  // bumped whenever the device tables are rebuilt
     void* c1 = nullptr;  // dt*D/dx^2 per substrate (FP64)
     AxisCoef ax[3];
     SweepPlan plan[3];  // per-axis launch shape chosen by the autotuner
@@ -69,6 +88,7 @@ struct Env {
     int* dir_idx = nullptr;
     void* dir_val = nullptr;          // n_dir*S, device precision
     unsigned char* dir_mask = nullptr;  // S bytes
+    unsigned source_epoch = 0;  // bumped whenever Dirichlet nodes or bulk fields are set
 
     // Bulk fields and the implicit-step terms built from them for one dt:
     // A = dt*(T*S), B = (U+S)*dt + 1 (all FP64 on the device).
@@ -84,6 +104,7 @@ struct Env {
     // Agents grouped by voxel (list order kept within a voxel).
     int n_agents = 0;
     int n_agent_voxels = 0;
+    unsigned agent_epoch = 0;  // bumped whenever the agent arrays change
     int* agent_seg_voxel = nullptr;  // n_agent_voxels
     int* agent_seg_start = nullptr;  // n_agent_voxels + 1
     void* agent_t1 = nullptr;        // n_agents*S, grouped order, FP64
@@ -105,6 +126,34 @@ hipError_t launch_bulk_prepare(Env& e, double dt);
 hipError_t launch_bulk(Env& e);
 hipError_t launch_agents(Env& e);
 hipError_t launch_sweep(Env& e, int axis);
+// Ensemble sweeps: whether a member's axis can join a launch with L-node
+// segments, its table entry, its dynamic LDS bytes, and the launch itself
+// (kMaxThreads threads per block, `blocks` blocks in total).
+bool batch_member_ok(const Env& e, int axis, int L);
+SweepArgs batch_args(const Env& e, int axis, int L);
+size_t batch_lds(const SweepArgs& A);
+hipError_t launch_sweep_batch(int precision, int L, const SweepArgs* d_args, const int* d_first,
+                              int nm, int blocks, size_t lds, hipStream_t st);
+
+// Agents of several grids in one launch: one entry per member.
+struct AgentBatch {
+    void* rho;
+    const int* gvox;
+    const int* gstart;
+    const double* t1;
+    const double* t2;
+    int S, n_groups;
+    long long first;  // first global thread of this member
+};
+hipError_t launch_agents_batch(int precision, const AgentBatch* d_tab, int nm, long long threads,
+                               hipStream_t st);
+
+// Thomas/Woodbury tables for dt (rebuilt when dt, D or lambda changed);
+// bumps Env::coef_epoch when it rebuilds.
+int ensure_coefficients(Env& e, double dt);
+// Bulk implicit terms for dt (rebuilt when dt changed).
+hipError_t ensure_bulk(Env& e, double dt);
+
 // Valid segmented shapes for a line of n nodes, for the autotuner.
 std::vector<SweepPlan> sweep_candidates(int n);
 // The launch shape a sweep of `axis` uses now (autotuned or default).

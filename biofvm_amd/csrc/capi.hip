@@ -219,9 +219,22 @@ int build_woodbury(Env& e, bfvm::AxisCoef& ax, int n, double dt)
     return BFVM_OK;
 }
 
-int ensure_coefficients(Env& e, double dt)
+// Waits for everything enqueued on the handle: its own stream and, for a
+// member of an ensemble, the ensemble's stream.
+hipError_t quiesce(Env& e)
+{
+    if (e.ensemble_stream) {
+        hipError_t st = hipStreamSynchronize(e.ensemble_stream);
+        if (st != hipSuccess) return st;
+    }
+    return hipStreamSynchronize(e.stream);
+}
+
+int ensure_coefficients_impl(Env& e, double dt)
 {
     if (e.coef_valid && e.coef_dt == dt) return BFVM_OK;
+    // kernels in flight may still read the tables about to be replaced
+    HIP_OR_FAIL(&e, quiesce(e));
     const int n_axis[3] = {e.nx, e.ny, e.nz};
     const int S = e.S;
     std::vector<double> denom, c, c1v;
@@ -237,8 +250,10 @@ int ensure_coefficients(Env& e, double dt)
     HIP_OR_FAIL(&e, upload_f64(e.c1, c1v.data(), S));
     e.coef_valid = true;
     e.coef_dt = dt;
+    ++e.coef_epoch;
     return BFVM_OK;
 }
+
 
 // Launch wrapper: optional per-kernel HIP events for bfvm_timed_run.
 template <typename F>
@@ -268,11 +283,7 @@ int step_impl(Env& e, double dt, int flags)
 {
     int rc;
     if ((flags & BFVM_BULK) && e.have_bulk) {
-        if (!e.bulk_valid || e.bulk_dt != dt) {
-            HIP_OR_FAIL(&e, bfvm::launch_bulk_prepare(e, dt));
-            e.bulk_valid = true;
-            e.bulk_dt = dt;
-        }
+        HIP_OR_FAIL(&e, bfvm::ensure_bulk(e, dt));
         if ((rc = run_kernel(e, BFVM_K_BULK, [&] { return bfvm::launch_bulk(e); }))) return rc;
     }
     if ((flags & BFVM_CELLS) && e.n_agents > 0) {
@@ -283,7 +294,7 @@ int step_impl(Env& e, double dt, int flags)
             return fail(&e, BFVM_E_MESH,
                         "This algorithm is written for uniform Cartesian meshes (dx == dy == dz).");
         if (e.D.empty()) return fail(&e, BFVM_E_STATE, "coefficients not set (bfvm_set_coefficients)");
-        if ((rc = ensure_coefficients(e, dt))) return rc;
+        if ((rc = ensure_coefficients_impl(e, dt))) return rc;
         for (int axis = 0; axis < 3; ++axis) {
             if (e.n_dir > 0)
                 if ((rc = run_kernel(e, BFVM_K_DIRICHLET, [&] { return bfvm::launch_dirichlet(e); })))
@@ -308,6 +319,7 @@ void free_agents(Env& e)
 int set_agents_impl(Env& e, int n, const int* voxel, const double* t1, const double* t2)
 {
     free_agents(e);
+    ++e.agent_epoch;
     if (n == 0) return BFVM_OK;
     const int S = e.S;
     for (int a = 0; a < n; ++a)
@@ -345,6 +357,21 @@ int set_agents_impl(Env& e, int n, const int* voxel, const double* t1, const dou
 }
 
 }  // namespace
+
+namespace bfvm {
+int ensure_coefficients(Env& e, double dt) { return ensure_coefficients_impl(e, dt); }
+
+hipError_t ensure_bulk(Env& e, double dt)
+{
+    if (!e.have_bulk || (e.bulk_valid && e.bulk_dt == dt)) return hipSuccess;
+    hipError_t st = launch_bulk_prepare(e, dt);
+    if (st == hipSuccess) {
+        e.bulk_valid = true;
+        e.bulk_dt = dt;
+    }
+    return st;
+}
+}  // namespace bfvm
 
 extern "C" {
 
@@ -406,6 +433,8 @@ int bfvm_create(int nx, int ny, int nz, int n_substrates, double dx, int precisi
 int bfvm_destroy(bfvm_handle h)
 {
     if (!h) return BFVM_OK;
+    if (h->ensemble_stream)
+        return fail(h, BFVM_E_STATE, "the handle belongs to an ensemble: destroy the ensemble first");
     bfvm_env* e = h;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     dfree(e->rho);
@@ -446,7 +475,7 @@ int bfvm_upload_density(bfvm_handle h, const double* host)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (!host) return fail(h, BFVM_E_ARG, "host is NULL");
-    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    HIP_OR_FAIL(h, quiesce(*h));
     HIP_OR_FAIL(h, upload_values(*h, h->rho, host, h->nvs));
     return BFVM_OK;
 }
@@ -455,7 +484,7 @@ int bfvm_download_density(bfvm_handle h, double* host)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (!host) return fail(h, BFVM_E_ARG, "host is NULL");
-    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    HIP_OR_FAIL(h, quiesce(*h));
     if (h->precision == BFVM_FP64) {
         HIP_OR_FAIL(h, hipMemcpy(host, h->rho, h->nvs * sizeof(double), hipMemcpyDeviceToHost));
     } else {
@@ -471,7 +500,8 @@ int bfvm_set_dirichlet(bfvm_handle h, int n, const int* voxel_index, const doubl
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (n < 0) return fail(h, BFVM_E_ARG, "n < 0");
-    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    HIP_OR_FAIL(h, quiesce(*h));
+    ++h->source_epoch;
     dfree_t(h->dir_idx);
     dfree(h->dir_val);
     h->n_dir = 0;
@@ -496,10 +526,11 @@ int bfvm_set_bulk(bfvm_handle h, const double* supply_rate, const double* supply
                   const double* uptake_rate)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
-    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    HIP_OR_FAIL(h, quiesce(*h));
     bool none = !supply_rate && !supply_target && !uptake_rate;
     bool all = supply_rate && supply_target && uptake_rate;
     if (!none && !all) return fail(h, BFVM_E_ARG, "give all three bulk fields or none");
+    ++h->source_epoch;
     h->have_bulk = false;
     h->bulk_valid = false;
     if (none) {
@@ -532,7 +563,7 @@ int bfvm_set_agents(bfvm_handle h, int n, const int* voxel_index, const double* 
     if (n < 0) return fail(h, BFVM_E_ARG, "n < 0");
     if (n > 0 && (!voxel_index || !temp1 || !temp2))
         return fail(h, BFVM_E_ARG, "voxel_index, temp1 and temp2 must be non-NULL");
-    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    HIP_OR_FAIL(h, quiesce(*h));
     return set_agents_impl(*h, n, voxel_index, temp1, temp2);
 }
 
@@ -565,13 +596,14 @@ int bfvm_set_agent_rates(bfvm_handle h, int n, const double* position, const dou
                         uptake + (size_t)a * S, &t1[(size_t)a * S], &t2[(size_t)a * S]);
     }
     if (out_voxel) std::copy(vox.begin(), vox.end(), out_voxel);
-    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    HIP_OR_FAIL(h, quiesce(*h));
     return set_agents_impl(*h, n, vox.data(), t1.data(), t2.data());
 }
 
 int bfvm_step(bfvm_handle h, double dt, int flags)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (h->ensemble_stream) HIP_OR_FAIL(h, hipStreamSynchronize(h->ensemble_stream));
     return step_impl(*h, dt, flags);
 }
 
@@ -579,6 +611,7 @@ int bfvm_run(bfvm_handle h, double dt, int steps, int flags)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (steps < 0) return fail(h, BFVM_E_ARG, "steps < 0");
+    if (h->ensemble_stream) HIP_OR_FAIL(h, hipStreamSynchronize(h->ensemble_stream));
     for (int n = 0; n < steps; ++n) {
         int rc = step_impl(*h, dt, flags);
         if (rc) return rc;
@@ -589,7 +622,7 @@ int bfvm_run(bfvm_handle h, double dt, int steps, int flags)
 int bfvm_sync(bfvm_handle h)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
-    HIP_OR_FAIL(h, hipStreamSynchronize(h->stream));
+    HIP_OR_FAIL(h, quiesce(*h));
     return BFVM_OK;
 }
 
@@ -602,19 +635,15 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
     // Factorisation and bulk terms are prepared outside the timed region.
     int rc;
     if ((flags & BFVM_DIFFUSION) && e.uniform && !e.D.empty())
-        if ((rc = ensure_coefficients(e, dt))) return rc;
-    if ((flags & BFVM_BULK) && e.have_bulk && (!e.bulk_valid || e.bulk_dt != dt)) {
-        HIP_OR_FAIL(&e, bfvm::launch_bulk_prepare(e, dt));
-        e.bulk_valid = true;
-        e.bulk_dt = dt;
-    }
+        if ((rc = ensure_coefficients_impl(e, dt))) return rc;
+    if (flags & BFVM_BULK) HIP_OR_FAIL(&e, bfvm::ensure_bulk(e, dt));
     hipEvent_t t0, t1;
     HIP_OR_FAIL(&e, hipEventCreate(&t0));
     HIP_OR_FAIL(&e, hipEventCreate(&t1));
     e.kernel_timing = per_kernel_ms != nullptr;
     e.ev_next = 0;
     e.ev_marks.clear();
-    HIP_OR_FAIL(&e, hipStreamSynchronize(e.stream));
+    HIP_OR_FAIL(&e, quiesce(e));
     HIP_OR_FAIL(&e, hipEventRecord(t0, e.stream));
     rc = BFVM_OK;
     for (int n = 0; n < steps && rc == BFVM_OK; ++n) rc = step_impl(e, dt, flags);
@@ -649,7 +678,8 @@ int bfvm_autotune(bfvm_handle h, double dt)
     Env& e = *h;
     if (e.D.empty()) return fail(&e, BFVM_E_STATE, "coefficients not set (bfvm_set_coefficients)");
     if (!e.uniform) return fail(&e, BFVM_E_MESH, "non-uniform mesh: the LOD solver is not used");
-    int rc = ensure_coefficients(e, dt);
+    HIP_OR_FAIL(&e, quiesce(e));
+    int rc = ensure_coefficients_impl(e, dt);
     if (rc) return rc;
     void* scratch = nullptr;
     HIP_OR_FAIL(&e, hipMalloc(&scratch, e.nvs * e.elem));

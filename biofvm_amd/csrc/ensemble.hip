@@ -1,0 +1,409 @@
+// Ensembles: many independent grids advanced together, one launch per phase
+// for all of them, optionally replayed from a HIP graph of one step.
+//
+// Members are ordinary handles (each with its own store, coefficients,
+// Dirichlet nodes, bulk fields and agents, all of one precision).  A step of
+// the ensemble runs, in the reference's order (bulk -> cells -> diffusion):
+//   - bulk: one k_bulk launch per member that has bulk fields;
+//   - cells: one k_agents_batch launch over every member's agent groups;
+//   - diffusion, per axis: Dirichlet launches for members that have nodes,
+//     then one k_sweep_batch launch over every member (members whose lines
+//     do not fit the batched kernel take their own sweep launch).
+// The launch shape of the batched sweep (nodes per segment L) is picked by
+// bfvm_ensemble_autotune.
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "bfvm_internal.h"
+
+using bfvm::Env;
+
+struct bfvm_ensemble_t {
+    std::vector<Env*> m;
+    int precision = BFVM_FP64;
+    hipStream_t stream = nullptr;
+    int L = 8;  // nodes per segment of the batched sweep
+
+    // tables, valid for (dt, L, member coefficient/agent epochs)
+    bool tables_valid = false;
+    double tab_dt = 0;
+    int tab_L = 0;
+    std::vector<unsigned> coef_epoch, agent_epoch, source_epoch;
+    bfvm::SweepArgs* d_args[3] = {nullptr, nullptr, nullptr};
+    int* d_first[3] = {nullptr, nullptr, nullptr};
+    int nbatch[3] = {0, 0, 0};       // members in the batched launch
+    int blocks[3] = {0, 0, 0};
+    size_t lds[3] = {0, 0, 0};
+    std::vector<int> solo[3];        // members sweeping on their own
+    bfvm::AgentBatch* d_agents = nullptr;
+    int n_agent_members = 0;
+    long long agent_threads = 0;
+
+    // graph of one step
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    double graph_dt = 0;
+    int graph_flags = -1;
+
+    std::string err;
+};
+
+namespace {
+
+int efail(bfvm_ensemble_t* e, int code, const std::string& msg)
+{
+    if (e) e->err = msg;
+    return code;
+}
+
+#define EHIP(e, call)                                                                          \
+    do {                                                                                       \
+        hipError_t st_ = (call);                                                               \
+        if (st_ != hipSuccess) return efail(e, BFVM_E_DEVICE, std::string(#call) + ": " +      \
+                                                                  hipGetErrorString(st_));     \
+    } while (0)
+
+// Runs `f` with member `m` launching on the ensemble's stream.
+template <typename F>
+hipError_t on_stream(Env& m, hipStream_t st, F&& f)
+{
+    hipStream_t keep = m.stream;
+    m.stream = st;
+    hipError_t r = f();
+    m.stream = keep;
+    return r;
+}
+
+void free_tables(bfvm_ensemble_t& E)
+{
+    for (int a = 0; a < 3; ++a) {
+        if (E.d_args[a]) (void)hipFree(E.d_args[a]);
+        if (E.d_first[a]) (void)hipFree(E.d_first[a]);
+        E.d_args[a] = nullptr;
+        E.d_first[a] = nullptr;
+        E.solo[a].clear();
+    }
+    if (E.d_agents) (void)hipFree(E.d_agents);
+    E.d_agents = nullptr;
+    E.tables_valid = false;
+}
+
+void drop_graph(bfvm_ensemble_t& E)
+{
+    if (E.exec) (void)hipGraphExecDestroy(E.exec);
+    if (E.graph) (void)hipGraphDestroy(E.graph);
+    E.exec = nullptr;
+    E.graph = nullptr;
+    E.graph_flags = -1;
+}
+
+// Host-side preparation of one step: coefficients and bulk terms of every
+// member for dt, then the batched launch tables.  No kernel is enqueued
+// here except bulk preparation, so a step captured after this is launches only.
+int prepare(bfvm_ensemble_t& E, double dt, int flags)
+{
+    bool changed = !E.tables_valid || E.tab_dt != dt || E.tab_L != E.L;
+    for (size_t i = 0; i < E.m.size(); ++i) {
+        Env& m = *E.m[i];
+        if (flags & BFVM_DIFFUSION) {
+            if (!m.uniform) return efail(&E, BFVM_E_MESH, "member " + std::to_string(i) + " has a non-uniform mesh");
+            if (m.D.empty()) return efail(&E, BFVM_E_STATE, "member " + std::to_string(i) + " has no coefficients");
+            int rc = bfvm::ensure_coefficients(m, dt);
+            if (rc) return efail(&E, rc, "member " + std::to_string(i) + ": " + m.err);
+        }
+        if (flags & BFVM_BULK) EHIP(&E, on_stream(m, E.stream, [&] { return bfvm::ensure_bulk(m, dt); }));
+        if (E.coef_epoch[i] != m.coef_epoch || E.agent_epoch[i] != m.agent_epoch ||
+            E.source_epoch[i] != m.source_epoch)
+            changed = true;
+    }
+    if (!changed) return BFVM_OK;
+    drop_graph(E);
+    free_tables(E);
+    for (int axis = 0; axis < 3; ++axis) {
+        std::vector<bfvm::SweepArgs> args;
+        std::vector<int> first{0};
+        size_t lds = 0;
+        for (size_t i = 0; i < E.m.size(); ++i) {
+            Env& m = *E.m[i];
+            if (!m.coef_valid || !bfvm::batch_member_ok(m, axis, E.L)) {
+                E.solo[axis].push_back((int)i);
+                continue;
+            }
+            bfvm::SweepArgs A = bfvm::batch_args(m, axis, E.L);
+            args.push_back(A);
+            first.push_back(first.back() + A.tiles * A.planes);
+            lds = std::max(lds, bfvm::batch_lds(A));
+        }
+        E.nbatch[axis] = (int)args.size();
+        E.blocks[axis] = first.back();
+        E.lds[axis] = lds;
+        if (!args.empty()) {
+            EHIP(&E, hipMalloc((void**)&E.d_args[axis], sizeof(bfvm::SweepArgs) * args.size()));
+            EHIP(&E, hipMalloc((void**)&E.d_first[axis], sizeof(int) * first.size()));
+            EHIP(&E, hipMemcpy(E.d_args[axis], args.data(), sizeof(bfvm::SweepArgs) * args.size(),
+                               hipMemcpyHostToDevice));
+            EHIP(&E, hipMemcpy(E.d_first[axis], first.data(), sizeof(int) * first.size(),
+                               hipMemcpyHostToDevice));
+        }
+    }
+    std::vector<bfvm::AgentBatch> ag;
+    long long threads = 0;
+    for (Env* pm : E.m) {
+        if (pm->n_agent_voxels == 0) continue;
+        bfvm::AgentBatch b;
+        b.rho = pm->rho;
+        b.gvox = pm->agent_seg_voxel;
+        b.gstart = pm->agent_seg_start;
+        b.t1 = (const double*)pm->agent_t1;
+        b.t2 = (const double*)pm->agent_t2;
+        b.S = pm->S;
+        b.n_groups = pm->n_agent_voxels;
+        b.first = threads;
+        threads += (long long)pm->n_agent_voxels * pm->S;
+        ag.push_back(b);
+    }
+    E.n_agent_members = (int)ag.size();
+    E.agent_threads = threads;
+    if (!ag.empty()) {
+        EHIP(&E, hipMalloc((void**)&E.d_agents, sizeof(bfvm::AgentBatch) * ag.size()));
+        EHIP(&E, hipMemcpy(E.d_agents, ag.data(), sizeof(bfvm::AgentBatch) * ag.size(),
+                           hipMemcpyHostToDevice));
+    }
+    for (size_t i = 0; i < E.m.size(); ++i) {
+        E.coef_epoch[i] = E.m[i]->coef_epoch;
+        E.agent_epoch[i] = E.m[i]->agent_epoch;
+        E.source_epoch[i] = E.m[i]->source_epoch;
+    }
+    E.tab_dt = dt;
+    E.tab_L = E.L;
+    E.tables_valid = true;
+    return BFVM_OK;
+}
+
+// The launches of one step (after prepare), all on the ensemble's stream.
+hipError_t enqueue(bfvm_ensemble_t& E, int flags)
+{
+    hipError_t st = hipSuccess;
+    if (flags & BFVM_BULK)
+        for (Env* pm : E.m)
+            if (pm->have_bulk && (st = on_stream(*pm, E.stream, [&] { return bfvm::launch_bulk(*pm); })))
+                return st;
+    if ((flags & BFVM_CELLS) && E.agent_threads > 0)
+        if ((st = bfvm::launch_agents_batch(E.precision, E.d_agents, E.n_agent_members,
+                                            E.agent_threads, E.stream)))
+            return st;
+    if (flags & BFVM_DIFFUSION) {
+        for (int axis = 0; axis < 3; ++axis) {
+            for (Env* pm : E.m)
+                if (pm->n_dir > 0 &&
+                    (st = on_stream(*pm, E.stream, [&] { return bfvm::launch_dirichlet(*pm); })))
+                    return st;
+            if (E.nbatch[axis] > 0 &&
+                (st = bfvm::launch_sweep_batch(E.precision, E.L, E.d_args[axis], E.d_first[axis],
+                                               E.nbatch[axis], E.blocks[axis], E.lds[axis], E.stream)))
+                return st;
+            for (int i : E.solo[axis])
+                if ((st = on_stream(*E.m[i], E.stream, [&] { return bfvm::launch_sweep(*E.m[i], axis); })))
+                    return st;
+        }
+    }
+    return hipSuccess;
+}
+
+int ensure_graph(bfvm_ensemble_t& E, double dt, int flags)
+{
+    if (E.exec && E.graph_dt == dt && E.graph_flags == flags) return BFVM_OK;
+    drop_graph(E);
+    EHIP(&E, hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+    hipError_t st = enqueue(E, flags);
+    hipGraph_t g = nullptr;
+    hipError_t st2 = hipStreamEndCapture(E.stream, &g);
+    if (st != hipSuccess || st2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return efail(&E, BFVM_E_DEVICE, std::string("graph capture: ") +
+                                            hipGetErrorString(st != hipSuccess ? st : st2));
+    }
+    E.graph = g;
+    EHIP(&E, hipGraphInstantiate(&E.exec, g, nullptr, nullptr, 0));
+    E.graph_dt = dt;
+    E.graph_flags = flags;
+    return BFVM_OK;
+}
+
+// Everything before the launches of `steps` steps: members' own work
+// drained, coefficients and tables for dt, the graph if asked for.
+int ready(bfvm_ensemble_t& E, double dt, int flags, int use_graph)
+{
+    for (Env* pm : E.m) EHIP(&E, hipStreamSynchronize(pm->stream));
+    int rc = prepare(E, dt, flags);
+    if (rc == BFVM_OK && use_graph) rc = ensure_graph(E, dt, flags);
+    return rc;
+}
+
+int launch_steps(bfvm_ensemble_t& E, int steps, int flags, int use_graph)
+{
+    for (int n = 0; n < steps; ++n) {
+        if (use_graph) EHIP(&E, hipGraphLaunch(E.exec, E.stream));
+        else EHIP(&E, enqueue(E, flags));
+    }
+    return BFVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bfvm_ensemble_create(int count, const bfvm_handle* members, bfvm_ensemble* out)
+{
+    if (!out) return BFVM_E_ARG;
+    *out = nullptr;
+    if (count < 1 || !members) return BFVM_E_ARG;
+    for (int i = 0; i < count; ++i) {
+        if (!members[i] || members[i]->ensemble_stream) return BFVM_E_ARG;  // one ensemble per handle
+        for (int j = 0; j < i; ++j)
+            if (members[j] == members[i]) return BFVM_E_ARG;
+    }
+    const int prec = members[0]->precision;
+    for (int i = 1; i < count; ++i)
+        if (members[i]->precision != prec) return BFVM_E_ARG;
+    auto* E = new bfvm_ensemble_t;
+    E->precision = prec;
+    for (int i = 0; i < count; ++i) E->m.push_back(members[i]);
+    E->coef_epoch.assign(count, ~0u);
+    E->agent_epoch.assign(count, ~0u);
+    E->source_epoch.assign(count, ~0u);
+    if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete E;
+        return BFVM_E_DEVICE;
+    }
+    for (Env* pm : E->m) pm->ensemble_stream = E->stream;
+    *out = E;
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_destroy(bfvm_ensemble e)
+{
+    if (!e) return BFVM_OK;
+    (void)hipStreamSynchronize(e->stream);
+    for (Env* pm : e->m) pm->ensemble_stream = nullptr;
+    drop_graph(*e);
+    free_tables(*e);
+    (void)hipStreamDestroy(e->stream);
+    delete e;
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph)
+{
+    if (!e) return BFVM_E_ARG;
+    if (steps < 0) return efail(e, BFVM_E_ARG, "steps < 0");
+    int rc = ready(*e, dt, flags, use_graph);
+    return rc ? rc : launch_steps(*e, steps, flags, use_graph);
+}
+
+int bfvm_ensemble_sync(bfvm_ensemble e)
+{
+    if (!e) return BFVM_E_ARG;
+    EHIP(e, hipStreamSynchronize(e->stream));
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph,
+                            double* ms)
+{
+    if (!e) return BFVM_E_ARG;
+    if (steps < 0) return efail(e, BFVM_E_ARG, "steps < 0");
+    int rc = ready(*e, dt, flags, use_graph);
+    if (rc) return rc;
+    hipEvent_t t0, t1;
+    EHIP(e, hipEventCreate(&t0));
+    EHIP(e, hipEventCreate(&t1));
+    EHIP(e, hipStreamSynchronize(e->stream));
+    EHIP(e, hipEventRecord(t0, e->stream));
+    rc = launch_steps(*e, steps, flags, use_graph);
+    EHIP(e, hipEventRecord(t1, e->stream));
+    EHIP(e, hipEventSynchronize(t1));
+    float f = 0;
+    EHIP(e, hipEventElapsedTime(&f, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (ms) *ms = f;
+    return rc;
+}
+
+int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
+{
+    if (!e) return BFVM_E_ARG;
+    // time the three batched sweeps for each segment length on scratch copies
+    for (Env* pm : e->m) EHIP(e, hipStreamSynchronize(pm->stream));
+    EHIP(e, hipStreamSynchronize(e->stream));
+    std::vector<void*> keep, scratch(e->m.size(), nullptr);
+    hipError_t st = hipSuccess;
+    for (size_t i = 0; i < e->m.size() && st == hipSuccess; ++i) {
+        Env& m = *e->m[i];
+        st = hipMalloc(&scratch[i], m.nvs * m.elem);
+        if (st == hipSuccess)
+            st = hipMemcpyAsync(scratch[i], m.rho, m.nvs * m.elem, hipMemcpyDeviceToDevice, e->stream);
+    }
+    if (st != hipSuccess) {
+        (void)hipStreamSynchronize(e->stream);
+        for (void* p : scratch)
+            if (p) (void)hipFree(p);
+        return efail(e, BFVM_E_DEVICE, std::string("autotune scratch: ") + hipGetErrorString(st));
+    }
+    for (size_t i = 0; i < e->m.size(); ++i) {
+        keep.push_back(e->m[i]->rho);
+        e->m[i]->rho = scratch[i];
+        ++e->m[i]->coef_epoch;  // launch tables point at the scratch stores
+    }
+    int best_L = e->L;
+    double best = 1e300;
+    int rc = BFVM_OK;
+    for (int L : {4, 8, 16, 32}) {
+        e->L = L;
+        double ms = 0;
+        if ((rc = bfvm_ensemble_timed_run(e, dt, 1, BFVM_DIFFUSION, 0, &ms))) break;
+        if ((rc = bfvm_ensemble_timed_run(e, dt, 3, BFVM_DIFFUSION, 0, &ms))) break;
+        if (ms < best) {
+            best = ms;
+            best_L = L;
+        }
+    }
+    (void)hipStreamSynchronize(e->stream);
+    for (size_t i = 0; i < e->m.size(); ++i) {
+        e->m[i]->rho = keep[i];
+        ++e->m[i]->coef_epoch;
+        (void)hipFree(scratch[i]);
+    }
+    e->L = best_L;
+    e->tables_valid = false;
+    drop_graph(*e);
+    return rc;
+}
+
+int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment)
+{
+    if (!e || !nodes_per_segment) return BFVM_E_ARG;
+    *nodes_per_segment = e->L;
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment)
+{
+    if (!e) return BFVM_E_ARG;
+    if (nodes_per_segment != 4 && nodes_per_segment != 8 && nodes_per_segment != 16 &&
+        nodes_per_segment != 32)
+        return efail(e, BFVM_E_ARG, "nodes per segment must be 4, 8, 16 or 32");
+    e->L = nodes_per_segment;
+    e->tables_valid = false;
+    drop_graph(*e);
+    return BFVM_OK;
+}
+
+const char* bfvm_ensemble_last_error(bfvm_ensemble e)
+{
+    return e ? e->err.c_str() : "null ensemble";
+}
+
+}  // extern "C"

@@ -66,30 +66,29 @@ __device__ __forceinline__ void buf_store(double*, __amdgpu_buffer_rsrc_t r, dou
 }
 
 // Carry chain of one column across its nseg segments, run by one thread:
-// slot u holds (alpha_u, beta_u) on entry and the carry into segment u on
-// exit, for C' = alpha_u + beta_u * C walked from segment 0 up (forward) or
-// from the top down (backward).  Returns the final C.
-template <int COLS>
-__device__ __forceinline__ double chain(double* xa, const double* xb, int lane, int nseg,
+// slot u*C + lane holds (alpha_u, beta_u) on entry and the carry into
+// segment u on exit, for c' = alpha_u + beta_u * c walked from segment 0 up
+// (forward) or from the top down (backward).  Returns the final carry.
+__device__ __forceinline__ double chain(double* xa, const double* xb, int lane, int C, int nseg,
                                         bool forward)
 {
-    double C = 0.0;
+    double C0 = 0.0;
     if (forward) {
 #pragma unroll 4
         for (int u = 0; u < nseg; ++u) {
-            const double a = xa[u * COLS + lane], b = xb[u * COLS + lane];
-            xa[u * COLS + lane] = C;
-            C = fma(b, C, a);
+            const double a = xa[u * C + lane], b = xb[u * C + lane];
+            xa[u * C + lane] = C0;
+            C0 = fma(b, C0, a);
         }
     } else {
 #pragma unroll 4
         for (int u = nseg - 1; u >= 0; --u) {
-            const double a = xa[u * COLS + lane], b = xb[u * COLS + lane];
-            xa[u * COLS + lane] = C;
-            C = fma(b, C, a);
+            const double a = xa[u * C + lane], b = xb[u * C + lane];
+            xa[u * C + lane] = C0;
+            C0 = fma(b, C0, a);
         }
     }
-    return C;
+    return C0;
 }
 
 // One kernel for the three sweeps.  The lines of a sweep are "columns": a
@@ -114,43 +113,39 @@ __device__ __forceinline__ double chain(double* xa, const double* xb, int lane, 
 // correction.  Only the segment holding node n-1 can be cut short: its nodes
 // past n-1 are clamped reads, are never stored, and cannot reach node n-1
 // because b is 0 there.  No loop reads a table or needs a mask.
-template <typename T, int L, int COLS>
-__global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4)
-k_sweep(T* __restrict__ rho, size_t nvs, int n, int nseg, unsigned line_stride, int cols_per_plane,
-        unsigned vstride, size_t plane_stride, int S, const double* __restrict__ wb,
-        const double* __restrict__ g0, const double* __restrict__ g1, int kg0, int kg1)
+// Block-level body of the segmented sweep for one plane's column tile.
+// C = lines per block (a power of two), tid -> column tid % C, segment tid / C.
+template <typename T, int L>
+__device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, int plane,
+                                           double* xa, double* xb, double* wfirst, double* wlast,
+                                           double* gtab)
 {
-    __shared__ double xa[kMaxThreads];
-    __shared__ double xb[kMaxThreads];
-    __shared__ double wfirst[COLS];  // w at node 0 of each column
-    __shared__ double wlast[COLS];   // w at node n-1 of each column
-    // correction tables with zero padding so that no index needs a clamp:
-    //   G0 rows 0..kg0-1 = g0, then kPad zero rows
-    //   G1 kPad zero rows, rows kPad+j = g1 at node n-1-j (j < kg1), kPad zero rows
-    //   then wb (8 values per substrate)
-    extern __shared__ double gtab[];
+    const int n = A.n, nseg = A.nseg, S = A.S, kg0 = A.kg0, kg1 = A.kg1;
+    const unsigned line_stride = A.line_stride;
     const int tid = threadIdx.x;
-    const int lane = tid % COLS;
-    const int g = tid / COLS;
-    const int col = blockIdx.x * COLS + lane;
-    const bool act = col < cols_per_plane && g < nseg;
+    const int lane = tid & (C - 1);
+    const int g = tid / C;
+    const int col = tile * C + lane;
+    const bool act = col < A.cols_per_plane && g < nseg;
     const int cc = act ? col : 0;
     const int s = cc % S;
-    const size_t plane0 = (size_t)blockIdx.y * plane_stride;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(rho + plane0, (nvs - plane0) * sizeof(T));
+    const size_t plane0 = (size_t)plane * A.plane_stride;
+    T* rho = (T*)A.rho;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(rho + plane0, (A.nvs - plane0) * sizeof(T));
     const int p0 = act ? g * L : 0;
     const int cnt = act ? min(L, n - p0) : 0;
     const bool has_last = p0 + L >= n;  // holds node n-1 (or nodes past it)
+    const double* wb = A.wb;
     const double r = wb[s * 8 + 0], a = wb[s * 8 + 1];
     // byte offset of node p0 of this lane's line; node p0+t adds t*step
-    const unsigned vo = ((unsigned)(cc / S) * vstride + (unsigned)s + (unsigned)p0 * line_stride) *
+    const unsigned vo = ((unsigned)(cc / S) * A.vstride + (unsigned)s + (unsigned)p0 * line_stride) *
                         (unsigned)sizeof(T);
     const unsigned step = line_stride * (unsigned)sizeof(T);
 
     // The last segment clamps its reads to node n-1 (every address stays in
     // the store); its nodes past n-1 are never stored and cannot reach node
-    // n-1 because b is 0 there.
-    // raw store values, widened to FP64 one by one in the forward pass
+    // n-1 because b is 0 there.  Raw store values are widened to FP64 one by
+    // one in the forward pass.
     T raw[L];
     if (!has_last) {
 #pragma unroll
@@ -163,17 +158,20 @@ k_sweep(T* __restrict__ rho, size_t nvs, int n, int nseg, unsigned line_stride, 
     }
 
     // the correction tables, staged in LDS while the loads are in flight
-    // (read after the first barrier below)
+    // (read after the first barrier below), zero padded so no index clamps:
+    //   G0 rows 0..kg0-1 = g0, then kPad zero rows
+    //   G1 kPad zero rows, rows kPad+j = g1 at node n-1-j (j < kg1), kPad zero rows
+    //   K  = wb (8 values per substrate)
     double* G0 = gtab;
     double* G1 = gtab + (kg0 + kPad) * S;
     double* K = G1 + (kg1 + 2 * kPad) * S;
     const int ng = (kg0 + kg1 + 3 * kPad) * S;
     for (int i = tid; i < ng + 8 * S; i += blockDim.x) {
-        const int row = i / S, col = i - row * S;
+        const int row = i / S, cl = i - row * S;
         double val = 0.0;
-        if (row < kg0) val = g0[i];
+        if (row < kg0) val = A.g0[i];
         else if (row >= kg0 + 2 * kPad && row < kg0 + kg1 + 2 * kPad)
-            val = g1[(row - kg0 - 2 * kPad) * S + col];
+            val = A.g1[(row - kg0 - 2 * kPad) * S + cl];
         else if (i >= ng) val = wb[i - ng];
         gtab[i] = val;
     }
@@ -189,7 +187,7 @@ k_sweep(T* __restrict__ rho, size_t nvs, int n, int nseg, unsigned line_stride, 
     xa[tid] = y;
     xb[tid] = aL;
     __syncthreads();
-    if (tid < COLS) chain<COLS>(xa, xb, lane, nseg, true);
+    if (tid < C) chain(xa, xb, lane, C, nseg, true);
     __syncthreads();
     {
         double PY = xa[tid];
@@ -221,7 +219,7 @@ k_sweep(T* __restrict__ rho, size_t nvs, int n, int nseg, unsigned line_stride, 
     xa[tid] = z;  // each thread overwrites only the slot it alone has read
     xb[tid] = bL;
     __syncthreads();
-    if (tid < COLS) wfirst[lane] = chain<COLS>(xa, xb, lane, nseg, false);
+    if (tid < C) wfirst[lane] = chain(xa, xb, lane, C, nseg, false);
     __syncthreads();
 
     // Last pass, from the top of the segment down, each node finished and
@@ -244,11 +242,47 @@ k_sweep(T* __restrict__ rho, size_t nvs, int n, int nseg, unsigned line_stride, 
 #pragma unroll
     for (int t = L - 1; t >= 0; --t) {
         QZ *= a;
-        double z = v[t] + QZ;
-        if (head) z = fma(mu0, g0r[t * S], z);
-        if (tail) z = fma(mu1, g1r[-t * S], z);
-        if (store_all || (act && t < cnt)) buf_store((T*)nullptr, rs, z, vo, t * step);
+        double zf = v[t] + QZ;
+        if (head) zf = fma(mu0, g0r[t * S], zf);
+        if (tail) zf = fma(mu1, g1r[-t * S], zf);
+        if (store_all || (act && t < cnt)) buf_store((T*)nullptr, rs, zf, vo, t * step);
     }
+}
+
+// One grid: block (x, y) = (column tile, plane), COLS lines per block.
+template <typename T, int L, int COLS>
+__global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4) k_sweep(SweepArgs A)
+{
+    __shared__ double xa[kMaxThreads];
+    __shared__ double xb[kMaxThreads];
+    __shared__ double wfirst[COLS];  // w at node 0 of each column
+    __shared__ double wlast[COLS];   // w at node n-1 of each column
+    extern __shared__ double gtab[];
+    sweep_tile<T, L>(A, COLS, blockIdx.x, blockIdx.y, xa, xb, wfirst, wlast, gtab);
+}
+
+// An ensemble: member m owns blocks [first[m], first[m+1]) of a 1-D grid,
+// laid out as planes x column tiles of A[m].cols lines; every block has
+// kMaxThreads threads (A[m].cols * A[m].nseg of them active).
+template <typename T, int L>
+__global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4)
+k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ first, int nm)
+{
+    __shared__ double xa[kMaxThreads];
+    __shared__ double xb[kMaxThreads];
+    __shared__ double wfirst[kMaxThreads];
+    __shared__ double wlast[kMaxThreads];
+    extern __shared__ double gtab[];
+    const int b = blockIdx.x;
+    int lo = 0, hi = nm;  // first[lo] <= b < first[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const SweepArgs A = args[lo];
+    const int local = b - first[lo];
+    sweep_tile<T, L>(A, A.cols, local % A.tiles, local / A.tiles, xa, xb, wfirst, wlast, gtab);
 }
 
 // One thread owns one whole (line, substrate) pair and runs the reference's
@@ -347,17 +381,44 @@ hipError_t sweep_naive(Env& e, int axis)
     return hipGetLastError();
 }
 
+// Kernel arguments of one axis of one grid, `cols` lines per block.
+SweepArgs sweep_args(const Env& e, int axis, int cols, int nseg)
+{
+    Geometry gm = geometry(e, axis);
+    const AxisCoef& a = e.ax[axis];
+    SweepArgs A;
+    A.rho = e.rho;
+    A.nvs = e.nvs;
+    A.plane_stride = gm.plane_stride;
+    A.wb = (const double*)a.wb;
+    A.g0 = (const double*)a.g0;
+    A.g1 = (const double*)a.g1;
+    A.n = gm.n;
+    A.nseg = nseg;
+    A.cols_per_plane = gm.ncols;
+    A.S = e.S;
+    A.kg0 = a.kg0;
+    A.kg1 = a.kg1;
+    A.line_stride = (unsigned)gm.line_stride;
+    A.vstride = (unsigned)gm.vstride;
+    A.cols = cols;
+    A.tiles = (gm.ncols + cols - 1) / cols;
+    A.planes = gm.nplanes;
+    return A;
+}
+
+size_t sweep_lds(const SweepArgs& A)
+{
+    return sizeof(double) * (size_t)(A.kg0 + A.kg1 + 3 * kPad + 8) * A.S;
+}
+
 template <typename T, int L, int COLS>
 hipError_t launch_seg(Env& e, int axis, const Geometry& gm, int nseg)
 {
-    const AxisCoef& a = e.ax[axis];
-    dim3 grid((gm.ncols + COLS - 1) / COLS, gm.nplanes);
+    const SweepArgs A = sweep_args(e, axis, COLS, nseg);
+    dim3 grid(A.tiles, gm.nplanes);
     int threads = (COLS * nseg + 63) / 64 * 64;
-    const size_t lds = sizeof(double) * (size_t)(a.kg0 + a.kg1 + 3 * kPad + 8) * e.S;
-    k_sweep<T, L, COLS><<<grid, threads, lds, e.stream>>>(
-        (T*)e.rho, e.nvs, gm.n, nseg, (unsigned)gm.line_stride, gm.ncols, (unsigned)gm.vstride,
-        gm.plane_stride, e.S, (const double*)a.wb, (const double*)a.g0, (const double*)a.g1, a.kg0,
-        a.kg1);
+    k_sweep<T, L, COLS><<<grid, threads, sweep_lds(A), e.stream>>>(A);
     return hipGetLastError();
 }
 
@@ -410,6 +471,65 @@ std::vector<SweepPlan> sweep_candidates(int n)
         for (int L : kNodes)
             if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n)) out.push_back(SweepPlan{cols, L});
     return out;
+}
+
+// Ensemble launch of one axis (see k_sweep_batch).  A member can join when
+// its lines fit 16 segments of L nodes and its tables fit the kernel.
+namespace {
+template <typename T>
+bool batch_member_ok_t(const Env& e, int axis, int L)
+{
+    Geometry gm = geometry(e, axis);
+    const size_t span = (axis == 2 ? e.nvs : (size_t)e.nx * e.ny * e.S) * sizeof(T);
+    const size_t gbytes =
+        sizeof(double) * (size_t)(e.ax[axis].kg0 + e.ax[axis].kg1 + 3 * kPad + 8) * e.S;
+    const int nseg = (gm.n + L - 1) / L;
+    return gm.n >= 2 && nseg <= kMaxThreads / 64 && span < 0xffffffffull && gbytes <= kMaxCorrectionLds;
+}
+
+template <typename T, int L>
+hipError_t launch_batch_t(const SweepArgs* d_args, const int* d_first, int nm, int blocks,
+                          size_t lds, hipStream_t st)
+{
+    k_sweep_batch<T, L><<<blocks, kMaxThreads, lds, st>>>(d_args, d_first, nm);
+    return hipGetLastError();
+}
+}  // namespace
+
+bool batch_member_ok(const Env& e, int axis, int L)
+{
+    return e.precision == BFVM_FP64 ? batch_member_ok_t<double>(e, axis, L)
+                                    : batch_member_ok_t<float>(e, axis, L);
+}
+
+SweepArgs batch_args(const Env& e, int axis, int L)
+{
+    Geometry gm = geometry(e, axis);
+    const int nseg = (gm.n + L - 1) / L;
+    int cols = 64;  // largest power of two with cols*nseg <= 1024, no wider than the plane needs
+    while (cols * 2 * nseg <= kMaxThreads && cols < gm.ncols) cols *= 2;
+    return sweep_args(e, axis, cols, nseg);
+}
+
+size_t batch_lds(const SweepArgs& A) { return sweep_lds(A); }
+
+hipError_t launch_sweep_batch(int precision, int L, const SweepArgs* d_args, const int* d_first,
+                              int nm, int blocks, size_t lds, hipStream_t st)
+{
+    if (precision == BFVM_FP64) {
+        switch (L) {
+        case 4: return launch_batch_t<double, 4>(d_args, d_first, nm, blocks, lds, st);
+        case 8: return launch_batch_t<double, 8>(d_args, d_first, nm, blocks, lds, st);
+        case 16: return launch_batch_t<double, 16>(d_args, d_first, nm, blocks, lds, st);
+        default: return launch_batch_t<double, 32>(d_args, d_first, nm, blocks, lds, st);
+        }
+    }
+    switch (L) {
+    case 4: return launch_batch_t<float, 4>(d_args, d_first, nm, blocks, lds, st);
+    case 8: return launch_batch_t<float, 8>(d_args, d_first, nm, blocks, lds, st);
+    case 16: return launch_batch_t<float, 16>(d_args, d_first, nm, blocks, lds, st);
+    default: return launch_batch_t<float, 32>(d_args, d_first, nm, blocks, lds, st);
+    }
 }
 
 hipError_t launch_sweep(Env& e, int axis)

@@ -82,7 +82,47 @@ __global__ void k_agents(T* __restrict__ rho, int S, int n_groups, const int* __
     *r = (T)v;
 }
 
+// k_agents for several grids: thread t belongs to the member whose range
+// [first, next first) holds it (binary search over the member table).
+template <typename T>
+__global__ void k_agents_batch(const AgentBatch* __restrict__ tab, int nm, long long threads)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= threads) return;
+    int lo = 0, hi = nm;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (tab[mid].first <= t) lo = mid;
+        else hi = mid;
+    }
+    const AgentBatch& m = tab[lo];
+    const long long local = t - m.first;
+    const int S = m.S;
+    const int g = (int)(local / S);
+    const int s = (int)(local - (long long)g * S);
+    T* r = (T*)m.rho + (size_t)m.gvox[g] * S + s;
+    double v = (double)*r;
+    const int a1 = m.gstart[g + 1];
+    for (int a = m.gstart[g]; a < a1; ++a) {
+        const size_t q = (size_t)a * S + s;
+        v = (v + m.t1[q]) / m.t2[q];
+    }
+    *r = (T)v;
+}
+
 }  // namespace
+
+hipError_t launch_agents_batch(int precision, const AgentBatch* d_tab, int nm, long long threads,
+                               hipStream_t st)
+{
+    if (threads == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((threads + kBlock - 1) / kBlock);
+    if (precision == BFVM_FP64)
+        k_agents_batch<double><<<grid, kBlock, 0, st>>>(d_tab, nm, threads);
+    else
+        k_agents_batch<float><<<grid, kBlock, 0, st>>>(d_tab, nm, threads);
+    return hipGetLastError();
+}
 
 hipError_t launch_dirichlet(Env& e)
 {

@@ -1,14 +1,16 @@
-"""Python handle over the C ABI: one device-resident microenvironment.
+"""Python handles over the C ABI.
 
-``DeviceMicroenvironment`` owns a ``bfvm_handle``; ``run_case`` drives one
-synthetic case (see ``biofvm_amd.workloads``) end to end through the C ABI.
+``DeviceMicroenvironment`` owns a ``bfvm_handle`` (one device-resident
+microenvironment); ``Ensemble`` advances several of them together with
+batched launches; ``run_case`` drives one synthetic case (see
+``biofvm_amd.workloads``) end to end through the C ABI.
 """
 import ctypes
 
 import numpy as np
 
 from . import _capi
-from ._capi import ALL, BULK, CELLS, DIFFUSION, FP32, FP64, check
+from ._capi import ALL, BULK, CELLS, DIFFUSION, FP32, FP64, check, check_ensemble
 
 
 def _dptr(a):
@@ -168,3 +170,66 @@ def run_case(case, precision=FP64):
     with setup_case(case, precision) as m:
         m.run(case["dt"], case["steps"], case["phases"])
         return m.download()
+
+
+class Ensemble:
+    """Several DeviceMicroenvironments (one precision) advanced together: one
+    batched launch per phase and sweep axis, optionally replayed from a HIP
+    graph.  The members stay usable; their entry points wait for the
+    ensemble's work.  Close the ensemble before closing its members."""
+
+    def __init__(self, members):
+        self.lib = _capi.load()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        self.e = ctypes.c_void_p()
+        rc = self.lib.bfvm_ensemble_create(len(self.members), arr, ctypes.byref(self.e))
+        if rc:
+            raise _capi.BfvmError(rc, "bfvm_ensemble_create: members must be distinct live handles "
+                                      "of one precision, in no other ensemble")
+
+    def close(self):
+        if self.e:
+            self.lib.bfvm_ensemble_destroy(self.e)
+            self.e = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        check_ensemble(rc, self.e)
+
+    def run(self, dt, steps, flags=ALL, graph=True):
+        self._check(self.lib.bfvm_ensemble_run(self.e, float(dt), int(steps), int(flags), int(graph)))
+
+    def sync(self):
+        self._check(self.lib.bfvm_ensemble_sync(self.e))
+
+    def timed_run(self, dt, steps, flags=ALL, graph=True):
+        ms = ctypes.c_double(0)
+        self._check(self.lib.bfvm_ensemble_timed_run(self.e, float(dt), int(steps), int(flags),
+                                                     int(graph), ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, dt):
+        self._check(self.lib.bfvm_ensemble_autotune(self.e, float(dt)))
+        return self.segment
+
+    @property
+    def segment(self):
+        v = ctypes.c_int(0)
+        self._check(self.lib.bfvm_ensemble_get_segment(self.e, ctypes.byref(v)))
+        return v.value
+
+    @segment.setter
+    def segment(self, nodes):
+        self._check(self.lib.bfvm_ensemble_set_segment(self.e, int(nodes)))

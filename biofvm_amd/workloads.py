@@ -11,6 +11,8 @@ The named workloads follow the five benchmark configurations of this path:
 * ``lod_50``       50^3 x 1 substrate, dx=20, D=1e5, lambda=0.1, dt=0.01, LOD only, 100 steps
 * ``drop_in_200``  200^3 x 4, dx=10, Dirichlet faces, bulk fields, 1e5 agents, 200 steps
 * ``precision_400`` 400^3 x 8, dx=10, log-uniform D/lambda, 1e6 agents (Gaussian cluster), 1000 steps
+* ``ensemble_64``  64 independent grids, edges from {32,48,64,96,128} per axis, 1-6 substrates,
+                   log-uniform D/lambda, 0-2e4 uniform agents, 500 steps
 """
 import numpy as np
 
@@ -106,3 +108,51 @@ def precision_400(steps=1000, seed=3, n=400, S=8, n_agents=1_000_000):
     case = make_case(n, n, n, S, 10.0, D, lam, 0.01, steps, CELLS | DIFFUSION, rho0)
     case.update(random_agents(rng, n_agents, S, n, n, n, 10.0, cluster_sigma=600.0))
     return case
+
+
+def ensemble_64(steps=500, seed=4, members=64, edges=(32, 48, 64, 96, 128), max_agents=20_000,
+                max_substrates=6):
+    """A list of `members` independent cases: each axis length drawn from
+    `edges` (non-cubic allowed), 1..max_substrates substrates, D and lambda
+    log-uniform as in ``precision_400``, 0..max_agents uniform agents; dx=10,
+    dt=0.01, agents + LOD each step."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(members):
+        nx, ny, nz = (int(v) for v in rng.choice(edges, size=3))
+        S = int(rng.integers(1, max_substrates + 1))
+        D = 10.0 ** rng.uniform(2.0, 5.0, S)
+        lam = 10.0 ** rng.uniform(-3.0, 0.0, S)
+        n_agents = int(rng.integers(0, max_agents + 1))
+        rho0 = rng.uniform(0.0, 1.0, nx * ny * nz * S)
+        case = make_case(nx, ny, nz, S, 10.0, D, lam, 0.01, steps, CELLS | DIFFUSION, rho0)
+        case.update(random_agents(rng, n_agents, S, nx, ny, nz, 10.0))
+        out.append(case)
+    return out
+
+
+def ensemble_mixed(seed=5, steps=3):
+    """A small uneven ensemble for parity tests: members with Dirichlet faces
+    (some masked), bulk fields, agents (some sharing a voxel), single-node and
+    long axes, and 1-5 substrates; all phases, `steps` steps."""
+    rng = np.random.default_rng(seed)
+    shapes = [(12, 9, 7, 3), (1, 16, 5, 2), (33, 1, 1, 1), (20, 14, 40, 5), (7, 30, 2, 4),
+              (64, 24, 8, 1), (9, 9, 9, 2)]
+    out = []
+    for m, (nx, ny, nz, S) in enumerate(shapes):
+        D = 10.0 ** rng.uniform(2.0, 5.0, S)
+        lam = 10.0 ** rng.uniform(-3.0, 0.0, S)
+        rho0 = rng.uniform(0.0, 1.0, nx * ny * nz * S)
+        case = make_case(nx, ny, nz, S, 20.0, D, lam, 0.01, steps, ALL, rho0)
+        if m % 2 == 0:
+            case.update(random_bulk(rng, nx * ny * nz, S))
+        if m % 3 != 2:
+            case.update(random_agents(rng, 40 + 10 * m, S, nx, ny, nz, 20.0, share_voxel=3 * (m % 2)))
+        if m in (0, 3, 5):
+            idx = outer_face_voxels(nx, ny, nz)
+            val = rng.uniform(0.0, 1.0, (idx.size, S))
+            case.update(dir_idx=idx, dir_val=val.reshape(-1))
+            if m == 3:
+                case["dir_mask"] = (np.arange(S) % 2 == 0).astype(np.uint8)
+        out.append(case)
+    return out

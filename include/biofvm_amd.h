@@ -189,6 +189,44 @@ int bfvm_density_device_ptr(bfvm_handle h, void** ptr);
 /* Message for the last failure on `h` (or of the calling thread when h is NULL). */
 const char* bfvm_last_error(bfvm_handle h);
 
+/*
+ * Ensembles.  The reference advances one Microenvironment per call of
+ * simulate_bulk_sources_and_sinks / simulate_cell_sources_and_sinks /
+ * simulate_diffusion_decay (BioFVM_microenvironment.cpp:502-633); an
+ * ensemble advances `count` independent handles (uneven grids and substrate
+ * counts, one store precision) with one batched launch per phase and sweep
+ * axis.  The members keep their own data and entry points; while an
+ * ensemble runs, do not enqueue work on a member's own handle without
+ * bfvm_ensemble_sync first.  Each member's results equal those of
+ * bfvm_step on that member alone.
+ */
+typedef struct bfvm_ensemble_t* bfvm_ensemble;
+
+/* Builds an ensemble over `count` existing handles (all of one precision). */
+int bfvm_ensemble_create(int count, const bfvm_handle* members, bfvm_ensemble* out);
+int bfvm_ensemble_destroy(bfvm_ensemble e);
+
+/* `steps` steps of every member (phases in `flags`, as bfvm_step).  With
+ * use_graph != 0 one step is captured into a HIP graph (once per dt and
+ * flags) and replayed `steps` times.  Asynchronous. */
+int bfvm_ensemble_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph);
+
+/* Waits for all work enqueued on the ensemble. */
+int bfvm_ensemble_sync(bfvm_ensemble e);
+
+/* bfvm_ensemble_run timed with HIP events on the ensemble's stream
+ * (preparation of coefficients and launch tables is outside the timing). */
+int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph,
+                            double* ms_total);
+
+/* Times the batched sweeps for each segment length (4, 8, 16, 32 nodes) on
+ * scratch copies of the stores and keeps the fastest.  Densities unchanged. */
+int bfvm_ensemble_autotune(bfvm_ensemble e, double dt);
+int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment);
+int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment);
+
+const char* bfvm_ensemble_last_error(bfvm_ensemble e);
+
 #ifdef __cplusplus
 }
 #endif
