@@ -273,9 +273,10 @@ def main():
     achieved = dom_bytes / (dom_avg * 1e-3) / 1e9
     traffic = None
     if dom.startswith("sweep"):
-        cols, L = plans["xyz".index(dom[-1])]
+        cols, L, streaming = plans["xyz".index(dom[-1])]
+        kname = "k_sweep_stream" if streaming else "k_sweep"
         traffic = measured_traffic(args.workload,
-                                   f"k_sweep<{'float' if main_prec == FP32 else 'double'}, {L}, {cols}>")
+                                   f"{kname}<{'float' if main_prec == FP32 else 'double'}, {L}, {cols}>")
     step_bytes = algorithmic_bytes(case, w_main)
 
     out_extra = {}
@@ -321,7 +322,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg},
         "kernels_ms_per_launch": {k: (v[0] / v[1] if v[1] else None) for k, v in per.items()},
-        "sweep_plans_lines_x_nodes": {"x": plans[0], "y": plans[1], "z": plans[2]},
+        "sweep_plans_lines_nodes_streaming": {"x": plans[0], "y": plans[1], "z": plans[2]},
     }
     line.update(out_extra)
     if not args.no_cpu_baseline:

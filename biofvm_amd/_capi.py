@@ -40,8 +40,8 @@ SIGNATURES = {
     "bfvm_sync": (_i, [_h]),
     "bfvm_timed_run": (_i, [_h, _d, _i, _i, _dp, _dp, _ip]),
     "bfvm_autotune": (_i, [_h, _d]),
-    "bfvm_get_sweep_plan": (_i, [_h, _i, _ip, _ip]),
-    "bfvm_set_sweep_plan": (_i, [_h, _i, _i, _i]),
+    "bfvm_get_sweep_plan": (_i, [_h, _i, _ip, _ip, _ip]),
+    "bfvm_set_sweep_plan": (_i, [_h, _i, _i, _i, _i]),
     "bfvm_get_info": (_i, [_h, _ip, _ip, _ip, _ip, _ip, _sp]),
     "bfvm_density_device_ptr": (_i, [_h, ctypes.POINTER(_vp)]),
     "bfvm_last_error": (ctypes.c_char_p, [_h]),

@@ -56,8 +56,9 @@ struct SweepArgs {
 
 // Launch shape of the segmented sweep of one axis (see lod.hip).
 struct SweepPlan {
-    int cols = 0;  // lines per block (16, 32 or 64); 0 = default
-    int L = 0;     // nodes per segment
+    int cols = 0;    // lines per block (16, 32 or 64); 0 = default
+    int L = 0;       // nodes per segment
+    int stream = 0;  // 1: resident blocks that load the next tile while solving one
 };
 
 struct Env {
@@ -70,6 +71,7 @@ struct Env {
     hipStream_t stream = nullptr;
     hipStream_t ensemble_stream = nullptr;  // stream of the ensemble holding this handle, if any
     int device = 0;
+    int cus = 0;  // compute units of the device
 
     void* rho = nullptr;  // density store, nvs values
 
@@ -154,8 +156,9 @@ int ensure_coefficients(Env& e, double dt);
 // Bulk implicit terms for dt (rebuilt when dt changed).
 hipError_t ensure_bulk(Env& e, double dt);
 
-// Valid segmented shapes for a line of n nodes, for the autotuner.
-std::vector<SweepPlan> sweep_candidates(int n);
+// Valid segmented shapes for a line of n nodes and a store precision, for
+// the autotuner.
+std::vector<SweepPlan> sweep_candidates(int n, int precision);
 // The launch shape a sweep of `axis` uses now (autotuned or default).
 SweepPlan effective_plan(const Env& e, int axis);
 

@@ -694,7 +694,7 @@ int bfvm_autotune(bfvm_handle h, double dt)
     for (int axis = 0; axis < 3 && st == hipSuccess; ++axis) {
         bfvm::SweepPlan best{};
         float best_ms = 1e30f;
-        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(n_axis[axis])) {
+        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(n_axis[axis], e.precision)) {
             e.plan[axis] = cand;
             if ((st = bfvm::launch_sweep(e, axis)) != hipSuccess) break;  // warm-up
             (void)hipEventRecord(t0, e.stream);
@@ -722,17 +722,20 @@ int bfvm_autotune(bfvm_handle h, double dt)
     return BFVM_OK;
 }
 
-int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment)
+int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment,
+                        int* streaming)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
     bfvm::SweepPlan p = bfvm::effective_plan(*h, axis);
     if (lines_per_block) *lines_per_block = p.cols;
     if (nodes_per_segment) *nodes_per_segment = p.L;
+    if (streaming) *streaming = p.stream;
     return BFVM_OK;
 }
 
-int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment)
+int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment,
+                        int streaming)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
@@ -741,8 +744,8 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
         return BFVM_OK;
     }
     const int n = axis == 0 ? h->nx : axis == 1 ? h->ny : h->nz;
-    for (const bfvm::SweepPlan& c : bfvm::sweep_candidates(n))
-        if (c.cols == lines_per_block && c.L == nodes_per_segment) {
+    for (const bfvm::SweepPlan& c : bfvm::sweep_candidates(n, h->precision))
+        if (c.cols == lines_per_block && c.L == nodes_per_segment && c.stream == (streaming ? 1 : 0)) {
             h->plan[axis] = c;
             return BFVM_OK;
         }

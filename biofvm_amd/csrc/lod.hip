@@ -16,12 +16,18 @@
 // elimination / back substitution (:186-201) up to rounding.
 //
 // Kernels
-//   k_sweep       all three sweeps: 16-64 lines per block, up to 1024/lines
-//                 segments of 4-32 nodes per line; see its comment.
-//   k_line_sweep  one thread per whole line with the reference's own
-//                 recurrences, for lines of one node or longer than the
-//                 segmented kernel holds (n > 64*32).
+//   k_sweep        all three sweeps: 16-64 lines per block, up to 1024/lines
+//                  segments of 4-32 nodes per line, one block per column tile.
+//   k_sweep_stream the same solve in resident blocks that walk the tiles and
+//                  load the next tile while solving the current one.
+//   k_sweep_batch  one launch over the tiles of many grids (ensembles).
+//   k_line_sweep   one thread per whole line with the reference's own
+//                  recurrences, for lines of one node or longer than the
+//                  segmented kernels hold (n > 64*32).
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "bfvm_internal.h"
 
@@ -38,6 +44,10 @@ constexpr int kPad = 32;  // zero rows around the correction tables (>= the larg
 // and writes there are dropped.  The descriptor's last word is the raw
 // 32-bit-element format used for gfx9-class buffers.
 constexpr int kRsrcWord3 = 0x00020000;
+// Cache policy bits of the loads and stores of the store (gfx950: bit 1 = NT,
+// streaming: each value is read and written once per sweep).
+constexpr int kLoadPolicy = 0;
+constexpr int kStorePolicy = 0;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, size_t bytes)
 {
@@ -47,45 +57,56 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, si
 
 __device__ __forceinline__ float buf_load(float*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, kLoadPolicy));
 }
 __device__ __forceinline__ double buf_load(double*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, kLoadPolicy));
 }
 __device__ __forceinline__ void buf_store(float*, __amdgpu_buffer_rsrc_t r, double x, unsigned vo,
                                           unsigned so)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)x), r, vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)x), r, vo, so, kStorePolicy);
 }
 __device__ __forceinline__ void buf_store(double*, __amdgpu_buffer_rsrc_t r, double x, unsigned vo,
                                           unsigned so)
 {
     using u64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, x), r, vo, so, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, x), r, vo, so, kStorePolicy);
+}
+
+// Block barrier that orders LDS only: waits for this wave's LDS accesses,
+// not for its global loads, so loads issued before it (the next tile's
+// values) stay in flight across it.  Threads exchange no global data.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Carry chain of one column across its nseg segments, run by one thread:
-// slot u*C + lane holds (alpha_u, beta_u) on entry and the carry into
-// segment u on exit, for c' = alpha_u + beta_u * c walked from segment 0 up
-// (forward) or from the top down (backward).  Returns the final carry.
-__device__ __forceinline__ double chain(double* xa, const double* xb, int lane, int C, int nseg,
-                                        bool forward)
+// slot u*C + lane holds alpha_u on entry and the carry into segment u on
+// exit, for c' = alpha_u + aL * c walked from segment 0 up (forward) or from
+// the top down (backward); aL = a^L is the same for every segment (the
+// backward step out of the last segment multiplies a zero carry).  Returns
+// the final carry.
+__device__ __forceinline__ double chain(double* x, double aL, int lane, int C, int nseg, bool forward)
 {
     double C0 = 0.0;
     if (forward) {
 #pragma unroll 4
         for (int u = 0; u < nseg; ++u) {
-            const double a = xa[u * C + lane], b = xb[u * C + lane];
-            xa[u * C + lane] = C0;
-            C0 = fma(b, C0, a);
+            const double al = x[u * C + lane];
+            x[u * C + lane] = C0;
+            C0 = fma(aL, C0, al);
         }
     } else {
 #pragma unroll 4
         for (int u = nseg - 1; u >= 0; --u) {
-            const double a = xa[u * C + lane], b = xb[u * C + lane];
-            xa[u * C + lane] = C0;
-            C0 = fma(b, C0, a);
+            const double al = x[u * C + lane];
+            x[u * C + lane] = C0;
+            C0 = fma(aL, C0, al);
         }
     }
     return C0;
@@ -111,70 +132,120 @@ __device__ __forceinline__ double chain(double* xa, const double* xb, int lane, 
 // carry chain, and each segment adds its carry times the running product
 // of a.  Segments within kg0 / kg1 of the line ends then add the rank-2
 // correction.  Only the segment holding node n-1 can be cut short: its nodes
-// past n-1 are clamped reads, are never stored, and cannot reach node n-1
-// because b is 0 there.  No loop reads a table or needs a mask.
-// Block-level body of the segmented sweep for one plane's column tile.
+// past n-1 are clamped reads and cannot reach node n-1 because b is 0 there.
+// No loop reads a table.  Barriers order LDS only (lds_barrier).
+
+// Where one thread's segment of one (column tile, plane) lies.
 // C = lines per block (a power of two), tid -> column tid % C, segment tid / C.
+struct SegPos {
+    size_t plane0;              // first value of the plane (the buffer runs from there to the end)
+    unsigned vo;                // byte offset of node p0 of this thread's line
+    int p0, s;                  // first node, substrate
+    int slot;                   // LDS slot of the segment: g * C + lane
+    bool act;                   // owns a real (line, segment), not a copy of one
+    bool last_writer;           // records w at node n-1 for its lane
+};
+
+// Threads past the plane's last column work on that column, and threads past
+// the last segment on that segment: every thread computes a real segment, so
+// a copy stores exactly the values (same inputs, same instructions) that the
+// real owner stores.
 template <typename T, int L>
-__device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, int plane,
-                                           double* xa, double* xb, double* wfirst, double* wlast,
-                                           double* gtab)
+__device__ __forceinline__ SegPos seg_pos(const SweepArgs& A, int C, int tile, int plane)
 {
-    const int n = A.n, nseg = A.nseg, S = A.S, kg0 = A.kg0, kg1 = A.kg1;
-    const unsigned line_stride = A.line_stride;
     const int tid = threadIdx.x;
     const int lane = tid & (C - 1);
-    const int g = tid / C;
-    const int col = tile * C + lane;
-    const bool act = col < A.cols_per_plane && g < nseg;
-    const int cc = act ? col : 0;
-    const int s = cc % S;
-    const size_t plane0 = (size_t)plane * A.plane_stride;
-    T* rho = (T*)A.rho;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(rho + plane0, (A.nvs - plane0) * sizeof(T));
-    const int p0 = act ? g * L : 0;
-    const int cnt = act ? min(L, n - p0) : 0;
-    const bool has_last = p0 + L >= n;  // holds node n-1 (or nodes past it)
-    const double* wb = A.wb;
-    const double r = wb[s * 8 + 0], a = wb[s * 8 + 1];
-    // byte offset of node p0 of this lane's line; node p0+t adds t*step
-    const unsigned vo = ((unsigned)(cc / S) * A.vstride + (unsigned)s + (unsigned)p0 * line_stride) *
-                        (unsigned)sizeof(T);
-    const unsigned step = line_stride * (unsigned)sizeof(T);
+    const int graw = tid / C;
+    const int craw = tile * C + lane;
+    const int col = min(craw, A.cols_per_plane - 1);
+    const int g = min(graw, A.nseg - 1);
+    SegPos P;
+    P.act = craw < A.cols_per_plane && graw < A.nseg;
+    P.last_writer = graw == A.nseg - 1;
+    P.s = col % A.S;
+    P.p0 = g * L;
+    P.slot = g * C + lane;
+    // plane is the same in every lane; say so, so the buffer descriptor
+    // built from it stays in scalar registers
+    P.plane0 = (size_t)__builtin_amdgcn_readfirstlane(plane) * A.plane_stride;
+    P.vo = ((unsigned)(col / A.S) * A.vstride + (unsigned)P.s + (unsigned)P.p0 * A.line_stride) *
+           (unsigned)sizeof(T);
+    return P;
+}
 
-    // The last segment clamps its reads to node n-1 (every address stays in
-    // the store); its nodes past n-1 are never stored and cannot reach node
-    // n-1 because b is 0 there.  Raw store values are widened to FP64 one by
-    // one in the forward pass.
-    T raw[L];
-    if (!has_last) {
-#pragma unroll
-        for (int t = 0; t < L; ++t) raw[t] = buf_load((T*)nullptr, rs, vo, t * step);
-    } else {
-        const unsigned vl = vo + (unsigned)(n - 1 - p0) * step;  // node n-1
+// The segment's L raw values.  The segment holding node n-1 clamps its reads
+// to node n-1 (every address stays in the store); its nodes past n-1 are
+// never stored and cannot reach node n-1 because b is 0 there.
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const SweepArgs& A, const SegPos& P)
+{
+    return make_rsrc((T*)A.rho + P.plane0, (A.nvs - P.plane0) * sizeof(T));
+}
+
+// STREAM: one load per node for every lane with a per-lane offset and no
+// branch, so a wait placed before the arithmetic can count every load issued
+// after these (the prefetch of the streaming kernel).  Otherwise the
+// segments below the last one use a scalar offset per node (fewer VGPRs) and
+// only the last segment computes clamped offsets.
+template <typename T, int L, bool STREAM>
+__device__ __forceinline__ void seg_load(const SweepArgs& A, const SegPos& P, T (&raw)[L])
+{
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc<T>(A, P);
+    const unsigned step = A.line_stride * (unsigned)sizeof(T);
+    const int last = min(L, A.n - P.p0) - 1;  // nodes past n-1 read node n-1
+    if (STREAM || P.p0 + L >= A.n) {
 #pragma unroll
         for (int t = 0; t < L; ++t)
-            raw[t] = buf_load((T*)nullptr, rs, p0 + t < n ? vo + t * step : vl, 0);
+            raw[t] = buf_load((T*)nullptr, rs, P.vo + (unsigned)min(t, last) * step, 0);
+    } else {
+#pragma unroll
+        for (int t = 0; t < L; ++t) raw[t] = buf_load((T*)nullptr, rs, P.vo, t * step);
     }
+}
 
-    // the correction tables, staged in LDS while the loads are in flight
-    // (read after the first barrier below), zero padded so no index clamps:
-    //   G0 rows 0..kg0-1 = g0, then kPad zero rows
-    //   G1 kPad zero rows, rows kPad+j = g1 at node n-1-j (j < kg1), kPad zero rows
-    //   K  = wb (8 values per substrate)
-    double* G0 = gtab;
-    double* G1 = gtab + (kg0 + kPad) * S;
-    double* K = G1 + (kg1 + 2 * kPad) * S;
+// The correction tables staged in LDS, zero padded so no index clamps:
+//   G0 rows 0..kg0-1 = g0, then kPad zero rows
+//   G1 kPad zero rows, rows kPad+j = g1 at node n-1-j (j < kg1), kPad zero rows
+//   K  = wb (8 values per substrate)
+// Read only after a barrier.
+__device__ __forceinline__ void stage_tables(const SweepArgs& A, double* gtab)
+{
+    const int S = A.S, kg0 = A.kg0, kg1 = A.kg1;
     const int ng = (kg0 + kg1 + 3 * kPad) * S;
-    for (int i = tid; i < ng + 8 * S; i += blockDim.x) {
+    for (int i = threadIdx.x; i < ng + 8 * S; i += blockDim.x) {
         const int row = i / S, cl = i - row * S;
         double val = 0.0;
         if (row < kg0) val = A.g0[i];
         else if (row >= kg0 + 2 * kPad && row < kg0 + kg1 + 2 * kPad)
             val = A.g1[(row - kg0 - 2 * kPad) * S + cl];
-        else if (i >= ng) val = wb[i - ng];
+        else if (i >= ng) val = A.wb[i - ng];
         gtab[i] = val;
     }
+}
+
+// The solve of one tile from its raw values, and the stores; r = 1/d and
+// a = c1/d of the thread's substrate.  Every thread of the block calls it (it
+// holds barriers); raw store values are widened to FP64 one by one in the
+// forward pass.  xa holds the forward partials and then the forward carries,
+// xb the backward ones: a copy thread reads its segment's carries while the
+// owner goes on writing, so the two phases do not share slots.
+// With STREAM, every thread stores every node, nodes past n-1 to node n-1
+// (copies store the owner's values; the true value of node n-1 is stored
+// after the others by the same lane), so no store sits behind a lane mask;
+// otherwise only owners store, and only their real nodes.
+template <typename T, int L, bool STREAM>
+__device__ __forceinline__ void seg_solve(const SweepArgs& A, int C, const SegPos& P, const T (&raw)[L],
+                                          double r, double a, double* xa, double* xb, double* wfirst,
+                                          double* wlast, const double* gtab)
+{
+    const int n = A.n, nseg = A.nseg, S = A.S, kg0 = A.kg0, kg1 = A.kg1;
+    const int tid = threadIdx.x;
+    const int lane = tid & (C - 1);
+    const int s = P.s, p0 = P.p0;
+    const int cnt = min(L, n - p0);     // real nodes of the segment (>= 1)
+    const bool has_last = p0 + L >= n;  // holds node n-1 (and nodes past it)
+    const double* K = gtab + (kg0 + kg1 + 3 * kPad) * S;  // read after the first barrier
+    const unsigned step = A.line_stride * (unsigned)sizeof(T);
 
     // forward, zero carry-in
     double v[L];
@@ -185,19 +256,18 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, 
 #pragma unroll
     for (int t = 1; t < L; ++t) aL *= a;
     xa[tid] = y;
-    xb[tid] = aL;
-    __syncthreads();
-    if (tid < C) chain(xa, xb, lane, C, nseg, true);
-    __syncthreads();
+    lds_barrier();
+    if (tid < C) chain(xa, aL, lane, C, nseg, true);
+    lds_barrier();
     {
-        double PY = xa[tid];
+        double PY = xa[P.slot];
 #pragma unroll
         for (int t = 0; t < L; ++t) {
             PY *= a;
             v[t] += PY;
         }
     }
-    if (has_last && act) {  // z_{n-1} = y_{n-1}
+    if (has_last && P.last_writer) {  // z_{n-1} = y_{n-1}
         double wl = 0.0;
 #pragma unroll
         for (int t = 0; t < L; ++t)
@@ -206,26 +276,23 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, 
     }
 
     // backward, zero carry-in from above
-    double z = 0.0, bL;
+    double z = 0.0;
     if (!has_last) {
 #pragma unroll
         for (int t = L - 1; t >= 0; --t) v[t] = z = fma(a, z, v[t]);
-        bL = aL;
     } else {
 #pragma unroll
         for (int t = L - 1; t >= 0; --t) v[t] = z = fma(p0 + t < n - 1 ? a : 0.0, z, v[t]);
-        bL = 0.0;  // nothing lies above the last segment
     }
-    xa[tid] = z;  // each thread overwrites only the slot it alone has read
-    xb[tid] = bL;
-    __syncthreads();
-    if (tid < C) wfirst[lane] = chain(xa, xb, lane, C, nseg, false);
-    __syncthreads();
+    xb[tid] = z;
+    lds_barrier();
+    if (tid < C) wfirst[lane] = chain(xb, aL, lane, C, nseg, false);
+    lds_barrier();
 
     // Last pass, from the top of the segment down, each node finished and
     // stored as soon as it is final: the carry from above (a^(q-i) Z), the
     // rank-2 correction near the line ends (mu0 g0_i + mu1 g1_i), store.
-    const double QZ0 = has_last ? 0.0 : xa[tid];  // the last segment has nothing above
+    const double QZ0 = has_last ? 0.0 : xb[P.slot];  // the last segment has nothing above
     const bool head = p0 < kg0, tail = p0 + L > n - kg1;
     double mu0 = 0.0, mu1 = 0.0;
     if (head || tail) {
@@ -235,9 +302,11 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, 
         mu1 = tail ? k[2] * w0 + k[3] * wn : 0.0;
     }
     // g0 rows from node p0 up; g1 rows from node p0 down (zero padded)
+    const double* G0 = gtab;
+    const double* G1 = gtab + (kg0 + kPad) * S;
     const double* g0r = G0 + (head ? p0 : 0) * S + s;
     const double* g1r = G1 + (tail ? kPad + n - 1 - p0 : 0) * S + s;
-    const bool store_all = act && !has_last;
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc<T>(A, P);
     double QZ = QZ0;
 #pragma unroll
     for (int t = L - 1; t >= 0; --t) {
@@ -245,8 +314,23 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, 
         double zf = v[t] + QZ;
         if (head) zf = fma(mu0, g0r[t * S], zf);
         if (tail) zf = fma(mu1, g1r[-t * S], zf);
-        if (store_all || (act && t < cnt)) buf_store((T*)nullptr, rs, zf, vo, t * step);
+        if (STREAM) buf_store((T*)nullptr, rs, zf, P.vo + (unsigned)min(t, cnt - 1) * step, 0);
+        else if (P.act && t < cnt) buf_store((T*)nullptr, rs, zf, P.vo, t * step);
     }
+}
+
+// Block-level body of the segmented sweep for one plane's column tile.
+template <typename T, int L>
+__device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, int plane,
+                                           double* xa, double* xb, double* wfirst, double* wlast,
+                                           double* gtab)
+{
+    const SegPos P = seg_pos<T, L>(A, C, tile, plane);
+    const double r = A.wb[P.s * 8 + 0], a = A.wb[P.s * 8 + 1];
+    T raw[L];
+    seg_load<T, L, false>(A, P, raw);
+    stage_tables(A, gtab);  // while the loads are in flight
+    seg_solve<T, L, false>(A, C, P, raw, r, a, xa, xb, wfirst, wlast, gtab);
 }
 
 // One grid: block (x, y) = (column tile, plane), COLS lines per block.
@@ -259,6 +343,47 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4) k_sweep(SweepArg
     __shared__ double wlast[COLS];   // w at node n-1 of each column
     extern __shared__ double gtab[];
     sweep_tile<T, L>(A, COLS, blockIdx.x, blockIdx.y, xa, xb, wfirst, wlast, gtab);
+}
+
+// One grid, resident blocks: block b solves tiles b, b + gridDim.x, ... (tile
+// w = plane * tiles + column tile) and loads tile w + gridDim.x while it
+// solves tile w, so the loads of one tile overlap the arithmetic, barriers
+// and stores of the previous one.  The tables are staged once per block and
+// r, a come from LDS, so no global load is issued behind the prefetch (the
+// vector memory counter is in order: waiting for a later load would wait for
+// the prefetch too).  nblocks = gridDim.x <= tiles * planes, passed as an
+// argument so it sits in a scalar register.
+template <typename T, int L, int COLS>
+__global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_stream(SweepArgs A, int nblocks)
+{
+    __shared__ double xa[kMaxThreads];
+    __shared__ double xb[kMaxThreads];
+    __shared__ double wfirst[COLS];
+    __shared__ double wlast[COLS];
+    extern __shared__ double gtab[];
+    const int total = A.tiles * A.planes;
+    int w = blockIdx.x;  // the same in every lane, as is everything derived from it
+    T raw[L];
+    seg_load<T, L, true>(A, seg_pos<T, L>(A, COLS, w % A.tiles, w / A.tiles), raw);
+    stage_tables(A, gtab);
+    lds_barrier();
+    const double* K = gtab + (A.kg0 + A.kg1 + 3 * kPad) * A.S;
+    for (;;) {
+        const int wn = w + nblocks;
+        const bool more = wn < total;
+        // the last pass loads its own tile again rather than branch around
+        // the loads: with no load behind a branch, the waits before the
+        // arithmetic below count the prefetch as in flight
+        const int wl = more ? wn : w;
+        T nxt[L];
+        seg_load<T, L, true>(A, seg_pos<T, L>(A, COLS, wl % A.tiles, wl / A.tiles), nxt);
+        const SegPos P = seg_pos<T, L>(A, COLS, w % A.tiles, w / A.tiles);
+        seg_solve<T, L, true>(A, COLS, P, raw, K[P.s * 8 + 0], K[P.s * 8 + 1], xa, xb, wfirst, wlast, gtab);
+        if (!more) break;
+#pragma unroll
+        for (int t = 0; t < L; ++t) raw[t] = nxt[t];
+        w = wn;
+    }
 }
 
 // An ensemble: member m owns blocks [first[m], first[m+1]) of a 1-D grid,
@@ -331,11 +456,22 @@ Geometry geometry(const Env& e, int axis)
 }
 
 // Launch shape of one sweep: COLS lines per block and L nodes per segment,
-// L one of kNodes, nseg = ceil(n/L) segments with COLS*nseg <= 1024.
+// L one of kNodes, nseg = ceil(n/L) segments with COLS*nseg <= 1024, either
+// one block per tile (k_sweep) or resident blocks streaming through the
+// tiles (k_sweep_stream, L one of kStreamNodes; it holds two tiles' raw
+// values, so an FP64 store is limited to 16 nodes to keep within 128 VGPRs).
 // cols == 0 means the line is too long for the segmented kernel.  A plan set
 // by the autotuner (bfvm_autotune) wins over the default.
 constexpr int kNodes[] = {4, 8, 12, 16, 24, 32};
+constexpr int kStreamNodes[] = {8, 12, 16, 24};
 constexpr int kColsSet[] = {16, 32, 64};
+
+bool stream_nodes_ok(int L, int precision)
+{
+    for (int v : kStreamNodes)
+        if (v == L) return precision == BFVM_FP32 || L <= 16;
+    return false;
+}
 
 bool plan_valid(int n, int cols, int L)
 {
@@ -435,6 +571,60 @@ hipError_t launch_cols(Env& e, int axis, const Geometry& gm, int L, int nseg)
     }
 }
 
+// Blocks of `fn` resident on one compute unit (cached per kernel and shape).
+int resident_blocks(const void* fn, int threads, size_t lds)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, size_t>, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto key = std::make_tuple(fn, threads, lds);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, lds) != hipSuccess || nb < 1)
+        nb = 1;
+    cache[key] = nb;
+    return nb;
+}
+
+int compute_units(Env& e)
+{
+    if (e.cus == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e.device) != hipSuccess ||
+            cus < 1)
+            cus = 1;
+        e.cus = cus;
+    }
+    return e.cus;
+}
+
+template <typename T, int L, int COLS>
+hipError_t launch_stream(Env& e, int axis, int nseg)
+{
+    const SweepArgs A = sweep_args(e, axis, COLS, nseg);
+    const int threads = (COLS * nseg + 63) / 64 * 64;
+    const size_t lds = sweep_lds(A);
+    const long long total = (long long)A.tiles * A.planes;
+    const int nb = resident_blocks((const void*)k_sweep_stream<T, L, COLS>, threads, lds);
+    const int grid = (int)std::min<long long>(total, (long long)nb * compute_units(e));
+    k_sweep_stream<T, L, COLS><<<grid, threads, lds, e.stream>>>(A, grid);
+    return hipGetLastError();
+}
+
+template <typename T, int COLS>
+hipError_t launch_cols_stream(Env& e, int axis, int L, int nseg)
+{
+    switch (L) {
+    case 8: return launch_stream<T, 8, COLS>(e, axis, nseg);
+    case 12: return launch_stream<T, 12, COLS>(e, axis, nseg);
+    case 16: return launch_stream<T, 16, COLS>(e, axis, nseg);
+    default:
+        if constexpr (sizeof(T) == 4) return launch_stream<T, 24, COLS>(e, axis, nseg);
+        return hipErrorInvalidValue;
+    }
+}
+
 template <typename T>
 hipError_t sweep(Env& e, int axis)
 {
@@ -450,6 +640,14 @@ hipError_t sweep(Env& e, int axis)
         gbytes > kMaxCorrectionLds)
         return sweep_naive<T>(e, axis);
     int nseg = (gm.n + p.L - 1) / p.L;
+    const long long tiles = ((long long)gm.ncols + p.cols - 1) / p.cols;
+    if (p.stream && stream_nodes_ok(p.L, e.precision) && tiles * gm.nplanes < (1ll << 31)) {
+        switch (p.cols) {
+        case 16: return launch_cols_stream<T, 16>(e, axis, p.L, nseg);
+        case 32: return launch_cols_stream<T, 32>(e, axis, p.L, nseg);
+        default: return launch_cols_stream<T, 64>(e, axis, p.L, nseg);
+        }
+    }
     switch (p.cols) {
     case 16: return launch_cols<T, 16>(e, axis, gm, p.L, nseg);
     case 32: return launch_cols<T, 32>(e, axis, gm, p.L, nseg);
@@ -464,12 +662,15 @@ SweepPlan effective_plan(const Env& e, int axis)
     return plan_for(e, axis);
 }
 
-std::vector<SweepPlan> sweep_candidates(int n)
+std::vector<SweepPlan> sweep_candidates(int n, int precision)
 {
     std::vector<SweepPlan> out;
     for (int cols : kColsSet)
         for (int L : kNodes)
-            if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n)) out.push_back(SweepPlan{cols, L});
+            if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n)) {
+                out.push_back(SweepPlan{cols, L, 0});
+                if (stream_nodes_ok(L, precision)) out.push_back(SweepPlan{cols, L, 1});
+            }
     return out;
 }
 

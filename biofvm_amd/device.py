@@ -124,17 +124,18 @@ class DeviceMicroenvironment:
         """Picks the fastest launch shape of each sweep (on a scratch copy)."""
         self._check(self.lib.bfvm_autotune(self.h, float(dt)))
 
-    def set_sweep_plan(self, axis, lines_per_block, nodes_per_segment):
+    def set_sweep_plan(self, axis, lines_per_block, nodes_per_segment, streaming=False):
         self._check(self.lib.bfvm_set_sweep_plan(self.h, int(axis), int(lines_per_block),
-                                                 int(nodes_per_segment)))
+                                                 int(nodes_per_segment), int(bool(streaming))))
 
     def sweep_plans(self):
-        """[(lines_per_block, nodes_per_segment)] for the x, y and z sweeps."""
+        """[(lines_per_block, nodes_per_segment, streaming)] for the x, y and z sweeps."""
         out = []
         for axis in range(3):
-            a, b = ctypes.c_int(), ctypes.c_int()
-            self._check(self.lib.bfvm_get_sweep_plan(self.h, axis, ctypes.byref(a), ctypes.byref(b)))
-            out.append((a.value, b.value))
+            a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            self._check(self.lib.bfvm_get_sweep_plan(self.h, axis, ctypes.byref(a), ctypes.byref(b),
+                                                     ctypes.byref(c)))
+            out.append((a.value, b.value, c.value))
         return out
 
     def timed_run(self, dt, steps, flags=ALL, per_kernel=False):

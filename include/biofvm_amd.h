@@ -171,13 +171,17 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
 int bfvm_autotune(bfvm_handle h, double dt);
 
 /* The launch shape in use for one axis (0 = x, 1 = y, 2 = z):
- * lines per block and nodes per segment; *lines_per_block = 0 means the
- * one-thread-per-line fallback. */
-int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment);
+ * lines per block, nodes per segment, and streaming (1: resident blocks
+ * that load the next tile while solving one; 0: one block per tile);
+ * *lines_per_block = 0 means the one-thread-per-line fallback.  Any output
+ * pointer may be NULL. */
+int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment,
+                        int* streaming);
 
 /* Sets the launch shape of one axis by hand (lines_per_block = 0 restores
  * the default).  Returns BFVM_E_ARG for a shape the kernel cannot run. */
-int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment);
+int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment,
+                        int streaming);
 
 /* Grid and store description. */
 int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, int* precision,

@@ -164,7 +164,55 @@ def test_autotune_keeps_state_and_parity(precision):
         before = m.download()
         m.autotune(case["dt"])
         np.testing.assert_array_equal(m.download(), before)
-        for cols, L in m.sweep_plans():
-            assert cols in (16, 32, 64) and 4 <= L <= 32
+        for cols, L, streaming in m.sweep_plans():
+            assert cols in (16, 32, 64) and 4 <= L <= 32 and streaming in (0, 1)
         m.run(case["dt"], case["steps"], case["phases"])
         assert rel_err(m.download(), expected) <= TOL[precision]
+
+
+@pytest.fixture(scope="module")
+def plan_case():
+    # large enough that resident (streaming) blocks loop over several tiles on
+    # every axis: x and y have 80 planes of 3*128/cols or 3*112/cols tiles
+    rng = np.random.default_rng(77)
+    S = 3
+    case = W.make_case(112, 128, 80, S, 10.0, [1e5, 2e3, 4e4], [0.1, 0.01, 1.0], 0.01, 2,
+                       W.DIFFUSION, rng.uniform(0, 1, 112 * 128 * 80 * S))
+    return case, oracle.oracle_run(case)
+
+
+def _plans(n, precision):
+    for cols in (16, 32, 64):
+        for L in (4, 8, 12, 16, 24, 32):
+            if cols * -(-n // L) > 1024 or (L > 8 and L >= 2 * n):
+                continue
+            yield cols, L, False
+            if L in (8, 12, 16) or (L == 24 and precision == FP32):
+                yield cols, L, True
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_every_launch_shape_against_oracle(plan_case, precision):
+    case, ref = plan_case
+    dims = (case["nx"], case["ny"], case["nz"])
+    with setup_case(case, precision) as m:
+        start = case["rho0"]
+        for axis in range(3):
+            for cols, L, streaming in _plans(dims[axis], precision):
+                m.upload(start)
+                m.set_sweep_plan(axis, cols, L, streaming)
+                assert m.sweep_plans()[axis] == (cols, L, int(streaming))
+                m.run(case["dt"], case["steps"], case["phases"])
+                err = rel_err(m.download(), ref)
+                assert err <= TOL[precision], (axis, cols, L, streaming, err)
+            m.set_sweep_plan(axis, 0, 0)
+
+
+def test_unrunnable_launch_shapes_are_refused():
+    with DeviceMicroenvironment(40, 40, 300, 2, 10.0, FP64) as m:
+        with pytest.raises(BfvmError):
+            m.set_sweep_plan(2, 64, 4)  # 64 lines x 75 segments > 1024 threads
+        with pytest.raises(BfvmError):
+            m.set_sweep_plan(0, 32, 24, True)  # no FP64 streaming kernel with 24 nodes
+        with pytest.raises(BfvmError):
+            m.set_sweep_plan(0, 48, 8)
