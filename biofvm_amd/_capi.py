@@ -54,6 +54,11 @@ SIGNATURES = {
     "bfvm_ensemble_get_segment": (_i, [_vp, _ip]),
     "bfvm_ensemble_set_segment": (_i, [_vp, _i]),
     "bfvm_ensemble_last_error": (ctypes.c_char_p, [_vp]),
+    "bfvm_tape_record": (_i, [_h, _d, _i, _i, _i]),
+    "bfvm_tape_seed": (_i, [_h, _dp]),
+    "bfvm_tape_seed_l2": (_i, [_h, _dp, _dp]),
+    "bfvm_tape_backward": (_i, [_h, _dp, _dp, _dp, _dp, _dp]),
+    "bfvm_tape_free": (_i, [_h]),
 }
 
 _lib = None

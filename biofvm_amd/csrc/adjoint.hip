@@ -1,0 +1,614 @@
+// Differentiation of the time-stepped model: a hand-written adjoint of the
+// step (bulk -> cells -> diffusion) with segment checkpointing.
+//
+// The step's operators and their transposes:
+//   bulk     y = (x + A) / B                      x' = y' / B
+//   agents   per agent in list order, per voxel:  y = (x + t1) / t2
+//            x' = y' / t2, t1' = y' / t2, t2' = -y' y / t2, taken in
+//            reverse list order with the intermediate values recomputed;
+//            t1 = c S T, t2 = 1 + c S + c U (c = dt V / V_voxel) give the
+//            rate gradients S' += c (T t1' + t2'), U' += c t2'
+//   Dirichlet y = x except at the nodes (masked substrates)   x' = y' there zeroed
+//   sweep    y = A^-1 x, A = (1 + c2) I + c1 L (L the line's Neumann Laplacian,
+//            BioFVM_solvers.cpp:104-142) is symmetric, so x' = A^-1 y' — the
+//            forward sweep kernels run on the adjoint unchanged — and
+//            c1' -= sum_edges (x'_i - x'_j)(y_i - y_j),  c2' -= sum x' . y
+//            with c1 = dt D / dx^2 and c2 = dt lambda / 3 per substrate.
+// The loss seed dJ/drho_N comes from the caller or from an L2 misfit.
+//
+// Tape: the state at every K-th step is kept (checkpoints); the backward
+// pass walks the segments from the last, recomputes the segment's step
+// starts from its checkpoint into a ring of K states, then for each step in
+// reverse recomputes the step once more keeping the states the transposes
+// need (after bulk, after each sweep) and applies the transposed step.
+// Device memory: (steps/K + K + 5) stores, plus FP64 gradients.
+// Reductions (gradients of D, lambda, the loss) are FP64 and deterministic:
+// fixed-size partial sums per block, then one block adds them in order.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bfvm_internal.h"
+
+using bfvm::Env;
+
+namespace bfvm {
+
+struct Tape {
+    double dt = 0;
+    int steps = 0, flags = 0, K = 0;
+    std::vector<double> D, lambda;  // coefficients the tape was recorded with
+    unsigned agent_epoch = 0, source_epoch = 0;
+    std::vector<void*> cps;   // state at steps 0, K, 2K, ...
+    std::vector<void*> ring;  // starts of the steps of one segment
+    void* s1 = nullptr;       // after bulk (the agents' input)
+    void* s4 = nullptr;       // after the x sweep
+    void* s6 = nullptr;       // after the y sweep
+    void* s8 = nullptr;       // after the z sweep (the step's result)
+    void* lam = nullptr;      // adjoint state
+    bool seeded = false;
+    double* rk = nullptr;      // agents: intermediate densities (n_agents x S)
+    double* gsec = nullptr;    // agents: secretion gradient (grouped order)
+    double* gupt = nullptr;    // agents: uptake gradient (grouped order)
+    double* gc = nullptr;      // 2*S: dJ/dc1, dJ/dc2
+    double* partial = nullptr; // reduction partials
+};
+
+}  // namespace bfvm
+
+using bfvm::Tape;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRedBlocks = 512;  // blocks of the reduction kernels (fixed: deterministic sums)
+constexpr int kChunk = 8;        // substrates per reduction pass
+
+inline unsigned grid_for(size_t n, int block = kBlock)
+{
+    return (unsigned)((n + block - 1) / block);
+}
+
+int afail(Env& e, int code, const std::string& msg)
+{
+    e.err = msg;
+    return code;
+}
+
+#define AHIP(e, call)                                                                            \
+    do {                                                                                         \
+        hipError_t st_ = (call);                                                                 \
+        if (st_ != hipSuccess) return afail(e, BFVM_E_DEVICE, std::string(#call) + ": " +        \
+                                                                   hipGetErrorString(st_));      \
+    } while (0)
+
+// ---------------------------------------------------------------- kernels
+
+template <typename T>
+__global__ void k_dirichlet_zero(T* __restrict__ lam, int S, int n_dir, const int* __restrict__ idx,
+                                 const unsigned char* __restrict__ mask)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n_dir * S) return;
+    int d = (int)(t / S);
+    int s = (int)(t - (size_t)d * S);
+    if (mask[s]) lam[(size_t)idx[d] * S + s] = (T)0;
+}
+
+template <typename T>
+__global__ void k_bulk_adjoint(T* __restrict__ lam, const double* __restrict__ B, size_t nvs)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nvs) lam[e] = (T)((double)lam[e] / B[e]);
+}
+
+// One thread per (voxel group, substrate): the group's agents recomputed in
+// list order from the phase input x (intermediate values kept in rk), then
+// walked back in reverse order.
+template <typename T>
+__global__ void k_agents_adjoint(T* __restrict__ lam, const T* __restrict__ x, int S, int n_groups,
+                                 const int* __restrict__ gvox, const int* __restrict__ gstart,
+                                 const double* __restrict__ t1, const double* __restrict__ t2,
+                                 const double* __restrict__ c, const double* __restrict__ sat,
+                                 double* __restrict__ rk, double* __restrict__ gsec,
+                                 double* __restrict__ gupt)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n_groups * S) return;
+    const int g = (int)(t / S);
+    const int s = (int)(t - (size_t)g * S);
+    const size_t v = (size_t)gvox[g] * S + s;
+    const int a0 = gstart[g], a1 = gstart[g + 1];
+    double r = (double)x[v];
+    for (int a = a0; a < a1; ++a) {
+        const size_t q = (size_t)a * S + s;
+        r = (r + t1[q]) / t2[q];
+        rk[q] = r;
+    }
+    double l = (double)lam[v];
+    for (int a = a1 - 1; a >= a0; --a) {
+        const size_t q = (size_t)a * S + s;
+        const double inv = 1.0 / t2[q];
+        if (c) {  // agents set from rates: accumulate the rate gradients
+            const double gt1 = l * inv;
+            const double gt2 = -l * rk[q] * inv;
+            gsec[q] += c[a] * (sat[q] * gt1 + gt2);
+            gupt[q] += c[a] * gt2;
+        }
+        l *= inv;
+    }
+    lam[v] = (T)l;
+}
+
+// out[m] = sum over the block's threads of acc[m], m < M, in a fixed order
+// (wave shuffles, then the waves in index order).
+template <int M>
+__device__ void block_sums(double (&acc)[M], double* __restrict__ out)
+{
+    __shared__ double red[kBlock / 64][M];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        double v = acc[m];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wave][m] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < M) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) v += red[w][threadIdx.x];
+        out[threadIdx.x] = v;
+    }
+}
+
+// Per block: for substrates s0 .. s0+sc-1 (sc <= kChunk), the sums over the
+// block's voxels of (l_v - l_w)(y_v - y_w) over edges v -> w = v + step along
+// `axis`, and of l_v y_v.  partial[block][0..kChunk) and [kChunk..2 kChunk).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_axis_grad(const T* __restrict__ lam, const T* __restrict__ y,
+                                                      int nx, int ny, int nz, int S, int axis, int s0,
+                                                      int sc, double* __restrict__ partial)
+{
+    double acc[2 * kChunk];
+#pragma unroll
+    for (int m = 0; m < 2 * kChunk; ++m) acc[m] = 0.0;
+    const size_t nvox = (size_t)nx * ny * nz;
+    const size_t stride = axis == 0 ? 1 : axis == 1 ? (size_t)nx : (size_t)nx * ny;
+    const int n = axis == 0 ? nx : axis == 1 ? ny : nz;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+         v += (size_t)gridDim.x * blockDim.x) {
+        const int coord = axis == 0 ? (int)(v % nx) : axis == 1 ? (int)((v / nx) % ny) : (int)(v / ((size_t)nx * ny));
+        const bool edge = coord < n - 1;
+        const size_t w = v + stride;
+#pragma unroll
+        for (int m = 0; m < kChunk; ++m) {
+            if (m < sc) {
+                const size_t q = v * S + s0 + m;
+                const double l = (double)lam[q], yv = (double)y[q];
+                acc[kChunk + m] += l * yv;
+                if (edge) {
+                    const size_t p = w * S + s0 + m;
+                    acc[m] += (l - (double)lam[p]) * (yv - (double)y[p]);
+                }
+            }
+        }
+    }
+    block_sums<2 * kChunk>(acc, partial + (size_t)blockIdx.x * 2 * kChunk);
+}
+
+// gc[s] -= sum of the blocks' edge sums, gc[S + s] -= sum of the l.y sums.
+__global__ void k_axis_grad_finish(const double* __restrict__ partial, int nblocks, int S, int s0, int sc,
+                                   double* __restrict__ gc)
+{
+    const int m = threadIdx.x;
+    if (m >= 2 * kChunk) return;
+    const int s = m % kChunk;
+    if (s >= sc) return;
+    double v = 0.0;
+    for (int b = 0; b < nblocks; ++b) v += partial[(size_t)b * 2 * kChunk + m];
+    if (m < kChunk) gc[s0 + s] -= v;
+    else gc[S + s0 + s] -= v;
+}
+
+// lam = rho - target; partial sums of (rho - target)^2 / 2.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_l2_seed(const T* __restrict__ rho, const double* __restrict__ target,
+                                                    T* __restrict__ lam, size_t nvs, double* __restrict__ partial)
+{
+    double acc[1] = {0.0};
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nvs;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const double d = (double)rho[e] - target[e];
+        lam[e] = (T)d;
+        acc[0] += 0.5 * d * d;
+    }
+    block_sums<1>(acc, partial + blockIdx.x);
+}
+
+__global__ void k_sum(const double* __restrict__ partial, int n, double* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double v = 0.0;
+    for (int b = 0; b < n; ++b) v += partial[b];
+    *out = v;
+}
+
+// ---------------------------------------------------------------- host side
+
+template <typename T>
+hipError_t dirichlet_zero(Env& e, void* lam)
+{
+    if (e.n_dir == 0) return hipSuccess;
+    k_dirichlet_zero<T><<<grid_for((size_t)e.n_dir * e.S), kBlock, 0, e.stream>>>(
+        (T*)lam, e.S, e.n_dir, e.dir_idx, e.dir_mask);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t axis_grad(Env& e, Tape& tp, const void* y, int axis)
+{
+    for (int s0 = 0; s0 < e.S; s0 += kChunk) {
+        const int sc = std::min(kChunk, e.S - s0);
+        k_axis_grad<T><<<kRedBlocks, kBlock, 0, e.stream>>>((const T*)tp.lam, (const T*)y, e.nx, e.ny, e.nz,
+                                                           e.S, axis, s0, sc, tp.partial);
+        hipError_t st = hipGetLastError();
+        if (st != hipSuccess) return st;
+        k_axis_grad_finish<<<1, 64, 0, e.stream>>>(tp.partial, kRedBlocks, e.S, s0, sc, tp.gc);
+        if ((st = hipGetLastError()) != hipSuccess) return st;
+    }
+    return hipSuccess;
+}
+
+// Runs launcher f with the handle's store pointing at `buf`.
+template <typename F>
+hipError_t on_buffer(Env& e, void* buf, F&& f)
+{
+    void* keep = e.rho;
+    e.rho = buf;
+    hipError_t st = f();
+    e.rho = keep;
+    return st;
+}
+
+hipError_t copy_store(Env& e, void* dst, const void* src)
+{
+    return hipMemcpyAsync(dst, src, e.nvs * e.elem, hipMemcpyDeviceToDevice, e.stream);
+}
+
+// One forward step on buffer `buf` (the phases of bfvm_step).
+hipError_t forward_step(Env& e, const Tape& tp, void* buf)
+{
+    return on_buffer(e, buf, [&]() -> hipError_t {
+        hipError_t st;
+        if ((tp.flags & BFVM_BULK) && e.have_bulk && (st = bfvm::launch_bulk(e))) return st;
+        if ((tp.flags & BFVM_CELLS) && e.n_agents > 0 && (st = bfvm::launch_agents(e))) return st;
+        if (tp.flags & BFVM_DIFFUSION)
+            for (int axis = 0; axis < 3; ++axis) {
+                if (e.n_dir > 0 && (st = bfvm::launch_dirichlet(e))) return st;
+                if ((st = bfvm::launch_sweep(e, axis))) return st;
+            }
+        return hipSuccess;
+    });
+}
+
+// The step from `start` again, keeping s1 (after bulk), s4, s6, s8 (after
+// each sweep; without diffusion all three hold the state after the agents).
+hipError_t forward_step_keep(Env& e, Tape& tp, const void* start)
+{
+    hipError_t st;
+    if ((st = copy_store(e, tp.s1, start))) return st;
+    if ((tp.flags & BFVM_BULK) && e.have_bulk &&
+        (st = on_buffer(e, tp.s1, [&] { return bfvm::launch_bulk(e); })))
+        return st;
+    if ((st = copy_store(e, tp.s4, tp.s1))) return st;
+    if ((tp.flags & BFVM_CELLS) && e.n_agents > 0 &&
+        (st = on_buffer(e, tp.s4, [&] { return bfvm::launch_agents(e); })))
+        return st;
+    const bool diff = tp.flags & BFVM_DIFFUSION;
+    void* bufs[3] = {tp.s4, tp.s6, tp.s8};
+    for (int axis = 0; axis < 3; ++axis) {
+        if (axis > 0 && (st = copy_store(e, bufs[axis], bufs[axis - 1]))) return st;
+        if (!diff) continue;
+        st = on_buffer(e, bufs[axis], [&]() -> hipError_t {
+            hipError_t q;
+            if (e.n_dir > 0 && (q = bfvm::launch_dirichlet(e))) return q;
+            return bfvm::launch_sweep(e, axis);
+        });
+        if (st) return st;
+    }
+    return hipSuccess;
+}
+
+// The transposed step: tp.lam holds dJ/d(step result) on entry and
+// dJ/d(step start) on exit; the D / lambda / rate gradients are accumulated.
+template <typename T>
+hipError_t adjoint_step(Env& e, Tape& tp)
+{
+    hipError_t st;
+    if (tp.flags & BFVM_DIFFUSION) {
+        const void* ys[3] = {tp.s4, tp.s6, tp.s8};
+        for (int axis = 2; axis >= 0; --axis) {
+            if ((st = on_buffer(e, tp.lam, [&] { return bfvm::launch_sweep(e, axis); }))) return st;
+            if ((st = axis_grad<T>(e, tp, ys[axis], axis))) return st;
+            if ((st = dirichlet_zero<T>(e, tp.lam))) return st;
+        }
+    }
+    if ((tp.flags & BFVM_CELLS) && e.n_agent_voxels > 0) {
+        const size_t n = (size_t)e.n_agent_voxels * e.S;
+        k_agents_adjoint<T><<<grid_for(n), kBlock, 0, e.stream>>>(
+            (T*)tp.lam, (const T*)tp.s1, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
+            (const double*)e.agent_t1, (const double*)e.agent_t2, e.agent_rates ? e.agent_c : nullptr,
+            e.agent_sat, tp.rk, tp.gsec, tp.gupt);
+        if ((st = hipGetLastError())) return st;
+    }
+    if ((tp.flags & BFVM_BULK) && e.have_bulk) {
+        k_bulk_adjoint<T><<<grid_for(e.nvs), kBlock, 0, e.stream>>>((T*)tp.lam, (const double*)e.bulk_B, e.nvs);
+        if ((st = hipGetLastError())) return st;
+    }
+    return hipSuccess;
+}
+
+void free_ptr(void*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+void free_ptr(double*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+hipError_t alloc_store(Env& e, void** p)
+{
+    return hipMalloc(p, e.nvs * e.elem);
+}
+
+int check_tape_handle(Env& e)
+{
+    if (e.ensemble_stream) return afail(e, BFVM_E_STATE, "the handle belongs to an ensemble");
+    return BFVM_OK;
+}
+
+// The tape is valid only for the coefficients, agents and sources it was
+// recorded with.
+int check_unchanged(Env& e, const Tape& tp)
+{
+    if (e.D != tp.D || e.lambda != tp.lambda || e.agent_epoch != tp.agent_epoch ||
+        e.source_epoch != tp.source_epoch)
+        return afail(e, BFVM_E_STATE,
+                     "coefficients, agents or sources changed since bfvm_tape_record");
+    return BFVM_OK;
+}
+
+void host_to_store(const Env& e, std::vector<unsigned char>& buf, const double* src)
+{
+    buf.resize(e.nvs * e.elem);
+    if (e.precision == BFVM_FP64) {
+        std::memcpy(buf.data(), src, e.nvs * sizeof(double));
+    } else {
+        float* f = (float*)buf.data();
+        for (size_t i = 0; i < e.nvs; ++i) f[i] = (float)src[i];
+    }
+}
+
+void store_to_host(const Env& e, const std::vector<unsigned char>& buf, double* dst)
+{
+    if (e.precision == BFVM_FP64) {
+        std::memcpy(dst, buf.data(), e.nvs * sizeof(double));
+    } else {
+        const float* f = (const float*)buf.data();
+        for (size_t i = 0; i < e.nvs; ++i) dst[i] = f[i];
+    }
+}
+
+}  // namespace
+
+namespace bfvm {
+
+void free_tape(Env& e)
+{
+    Tape* tp = e.tape;
+    if (!tp) return;
+    (void)hipStreamSynchronize(e.stream);
+    for (void*& p : tp->cps) free_ptr(p);
+    for (void*& p : tp->ring) free_ptr(p);
+    free_ptr(tp->s1);
+    free_ptr(tp->s4);
+    free_ptr(tp->s6);
+    free_ptr(tp->s8);
+    free_ptr(tp->lam);
+    free_ptr(tp->rk);
+    free_ptr(tp->gsec);
+    free_ptr(tp->gupt);
+    free_ptr(tp->gc);
+    free_ptr(tp->partial);
+    delete tp;
+    e.tape = nullptr;
+}
+
+}  // namespace bfvm
+
+extern "C" {
+
+int bfvm_tape_record(bfvm_handle h, double dt, int steps, int flags, int checkpoint_every)
+{
+    if (!h) return BFVM_E_ARG;
+    Env& e = *h;
+    int rc;
+    if ((rc = check_tape_handle(e))) return rc;
+    if (steps < 1) return afail(e, BFVM_E_ARG, "steps must be >= 1");
+    if (checkpoint_every < 1) return afail(e, BFVM_E_ARG, "checkpoint_every must be >= 1");
+    if ((flags & ~BFVM_ALL) != 0) return afail(e, BFVM_E_ARG, "unknown phase flags");
+    if ((flags & BFVM_CELLS) && e.n_agents > 0 && e.agent_rates && e.agent_dt != dt)
+        return afail(e, BFVM_E_ARG, "dt differs from the dt the agent rates were set with");
+    bfvm::free_tape(e);
+    Tape* tp = new Tape;
+    e.tape = tp;
+    tp->dt = dt;
+    tp->steps = steps;
+    tp->flags = flags;
+    tp->K = std::min(checkpoint_every, steps);
+    tp->D = e.D;
+    tp->lambda = e.lambda;
+    tp->agent_epoch = e.agent_epoch;
+    tp->source_epoch = e.source_epoch;
+    const int ncp = (steps + tp->K - 1) / tp->K;
+    tp->cps.assign(ncp, nullptr);
+    for (void*& p : tp->cps) AHIP(e, alloc_store(e, &p));
+    for (int n = 0; n < steps; ++n) {
+        if (n % tp->K == 0) AHIP(e, copy_store(e, tp->cps[n / tp->K], e.rho));
+        if ((rc = bfvm_step(h, dt, flags))) return rc;
+    }
+    return BFVM_OK;
+}
+
+int bfvm_tape_seed(bfvm_handle h, const double* seed)
+{
+    if (!h) return BFVM_E_ARG;
+    Env& e = *h;
+    Tape* tp = e.tape;
+    if (!tp) return afail(e, BFVM_E_STATE, "no tape (bfvm_tape_record)");
+    if (!seed) return afail(e, BFVM_E_ARG, "seed is NULL");
+    if (!tp->lam) AHIP(e, alloc_store(e, &tp->lam));
+    std::vector<unsigned char> buf;
+    host_to_store(e, buf, seed);
+    AHIP(e, hipStreamSynchronize(e.stream));
+    AHIP(e, hipMemcpy(tp->lam, buf.data(), e.nvs * e.elem, hipMemcpyHostToDevice));
+    tp->seeded = true;
+    return BFVM_OK;
+}
+
+int bfvm_tape_seed_l2(bfvm_handle h, const double* target, double* loss)
+{
+    if (!h) return BFVM_E_ARG;
+    Env& e = *h;
+    Tape* tp = e.tape;
+    if (!tp) return afail(e, BFVM_E_STATE, "no tape (bfvm_tape_record)");
+    if (!target) return afail(e, BFVM_E_ARG, "target is NULL");
+    if (!tp->lam) AHIP(e, alloc_store(e, &tp->lam));
+    if (!tp->partial) AHIP(e, hipMalloc((void**)&tp->partial, sizeof(double) * kRedBlocks * 2 * kChunk));
+    double* dtarget = nullptr;
+    double* dloss = nullptr;
+    AHIP(e, hipMalloc((void**)&dtarget, e.nvs * sizeof(double)));
+    AHIP(e, hipMalloc((void**)&dloss, sizeof(double)));
+    hipError_t st = hipMemcpyAsync(dtarget, target, e.nvs * sizeof(double), hipMemcpyHostToDevice, e.stream);
+    if (st == hipSuccess) {
+        if (e.precision == BFVM_FP64)
+            k_l2_seed<double><<<kRedBlocks, kBlock, 0, e.stream>>>((const double*)e.rho, dtarget,
+                                                                  (double*)tp->lam, e.nvs, tp->partial);
+        else
+            k_l2_seed<float><<<kRedBlocks, kBlock, 0, e.stream>>>((const float*)e.rho, dtarget,
+                                                                 (float*)tp->lam, e.nvs, tp->partial);
+        st = hipGetLastError();
+    }
+    if (st == hipSuccess) {
+        k_sum<<<1, 64, 0, e.stream>>>(tp->partial, kRedBlocks, dloss);
+        st = hipGetLastError();
+    }
+    double v = 0;
+    if (st == hipSuccess) st = hipMemcpyAsync(&v, dloss, sizeof(double), hipMemcpyDeviceToHost, e.stream);
+    if (st == hipSuccess) st = hipStreamSynchronize(e.stream);
+    (void)hipFree(dtarget);
+    (void)hipFree(dloss);
+    AHIP(e, st);
+    if (loss) *loss = v;
+    tp->seeded = true;
+    return BFVM_OK;
+}
+
+int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double* grad_lambda,
+                       double* grad_secretion, double* grad_uptake)
+{
+    if (!h) return BFVM_E_ARG;
+    Env& e = *h;
+    Tape* tp = e.tape;
+    int rc;
+    if ((rc = check_tape_handle(e))) return rc;
+    if (!tp) return afail(e, BFVM_E_STATE, "no tape (bfvm_tape_record)");
+    if (!tp->seeded) return afail(e, BFVM_E_STATE, "no seed (bfvm_tape_seed / bfvm_tape_seed_l2)");
+    if ((rc = check_unchanged(e, *tp))) return rc;
+    if ((grad_secretion || grad_uptake) && e.n_agents > 0 && !e.agent_rates)
+        return afail(e, BFVM_E_STATE, "rate gradients need agents set from rates");
+    const int S = e.S;
+    const double dt = tp->dt;
+    if (tp->flags & BFVM_DIFFUSION) {
+        if (!e.uniform) return afail(e, BFVM_E_MESH, "non-uniform mesh");
+        if ((rc = bfvm::ensure_coefficients(e, dt))) return rc;
+    }
+    if (tp->flags & BFVM_BULK) AHIP(e, bfvm::ensure_bulk(e, dt));
+    // work buffers
+    if ((int)tp->ring.size() != tp->K) {
+        tp->ring.assign(tp->K, nullptr);
+        for (void*& p : tp->ring) AHIP(e, alloc_store(e, &p));
+    }
+    for (void** p : {&tp->s1, &tp->s4, &tp->s6, &tp->s8})
+        if (!*p) AHIP(e, alloc_store(e, p));
+    if (!tp->partial) AHIP(e, hipMalloc((void**)&tp->partial, sizeof(double) * kRedBlocks * 2 * kChunk));
+    if (!tp->gc) AHIP(e, hipMalloc((void**)&tp->gc, sizeof(double) * 2 * S));
+    AHIP(e, hipMemsetAsync(tp->gc, 0, sizeof(double) * 2 * S, e.stream));
+    const size_t na = (size_t)e.n_agents * S;
+    if (na > 0) {
+        free_ptr(tp->rk);
+        free_ptr(tp->gsec);
+        free_ptr(tp->gupt);
+        AHIP(e, hipMalloc((void**)&tp->rk, sizeof(double) * na));
+        AHIP(e, hipMalloc((void**)&tp->gsec, sizeof(double) * na));
+        AHIP(e, hipMalloc((void**)&tp->gupt, sizeof(double) * na));
+        AHIP(e, hipMemsetAsync(tp->gsec, 0, sizeof(double) * na, e.stream));
+        AHIP(e, hipMemsetAsync(tp->gupt, 0, sizeof(double) * na, e.stream));
+    }
+    // segments from the last
+    const int K = tp->K;
+    for (int j = (int)tp->cps.size() - 1; j >= 0; --j) {
+        const int n0 = j * K, n1 = std::min(n0 + K, tp->steps);
+        AHIP(e, copy_store(e, tp->ring[0], tp->cps[j]));
+        for (int n = n0; n + 1 < n1; ++n) {
+            AHIP(e, copy_store(e, tp->ring[n - n0 + 1], tp->ring[n - n0]));
+            AHIP(e, forward_step(e, *tp, tp->ring[n - n0 + 1]));
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            AHIP(e, forward_step_keep(e, *tp, tp->ring[n - n0]));
+            AHIP(e, e.precision == BFVM_FP64 ? adjoint_step<double>(e, *tp) : adjoint_step<float>(e, *tp));
+        }
+    }
+    tp->seeded = false;  // the adjoint state now holds dJ/drho_0
+    // results
+    std::vector<double> gc(2 * S);
+    AHIP(e, hipMemcpyAsync(gc.data(), tp->gc, sizeof(double) * 2 * S, hipMemcpyDeviceToHost, e.stream));
+    std::vector<unsigned char> buf;
+    if (grad_rho0) {
+        buf.resize(e.nvs * e.elem);
+        AHIP(e, hipMemcpyAsync(buf.data(), tp->lam, e.nvs * e.elem, hipMemcpyDeviceToHost, e.stream));
+    }
+    std::vector<double> gs(na), gu(na);
+    if (na > 0 && (grad_secretion || grad_uptake)) {
+        AHIP(e, hipMemcpyAsync(gs.data(), tp->gsec, sizeof(double) * na, hipMemcpyDeviceToHost, e.stream));
+        AHIP(e, hipMemcpyAsync(gu.data(), tp->gupt, sizeof(double) * na, hipMemcpyDeviceToHost, e.stream));
+    }
+    AHIP(e, hipStreamSynchronize(e.stream));
+    if (grad_rho0) store_to_host(e, buf, grad_rho0);
+    // c1 = dt D / dx^2 (BioFVM_solvers.cpp:95), c2 = dt lambda / 3 (:96)
+    for (int s = 0; s < S; ++s) {
+        if (grad_D) grad_D[s] = gc[s] * dt / (e.dx * e.dx);
+        if (grad_lambda) grad_lambda[s] = gc[S + s] * dt / 3.0;
+    }
+    // grouped order -> list order
+    for (int q = 0; q < e.n_agents; ++q) {
+        const int a = e.agent_order[q];
+        for (int s = 0; s < S; ++s) {
+            if (grad_secretion) grad_secretion[(size_t)a * S + s] = gs[(size_t)q * S + s];
+            if (grad_uptake) grad_uptake[(size_t)a * S + s] = gu[(size_t)q * S + s];
+        }
+    }
+    return BFVM_OK;
+}
+
+int bfvm_tape_free(bfvm_handle h)
+{
+    if (!h) return BFVM_E_ARG;
+    bfvm::free_tape(*h);
+    return BFVM_OK;
+}
+
+}  // extern "C"

@@ -111,6 +111,17 @@ struct Env {
     int* agent_seg_start = nullptr;  // n_agent_voxels + 1
     void* agent_t1 = nullptr;        // n_agents*S, grouped order, FP64
     void* agent_t2 = nullptr;
+    // Agents set from rates, for the adjoint: c = dt*V/V_voxel per agent and
+    // the saturation densities (n_agents*S), grouped order, the dt they were
+    // built for, and the list index of each grouped agent.
+    bool agent_rates = false;
+    double agent_dt = 0;
+    double* agent_c = nullptr;
+    double* agent_sat = nullptr;
+    std::vector<int> agent_order;
+
+    // Recorded steps for the adjoint (adjoint.hip).
+    struct Tape* tape = nullptr;
 
     // Per-launch timing (bfvm_timed_run with per-kernel events).
     bool kernel_timing = false;
@@ -155,6 +166,9 @@ hipError_t launch_agents_batch(int precision, const AgentBatch* d_tab, int nm, l
 int ensure_coefficients(Env& e, double dt);
 // Bulk implicit terms for dt (rebuilt when dt changed).
 hipError_t ensure_bulk(Env& e, double dt);
+
+// Frees the adjoint's tape, if any.
+void free_tape(Env& e);
 
 // Valid segmented shapes for a line of n nodes and a store precision, for
 // the autotuner.

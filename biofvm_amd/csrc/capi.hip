@@ -312,11 +312,18 @@ void free_agents(Env& e)
     dfree_t(e.agent_seg_start);
     dfree(e.agent_t1);
     dfree(e.agent_t2);
+    dfree_t(e.agent_c);
+    dfree_t(e.agent_sat);
+    e.agent_rates = false;
+    e.agent_order.clear();
     e.n_agents = 0;
     e.n_agent_voxels = 0;
 }
 
-int set_agents_impl(Env& e, int n, const int* voxel, const double* t1, const double* t2)
+// c (n) and sat (n*S) in list order, or NULL: kept in grouped order for the
+// adjoint's rate gradients.
+int set_agents_impl(Env& e, int n, const int* voxel, const double* t1, const double* t2,
+                    const double* c = nullptr, const double* sat = nullptr, double dt = 0)
 {
     free_agents(e);
     ++e.agent_epoch;
@@ -351,6 +358,20 @@ int set_agents_impl(Env& e, int n, const int* voxel, const double* t1, const dou
                               hipMemcpyHostToDevice));
     HIP_OR_FAIL(&e, upload_f64(e.agent_t1, s1.data(), (size_t)n * S));
     HIP_OR_FAIL(&e, upload_f64(e.agent_t2, s2.data(), (size_t)n * S));
+    e.agent_order = order;
+    if (c && sat) {
+        std::vector<double> cg(n), sg((size_t)n * S);
+        for (int q = 0; q < n; ++q) {
+            cg[q] = c[order[q]];
+            std::memcpy(&sg[(size_t)q * S], sat + (size_t)order[q] * S, sizeof(double) * S);
+        }
+        HIP_OR_FAIL(&e, hipMalloc((void**)&e.agent_c, sizeof(double) * n));
+        HIP_OR_FAIL(&e, hipMalloc((void**)&e.agent_sat, sizeof(double) * n * S));
+        HIP_OR_FAIL(&e, hipMemcpy(e.agent_c, cg.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        HIP_OR_FAIL(&e, hipMemcpy(e.agent_sat, sg.data(), sizeof(double) * n * S, hipMemcpyHostToDevice));
+        e.agent_rates = true;
+        e.agent_dt = dt;
+    }
     e.n_agents = n;
     e.n_agent_voxels = ng;
     return BFVM_OK;
@@ -436,6 +457,7 @@ int bfvm_destroy(bfvm_handle h)
     if (h->ensemble_stream)
         return fail(h, BFVM_E_STATE, "the handle belongs to an ensemble: destroy the ensemble first");
     bfvm_env* e = h;
+    bfvm::free_tape(*e);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     dfree(e->rho);
     dfree(e->c1);
@@ -577,7 +599,7 @@ int bfvm_set_agent_rates(bfvm_handle h, int n, const double* position, const dou
         return fail(h, BFVM_E_ARG, "agent arrays must be non-NULL");
     const int S = h->S;
     std::vector<int> vox(n);
-    std::vector<double> t1((size_t)n * S), t2((size_t)n * S);
+    std::vector<double> t1((size_t)n * S), t2((size_t)n * S), cv(n);
     const double ext[3] = {h->nx * h->dx, h->ny * h->dy, h->nz * h->dz};
     const double edge[3] = {h->dx, h->dy, h->dz};
     const int cnt[3] = {h->nx, h->ny, h->nz};
@@ -594,10 +616,11 @@ int bfvm_set_agent_rates(bfvm_handle h, int n, const double* position, const dou
         vox[a] = (ijk[2] * h->ny + ijk[1]) * h->nx + ijk[0];
         agent_constants(S, dt, dV, volume[a], secretion + (size_t)a * S, saturation + (size_t)a * S,
                         uptake + (size_t)a * S, &t1[(size_t)a * S], &t2[(size_t)a * S]);
+        cv[a] = dt * volume[a] / dV;  // as agent_constants
     }
     if (out_voxel) std::copy(vox.begin(), vox.end(), out_voxel);
     HIP_OR_FAIL(h, quiesce(*h));
-    return set_agents_impl(*h, n, vox.data(), t1.data(), t2.data());
+    return set_agents_impl(*h, n, vox.data(), t1.data(), t2.data(), cv.data(), saturation, dt);
 }
 
 int bfvm_step(bfvm_handle h, double dt, int flags)

@@ -138,6 +138,48 @@ class DeviceMicroenvironment:
             out.append((a.value, b.value, c.value))
         return out
 
+    # ---- differentiation (adjoint) -------------------------------------
+    def tape_record(self, dt, steps, flags=ALL, checkpoint_every=50):
+        """Runs `steps` steps from the current store, recording checkpoints."""
+        self._check(self.lib.bfvm_tape_record(self.h, float(dt), int(steps), int(flags),
+                                              int(checkpoint_every)))
+
+    def tape_seed(self, seed):
+        """Sets dJ/drho_N (nvox*S values)."""
+        seed = _f64(seed)
+        assert seed.size == self.nvox * self.S
+        self._check(self.lib.bfvm_tape_seed(self.h, _dptr(seed)))
+
+    def tape_seed_l2(self, target):
+        """J = sum (rho_N - target)^2 / 2 (FP64); the seed becomes rho_N - target. Returns J."""
+        target = _f64(target)
+        assert target.size == self.nvox * self.S
+        loss = ctypes.c_double(0)
+        self._check(self.lib.bfvm_tape_seed_l2(self.h, _dptr(target), ctypes.byref(loss)))
+        return loss.value
+
+    def tape_backward(self, n_agents=0, rates=True):
+        """Runs the adjoint; returns dict of gradients (grad_rho0, grad_D,
+        grad_lambda and, with agents set from rates, grad_secretion /
+        grad_uptake of shape (n_agents, S))."""
+        g0 = np.empty(self.nvox * self.S)
+        gD = np.empty(self.S)
+        gL = np.empty(self.S)
+        gs = np.empty(max(n_agents, 1) * self.S)
+        gu = np.empty(max(n_agents, 1) * self.S)
+        want = rates and n_agents > 0
+        self._check(self.lib.bfvm_tape_backward(self.h, _dptr(g0), _dptr(gD), _dptr(gL),
+                                                _dptr(gs) if want else None,
+                                                _dptr(gu) if want else None))
+        out = dict(grad_rho0=g0, grad_D=gD, grad_lambda=gL)
+        if want:
+            out["grad_secretion"] = gs.reshape(n_agents, self.S)
+            out["grad_uptake"] = gu.reshape(n_agents, self.S)
+        return out
+
+    def tape_free(self):
+        self._check(self.lib.bfvm_tape_free(self.h))
+
     def timed_run(self, dt, steps, flags=ALL, per_kernel=False):
         """Returns (ms_total, {kernel: (ms_sum, launches)} or None)."""
         ms = ctypes.c_double(0)

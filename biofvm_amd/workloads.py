@@ -156,3 +156,31 @@ def ensemble_mixed(seed=5, steps=3):
                 case["dir_mask"] = (np.arange(S) % 2 == 0).astype(np.uint8)
         out.append(case)
     return out
+
+
+def differentiation_128(steps=500, seed=6, n=128, S=2, n_agents=20_000):
+    """128^3 x 2 substrates, 2e4 uniform agents, dt=0.01, agents + LOD each
+    step.  The case holds the hidden parameters (per-agent secretion, D,
+    lambda drawn from the seed); ``target`` is produced by running it."""
+    rng = np.random.default_rng(seed)
+    D = 10.0 ** rng.uniform(2.0, 5.0, S)
+    lam = 10.0 ** rng.uniform(-3.0, 0.0, S)
+    rho0 = rng.uniform(0.0, 1.0, n ** 3 * S)
+    case = make_case(n, n, n, S, 10.0, D, lam, 0.01, steps, CELLS | DIFFUSION, rho0)
+    case.update(random_agents(rng, n_agents, S, n, n, n, 10.0))
+    return case
+
+
+def adjoint_small(seed=7, steps=6):
+    """A small case with every phase (bulk, agents sharing voxels, masked
+    Dirichlet faces) for gradient checks against finite differences."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz, S = 9, 7, 5, 2
+    case = make_case(nx, ny, nz, S, 20.0, [3e4, 2e3], [0.1, 0.5], 0.01, steps, ALL,
+                     rng.uniform(0.0, 1.0, nx * ny * nz * S))
+    case.update(random_bulk(rng, nx * ny * nz, S))
+    case.update(random_agents(rng, 30, S, nx, ny, nz, 20.0, share_voxel=4))
+    idx = outer_face_voxels(nx, ny, nz)[::3]
+    case.update(dir_idx=idx, dir_val=rng.uniform(0.0, 1.0, idx.size * S),
+                dir_mask=np.array([1, 0], dtype=np.uint8))
+    return case

@@ -231,6 +231,38 @@ int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment);
 
 const char* bfvm_ensemble_last_error(bfvm_ensemble e);
 
+/*
+ * Differentiation: gradients of a scalar loss J(rho after `steps` steps)
+ * with respect to the initial densities, D, lambda and the agents'
+ * secretion and uptake rates, by a hand-written adjoint of the step.  No
+ * counterpart in the reference (it has no derivatives); the forward steps
+ * are exactly bfvm_step's.
+ *
+ *   bfvm_tape_record    runs `steps` steps from the current store (as
+ *                       bfvm_run) keeping the state every checkpoint_every
+ *                       steps; the store ends at rho_N.
+ *   bfvm_tape_seed      dJ/drho_N from the caller (host, nvox*S), or
+ *   bfvm_tape_seed_l2   J = sum (rho_N - target)^2 / 2, seed rho_N - target;
+ *                       *loss receives J (FP64 sum).
+ *   bfvm_tape_backward  runs the adjoint (recomputing each checkpoint
+ *                       segment) and writes the gradients: grad_rho0 (nvox*S),
+ *                       grad_D and grad_lambda (S), grad_secretion and
+ *                       grad_uptake (n_agents*S, list order; agents must have
+ *                       been set with bfvm_set_agent_rates).  Any output may be
+ *                       NULL.  The coefficients, sources and agents must be
+ *                       those the tape was recorded with.
+ *   bfvm_tape_free      releases the tape's device memory
+ *                       ((steps/K + K + 5) stores).
+ * The adjoint runs in the store's precision with FP64 arithmetic and FP64
+ * gradient sums.
+ */
+int bfvm_tape_record(bfvm_handle h, double dt, int steps, int flags, int checkpoint_every);
+int bfvm_tape_seed(bfvm_handle h, const double* seed);
+int bfvm_tape_seed_l2(bfvm_handle h, const double* target, double* loss);
+int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double* grad_lambda,
+                       double* grad_secretion, double* grad_uptake);
+int bfvm_tape_free(bfvm_handle h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,163 @@
+"""GPU adjoint (differentiation form) against finite differences and the
+transpose identity.
+
+The adjoint has no counterpart in the reference; it is pinned by
+  - central differences of the oracle's forward run (FP64, h = 1e-6
+    relative) on a small case with every phase: |g.d - FD| <= 1e-6 |FD|
+    for the FP64 store; for the FP32 store 1e-4 of sum_i |g_i d_i| (the size
+    of the terms a random direction's derivative sums, which can cancel);
+  - the transpose identity of the linear map rho_0 -> rho_N:
+    <M x, y> = <x, M^T y>  (FP64, 1e-12);
+  - central differences of the FP64 GPU forward at 32^3 x 2, 40 steps, for
+    FP32 adjoint gradients (1e-4 of sum_i |g_i d_i|);
+  - torch.autograd.gradcheck of the autograd bridge on a tiny case.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from biofvm_amd import ALL, DIFFUSION, FP32, FP64, BfvmError, DeviceMicroenvironment, setup_case
+from biofvm_amd import workloads as W
+from biofvm_amd.autograd import LodSteps, Model
+
+pytestmark = pytest.mark.gpu
+
+H = 1e-6
+
+
+def loss_of(rho, target):
+    d = rho - target
+    return 0.5 * float(d @ d)
+
+
+def with_params(case, rho0=None, D=None, lam=None, sec=None, upt=None):
+    c = dict(case)
+    if rho0 is not None:
+        c["rho0"] = rho0
+    if D is not None:
+        c["D"] = D
+    if lam is not None:
+        c["lam"] = lam
+    if sec is not None:
+        c["agent_secretion"] = sec
+    if upt is not None:
+        c["agent_uptake"] = upt
+    return c
+
+
+def adjoint_grads(case, target, precision, checkpoint_every=2):
+    with setup_case(case, precision) as m:
+        m.tape_record(case["dt"], case["steps"], case["phases"], checkpoint_every)
+        loss = m.tape_seed_l2(target)
+        g = m.tape_backward(len(case["agent_volume"]))
+    return loss, g
+
+
+@pytest.fixture(scope="module")
+def small():
+    case = W.adjoint_small()
+    rng = np.random.default_rng(11)
+    target = oracle.oracle_run(case) + rng.normal(0, 0.05, case["rho0"].size)
+    return case, target
+
+
+GROUPS = [("rho0", "grad_rho0", "rho0"), ("D", "grad_D", "D"), ("lam", "grad_lambda", "lam"),
+          ("sec", "grad_secretion", "agent_secretion"), ("upt", "grad_uptake", "agent_uptake")]
+
+
+@pytest.mark.parametrize("precision,tol", [(FP64, 1e-6), (FP32, 1e-4)], ids=["fp64", "fp32"])
+def test_gradients_against_oracle_central_differences(small, precision, tol):
+    case, target = small
+    loss, g = adjoint_grads(case, target, precision)
+    assert abs(loss - loss_of(oracle.oracle_run(case), target)) <= 1e-5 * loss
+    rng = np.random.default_rng(5)
+    for key, gname, ckey in GROUPS:
+        p = np.asarray(case[ckey], dtype=np.float64)
+        for _ in range(3):
+            d = rng.normal(size=p.shape) * p  # relative direction
+            jp = loss_of(oracle.oracle_run(with_params(case, **{key: p + H * d})), target)
+            jm = loss_of(oracle.oracle_run(with_params(case, **{key: p - H * d})), target)
+            fd = (jp - jm) / (2 * H)
+            terms = g[gname].reshape(-1) * d.reshape(-1)
+            ad = float(np.sum(terms))
+            scale = abs(fd) if precision == FP64 else float(np.sum(np.abs(terms)))
+            assert abs(ad - fd) <= tol * scale + 1e-12, (key, ad, fd, scale)
+
+
+def test_transpose_identity():
+    # rho_0 -> rho_N is linear without sources: <M x, y> = <x, M^T y>
+    rng = np.random.default_rng(3)
+    case = W.make_case(11, 8, 6, 2, 10.0, [4e4, 1e3], [0.2, 0.01], 0.01, 5, DIFFUSION,
+                       rng.uniform(0, 1, 11 * 8 * 6 * 2))
+    y = rng.normal(size=case["rho0"].size)
+    with setup_case(case, FP64) as m:
+        m.tape_record(case["dt"], case["steps"], case["phases"], 2)
+        mx = m.download()
+        m.tape_seed(y)
+        mty = m.tape_backward()["grad_rho0"]
+    lhs, rhs = float(mx @ y), float(case["rho0"] @ mty)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_checkpoint_interval_does_not_change_gradients(small):
+    case, target = small
+    ref = adjoint_grads(case, target, FP64, checkpoint_every=1)[1]
+    for K in (2, 4, case["steps"]):
+        g = adjoint_grads(case, target, FP64, checkpoint_every=K)[1]
+        for name in ref:
+            np.testing.assert_array_equal(g[name], ref[name], err_msg=f"K={K} {name}")
+
+
+def test_fp32_adjoint_against_fp64_gpu_differences():
+    case = W.differentiation_128(steps=40, n=32, n_agents=300)
+    rng = np.random.default_rng(9)
+    with setup_case(case, FP64) as m:
+        m.run(case["dt"], case["steps"], case["phases"])
+        target = m.download() * (1 + 0.05 * rng.normal(size=case["rho0"].size))
+    loss, g = adjoint_grads(case, target, FP32, checkpoint_every=10)
+
+    def J(c):
+        with setup_case(c, FP64) as m:
+            m.run(c["dt"], c["steps"], c["phases"])
+            return loss_of(m.download(), target)
+
+    for key, gname, ckey in GROUPS:
+        p = np.asarray(case[ckey], dtype=np.float64)
+        for _ in range(2):
+            d = rng.normal(size=p.shape) * p
+            fd = (J(with_params(case, **{key: p + H * d})) - J(with_params(case, **{key: p - H * d}))) / (2 * H)
+            terms = g[gname].reshape(-1) * d.reshape(-1)
+            ad = float(np.sum(terms))
+            assert abs(ad - fd) <= 1e-4 * float(np.sum(np.abs(terms))), (key, ad, fd)
+
+
+def test_autograd_gradcheck_tiny():
+    rng = np.random.default_rng(2)
+    case = W.make_case(4, 3, 2, 1, 20.0, [2e3], [0.3], 0.01, 3, ALL, rng.uniform(0, 1, 24))
+    case.update(W.random_bulk(rng, 24, 1))
+    case.update(W.random_agents(rng, 2, 1, 4, 3, 2, 20.0))
+    with Model(case, FP64, checkpoint_every=2) as model:
+        args = [t.clone().requires_grad_() for t in model.params()]
+
+        def f(rho0, D, lam, sec, upt):
+            return LodSteps.apply(rho0, D, lam, sec, upt, model)
+
+        assert torch.autograd.gradcheck(f, tuple(args), eps=1e-6, atol=1e-9, rtol=1e-6)
+
+
+def test_tape_errors(small):
+    case, target = small
+    with setup_case(case, FP64) as m:
+        with pytest.raises(BfvmError):
+            m.tape_backward(len(case["agent_volume"]))  # no tape
+        m.tape_record(case["dt"], 2, case["phases"], 1)
+        with pytest.raises(BfvmError):
+            m.tape_backward(len(case["agent_volume"]))  # no seed
+        m.tape_seed_l2(target)
+        m.set_coefficients(case["D"] * 2, case["lam"])
+        with pytest.raises(BfvmError):
+            m.tape_backward(len(case["agent_volume"]))  # coefficients changed
+        with pytest.raises(BfvmError):
+            m.tape_record(case["dt"] * 2, 2, case["phases"], 1)  # dt differs from the agents' dt
+        m.tape_free()
