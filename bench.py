@@ -1,6 +1,7 @@
 """Benchmark of the diffusion-decay step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload precision_400|drop_in_200|lod_50|ensemble_64]
+    python bench.py [--steps K] [--warmup W]
+                    [--workload precision_400|drop_in_200|lod_50|ensemble_64|differentiation_128]
 
 Default workload (the configuration the headline metric is quoted on):
 ``precision_400`` — 400^3 voxels x 8 substrates (dx=10 um, dt=0.01 min), D
@@ -19,7 +20,8 @@ source, oracle/_ref) on a slab of the same workload on this host's cores.
 ``--workload ensemble_64`` times the batching form instead: 64 independent
 uneven grids (FP32 stores) advanced together, one step = one HIP graph replay
 of batched agent and sweep launches; its roofline prices the three batched
-sweeps of a diffusion-only step.
+sweeps of a diffusion-only step.  ``--workload differentiation_128`` times the
+adjoint (see differentiation_bench).
 """
 import argparse
 import json
@@ -77,6 +79,13 @@ def measured_traffic(workload, kernel_name):
     return t["traffic_bytes_per_launch"] if t.get("kernel") == kernel_name else None
 
 
+def lod_only(per, nvs, w):
+    """The three sweeps alone (HIP-event launch times): updates/s and HBM fraction."""
+    ms = sum(per[k][0] / per[k][1] for k in ("sweep_x", "sweep_y", "sweep_z") if per[k][1])
+    return {"value": nvs / (ms * 1e-3), "ms_per_step": ms,
+            "hbm_frac": 6 * nvs * w / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+
+
 def time_precision(case, precision, steps, warmup):
     from biofvm_amd import setup_case
     m = setup_case(case, precision)
@@ -94,6 +103,34 @@ def time_precision(case, precision, steps, warmup):
     m.sync()
     total_steps = warmup + steps + prof_steps
     return m, wall, ev_ms, per, total_steps
+
+
+def error_curve(case, total, every):
+    """FP32 store against the FP64 store on the same inputs, from step 0, every
+    `every` steps up to `total`: max normwise, max elementwise (values above
+    1e-3 of the largest) and L2 relative errors."""
+    from biofvm_amd import FP32, FP64, setup_case
+    dt, ph = case["dt"], case["phases"]
+    m32 = setup_case(case, FP32)
+    m64 = setup_case(case, FP64)
+    m32.autotune(dt)
+    m64.autotune(dt)
+    curve = []
+    for n in range(every, total + 1, every):
+        m32.run(dt, every, ph)
+        m64.run(dt, every, ph)
+        a, b = m32.download(), m64.download()
+        diff = np.abs(a - b)
+        scale = np.abs(b)
+        big = scale > 1e-3 * scale.max()
+        curve.append({"step": n, "max_rel_normwise": float(diff.max() / scale.max()),
+                      "max_rel_elementwise": float((diff[big] / scale[big]).max()),
+                      "l2_rel": float(np.linalg.norm(a - b) / np.linalg.norm(b))})
+        del a, b, diff, scale, big
+        print(f"[bench] error curve step {n}: {curve[-1]['max_rel_normwise']:.3e}", file=sys.stderr, flush=True)
+    m32.close()
+    m64.close()
+    return curve
 
 
 def cpu_baseline(case, workload):
@@ -227,15 +264,101 @@ def ensemble_bench(args):
     print(json.dumps(line), flush=True)
 
 
+def differentiation_bench(args):
+    """Adjoint of 500 steps at 128^3 x 2 with 2e4 agents (FP32 store,
+    checkpoints every 50 steps).  The target fields come from the hidden
+    parameters (FP64 run); the gradient is taken at the start of a fit (D,
+    lambda and secretion rates off by a factor exp(U[-0.3, 0.3])).  Times the
+    recorded forward and the backward pass, and checks the gradient against
+    FP64 central differences (h = 1e-6 relative) on 32 random directions in
+    the joint parameter space (initial field, D, lambda, secretion, uptake).
+    Per-group checks of D and lambda at h = 1e-6 and 1e-4 are reported too:
+    their directional derivatives are small next to J, so h = 1e-6 differences
+    carry J's rounding noise (~1e-13 J / h)."""
+    from biofvm_amd import FP32, FP64, setup_case
+    from biofvm_amd import workloads as W
+    hidden = W.differentiation_128()
+    nvs = hidden["nx"] * hidden["ny"] * hidden["nz"] * hidden["S"]
+    dt, ph, steps = hidden["dt"], hidden["phases"], hidden["steps"]
+    n_agents = len(hidden["agent_volume"])
+    rng = np.random.default_rng(12)
+    with setup_case(hidden, FP64) as m:
+        m.run(dt, steps, ph)
+        target = m.download()
+    case = dict(hidden)
+    case["D"] = hidden["D"] * np.exp(rng.uniform(-0.3, 0.3, hidden["S"]))
+    case["lam"] = hidden["lam"] * np.exp(rng.uniform(-0.3, 0.3, hidden["S"]))
+    case["agent_secretion"] = hidden["agent_secretion"] * np.exp(
+        rng.uniform(-0.3, 0.3, hidden["agent_secretion"].shape))
+    m = setup_case(case, FP32)
+    m.autotune(dt)
+    m.sync()
+    t0 = time.perf_counter()
+    m.tape_record(dt, steps, ph, 50)
+    m.sync()
+    t_fwd = time.perf_counter() - t0
+    loss = m.tape_seed_l2(target)
+    t0 = time.perf_counter()
+    g = m.tape_backward(n_agents)
+    t_bwd = time.perf_counter() - t0
+    m.close()
+    out = {"forward_s": t_fwd, "backward_s": t_bwd, "loss": loss}
+    if not args.no_fp64:
+        keys = ["rho0", "D", "lam", "agent_secretion", "agent_uptake"]
+        gk = {"rho0": "grad_rho0", "D": "grad_D", "lam": "grad_lambda", "agent_secretion": "grad_secretion",
+              "agent_uptake": "grad_uptake"}
+
+        def J(ds, h):
+            c = dict(case)
+            for k, d in ds.items():
+                c[k] = np.asarray(case[k]) + h * d
+            with setup_case(c, FP64) as mm:
+                mm.run(dt, steps, ph)
+                r = mm.download() - target
+                return 0.5 * float(r @ r)
+
+        def check(ks, h):
+            ds = {k: rng.normal(size=np.shape(case[k])) * np.asarray(case[k]) for k in ks}
+            fd = (J(ds, h) - J(ds, -h)) / (2 * h)
+            terms = np.concatenate([g[gk[k]].reshape(-1) * ds[k].reshape(-1) for k in ks])
+            ad = float(terms.sum())
+            return abs(ad - fd) / abs(fd), abs(ad - fd) / float(np.abs(terms).sum())
+
+        joint = [check(keys, 1e-6) for _ in range(32)]
+        out["fd_check"] = {"directions": 32, "h": 1e-6, "space": "joint (rho0, D, lambda, secretion, uptake)",
+                           "max_rel_err": max(a for a, _ in joint),
+                           "max_rel_err_vs_term_scale": max(b for _, b in joint)}
+        dl = {h: [check(["D", "lam"], h) for _ in range(4)] for h in (1e-6, 1e-4)}
+        out["fd_check_D_lambda"] = {str(h): {"max_rel_err": max(a for a, _ in v),
+                                             "max_rel_err_vs_term_scale": max(b for _, b in v)}
+                                    for h, v in dl.items()}
+    line = {
+        "metric": "voxel-substrate adjoint step updates/s (backward pass incl. checkpoint recomputation)",
+        "value": nvs * steps / t_bwd,
+        "unit": "voxel-substrate updates/s",
+        "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": 1e3 * t_bwd / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded numpy default_rng; see biofvm_amd/workloads.py)",
+        "config": {"workload": f"differentiation_128: {case['nx']}^3 voxels x {case['S']} substrates, "
+                               f"{n_agents} agents, {steps} steps, checkpoints every 50"},
+    }
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="precision_400",
-                    choices=["precision_400", "drop_in_200", "lod_50", "ensemble_64"])
+                    choices=["precision_400", "drop_in_200", "lod_50", "ensemble_64",
+                             "differentiation_128"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=1000,
+                    help="FP32-vs-FP64 error curve every 100 steps up to this step (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device hipMemcpy of MIB MiB (a plain copy "
                          "kernel of known size, to scale HBM counters) and exit")
@@ -253,6 +376,9 @@ def main():
     _capi.load()
     if args.workload == "ensemble_64":
         ensemble_bench(args)
+        return
+    if args.workload == "differentiation_128":
+        differentiation_bench(args)
         return
 
     case = getattr(W, args.workload)()
@@ -290,6 +416,7 @@ def main():
         scale = np.abs(b)
         big = scale > 1e-3 * scale.max()
         out_extra["fp64"] = {"value": nvs * steps / wall64, "ms_per_step": 1e3 * wall64 / steps,
+                             "lod_only": lod_only(per64, nvs, 8),
                              "hbm_frac_step": algorithmic_bytes(case, 8) / (wall64 / steps) / 1e9 / HBM_PEAK_GBS}
         out_extra["fp32_speedup_vs_fp64"] = wall64 / wall
         out_extra["error_vs_fp64"] = {
@@ -298,6 +425,8 @@ def main():
             "l2_rel": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
         del a, b
     m32.close()
+    if main_prec == FP32 and not args.no_fp64 and args.error_steps > 0:
+        out_extra["error_vs_fp64_curve"] = error_curve(case, args.error_steps, 100)
 
     line = {
         "metric": "voxel-substrate updates/s (full step: agents + LOD diffusion-decay)",
@@ -322,6 +451,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg},
         "kernels_ms_per_launch": {k: (v[0] / v[1] if v[1] else None) for k, v in per.items()},
+        "lod_only": lod_only(per, nvs, w_main),
         "sweep_plans_lines_nodes_streaming": {"x": plans[0], "y": plans[1], "z": plans[2]},
     }
     line.update(out_extra)

@@ -21,9 +21,14 @@
 // starts from its checkpoint into a ring of K states, then for each step in
 // reverse recomputes the step once more keeping the states the transposes
 // need (after bulk, after each sweep) and applies the transposed step.
-// Device memory: (steps/K + K + 5) stores, plus FP64 gradients.
+// Device memory: (steps/K + K + 4) stores, an FP64 adjoint state, FP64 gradients.
 // Reductions (gradients of D, lambda, the loss) are FP64 and deterministic:
 // fixed-size partial sums per block, then one block adds them in order.
+// The adjoint state is FP64 for either store precision: the adjoint of a
+// rough misfit decays fast under the transposed sweeps, and an FP32 copy
+// keeps the rounding of its early, large values (an FP32 adjoint state
+// moved the lambda gradient by 5e-4 relative after 500 steps at 128^3).  The
+// recomputed forward states stay in the store's precision.
 #include <cstring>
 #include <string>
 #include <vector>
@@ -45,7 +50,7 @@ struct Tape {
     void* s4 = nullptr;       // after the x sweep
     void* s6 = nullptr;       // after the y sweep
     void* s8 = nullptr;       // after the z sweep (the step's result)
-    void* lam = nullptr;      // adjoint state
+    double* lam = nullptr;    // adjoint state (FP64)
     bool seeded = false;
     double* rk = nullptr;      // agents: intermediate densities (n_agents x S)
     double* gsec = nullptr;    // agents: secretion gradient (grouped order)
@@ -106,7 +111,7 @@ __global__ void k_bulk_adjoint(T* __restrict__ lam, const double* __restrict__ B
 // list order from the phase input x (intermediate values kept in rk), then
 // walked back in reverse order.
 template <typename T>
-__global__ void k_agents_adjoint(T* __restrict__ lam, const T* __restrict__ x, int S, int n_groups,
+__global__ void k_agents_adjoint(double* __restrict__ lam, const T* __restrict__ x, int S, int n_groups,
                                  const int* __restrict__ gvox, const int* __restrict__ gstart,
                                  const double* __restrict__ t1, const double* __restrict__ t2,
                                  const double* __restrict__ c, const double* __restrict__ sat,
@@ -137,7 +142,7 @@ __global__ void k_agents_adjoint(T* __restrict__ lam, const T* __restrict__ x, i
         }
         l *= inv;
     }
-    lam[v] = (T)l;
+    lam[v] = l;
 }
 
 // out[m] = sum over the block's threads of acc[m], m < M, in a fixed order
@@ -165,7 +170,7 @@ __device__ void block_sums(double (&acc)[M], double* __restrict__ out)
 // block's voxels of (l_v - l_w)(y_v - y_w) over edges v -> w = v + step along
 // `axis`, and of l_v y_v.  partial[block][0..kChunk) and [kChunk..2 kChunk).
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_axis_grad(const T* __restrict__ lam, const T* __restrict__ y,
+__global__ void __launch_bounds__(kBlock) k_axis_grad(const double* __restrict__ lam, const T* __restrict__ y,
                                                       int nx, int ny, int nz, int S, int axis, int s0,
                                                       int sc, double* __restrict__ partial)
 {
@@ -213,13 +218,14 @@ __global__ void k_axis_grad_finish(const double* __restrict__ partial, int nbloc
 // lam = rho - target; partial sums of (rho - target)^2 / 2.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_l2_seed(const T* __restrict__ rho, const double* __restrict__ target,
-                                                    T* __restrict__ lam, size_t nvs, double* __restrict__ partial)
+                                                    double* __restrict__ lam, size_t nvs,
+                                                    double* __restrict__ partial)
 {
     double acc[1] = {0.0};
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nvs;
          e += (size_t)gridDim.x * blockDim.x) {
         const double d = (double)rho[e] - target[e];
-        lam[e] = (T)d;
+        lam[e] = d;
         acc[0] += 0.5 * d * d;
     }
     block_sums<1>(acc, partial + blockIdx.x);
@@ -235,12 +241,11 @@ __global__ void k_sum(const double* __restrict__ partial, int n, double* __restr
 
 // ---------------------------------------------------------------- host side
 
-template <typename T>
-hipError_t dirichlet_zero(Env& e, void* lam)
+hipError_t dirichlet_zero(Env& e, double* lam)
 {
     if (e.n_dir == 0) return hipSuccess;
-    k_dirichlet_zero<T><<<grid_for((size_t)e.n_dir * e.S), kBlock, 0, e.stream>>>(
-        (T*)lam, e.S, e.n_dir, e.dir_idx, e.dir_mask);
+    k_dirichlet_zero<double><<<grid_for((size_t)e.n_dir * e.S), kBlock, 0, e.stream>>>(
+        lam, e.S, e.n_dir, e.dir_idx, e.dir_mask);
     return hipGetLastError();
 }
 
@@ -249,7 +254,7 @@ hipError_t axis_grad(Env& e, Tape& tp, const void* y, int axis)
 {
     for (int s0 = 0; s0 < e.S; s0 += kChunk) {
         const int sc = std::min(kChunk, e.S - s0);
-        k_axis_grad<T><<<kRedBlocks, kBlock, 0, e.stream>>>((const T*)tp.lam, (const T*)y, e.nx, e.ny, e.nz,
+        k_axis_grad<T><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny, e.nz,
                                                            e.S, axis, s0, sc, tp.partial);
         hipError_t st = hipGetLastError();
         if (st != hipSuccess) return st;
@@ -267,6 +272,17 @@ hipError_t on_buffer(Env& e, void* buf, F&& f)
     e.rho = buf;
     hipError_t st = f();
     e.rho = keep;
+    return st;
+}
+
+// The sweep of `axis` on the FP64 adjoint state (the FP64 kernels, whatever
+// the store's precision).
+hipError_t sweep_adjoint(Env& e, double* lam, int axis)
+{
+    const int keep = e.precision;
+    e.precision = BFVM_FP64;
+    hipError_t st = on_buffer(e, lam, [&] { return bfvm::launch_sweep(e, axis); });
+    e.precision = keep;
     return st;
 }
 
@@ -328,21 +344,21 @@ hipError_t adjoint_step(Env& e, Tape& tp)
     if (tp.flags & BFVM_DIFFUSION) {
         const void* ys[3] = {tp.s4, tp.s6, tp.s8};
         for (int axis = 2; axis >= 0; --axis) {
-            if ((st = on_buffer(e, tp.lam, [&] { return bfvm::launch_sweep(e, axis); }))) return st;
+            if ((st = sweep_adjoint(e, tp.lam, axis))) return st;
             if ((st = axis_grad<T>(e, tp, ys[axis], axis))) return st;
-            if ((st = dirichlet_zero<T>(e, tp.lam))) return st;
+            if ((st = dirichlet_zero(e, tp.lam))) return st;
         }
     }
     if ((tp.flags & BFVM_CELLS) && e.n_agent_voxels > 0) {
         const size_t n = (size_t)e.n_agent_voxels * e.S;
         k_agents_adjoint<T><<<grid_for(n), kBlock, 0, e.stream>>>(
-            (T*)tp.lam, (const T*)tp.s1, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
+            tp.lam, (const T*)tp.s1, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
             (const double*)e.agent_t1, (const double*)e.agent_t2, e.agent_rates ? e.agent_c : nullptr,
             e.agent_sat, tp.rk, tp.gsec, tp.gupt);
         if ((st = hipGetLastError())) return st;
     }
     if ((tp.flags & BFVM_BULK) && e.have_bulk) {
-        k_bulk_adjoint<T><<<grid_for(e.nvs), kBlock, 0, e.stream>>>((T*)tp.lam, (const double*)e.bulk_B, e.nvs);
+        k_bulk_adjoint<double><<<grid_for(e.nvs), kBlock, 0, e.stream>>>(tp.lam, (const double*)e.bulk_B, e.nvs);
         if ((st = hipGetLastError())) return st;
     }
     return hipSuccess;
@@ -379,27 +395,6 @@ int check_unchanged(Env& e, const Tape& tp)
         return afail(e, BFVM_E_STATE,
                      "coefficients, agents or sources changed since bfvm_tape_record");
     return BFVM_OK;
-}
-
-void host_to_store(const Env& e, std::vector<unsigned char>& buf, const double* src)
-{
-    buf.resize(e.nvs * e.elem);
-    if (e.precision == BFVM_FP64) {
-        std::memcpy(buf.data(), src, e.nvs * sizeof(double));
-    } else {
-        float* f = (float*)buf.data();
-        for (size_t i = 0; i < e.nvs; ++i) f[i] = (float)src[i];
-    }
-}
-
-void store_to_host(const Env& e, const std::vector<unsigned char>& buf, double* dst)
-{
-    if (e.precision == BFVM_FP64) {
-        std::memcpy(dst, buf.data(), e.nvs * sizeof(double));
-    } else {
-        const float* f = (const float*)buf.data();
-        for (size_t i = 0; i < e.nvs; ++i) dst[i] = f[i];
-    }
 }
 
 }  // namespace
@@ -470,11 +465,9 @@ int bfvm_tape_seed(bfvm_handle h, const double* seed)
     Tape* tp = e.tape;
     if (!tp) return afail(e, BFVM_E_STATE, "no tape (bfvm_tape_record)");
     if (!seed) return afail(e, BFVM_E_ARG, "seed is NULL");
-    if (!tp->lam) AHIP(e, alloc_store(e, &tp->lam));
-    std::vector<unsigned char> buf;
-    host_to_store(e, buf, seed);
+    if (!tp->lam) AHIP(e, hipMalloc((void**)&tp->lam, e.nvs * sizeof(double)));
     AHIP(e, hipStreamSynchronize(e.stream));
-    AHIP(e, hipMemcpy(tp->lam, buf.data(), e.nvs * e.elem, hipMemcpyHostToDevice));
+    AHIP(e, hipMemcpy(tp->lam, seed, e.nvs * sizeof(double), hipMemcpyHostToDevice));
     tp->seeded = true;
     return BFVM_OK;
 }
@@ -486,7 +479,7 @@ int bfvm_tape_seed_l2(bfvm_handle h, const double* target, double* loss)
     Tape* tp = e.tape;
     if (!tp) return afail(e, BFVM_E_STATE, "no tape (bfvm_tape_record)");
     if (!target) return afail(e, BFVM_E_ARG, "target is NULL");
-    if (!tp->lam) AHIP(e, alloc_store(e, &tp->lam));
+    if (!tp->lam) AHIP(e, hipMalloc((void**)&tp->lam, e.nvs * sizeof(double)));
     if (!tp->partial) AHIP(e, hipMalloc((void**)&tp->partial, sizeof(double) * kRedBlocks * 2 * kChunk));
     double* dtarget = nullptr;
     double* dloss = nullptr;
@@ -495,11 +488,11 @@ int bfvm_tape_seed_l2(bfvm_handle h, const double* target, double* loss)
     hipError_t st = hipMemcpyAsync(dtarget, target, e.nvs * sizeof(double), hipMemcpyHostToDevice, e.stream);
     if (st == hipSuccess) {
         if (e.precision == BFVM_FP64)
-            k_l2_seed<double><<<kRedBlocks, kBlock, 0, e.stream>>>((const double*)e.rho, dtarget,
-                                                                  (double*)tp->lam, e.nvs, tp->partial);
+            k_l2_seed<double><<<kRedBlocks, kBlock, 0, e.stream>>>((const double*)e.rho, dtarget, tp->lam,
+                                                                  e.nvs, tp->partial);
         else
-            k_l2_seed<float><<<kRedBlocks, kBlock, 0, e.stream>>>((const float*)e.rho, dtarget,
-                                                                 (float*)tp->lam, e.nvs, tp->partial);
+            k_l2_seed<float><<<kRedBlocks, kBlock, 0, e.stream>>>((const float*)e.rho, dtarget, tp->lam,
+                                                                 e.nvs, tp->partial);
         st = hipGetLastError();
     }
     if (st == hipSuccess) {
@@ -576,18 +569,14 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
     // results
     std::vector<double> gc(2 * S);
     AHIP(e, hipMemcpyAsync(gc.data(), tp->gc, sizeof(double) * 2 * S, hipMemcpyDeviceToHost, e.stream));
-    std::vector<unsigned char> buf;
-    if (grad_rho0) {
-        buf.resize(e.nvs * e.elem);
-        AHIP(e, hipMemcpyAsync(buf.data(), tp->lam, e.nvs * e.elem, hipMemcpyDeviceToHost, e.stream));
-    }
+    if (grad_rho0)
+        AHIP(e, hipMemcpyAsync(grad_rho0, tp->lam, e.nvs * sizeof(double), hipMemcpyDeviceToHost, e.stream));
     std::vector<double> gs(na), gu(na);
     if (na > 0 && (grad_secretion || grad_uptake)) {
         AHIP(e, hipMemcpyAsync(gs.data(), tp->gsec, sizeof(double) * na, hipMemcpyDeviceToHost, e.stream));
         AHIP(e, hipMemcpyAsync(gu.data(), tp->gupt, sizeof(double) * na, hipMemcpyDeviceToHost, e.stream));
     }
     AHIP(e, hipStreamSynchronize(e.stream));
-    if (grad_rho0) store_to_host(e, buf, grad_rho0);
     // c1 = dt D / dx^2 (BioFVM_solvers.cpp:95), c2 = dt lambda / 3 (:96)
     for (int s = 0; s < S; ++s) {
         if (grad_D) grad_D[s] = gc[s] * dt / (e.dx * e.dx);

@@ -252,9 +252,9 @@ const char* bfvm_ensemble_last_error(bfvm_ensemble e);
  *                       NULL.  The coefficients, sources and agents must be
  *                       those the tape was recorded with.
  *   bfvm_tape_free      releases the tape's device memory
- *                       ((steps/K + K + 5) stores).
- * The adjoint runs in the store's precision with FP64 arithmetic and FP64
- * gradient sums.
+ *                       ((steps/K + K + 4) stores and an FP64 adjoint state).
+ * The recomputed forward states are in the store's precision; the adjoint
+ * state, all arithmetic and the gradient sums are FP64.
  */
 int bfvm_tape_record(bfvm_handle h, double dt, int steps, int flags, int checkpoint_every);
 int bfvm_tape_seed(bfvm_handle h, const double* seed);
