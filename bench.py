@@ -55,7 +55,8 @@ def algorithmic_bytes(case, w):
 
 
 def kernel_bytes(case, kernel, w):
-    """Algorithmic bytes of one launch of `kernel`."""
+    """Algorithmic bytes of one step's launches of `kernel` (for the sweeps: one
+    whole sweep, whether it runs as one launch or plane chunk by chunk)."""
     nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
     if kernel.startswith("sweep"):
         return 2 * nvs * w
@@ -80,8 +81,8 @@ def measured_traffic(workload, kernel_name):
 
 
 def lod_only(per, nvs, w):
-    """The three sweeps alone (HIP-event launch times): updates/s and HBM fraction."""
-    ms = sum(per[k][0] / per[k][1] for k in ("sweep_x", "sweep_y", "sweep_z") if per[k][1])
+    """The three sweeps alone (HIP-event launch times per step): updates/s and HBM fraction."""
+    ms = sum(per[k][0] for k in ("sweep_x", "sweep_y", "sweep_z"))
     return {"value": nvs / (ms * 1e-3), "ms_per_step": ms,
             "hbm_frac": 6 * nvs * w / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
@@ -100,6 +101,8 @@ def time_precision(case, precision, steps, warmup):
     # a short second pass with an event pair around every launch
     prof_steps = max(1, min(steps, 10))
     _, per = m.timed_run(dt, prof_steps, ph, per_kernel=True)
+    # per kernel: (ms per step, launches per step)
+    per = {k: (v[0] / prof_steps, v[1] / prof_steps) for k, v in per.items()}
     m.sync()
     total_steps = warmup + steps + prof_steps
     return m, wall, ev_ms, per, total_steps
@@ -389,14 +392,15 @@ def main():
     w_main = 4 if main_prec == FP32 else 8
 
     m32, wall, ev_ms, per, n_done = time_precision(case, main_prec, steps, warmup)
+    m32_chunk = m32.sweep_chunk
     plans = m32.sweep_plans()
     value = nvs * steps / wall
     step_ms = 1e3 * wall / steps
     dom = max(per, key=lambda k: per[k][0])
-    dom_ms, dom_n = per[dom]
-    dom_avg = dom_ms / dom_n
+    dom_ms, dom_n = per[dom]  # per step
+    dom_avg = dom_ms / dom_n  # per launch
     dom_bytes = kernel_bytes(case, dom, w_main)
-    achieved = dom_bytes / (dom_avg * 1e-3) / 1e9
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9  # one sweep's bytes over its time per step
     traffic = None
     if dom.startswith("sweep"):
         cols, L, streaming = plans["xyz".index(dom[-1])]
@@ -449,8 +453,11 @@ def main():
         "hbm_frac_step": step_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_avg},
-        "kernels_ms_per_launch": {k: (v[0] / v[1] if v[1] else None) for k, v in per.items()},
+                     "bytes_per_step": dom_bytes, "ms_per_step": dom_ms, "launches_per_step": dom_n,
+                     "avg_launch_ms": dom_avg},
+        "kernels_ms_per_step": {k: v[0] for k, v in per.items()},
+        "kernel_launches_per_step": {k: v[1] for k, v in per.items()},
+        "sweep_xy_chunk_planes": m32_chunk,
         "lod_only": lod_only(per, nvs, w_main),
         "sweep_plans_lines_nodes_streaming": {"x": plans[0], "y": plans[1], "z": plans[2]},
     }

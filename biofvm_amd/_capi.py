@@ -42,6 +42,11 @@ SIGNATURES = {
     "bfvm_autotune": (_i, [_h, _d]),
     "bfvm_get_sweep_plan": (_i, [_h, _i, _ip, _ip, _ip]),
     "bfvm_set_sweep_plan": (_i, [_h, _i, _i, _i, _i]),
+    "bfvm_compute_gradients": (_i, [_h]),
+    "bfvm_download_gradients": (_i, [_h, _dp]),
+    "bfvm_gradient_device_ptr": (_i, [_h, ctypes.POINTER(_vp)]),
+    "bfvm_get_sweep_chunk": (_i, [_h, _ip]),
+    "bfvm_set_sweep_chunk": (_i, [_h, _i]),
     "bfvm_get_info": (_i, [_h, _ip, _ip, _ip, _ip, _ip, _sp]),
     "bfvm_density_device_ptr": (_i, [_h, ctypes.POINTER(_vp)]),
     "bfvm_last_error": (ctypes.c_char_p, [_h]),

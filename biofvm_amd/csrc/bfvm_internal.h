@@ -84,6 +84,10 @@ struct Env {
     void* c1 = nullptr;  // dt*D/dx^2 per substrate (FP64)
     AxisCoef ax[3];
     SweepPlan plan[3];  // per-axis launch shape chosen by the autotuner
+    // x and y sweeps run in chunks of this many z planes (x then y per chunk),
+    // so the y sweep reads the x sweep's output from the last-level cache;
+    // 0 = whole-store sweeps.  Chosen by the autotuner.
+    int xy_chunk = 0;
 
     // Dirichlet nodes.
     int n_dir = 0;
@@ -123,6 +127,9 @@ struct Env {
     // Recorded steps for the adjoint (adjoint.hip).
     struct Tape* tape = nullptr;
 
+    // Gradient vectors (nvs*3, store precision), allocated on first use.
+    void* grad = nullptr;
+
     // Per-launch timing (bfvm_timed_run with per-kernel events).
     bool kernel_timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -138,7 +145,13 @@ hipError_t launch_dirichlet(Env& e);
 hipError_t launch_bulk_prepare(Env& e, double dt);
 hipError_t launch_bulk(Env& e);
 hipError_t launch_agents(Env& e);
+// Gradient vectors of the store into out (nvs*3 values, store precision).
+hipError_t launch_gradient(Env& e, void* out);
 hipError_t launch_sweep(Env& e, int axis);
+// x or y sweep of planes (z index) [k0, k1) only, for plane-chunked steps;
+// valid when sweep_chunkable(e, axis) (the segmented kernels run the axis).
+bool sweep_chunkable(const Env& e, int axis);
+hipError_t launch_sweep_planes(Env& e, int axis, int k0, int k1);
 // Ensemble sweeps: whether a member's axis can join a launch with L-node
 // segments, its table entry, its dynamic LDS bytes, and the launch itself
 // (kMaxThreads threads per block, `blocks` blocks in total).

@@ -295,10 +295,29 @@ int step_impl(Env& e, double dt, int flags)
                         "This algorithm is written for uniform Cartesian meshes (dx == dy == dz).");
         if (e.D.empty()) return fail(&e, BFVM_E_STATE, "coefficients not set (bfvm_set_coefficients)");
         if ((rc = ensure_coefficients_impl(e, dt))) return rc;
-        for (int axis = 0; axis < 3; ++axis) {
-            if (e.n_dir > 0)
-                if ((rc = run_kernel(e, BFVM_K_DIRICHLET, [&] { return bfvm::launch_dirichlet(e); })))
-                    return rc;
+        auto dirichlet = [&] {
+            return e.n_dir > 0 ? run_kernel(e, BFVM_K_DIRICHLET, [&] { return bfvm::launch_dirichlet(e); })
+                               : BFVM_OK;
+        };
+        const int chunk = e.xy_chunk;
+        int axis0 = 0;
+        if (chunk > 0 && chunk < e.nz && bfvm::sweep_chunkable(e, 0) && bfvm::sweep_chunkable(e, 1)) {
+            // x then y per chunk of z planes.  Every Dirichlet launch writes all
+            // nodes; the overwrite is idempotent, so each plane still sees it
+            // before its x, y and z sweeps and never in the middle of one.
+            for (int k0 = 0; k0 < e.nz; k0 += chunk) {
+                const int k1 = std::min(e.nz, k0 + chunk);
+                for (int axis = 0; axis < 2; ++axis) {
+                    if ((rc = dirichlet())) return rc;
+                    if ((rc = run_kernel(e, BFVM_K_SWEEP_X + axis,
+                                         [&] { return bfvm::launch_sweep_planes(e, axis, k0, k1); })))
+                        return rc;
+                }
+            }
+            axis0 = 2;
+        }
+        for (int axis = axis0; axis < 3; ++axis) {
+            if ((rc = dirichlet())) return rc;
             if ((rc = run_kernel(e, BFVM_K_SWEEP_X + axis, [&] { return bfvm::launch_sweep(e, axis); })))
                 return rc;
         }
@@ -476,6 +495,7 @@ int bfvm_destroy(bfvm_handle h)
     dfree_t(e->bulk_uptake);
     dfree(e->bulk_A);
     dfree(e->bulk_B);
+    dfree(e->grad);
     free_agents(*e);
     for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -733,6 +753,30 @@ int bfvm_autotune(bfvm_handle h, double dt)
         }
         e.plan[axis] = best;
     }
+    // plane chunking of the x and y sweeps: whole diffusion phases, timed
+    if (st == hipSuccess && bfvm::sweep_chunkable(e, 0) && bfvm::sweep_chunkable(e, 1)) {
+        int best_chunk = 0;
+        float best_ms = 1e30f;
+        for (int chunk : {0, 4, 8, 16, 32, 64}) {
+            if (chunk >= e.nz) break;
+            e.xy_chunk = chunk;
+            int rc2 = step_impl(e, dt, BFVM_DIFFUSION);  // warm-up
+            (void)hipEventRecord(t0, e.stream);
+            for (int rep = 0; rep < 2 && rc2 == BFVM_OK; ++rep) rc2 = step_impl(e, dt, BFVM_DIFFUSION);
+            (void)hipEventRecord(t1, e.stream);
+            if (rc2 != BFVM_OK || (st = hipEventSynchronize(t1)) != hipSuccess) {
+                if (st == hipSuccess) st = hipErrorUnknown;
+                break;
+            }
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_chunk = chunk;
+            }
+        }
+        e.xy_chunk = best_chunk;
+    }
     (void)hipStreamSynchronize(e.stream);
     e.rho = keep;
     (void)hipFree(scratch);
@@ -773,6 +817,54 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
             return BFVM_OK;
         }
     return fail(h, BFVM_E_ARG, "launch shape not valid for this axis length");
+}
+
+int bfvm_compute_gradients(bfvm_handle h)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (h->ensemble_stream) HIP_OR_FAIL(h, hipStreamSynchronize(h->ensemble_stream));
+    if (!h->grad) HIP_OR_FAIL(h, hipMalloc(&h->grad, h->nvs * 3 * h->elem));
+    HIP_OR_FAIL(h, bfvm::launch_gradient(*h, h->grad));
+    return BFVM_OK;
+}
+
+int bfvm_download_gradients(bfvm_handle h, double* host)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (!host) return fail(h, BFVM_E_ARG, "host is NULL");
+    if (!h->grad) return fail(h, BFVM_E_STATE, "no gradients computed (bfvm_compute_gradients)");
+    HIP_OR_FAIL(h, quiesce(*h));
+    const size_t n = h->nvs * 3;
+    if (h->precision == BFVM_FP64) {
+        HIP_OR_FAIL(h, hipMemcpy(host, h->grad, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        HIP_OR_FAIL(h, hipMemcpy(tmp.data(), h->grad, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) host[i] = tmp[i];
+    }
+    return BFVM_OK;
+}
+
+int bfvm_gradient_device_ptr(bfvm_handle h, void** ptr)
+{
+    if (!h || !ptr) return fail(h, BFVM_E_ARG, "null argument");
+    *ptr = h->grad;
+    return BFVM_OK;
+}
+
+int bfvm_get_sweep_chunk(bfvm_handle h, int* planes)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (planes) *planes = h->xy_chunk;
+    return BFVM_OK;
+}
+
+int bfvm_set_sweep_chunk(bfvm_handle h, int planes)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (planes < 0) return fail(h, BFVM_E_ARG, "planes < 0");
+    h->xy_chunk = planes;
+    return BFVM_OK;
 }
 
 int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, int* precision,

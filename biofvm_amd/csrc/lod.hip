@@ -517,14 +517,16 @@ hipError_t sweep_naive(Env& e, int axis)
     return hipGetLastError();
 }
 
-// Kernel arguments of one axis of one grid, `cols` lines per block.
-SweepArgs sweep_args(const Env& e, int axis, int cols, int nseg)
+// Kernel arguments of one axis of one grid, `cols` lines per block, planes
+// [k0, k1) (k1 < 0: all planes).
+SweepArgs sweep_args(const Env& e, int axis, int cols, int nseg, int k0 = 0, int k1 = -1)
 {
     Geometry gm = geometry(e, axis);
     const AxisCoef& a = e.ax[axis];
+    if (k1 < 0) k1 = gm.nplanes;
     SweepArgs A;
-    A.rho = e.rho;
-    A.nvs = e.nvs;
+    A.rho = (char*)e.rho + (size_t)k0 * gm.plane_stride * e.elem;
+    A.nvs = e.nvs - (size_t)k0 * gm.plane_stride;
     A.plane_stride = gm.plane_stride;
     A.wb = (const double*)a.wb;
     A.g0 = (const double*)a.g0;
@@ -539,7 +541,7 @@ SweepArgs sweep_args(const Env& e, int axis, int cols, int nseg)
     A.vstride = (unsigned)gm.vstride;
     A.cols = cols;
     A.tiles = (gm.ncols + cols - 1) / cols;
-    A.planes = gm.nplanes;
+    A.planes = k1 - k0;
     return A;
 }
 
@@ -549,25 +551,25 @@ size_t sweep_lds(const SweepArgs& A)
 }
 
 template <typename T, int L, int COLS>
-hipError_t launch_seg(Env& e, int axis, const Geometry& gm, int nseg)
+hipError_t launch_seg(Env& e, int axis, int nseg, int k0, int k1)
 {
-    const SweepArgs A = sweep_args(e, axis, COLS, nseg);
-    dim3 grid(A.tiles, gm.nplanes);
+    const SweepArgs A = sweep_args(e, axis, COLS, nseg, k0, k1);
+    dim3 grid(A.tiles, A.planes);
     int threads = (COLS * nseg + 63) / 64 * 64;
     k_sweep<T, L, COLS><<<grid, threads, sweep_lds(A), e.stream>>>(A);
     return hipGetLastError();
 }
 
 template <typename T, int COLS>
-hipError_t launch_cols(Env& e, int axis, const Geometry& gm, int L, int nseg)
+hipError_t launch_cols(Env& e, int axis, int L, int nseg, int k0, int k1)
 {
     switch (L) {
-    case 4: return launch_seg<T, 4, COLS>(e, axis, gm, nseg);
-    case 8: return launch_seg<T, 8, COLS>(e, axis, gm, nseg);
-    case 12: return launch_seg<T, 12, COLS>(e, axis, gm, nseg);
-    case 16: return launch_seg<T, 16, COLS>(e, axis, gm, nseg);
-    case 24: return launch_seg<T, 24, COLS>(e, axis, gm, nseg);
-    default: return launch_seg<T, 32, COLS>(e, axis, gm, nseg);
+    case 4: return launch_seg<T, 4, COLS>(e, axis, nseg, k0, k1);
+    case 8: return launch_seg<T, 8, COLS>(e, axis, nseg, k0, k1);
+    case 12: return launch_seg<T, 12, COLS>(e, axis, nseg, k0, k1);
+    case 16: return launch_seg<T, 16, COLS>(e, axis, nseg, k0, k1);
+    case 24: return launch_seg<T, 24, COLS>(e, axis, nseg, k0, k1);
+    default: return launch_seg<T, 32, COLS>(e, axis, nseg, k0, k1);
     }
 }
 
@@ -600,9 +602,9 @@ int compute_units(Env& e)
 }
 
 template <typename T, int L, int COLS>
-hipError_t launch_stream(Env& e, int axis, int nseg)
+hipError_t launch_stream(Env& e, int axis, int nseg, int k0, int k1)
 {
-    const SweepArgs A = sweep_args(e, axis, COLS, nseg);
+    const SweepArgs A = sweep_args(e, axis, COLS, nseg, k0, k1);
     const int threads = (COLS * nseg + 63) / 64 * 64;
     const size_t lds = sweep_lds(A);
     const long long total = (long long)A.tiles * A.planes;
@@ -613,45 +615,55 @@ hipError_t launch_stream(Env& e, int axis, int nseg)
 }
 
 template <typename T, int COLS>
-hipError_t launch_cols_stream(Env& e, int axis, int L, int nseg)
+hipError_t launch_cols_stream(Env& e, int axis, int L, int nseg, int k0, int k1)
 {
     switch (L) {
-    case 8: return launch_stream<T, 8, COLS>(e, axis, nseg);
-    case 12: return launch_stream<T, 12, COLS>(e, axis, nseg);
-    case 16: return launch_stream<T, 16, COLS>(e, axis, nseg);
+    case 8: return launch_stream<T, 8, COLS>(e, axis, nseg, k0, k1);
+    case 12: return launch_stream<T, 12, COLS>(e, axis, nseg, k0, k1);
+    case 16: return launch_stream<T, 16, COLS>(e, axis, nseg, k0, k1);
     default:
-        if constexpr (sizeof(T) == 4) return launch_stream<T, 24, COLS>(e, axis, nseg);
+        if constexpr (sizeof(T) == 4) return launch_stream<T, 24, COLS>(e, axis, nseg, k0, k1);
         return hipErrorInvalidValue;
     }
 }
 
+// Whether the segmented kernels run this axis (else the per-line fallback).
 template <typename T>
-hipError_t sweep(Env& e, int axis)
+bool segmented(const Env& e, int axis, const SweepPlan& p)
 {
     Geometry gm = geometry(e, axis);
-    SweepPlan p = plan_for(e, axis);
     // buffer offsets are 32-bit: the segmented kernel addresses at most 4 GiB
     // from a plane's start
     const size_t span = (axis == 2 ? e.nvs : (size_t)e.nx * e.ny * e.S) * sizeof(T);
     // the end-correction tables live in LDS beside the 16 KiB carry arrays
     const size_t gbytes =
         sizeof(double) * (size_t)(e.ax[axis].kg0 + e.ax[axis].kg1 + 3 * kPad + 8) * e.S;
-    if (gm.n < 2 || p.cols == 0 || !plan_valid(gm.n, p.cols, p.L) || span >= 0xffffffffull ||
-        gbytes > kMaxCorrectionLds)
-        return sweep_naive<T>(e, axis);
+    return gm.n >= 2 && p.cols != 0 && plan_valid(gm.n, p.cols, p.L) && span < 0xffffffffull &&
+           gbytes <= kMaxCorrectionLds;
+}
+
+// Sweep of planes [k0, k1) of `axis` (k1 < 0: all; the per-line fallback
+// always sweeps all planes).
+template <typename T>
+hipError_t sweep(Env& e, int axis, int k0 = 0, int k1 = -1)
+{
+    Geometry gm = geometry(e, axis);
+    SweepPlan p = plan_for(e, axis);
+    if (!segmented<T>(e, axis, p)) return sweep_naive<T>(e, axis);
+    if (k1 < 0) k1 = gm.nplanes;
     int nseg = (gm.n + p.L - 1) / p.L;
     const long long tiles = ((long long)gm.ncols + p.cols - 1) / p.cols;
-    if (p.stream && stream_nodes_ok(p.L, e.precision) && tiles * gm.nplanes < (1ll << 31)) {
+    if (p.stream && stream_nodes_ok(p.L, e.precision) && tiles * (k1 - k0) < (1ll << 31)) {
         switch (p.cols) {
-        case 16: return launch_cols_stream<T, 16>(e, axis, p.L, nseg);
-        case 32: return launch_cols_stream<T, 32>(e, axis, p.L, nseg);
-        default: return launch_cols_stream<T, 64>(e, axis, p.L, nseg);
+        case 16: return launch_cols_stream<T, 16>(e, axis, p.L, nseg, k0, k1);
+        case 32: return launch_cols_stream<T, 32>(e, axis, p.L, nseg, k0, k1);
+        default: return launch_cols_stream<T, 64>(e, axis, p.L, nseg, k0, k1);
         }
     }
     switch (p.cols) {
-    case 16: return launch_cols<T, 16>(e, axis, gm, p.L, nseg);
-    case 32: return launch_cols<T, 32>(e, axis, gm, p.L, nseg);
-    default: return launch_cols<T, 64>(e, axis, gm, p.L, nseg);
+    case 16: return launch_cols<T, 16>(e, axis, p.L, nseg, k0, k1);
+    case 32: return launch_cols<T, 32>(e, axis, p.L, nseg, k0, k1);
+    default: return launch_cols<T, 64>(e, axis, p.L, nseg, k0, k1);
     }
 }
 
@@ -736,6 +748,17 @@ hipError_t launch_sweep_batch(int precision, int L, const SweepArgs* d_args, con
 hipError_t launch_sweep(Env& e, int axis)
 {
     return e.precision == BFVM_FP64 ? sweep<double>(e, axis) : sweep<float>(e, axis);
+}
+
+bool sweep_chunkable(const Env& e, int axis)
+{
+    const SweepPlan p = plan_for(e, axis);
+    return e.precision == BFVM_FP64 ? segmented<double>(e, axis, p) : segmented<float>(e, axis, p);
+}
+
+hipError_t launch_sweep_planes(Env& e, int axis, int k0, int k1)
+{
+    return e.precision == BFVM_FP64 ? sweep<double>(e, axis, k0, k1) : sweep<float>(e, axis, k0, k1);
 }
 
 }  // namespace bfvm

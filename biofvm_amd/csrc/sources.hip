@@ -110,7 +110,47 @@ __global__ void k_agents_batch(const AgentBatch* __restrict__ tab, int nm, long 
     *r = (T)v;
 }
 
+// Microenvironment::compute_all_gradient_vectors (BioFVM_microenvironment.cpp:699-779):
+// per voxel and substrate, for each axis with the voxel strictly inside,
+// g = (rho[n+jump] - rho[n-jump]) / (2*d) in FP64; components on an axis's two
+// boundary layers are 0 (the value reset_all_gradient_vectors leaves there,
+// :826-842).  out[(n*S + s)*3 + axis].  One thread per (voxel, substrate).
+template <typename T>
+__global__ void k_gradient(const T* __restrict__ rho, T* __restrict__ out, int nx, int ny, int nz, int S,
+                           double two_dx, double two_dy, double two_dz)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nvs = (size_t)nx * ny * nz * S;
+    if (t >= nvs) return;
+    const size_t v = t / S;
+    const int i = (int)(v % nx);
+    const int j = (int)((v / nx) % ny);
+    const int k = (int)(v / ((size_t)nx * ny));
+    const size_t sx = (size_t)S, sy = (size_t)nx * S, sz = (size_t)nx * ny * S;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    if (i > 0 && i < nx - 1) gx = ((double)rho[t + sx] - (double)rho[t - sx]) / two_dx;
+    if (j > 0 && j < ny - 1) gy = ((double)rho[t + sy] - (double)rho[t - sy]) / two_dy;
+    if (k > 0 && k < nz - 1) gz = ((double)rho[t + sz] - (double)rho[t - sz]) / two_dz;
+    out[t * 3 + 0] = (T)gx;
+    out[t * 3 + 1] = (T)gy;
+    out[t * 3 + 2] = (T)gz;
+}
+
 }  // namespace
+
+hipError_t launch_gradient(Env& e, void* out)
+{
+    const size_t n = e.nvs;
+    // 2*d as the reference forms it (d *= 2.0, :701-710)
+    const double tx = e.dx * 2.0, ty = e.dy * 2.0, tz = e.dz * 2.0;
+    if (e.precision == BFVM_FP64)
+        k_gradient<double><<<grid_for(n), kBlock, 0, e.stream>>>((const double*)e.rho, (double*)out, e.nx,
+                                                                 e.ny, e.nz, e.S, tx, ty, tz);
+    else
+        k_gradient<float><<<grid_for(n), kBlock, 0, e.stream>>>((const float*)e.rho, (float*)out, e.nx, e.ny,
+                                                                e.nz, e.S, tx, ty, tz);
+    return hipGetLastError();
+}
 
 hipError_t launch_agents_batch(int precision, const AgentBatch* d_tab, int nm, long long threads,
                                hipStream_t st)

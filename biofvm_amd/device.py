@@ -128,6 +128,24 @@ class DeviceMicroenvironment:
         self._check(self.lib.bfvm_set_sweep_plan(self.h, int(axis), int(lines_per_block),
                                                  int(nodes_per_segment), int(bool(streaming))))
 
+    def gradients(self):
+        """Gradient vectors of the current densities, shape (nvox, S, 3)."""
+        self._check(self.lib.bfvm_compute_gradients(self.h))
+        out = np.empty(self.nvox * self.S * 3)
+        self._check(self.lib.bfvm_download_gradients(self.h, _dptr(out)))
+        return out.reshape(self.nvox, self.S, 3)
+
+    @property
+    def sweep_chunk(self):
+        """z planes per x/y chunk (0 = whole-store sweeps)."""
+        v = ctypes.c_int(0)
+        self._check(self.lib.bfvm_get_sweep_chunk(self.h, ctypes.byref(v)))
+        return v.value
+
+    @sweep_chunk.setter
+    def sweep_chunk(self, planes):
+        self._check(self.lib.bfvm_set_sweep_chunk(self.h, int(planes)))
+
     def sweep_plans(self):
         """[(lines_per_block, nodes_per_segment, streaming)] for the x, y and z sweeps."""
         out = []

@@ -163,8 +163,9 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
 
 /*
  * Autotuner: times every valid launch shape of the three segmented sweeps
- * (lines per block x nodes per segment) for this grid, substrate count and
- * precision on a scratch copy of the store, and keeps the fastest per axis.
+ * (lines per block x nodes per segment x kernel) for this grid, substrate
+ * count and precision on a scratch copy of the store, keeps the fastest per
+ * axis, then times the x/y plane chunk sizes (bfvm_set_sweep_chunk).
  * The densities are left unchanged.  No counterpart in the reference (its
  * OpenMP loops have no launch shape).
  */
@@ -182,6 +183,29 @@ int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* node
  * the default).  Returns BFVM_E_ARG for a shape the kernel cannot run. */
 int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment,
                         int streaming);
+
+/*
+ * Gradient vectors of the current densities (Microenvironment::
+ * compute_all_gradient_vectors, BioFVM_microenvironment.cpp:699-779): per
+ * voxel n and substrate s, for each axis with n strictly inside,
+ * (rho[n+jump] - rho[n-jump]) / (2*d_axis) in FP64, else 0.  Enqueued on the
+ * handle's stream into a device array of nvox*S*3 values in the store's
+ * precision, index (n*S + s)*3 + axis.  Unlike the reference (whose
+ * neighbour strides are set only once its solver has run, and whose 2*dx is
+ * fixed by the first mesh of the process), the strides and spacings are the
+ * handle's own.
+ */
+int bfvm_compute_gradients(bfvm_handle h);
+int bfvm_download_gradients(bfvm_handle h, double* host);
+int bfvm_gradient_device_ptr(bfvm_handle h, void** ptr);
+
+/* Plane chunking of the x and y sweeps: with planes > 0 (and fewer than nz)
+ * the x and y sweeps run chunk by chunk of `planes` z planes, x then y per
+ * chunk, so the y sweep reads what the x sweep just wrote from the GPU's
+ * last-level cache.  Results are bit-identical to whole-store sweeps;
+ * 0 = whole-store sweeps.  bfvm_autotune picks it. */
+int bfvm_get_sweep_chunk(bfvm_handle h, int* planes);
+int bfvm_set_sweep_chunk(bfvm_handle h, int planes);
 
 /* Grid and store description. */
 int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, int* precision,

@@ -175,3 +175,26 @@ def agent_constants(S, dt, voxel_volume, volume, secretion, saturation, uptake):
 def nearest_voxel(nx, ny, nz, dx, x, y, z):
     """Voxel index of a point, or -1 outside the closed box (BioFVM_mesh.cpp:930-948)."""
     return _load_oracle().oracle_nearest_voxel(nx, ny, nz, dx, 0.0, 0.0, 0.0, x, y, z)
+
+
+def gradients(nx, ny, nz, S, dx, rho, dy=None, dz=None):
+    """Gradient vectors of every voxel, shape (nvox, S, 3) (oracle_gradients)."""
+    lib = _load_oracle()
+    rho = _f64(rho)
+    out = np.empty(nx * ny * nz * S * 3)
+    lib.oracle_gradients(nx, ny, nz, S, ctypes.c_double(dx), ctypes.c_double(dx if dy is None else dy),
+                         ctypes.c_double(dx if dz is None else dz), _ptr(rho), _ptr(out))
+    return out.reshape(nx * ny * nz, S, 3)
+
+
+def ref_gradients(nx, ny, nz, S, dx, rho):
+    """The compiled reference's gradient vectors, shape (nvox, S, 3).  The
+    reference fixes 2*dx on its first call in a process: use one dx per process."""
+    lib = _load_ref()
+    rho = _f64(rho)
+    out = np.empty(nx * ny * nz * S * 3)
+    lib.ref_gradients.restype = ctypes.c_int
+    rc = lib.ref_gradients(nx, ny, nz, S, ctypes.c_double(dx), _ptr(rho), _ptr(out))
+    if rc != 0:
+        raise RuntimeError(f"ref_gradients failed ({rc})")
+    return out.reshape(nx * ny * nz, S, 3)

@@ -302,3 +302,39 @@ int oracle_run(int nx, int ny, int nz, int S, double dx,
     free(voxel); free(t1); free(t2);
     return rc;
 }
+
+/*
+ * Gradient vectors of every voxel (Microenvironment::compute_all_gradient_vectors,
+ * BioFVM_microenvironment.cpp:699-779): for each axis, interior voxels only,
+ *   g = rho[n + jump]; g -= rho[n - jump]; g /= (2 * d_axis)
+ * with 2*d computed as d*2.0 (:701-710).  Components on the two boundary
+ * layers of an axis are left untouched; here they are set to 0, the value the
+ * reference's reset_all_gradient_vectors gives them (:826-842).
+ * out[(n*S + s)*3 + axis].  The reference's two_dx is a function-static set
+ * from the first mesh seen (:701-710); here each call uses its own spacing.
+ */
+void oracle_gradients(int nx, int ny, int nz, int S, double dx, double dy, double dz,
+                      const double* rho, double* out)
+{
+    const size_t nvox = (size_t)nx * ny * nz;
+    const int n_axis[3] = {nx, ny, nz};
+    const size_t jump[3] = {1, (size_t)nx, (size_t)nx * ny};
+    double two_d[3] = {dx, dy, dz};
+    for (int a = 0; a < 3; ++a) two_d[a] *= 2.0;
+    memset(out, 0, sizeof(double) * nvox * S * 3);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const size_t n = ((size_t)k * ny + j) * nx + i;
+                const int coord[3] = {i, j, k};
+                for (int a = 0; a < 3; ++a) {
+                    if (coord[a] < 1 || coord[a] >= n_axis[a] - 1) continue;
+                    for (int q = 0; q < S; ++q) {
+                        double g = rho[(n + jump[a]) * S + q];
+                        g -= rho[(n - jump[a]) * S + q];
+                        g /= two_d[a];
+                        out[(n * S + q) * 3 + a] = g;
+                    }
+                }
+            }
+}

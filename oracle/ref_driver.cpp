@@ -156,4 +156,48 @@ int ref_run(int nx, int ny, int nz, int S, double dx,
     return rc;
 }
 
+/*
+ * Gradient vectors of the reference (Microenvironment::compute_all_gradient_vectors,
+ * BioFVM_microenvironment.cpp:699-779) of the densities rho on a uniform grid:
+ * out[(n*S + s)*3 + axis].  The reference keeps 2*dx in a function-static
+ * set on its first call in a process (:701-710): every call in one process
+ * must use the same dx.
+ */
+int ref_gradients(int nx, int ny, int nz, int S, double dx, const double* rho, double* out)
+{
+    Microenvironment* M = new Microenvironment;
+    set_default_microenvironment(M);
+    M->set_density(0, "s0", "dimensionless");
+    for (int s = 1; s < S; ++s) M->add_density("s", "dimensionless");
+    M->resize_space_uniform(0.0, nx * dx, 0.0, ny * dx, 0.0, nz * dx, dx);
+    if ((int)M->mesh.x_coordinates.size() != nx || (int)M->mesh.y_coordinates.size() != ny ||
+        (int)M->mesh.z_coordinates.size() != nz || M->number_of_densities() != S) {
+        set_default_microenvironment(nullptr);
+        delete M;
+        return -1;
+    }
+    const int nvox = nx * ny * nz;
+    for (int v = 0; v < nvox; ++v) {
+        std::vector<double>& d = M->density_vector(v);
+        for (int s = 0; s < S; ++s) d[s] = rho[(size_t)v * S + s];
+    }
+    // The neighbour strides the gradient uses (thomas_*_jump) are set only by
+    // the LOD solver's setup (BioFVM_solvers.cpp:100-103).  One step with
+    // D = lambda = 0 sets them and leaves every density unchanged (denom = 1,
+    // c = 0: each sweep is the identity).
+    M->diffusion_decay_solver = diffusion_decay_solver__constant_coefficients_LOD_3D;
+    for (int s = 0; s < S; ++s) {
+        M->diffusion_coefficients[s] = 0.0;
+        M->decay_rates[s] = 0.0;
+    }
+    M->simulate_diffusion_decay(1.0);
+    M->compute_all_gradient_vectors();
+    for (int v = 0; v < nvox; ++v)
+        for (int s = 0; s < S; ++s)
+            for (int a = 0; a < 3; ++a) out[((size_t)v * S + s) * 3 + a] = M->gradient_vector(v)[s][a];
+    set_default_microenvironment(nullptr);
+    delete M;
+    return 0;
+}
+
 }  // extern "C"

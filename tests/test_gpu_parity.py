@@ -216,3 +216,47 @@ def test_unrunnable_launch_shapes_are_refused():
             m.set_sweep_plan(0, 32, 24, True)  # no FP64 streaming kernel with 24 nodes
         with pytest.raises(BfvmError):
             m.set_sweep_plan(0, 48, 8)
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_plane_chunked_xy_sweeps_are_bit_identical(precision):
+    # x then y per chunk of z planes, with Dirichlet faces (masked), bulk and agents
+    rng = np.random.default_rng(61)
+    nx, ny, nz, S = 30, 20, 17, 3
+    case = W.make_case(nx, ny, nz, S, 10.0, [1e5, 2e3, 3e4], [0.1, 0.3, 0.01], 0.01, 3, W.ALL,
+                       rng.uniform(0, 1, nx * ny * nz * S))
+    case.update(W.random_bulk(rng, nx * ny * nz, S))
+    case.update(W.random_agents(rng, 200, S, nx, ny, nz, 10.0, share_voxel=5))
+    idx = W.outer_face_voxels(nx, ny, nz)
+    case.update(dir_idx=idx, dir_val=rng.uniform(0, 2, idx.size * S), dir_mask=np.array([1, 0, 1], np.uint8))
+    ref = oracle.oracle_run(case)
+    with setup_case(case, precision) as m:
+        m.run(case["dt"], case["steps"], case["phases"])
+        whole = m.download()
+    assert rel_err(whole, ref) <= TOL[precision]
+    for chunk in (1, 4, 16, 17, 40):
+        with setup_case(case, precision) as m:
+            m.sweep_chunk = chunk
+            assert m.sweep_chunk == chunk
+            m.run(case["dt"], case["steps"], case["phases"])
+            np.testing.assert_array_equal(m.download(), whole, err_msg=f"chunk {chunk}")
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_gradients_against_oracle(precision):
+    # gradient vectors of the device's densities after a few steps, against the
+    # oracle's gradients of the same densities (FP64: bit-identical formula)
+    rng = np.random.default_rng(70)
+    for dims in [(17, 13, 9, 3), (1, 12, 7, 2), (40, 2, 3, 1)]:
+        nx, ny, nz, S = dims
+        case = W.make_case(nx, ny, nz, S, 10.0, 10.0 ** rng.uniform(2, 5, S), 10.0 ** rng.uniform(-3, 0, S),
+                           0.01, 2, W.DIFFUSION, rng.uniform(0, 1, nx * ny * nz * S))
+        with setup_case(case, precision) as m:
+            m.run(case["dt"], case["steps"], case["phases"])
+            rho = m.download()
+            g = m.gradients()
+        ref = oracle.gradients(nx, ny, nz, S, 10.0, rho)
+        if precision == FP64:
+            np.testing.assert_array_equal(g, ref, err_msg=str(dims))
+        else:  # FP32 output of the FP64 difference of FP32 values
+            assert np.max(np.abs(g - ref)) <= 1e-6 * np.max(np.abs(ref)) + 1e-12, dims

@@ -214,3 +214,35 @@ def test_convergence_to_1d_analytic_solution():
         errs.append(np.max(np.abs(got - exact)))
     assert errs[1] < errs[0] / 3.0, errs
     assert errs[0] < 1e-2
+
+
+# ---- gradient vectors (compute_all_gradient_vectors) ----------------------
+
+def test_gradient_of_quadratic_fields_is_exact_inside_and_zero_on_boundary_layers():
+    # central differences are exact for polynomials of degree <= 2
+    nx, ny, nz, S, dx = 7, 6, 5, 2, 20.0
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y, z = (i + 0.5) * dx, (j + 0.5) * dx, (k + 0.5) * dx
+    f0 = 3.0 * x - 2.0 * y + 0.5 * z
+    f1 = 0.01 * x * x + 0.02 * y * y - 0.03 * z * z
+    rho = np.stack([f0.reshape(-1), f1.reshape(-1)], axis=1).reshape(-1)
+    g = oracle.gradients(nx, ny, nz, S, dx, rho)
+    coords = [i.reshape(-1), j.reshape(-1), k.reshape(-1)]
+    n_axis = [nx, ny, nz]
+    expect0 = [3.0, -2.0, 0.5]
+    expect1 = [0.02 * x.reshape(-1), 0.04 * y.reshape(-1), -0.06 * z.reshape(-1)]
+    for a in range(3):
+        inside = (coords[a] > 0) & (coords[a] < n_axis[a] - 1)
+        np.testing.assert_allclose(g[inside, 0, a], expect0[a], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(g[inside, 1, a], expect1[a][inside], rtol=1e-12, atol=1e-12)
+        assert np.all(g[~inside, :, a] == 0.0)
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="compiled reference not built")
+@pytest.mark.parametrize("shape", [(9, 7, 5, 3), (1, 6, 4, 2), (5, 1, 1, 1), (2, 3, 2, 2)])
+def test_gradient_matches_compiled_reference(shape):
+    # one spacing in this process: the reference keeps 2*dx in a function-static
+    nx, ny, nz, S = shape
+    rho = np.random.default_rng(sum(shape)).uniform(0, 1, nx * ny * nz * S)
+    np.testing.assert_array_equal(oracle.gradients(nx, ny, nz, S, 20.0, rho),
+                                  oracle.ref_gradients(nx, ny, nz, S, 20.0, rho))
