@@ -45,6 +45,9 @@ SIGNATURES = {
     "bfvm_compute_gradients": (_i, [_h]),
     "bfvm_download_gradients": (_i, [_h, _dp]),
     "bfvm_gradient_device_ptr": (_i, [_h, ctypes.POINTER(_vp)]),
+    "bfvm_write_matlab": (_i, [_h, ctypes.c_char_p, _d, _d, _d]),
+    "bfvm_write_matlab_async": (_i, [_h, ctypes.c_char_p, _d, _d, _d]),
+    "bfvm_io_wait": (_i, [_h]),
     "bfvm_get_sweep_chunk": (_i, [_h, _ip]),
     "bfvm_set_sweep_chunk": (_i, [_h, _i]),
     "bfvm_get_info": (_i, [_h, _ip, _ip, _ip, _ip, _ip, _sp]),

@@ -130,6 +130,9 @@ struct Env {
     // Gradient vectors (nvs*3, store precision), allocated on first use.
     void* grad = nullptr;
 
+    // MAT v4 snapshot buffers and writer thread (io.hip).
+    struct MatWriter* mat = nullptr;
+
     // Per-launch timing (bfvm_timed_run with per-kernel events).
     bool kernel_timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -182,6 +185,8 @@ hipError_t ensure_bulk(Env& e, double dt);
 
 // Frees the adjoint's tape, if any.
 void free_tape(Env& e);
+// Waits for and frees the snapshot writer, if any.
+void free_writer(Env& e);
 
 // Valid segmented shapes for a line of n nodes and a store precision, for
 // the autotuner.

@@ -477,6 +477,7 @@ int bfvm_destroy(bfvm_handle h)
         return fail(h, BFVM_E_STATE, "the handle belongs to an ensemble: destroy the ensemble first");
     bfvm_env* e = h;
     bfvm::free_tape(*e);
+    bfvm::free_writer(*e);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     dfree(e->rho);
     dfree(e->c1);

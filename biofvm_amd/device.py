@@ -128,6 +128,14 @@ class DeviceMicroenvironment:
         self._check(self.lib.bfvm_set_sweep_plan(self.h, int(axis), int(lines_per_block),
                                                  int(nodes_per_segment), int(bool(streaming))))
 
+    def write_matlab(self, filename, origin=(0.0, 0.0, 0.0), asynchronous=False):
+        """MAT v4 snapshot in write_to_matlab's layout (see biofvm_amd.matlab.read_matlab)."""
+        fn = self.lib.bfvm_write_matlab_async if asynchronous else self.lib.bfvm_write_matlab
+        self._check(fn(self.h, str(filename).encode(), float(origin[0]), float(origin[1]), float(origin[2])))
+
+    def io_wait(self):
+        self._check(self.lib.bfvm_io_wait(self.h))
+
     def gradients(self):
         """Gradient vectors of the current densities, shape (nvox, S, 3)."""
         self._check(self.lib.bfvm_compute_gradients(self.h))

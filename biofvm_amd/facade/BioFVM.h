@@ -22,8 +22,10 @@
  * the std::vector<double> returned by density_vector() is a host mirror
  * synchronised lazily — any density access first copies the device state
  * down if it is newer, and marks the mirror as the newer copy, so the next
- * simulate_* call uploads it.  Only the hot path is here; I/O (MATLAB,
- * MultiCellDS), gradients and the general mesh are not part of this façade.
+ * simulate_* call uploads it.  Also here: the gradient vectors
+ * (compute_all_gradient_vectors on the GPU, BioFVM_microenvironment.cpp:658-842)
+ * and write_to_matlab (:557-580, packed on the GPU).  MultiCellDS output and
+ * the general (non-Cartesian) mesh are not part of this façade.
  */
 #ifndef BIOFVM_AMD_FACADE_H
 #define BIOFVM_AMD_FACADE_H
@@ -78,6 +80,8 @@ class Cartesian_Mesh {
 };
 
 struct DeviceState;  // the biofvm_amd handle and its synchronisation flags
+
+typedef std::vector<double> gradient;  // BioFVM_microenvironment.h:39
 
 class Microenvironment {
  public:
@@ -145,6 +149,21 @@ class Microenvironment {
     void apply_dirichlet_conditions(void);
 
     void display_information(std::ostream& os);
+
+    // Gradients (BioFVM_microenvironment.h:65-66, 176-184): vectors of S
+    // 3-vectors per voxel; compute_all_gradient_vectors runs on the GPU.
+    std::vector<std::vector<gradient> > gradient_vectors;
+    std::vector<bool> gradient_vector_computed;
+    std::vector<gradient>& gradient_vector(int i, int j, int k);
+    std::vector<gradient>& gradient_vector(int i, int j);
+    std::vector<gradient>& gradient_vector(int n);
+    std::vector<gradient>& nearest_gradient_vector(std::vector<double>& position);
+    void compute_all_gradient_vectors(void);
+    void compute_gradient_vector(int n);
+    void reset_all_gradient_vectors(void);
+
+    // MAT v4 snapshot (BioFVM_microenvironment.cpp:557-580), packed on the GPU.
+    void write_to_matlab(std::string filename);
 
     /* Extensions (no counterpart in the reference). */
     // Restrict Dirichlet overwrites to substrates with mask[s] != 0

@@ -485,6 +485,92 @@ int Microenvironment::push_to_device(void)
     return BFVM_OK;
 }
 
+// ---------------------------------------------------------------- gradients, output
+
+void Microenvironment::reset_all_gradient_vectors(void)
+{
+    const size_t S = densities[0].size();
+    gradient_vectors.assign(densities.size(), std::vector<gradient>(S, gradient(3, 0.0)));
+    gradient_vector_computed.assign(densities.size(), false);
+}
+
+// On the GPU from the device copy (pushed first if the mirror is newer); the
+// result is copied into gradient_vectors and every voxel marked computed.
+void Microenvironment::compute_all_gradient_vectors(void)
+{
+    const size_t nvox = densities.size(), S = densities[0].size();
+    if (gradient_vectors.size() != nvox || gradient_vectors[0].size() != S) reset_all_gradient_vectors();
+    DeviceState* d = device();
+    int rc = d->h ? push_to_device() : BFVM_E_DEVICE;
+    if (!rc) rc = bfvm_compute_gradients(d->h);
+    std::vector<double> g(nvox * S * 3);
+    if (!rc) rc = bfvm_download_gradients(d->h, g.data());
+    if (rc) {
+        report("compute_all_gradient_vectors", rc, d->h);
+        return;
+    }
+    for (size_t v = 0; v < nvox; ++v) {
+        for (size_t s = 0; s < S; ++s)
+            for (int a = 0; a < 3; ++a) gradient_vectors[v][s][a] = g[(v * S + s) * 3 + a];
+        gradient_vector_computed[v] = true;
+    }
+}
+
+// One voxel on the host from the mirror, as the reference
+// (BioFVM_microenvironment.cpp:781-824), with this mesh's spacings.
+void Microenvironment::compute_gradient_vector(int n)
+{
+    const size_t S = densities[0].size();
+    if (gradient_vectors.size() != densities.size() || gradient_vectors[0].size() != S)
+        reset_all_gradient_vectors();
+    host_view();
+    const std::vector<int> ijk = cartesian_indices(n);
+    const int counts[3] = {(int)mesh.x_coordinates.size(), (int)mesh.y_coordinates.size(),
+                           (int)mesh.z_coordinates.size()};
+    const int jumps[3] = {1, counts[0], counts[0] * counts[1]};
+    double two_d[3] = {mesh.dx, mesh.dy, mesh.dz};
+    for (int a = 0; a < 3; ++a) {
+        two_d[a] *= 2.0;
+        if (ijk[a] <= 0 || ijk[a] >= counts[a] - 1) continue;
+        for (size_t q = 0; q < S; ++q) {
+            double g = densities[n + jumps[a]][q];
+            g -= densities[n - jumps[a]][q];
+            g /= two_d[a];
+            gradient_vectors[n][q][a] = g;
+        }
+        gradient_vector_computed[n] = true;
+    }
+}
+
+std::vector<gradient>& Microenvironment::gradient_vector(int n)
+{
+    if (gradient_vector_computed.size() != densities.size() || !gradient_vector_computed[n])
+        compute_gradient_vector(n);
+    return gradient_vectors[n];
+}
+std::vector<gradient>& Microenvironment::gradient_vector(int i, int j, int k)
+{
+    return gradient_vector(voxel_index(i, j, k));
+}
+std::vector<gradient>& Microenvironment::gradient_vector(int i, int j)
+{
+    return gradient_vector(voxel_index(i, j, 0));
+}
+std::vector<gradient>& Microenvironment::nearest_gradient_vector(std::vector<double>& position)
+{
+    return gradient_vector(nearest_voxel_index(position));
+}
+
+void Microenvironment::write_to_matlab(std::string filename)
+{
+    DeviceState* d = device();
+    int rc = d->h ? push_to_device() : BFVM_E_DEVICE;
+    if (!rc)
+        rc = bfvm_write_matlab(d->h, filename.c_str(), mesh.bounding_box[0], mesh.bounding_box[1],
+                               mesh.bounding_box[2]);
+    if (rc) report("write_to_matlab", rc, d->h);
+}
+
 void Microenvironment::add_dirichlet_node(int voxel_index, std::vector<double>& value)
 {
     mesh.voxels[voxel_index].is_Dirichlet = true;

@@ -199,6 +199,22 @@ int bfvm_compute_gradients(bfvm_handle h);
 int bfvm_download_gradients(bfvm_handle h, double* host);
 int bfvm_gradient_device_ptr(bfvm_handle h, void** ptr);
 
+/*
+ * Snapshot of the densities as a MAT v4 file in the layout of
+ * Microenvironment::write_to_matlab (BioFVM_microenvironment.cpp:557-580,
+ * header BioFVM_matlab.cpp:515-559): a (4+S) x voxels FP64 matrix named
+ * "multiscale_microenvironment", one column per voxel: centre x, y, z
+ * (x0 + (i+0.5)*dx, ...: the mesh's origin is the caller's), volume, the S
+ * densities.  bfvm_write_matlab returns when the file is written;
+ * bfvm_write_matlab_async returns at once (the store is captured at that
+ * point of the handle's stream, later steps may run) and bfvm_io_wait waits
+ * for the file and reports its errors.  One asynchronous snapshot is in flight
+ * at a time (a second call waits for the first).
+ */
+int bfvm_write_matlab(bfvm_handle h, const char* filename, double x0, double y0, double z0);
+int bfvm_write_matlab_async(bfvm_handle h, const char* filename, double x0, double y0, double z0);
+int bfvm_io_wait(bfvm_handle h);
+
 /* Plane chunking of the x and y sweeps: with planes > 0 (and fewer than nz)
  * the x and y sweeps run chunk by chunk of `planes` z planes, x then y per
  * chunk, so the y sweep reads what the x sweep just wrote from the GPU's

@@ -198,3 +198,13 @@ def ref_gradients(nx, ny, nz, S, dx, rho):
     if rc != 0:
         raise RuntimeError(f"ref_gradients failed ({rc})")
     return out.reshape(nx * ny * nz, S, 3)
+
+
+def ref_write_matlab(nx, ny, nz, S, dx, rho, filename):
+    """The compiled reference's write_to_matlab of densities rho (origin 0)."""
+    lib = _load_ref()
+    rho = _f64(rho)
+    lib.ref_write_matlab.restype = ctypes.c_int
+    rc = lib.ref_write_matlab(nx, ny, nz, S, ctypes.c_double(dx), _ptr(rho), str(filename).encode())
+    if rc != 0:
+        raise RuntimeError(f"ref_write_matlab failed ({rc})")

@@ -200,4 +200,32 @@ int ref_gradients(int nx, int ny, int nz, int S, double dx, const double* rho, d
     return 0;
 }
 
+/*
+ * The reference's own MAT v4 snapshot (Microenvironment::write_to_matlab,
+ * BioFVM_microenvironment.cpp:557-580) of densities rho on a uniform grid
+ * with origin 0.
+ */
+int ref_write_matlab(int nx, int ny, int nz, int S, double dx, const double* rho, const char* filename)
+{
+    Microenvironment* M = new Microenvironment;
+    set_default_microenvironment(M);
+    M->set_density(0, "s0", "dimensionless");
+    for (int s = 1; s < S; ++s) M->add_density("s", "dimensionless");
+    M->resize_space_uniform(0.0, nx * dx, 0.0, ny * dx, 0.0, nz * dx, dx);
+    const int nvox = nx * ny * nz;
+    int rc = -1;
+    if ((int)M->mesh.x_coordinates.size() == nx && (int)M->mesh.y_coordinates.size() == ny &&
+        (int)M->mesh.z_coordinates.size() == nz && M->number_of_densities() == S) {
+        for (int v = 0; v < nvox; ++v) {
+            std::vector<double>& d = M->density_vector(v);
+            for (int s = 0; s < S; ++s) d[s] = rho[(size_t)v * S + s];
+        }
+        M->write_to_matlab(filename);
+        rc = 0;
+    }
+    set_default_microenvironment(nullptr);
+    delete M;
+    return rc;
+}
+
 }  // extern "C"

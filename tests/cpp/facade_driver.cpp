@@ -3,13 +3,15 @@
 // 152-241): build the microenvironment through its API, add Dirichlet nodes,
 // bulk callbacks and agents, then loop bulk -> cells -> diffusion.
 //
-//   facade_driver <case.bin> <out.bin>
+//   facade_driver <case.bin> <out.bin> [<grad.bin> <snapshot.mat>]
 //
 // case.bin (little endian): int32 nx ny nz S steps phases n_dir n_agents
 // has_bulk precision; float64 dx dt; D[S] lambda[S]; rho0[nvox*S];
 // int32 dir_idx[n_dir]; float64 dir_val[n_dir*S]; if has_bulk: supply,
 // target, uptake [nvox*S each]; agents: pos[3n] volume[n] secretion[nS]
-// saturation[nS] uptake[nS].  out.bin: float64 rho[nvox*S].
+// saturation[nS] uptake[nS].  out.bin: float64 rho[nvox*S].  With the two
+// optional paths, the final gradient vectors (compute_all_gradient_vectors,
+// float64 [nvox][S][3]) and a write_to_matlab snapshot are written too.
 // Halfway through, every density is read through density_vector() (which
 // synchronises the host mirror) and written back unchanged, so the result
 // also covers the mirror's download/upload path.
@@ -40,8 +42,8 @@ void rd(FILE* f, T* p, size_t n)
 
 int main(int argc, char** argv)
 {
-    if (argc != 3) {
-        fprintf(stderr, "usage: facade_driver case.bin out.bin\n");
+    if (argc != 3 && argc != 5) {
+        fprintf(stderr, "usage: facade_driver case.bin out.bin [grad.bin snapshot.mat]\n");
         return 2;
     }
     FILE* f = fopen(argv[1], "rb");
@@ -128,6 +130,15 @@ int main(int argc, char** argv)
     if (!o) return 2;
     for (size_t v = 0; v < nvox; ++v) fwrite(M.density_vector((int)v).data(), sizeof(double), S, o);
     fclose(o);
+    if (argc == 5) {
+        M.compute_all_gradient_vectors();
+        FILE* g = fopen(argv[3], "wb");
+        if (!g) return 2;
+        for (size_t v = 0; v < nvox; ++v)
+            for (int s = 0; s < S; ++s) fwrite(M.gradient_vector((int)v)[s].data(), sizeof(double), 3, g);
+        fclose(g);
+        M.write_to_matlab(argv[4]);
+    }
     while (!all_basic_agents.empty()) delete_basic_agent((int)all_basic_agents.size() - 1);
     return 0;
 }

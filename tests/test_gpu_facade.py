@@ -78,3 +78,33 @@ def test_facade_many_agents_per_voxel(tmp_path):
     ref = oracle.oracle_run(case)
     got = run_driver(tmp_path, case, FP64)
     assert float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_facade_gradients_and_matlab_snapshot(tmp_path, precision):
+    # compute_all_gradient_vectors and write_to_matlab through the façade,
+    # after the steps of a fixture with every phase
+    import oracle
+    from biofvm_amd.matlab import read_matlab
+    case, _, _ = gen_golden.load("g8_shell")
+    cin, cout, cgrad, cmat = (tmp_path / n for n in ("case.bin", "out.bin", "grad.bin", "snap.mat"))
+    write_case(cin, case, precision)
+    r = subprocess.run([DRIVER, str(cin), str(cout), str(cgrad), str(cmat)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rho = np.fromfile(cout, dtype="<f8")
+    nx, ny, nz, S, dx = case["nx"], case["ny"], case["nz"], case["S"], case["dx"]
+    grad = np.fromfile(cgrad, dtype="<f8").reshape(-1, S, 3)
+    ref = oracle.gradients(nx, ny, nz, S, dx, rho)
+    if precision == FP64:
+        np.testing.assert_array_equal(grad, ref)
+    else:
+        assert np.max(np.abs(grad - ref)) <= 1e-6 * np.max(np.abs(ref))
+    name, M = read_matlab(cmat)
+    assert name == "multiscale_microenvironment" and M.shape == (4 + S, nx * ny * nz)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    np.testing.assert_array_equal(M[0], ((i + 0.5) * dx).reshape(-1))
+    np.testing.assert_array_equal(M[1], ((j + 0.5) * dx).reshape(-1))
+    np.testing.assert_array_equal(M[2], ((k + 0.5) * dx).reshape(-1))
+    assert np.all(M[3] == dx * dx * dx)
+    np.testing.assert_array_equal(M[4:].T.reshape(-1), rho)
