@@ -35,6 +35,7 @@ SIGNATURES = {
     "bfvm_set_bulk": (_i, [_h, _dp, _dp, _dp]),
     "bfvm_set_agents": (_i, [_h, _i, _ip, _dp, _dp]),
     "bfvm_set_agent_rates": (_i, [_h, _i, _dp, _dp, _dp, _dp, _dp, _d, _ip]),
+    "bfvm_set_agent_rates_device": (_i, [_h, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp]),
     "bfvm_step": (_i, [_h, _d, _i]),
     "bfvm_run": (_i, [_h, _d, _i, _i]),
     "bfvm_sync": (_i, [_h]),
@@ -48,6 +49,7 @@ SIGNATURES = {
     "bfvm_write_matlab": (_i, [_h, ctypes.c_char_p, _d, _d, _d]),
     "bfvm_write_matlab_async": (_i, [_h, ctypes.c_char_p, _d, _d, _d]),
     "bfvm_io_wait": (_i, [_h]),
+    "bfvm_set_lod_dimensions": (_i, [_h, _i]),
     "bfvm_get_sweep_chunk": (_i, [_h, _ip]),
     "bfvm_set_sweep_chunk": (_i, [_h, _i]),
     "bfvm_get_info": (_i, [_h, _ip, _ip, _ip, _ip, _ip, _sp]),

@@ -26,6 +26,8 @@ class Model:
         self.checkpoint_every = int(checkpoint_every)
         self.env = DeviceMicroenvironment(case["nx"], case["ny"], case["nz"], case["S"], case["dx"],
                                           precision)
+        if case.get("lod_dims", 3) != 3:
+            self.env.set_lod_dimensions(case["lod_dims"])
         if case.get("dir_idx") is not None and len(case["dir_idx"]) > 0:
             self.env.set_dirichlet(case["dir_idx"], case["dir_val"], case.get("dir_mask"))
         if case.get("bulk_supply") is not None:

@@ -299,7 +299,7 @@ hipError_t forward_step(Env& e, const Tape& tp, void* buf)
         if ((tp.flags & BFVM_BULK) && e.have_bulk && (st = bfvm::launch_bulk(e))) return st;
         if ((tp.flags & BFVM_CELLS) && e.n_agents > 0 && (st = bfvm::launch_agents(e))) return st;
         if (tp.flags & BFVM_DIFFUSION)
-            for (int axis = 0; axis < 3; ++axis) {
+            for (int axis = 0; axis < e.lod_dims; ++axis) {
                 if (e.n_dir > 0 && (st = bfvm::launch_dirichlet(e))) return st;
                 if ((st = bfvm::launch_sweep(e, axis))) return st;
             }
@@ -324,7 +324,7 @@ hipError_t forward_step_keep(Env& e, Tape& tp, const void* start)
     void* bufs[3] = {tp.s4, tp.s6, tp.s8};
     for (int axis = 0; axis < 3; ++axis) {
         if (axis > 0 && (st = copy_store(e, bufs[axis], bufs[axis - 1]))) return st;
-        if (!diff) continue;
+        if (!diff || axis >= e.lod_dims) continue;
         st = on_buffer(e, bufs[axis], [&]() -> hipError_t {
             hipError_t q;
             if (e.n_dir > 0 && (q = bfvm::launch_dirichlet(e))) return q;
@@ -343,7 +343,7 @@ hipError_t adjoint_step(Env& e, Tape& tp)
     hipError_t st;
     if (tp.flags & BFVM_DIFFUSION) {
         const void* ys[3] = {tp.s4, tp.s6, tp.s8};
-        for (int axis = 2; axis >= 0; --axis) {
+        for (int axis = e.lod_dims - 1; axis >= 0; --axis) {
             if ((st = sweep_adjoint(e, tp.lam, axis))) return st;
             if ((st = axis_grad<T>(e, tp, ys[axis], axis))) return st;
             if ((st = dirichlet_zero(e, tp.lam))) return st;
@@ -577,10 +577,11 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
         AHIP(e, hipMemcpyAsync(gu.data(), tp->gupt, sizeof(double) * na, hipMemcpyDeviceToHost, e.stream));
     }
     AHIP(e, hipStreamSynchronize(e.stream));
-    // c1 = dt D / dx^2 (BioFVM_solvers.cpp:95), c2 = dt lambda / 3 (:96)
+    // c1 = dt D / dx^2 (BioFVM_solvers.cpp:111-113), c2 = dt lambda / 3 (:118-119;
+    // / 2 for LOD_2D, :313-314)
     for (int s = 0; s < S; ++s) {
         if (grad_D) grad_D[s] = gc[s] * dt / (e.dx * e.dx);
-        if (grad_lambda) grad_lambda[s] = gc[S + s] * dt / 3.0;
+        if (grad_lambda) grad_lambda[s] = gc[S + s] * dt * (e.lod_dims == 2 ? 0.5 : 1.0 / 3.0);
     }
     // grouped order -> list order
     for (int q = 0; q < e.n_agents; ++q) {

@@ -88,6 +88,10 @@ struct Env {
     // so the y sweep reads the x sweep's output from the last-level cache;
     // 0 = whole-store sweeps.  Chosen by the autotuner.
     int xy_chunk = 0;
+    // 3: diffusion_decay_solver__constant_coefficients_LOD_3D (x, y, z sweeps,
+    // decay split in thirds); 2: ..._LOD_2D (BioFVM_solvers.cpp:278-427: x and
+    // y sweeps, decay split in halves; nz must be 1).
+    int lod_dims = 3;
 
     // Dirichlet nodes.
     int n_dir = 0;
@@ -182,6 +186,17 @@ hipError_t launch_agents_batch(int precision, const AgentBatch* d_tab, int nm, l
 int ensure_coefficients(Env& e, double dt);
 // Bulk implicit terms for dt (rebuilt when dt changed).
 hipError_t ensure_bulk(Env& e, double dt);
+
+// The decay part of one sweep: dt*lambda/3 for LOD_3D (BioFVM_solvers.cpp:118-119),
+// dt*lambda*0.5 for LOD_2D (:313-314), in the reference's operation order.
+inline double decay_split(double lambda, double dt, int lod_dims)
+{
+    double c2 = lambda;
+    c2 *= dt;
+    if (lod_dims == 2) c2 *= 0.5;
+    else c2 /= 3.0;
+    return c2;
+}
 
 // Frees the adjoint's tape, if any.
 void free_tape(Env& e);

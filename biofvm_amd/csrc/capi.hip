@@ -77,7 +77,7 @@ hipError_t alloc_f64(void** p, size_t n)
 // Thomas factorisation of one axis in FP64, in the reference's operation
 // order (BioFVM_solvers.cpp:104-142).  denom/c are (n, S), element [i*S+s].
 void thomas_axis(int n, int S, double dx, double dt, const std::vector<double>& D,
-                 const std::vector<double>& lambda, std::vector<double>& denom,
+                 const std::vector<double>& lambda, int lod_dims, std::vector<double>& denom,
                  std::vector<double>& c, std::vector<double>& c1v)
 {
     denom.assign((size_t)n * S, 0.0);
@@ -89,9 +89,7 @@ void thomas_axis(int n, int S, double dx, double dt, const std::vector<double>& 
         c1 /= dx;
         c1 /= dx;
         double c1a = c1 * -1.0;
-        double c2 = lambda[s];
-        c2 *= dt;
-        c2 /= 3.0;
+        const double c2 = bfvm::decay_split(lambda[s], dt, lod_dims);
         double c3 = 1.0;
         c3 += c1;
         c3 += c1;
@@ -164,9 +162,7 @@ int build_woodbury(Env& e, bfvm::AxisCoef& ax, int n, double dt)
         c1 *= dt;
         c1 /= e.dx;
         c1 /= e.dx;
-        double c2 = e.lambda[s];
-        c2 *= dt;
-        c2 /= 3.0;
+        const double c2 = bfvm::decay_split(e.lambda[s], dt, e.lod_dims);
         const double c3 = 1.0 + c1 + c1 + c2, c3a = 1.0 + c1 + c2;
         // fixed point of the reference's recurrence denom_i = c3 + c1*c_{i-1},
         // c_i = -c1/denom_i, iterated from its first row
@@ -239,7 +235,7 @@ int ensure_coefficients_impl(Env& e, double dt)
     const int S = e.S;
     std::vector<double> denom, c, c1v;
     for (int a = 0; a < 3; ++a) {
-        thomas_axis(n_axis[a], S, e.dx, dt, e.D, e.lambda, denom, c, c1v);
+        thomas_axis(n_axis[a], S, e.dx, dt, e.D, e.lambda, e.lod_dims, denom, c, c1v);
         bfvm::AxisCoef& ax = e.ax[a];
         const size_t m = (size_t)n_axis[a] * S;
         HIP_OR_FAIL(&e, upload_f64(ax.denom, denom.data(), m));
@@ -316,7 +312,7 @@ int step_impl(Env& e, double dt, int flags)
             }
             axis0 = 2;
         }
-        for (int axis = axis0; axis < 3; ++axis) {
+        for (int axis = axis0; axis < e.lod_dims; ++axis) {
             if ((rc = dirichlet())) return rc;
             if ((rc = run_kernel(e, BFVM_K_SWEEP_X + axis, [&] { return bfvm::launch_sweep(e, axis); })))
                 return rc;
@@ -850,6 +846,19 @@ int bfvm_gradient_device_ptr(bfvm_handle h, void** ptr)
 {
     if (!h || !ptr) return fail(h, BFVM_E_ARG, "null argument");
     *ptr = h->grad;
+    return BFVM_OK;
+}
+
+int bfvm_set_lod_dimensions(bfvm_handle h, int dims)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (dims != 2 && dims != 3) return fail(h, BFVM_E_ARG, "dims must be 2 or 3");
+    if (dims == 2 && h->nz != 1) return fail(h, BFVM_E_MESH, "LOD_2D needs a mesh with nz == 1");
+    HIP_OR_FAIL(h, quiesce(*h));
+    if (h->lod_dims != dims) {
+        h->lod_dims = dims;
+        h->coef_valid = false;  // the decay split changes
+    }
     return BFVM_OK;
 }
 

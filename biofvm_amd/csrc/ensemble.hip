@@ -126,6 +126,7 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
         size_t lds = 0;
         for (size_t i = 0; i < E.m.size(); ++i) {
             Env& m = *E.m[i];
+            if (axis >= m.lod_dims) continue;  // LOD_2D members have no z sweep
             if (!m.coef_valid || !bfvm::batch_member_ok(m, axis, E.L)) {
                 E.solo[axis].push_back((int)i);
                 continue;
@@ -196,7 +197,7 @@ hipError_t enqueue(bfvm_ensemble_t& E, int flags)
     if (flags & BFVM_DIFFUSION) {
         for (int axis = 0; axis < 3; ++axis) {
             for (Env* pm : E.m)
-                if (pm->n_dir > 0 &&
+                if (pm->n_dir > 0 && axis < pm->lod_dims &&
                     (st = on_stream(*pm, E.stream, [&] { return bfvm::launch_dirichlet(*pm); })))
                     return st;
             if (E.nbatch[axis] > 0 &&

@@ -111,6 +111,14 @@ class DeviceMicroenvironment:
                                                   _dptr(sat), _dptr(upt), float(dt), _iptr(vox)))
         return vox[:n]
 
+    def set_agent_rates_device(self, pos, volume, secretion, saturation, uptake, dt, out_voxel=None):
+        """Agents from device arrays: tensors (or objects with data_ptr()) of float64 in
+        device memory, contiguous; out_voxel an int32 device tensor of n or None."""
+        n = int(volume.numel())
+        ptrs = [t.data_ptr() for t in (pos, volume, secretion, saturation, uptake)]
+        self._check(self.lib.bfvm_set_agent_rates_device(self.h, n, *ptrs, float(dt),
+                                                         out_voxel.data_ptr() if out_voxel is not None else None))
+
     def step(self, dt, flags=ALL):
         self._check(self.lib.bfvm_step(self.h, float(dt), int(flags)))
 
@@ -127,6 +135,10 @@ class DeviceMicroenvironment:
     def set_sweep_plan(self, axis, lines_per_block, nodes_per_segment, streaming=False):
         self._check(self.lib.bfvm_set_sweep_plan(self.h, int(axis), int(lines_per_block),
                                                  int(nodes_per_segment), int(bool(streaming))))
+
+    def set_lod_dimensions(self, dims):
+        """3 = LOD_3D (default), 2 = LOD_2D (nz must be 1)."""
+        self._check(self.lib.bfvm_set_lod_dimensions(self.h, int(dims)))
 
     def write_matlab(self, filename, origin=(0.0, 0.0, 0.0), asynchronous=False):
         """MAT v4 snapshot in write_to_matlab's layout (see biofvm_amd.matlab.read_matlab)."""
@@ -222,6 +234,8 @@ class DeviceMicroenvironment:
 def setup_case(case, precision=FP64):
     """Creates a DeviceMicroenvironment holding every input of `case`."""
     m = DeviceMicroenvironment(case["nx"], case["ny"], case["nz"], case["S"], case["dx"], precision)
+    if case.get("lod_dims", 3) != 3:
+        m.set_lod_dimensions(case["lod_dims"])
     m.set_coefficients(case["D"], case["lam"])
     m.upload(case["rho0"])
     if case.get("dir_idx") is not None and len(case["dir_idx"]) > 0:

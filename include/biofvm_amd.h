@@ -127,6 +127,19 @@ int bfvm_set_agent_rates(bfvm_handle h, int n, const double* position, const dou
                          const double* uptake, double dt, int* out_voxel);
 
 /*
+ * bfvm_set_agent_rates with every array in device memory (FP64, the caller's):
+ * voxels, constants and the grouping by voxel are computed on the GPU (a
+ * stable radix sort keeps list order inside a voxel), with the same results
+ * as bfvm_set_agent_rates on the same values.  Agents that move are re-binned
+ * by calling it again with the new positions.  out_voxel (optional) is a
+ * device array of n ints.  Returns BFVM_E_ARG if an agent lies outside the
+ * domain (after the kernels ran; the handle then has no agents).
+ */
+int bfvm_set_agent_rates_device(bfvm_handle h, int n, const double* position, const double* volume,
+                                const double* secretion, const double* saturation,
+                                const double* uptake, double dt, int* out_voxel);
+
+/*
  * One time step: the phases in `flags`, in the order bulk -> cells ->
  * diffusion.  BFVM_DIFFUSION replaces
  * diffusion_decay_solver__constant_coefficients_LOD_3D
@@ -214,6 +227,15 @@ int bfvm_gradient_device_ptr(bfvm_handle h, void** ptr);
 int bfvm_write_matlab(bfvm_handle h, const char* filename, double x0, double y0, double z0);
 int bfvm_write_matlab_async(bfvm_handle h, const char* filename, double x0, double y0, double z0);
 int bfvm_io_wait(bfvm_handle h);
+
+/*
+ * Solver choice: dims = 3 is diffusion_decay_solver__constant_coefficients_LOD_3D
+ * (the default), dims = 2 is ..._LOD_2D (BioFVM_solvers.cpp:278-427): Dirichlet,
+ * x sweep, Dirichlet, y sweep, with the decay split in halves
+ * (dt*lambda*0.5 per sweep).  dims = 2 needs nz == 1 (BFVM_E_MESH otherwise;
+ * the reference's 2-D solver sweeps only the k = 0 plane).
+ */
+int bfvm_set_lod_dimensions(bfvm_handle h, int dims);
 
 /* Plane chunking of the x and y sweeps: with planes > 0 (and fewer than nz)
  * the x and y sweeps run chunk by chunk of `planes` z planes, x then y per

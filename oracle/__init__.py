@@ -83,6 +83,11 @@ def _i32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
 
 
+def _phases(case):
+    """Phase flags plus 8 (the 2-D solver) for a case with lod_dims == 2."""
+    return int(case["phases"]) | (8 if case.get("lod_dims", 3) == 2 else 0)
+
+
 def _common(case):
     S = int(case["S"])
     D = _f64(case["D"])
@@ -110,7 +115,7 @@ def oracle_run(case, return_voxels=False):
     rc = lib.oracle_run(
         ctypes.c_int(case["nx"]), ctypes.c_int(case["ny"]), ctypes.c_int(case["nz"]),
         ctypes.c_int(S), ctypes.c_double(case["dx"]), _ptr(D), _ptr(lam),
-        ctypes.c_double(case["dt"]), ctypes.c_int(case["steps"]), ctypes.c_int(case["phases"]),
+        ctypes.c_double(case["dt"]), ctypes.c_int(case["steps"]), ctypes.c_int(_phases(case)),
         _ptr(rho), ctypes.c_int(n_dir), _ptr(dir_idx, _ip), _ptr(dir_val), _ptr(mask, _u8p),
         _ptr(bulk[0]), _ptr(bulk[1]), _ptr(bulk[2]),
         ctypes.c_int(n_agents), _ptr(ag[0]), _ptr(ag[1]), _ptr(ag[2]), _ptr(ag[3]), _ptr(ag[4]),
@@ -131,7 +136,7 @@ def ref_run(case, nthreads=1, return_voxels=False, return_seconds=False):
     rc = lib.ref_run(
         ctypes.c_int(case["nx"]), ctypes.c_int(case["ny"]), ctypes.c_int(case["nz"]),
         ctypes.c_int(S), ctypes.c_double(case["dx"]), _ptr(D), _ptr(lam),
-        ctypes.c_double(case["dt"]), ctypes.c_int(case["steps"]), ctypes.c_int(case["phases"]),
+        ctypes.c_double(case["dt"]), ctypes.c_int(case["steps"]), ctypes.c_int(_phases(case)),
         _ptr(rho), ctypes.c_int(n_dir), _ptr(dir_idx, _ip), _ptr(dir_val),
         _ptr(bulk[0]), _ptr(bulk[1]), _ptr(bulk[2]),
         ctypes.c_int(n_agents), _ptr(ag[0]), _ptr(ag[1]), _ptr(ag[2]), _ptr(ag[3]), _ptr(ag[4]),

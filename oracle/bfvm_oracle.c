@@ -28,15 +28,16 @@
 #define ORACLE_BULK 1
 #define ORACLE_CELLS 2
 #define ORACLE_DIFFUSION 4
+#define ORACLE_LOD_2D 8 /* with ORACLE_DIFFUSION: the 2-D solver instead of the 3-D one */
 
 /*
  * Thomas coefficients for one axis of n nodes, all S substrates
  * (BioFVM_solvers.cpp:104-142; the y and z axes repeat it at :144-170).
  * denom[i*S+s], c[i*S+s]; i in [0,n).
  */
-void oracle_thomas_coefficients(int n, int S, double dx, double dt,
-                                const double* D, const double* lambda,
-                                double* denom, double* c)
+static void thomas_coefficients_split(int n, int S, double dx, double dt,
+                                      const double* D, const double* lambda, int dims,
+                                      double* denom, double* c)
 {
     for (int s = 0; s < S; ++s) {
         double c1 = D[s];
@@ -46,7 +47,8 @@ void oracle_thomas_coefficients(int n, int S, double dx, double dt,
         double c1a = c1 * -1.0;         /* -dt*D/dx^2         :115-116 */
         double c2 = lambda[s];
         c2 *= dt;
-        c2 /= 3.0;                      /* dt*lambda/3        :118-119 */
+        if (dims == 2) c2 *= 0.5;       /* dt*lambda/2, LOD_2D :313-314 */
+        else c2 /= 3.0;                 /* dt*lambda/3        :118-119 */
         double c3 = 1.0;
         c3 += c1;
         c3 += c1;
@@ -71,6 +73,13 @@ void oracle_thomas_coefficients(int n, int S, double dx, double dt,
             c[i * S + s] /= denom[i * S + s];
         }
     }
+}
+
+void oracle_thomas_coefficients(int n, int S, double dx, double dt,
+                                const double* D, const double* lambda,
+                                double* denom, double* c)
+{
+    thomas_coefficients_split(n, S, dx, dt, D, lambda, 3, denom, c);
 }
 
 /* c1[s] = dt*D/dx^2 in the reference's order (BioFVM_solvers.cpp:111-113). */
@@ -163,6 +172,32 @@ void oracle_lod_step(int nx, int ny, int nz, int S, double dx, double dt,
             thomas_line(rho, (size_t)j * nx + i, jz, nz, S, c1, dnz, cz);
 
     free(dnx); free(cx); free(dny); free(cy); free(dnz); free(cz); free(c1);
+}
+
+/*
+ * One LOD_2D step (diffusion_decay_solver__constant_coefficients_LOD_2D,
+ * BioFVM_solvers.cpp:278-427): Dirichlet, x-sweep, Dirichlet, y-sweep, with
+ * the decay split in halves.  Only the k = 0 plane is swept (:370, :399).
+ */
+void oracle_lod2d_step(int nx, int ny, int nz, int S, double dx, double dt,
+                       const double* D, const double* lambda, double* rho,
+                       int n_dir, const int* dir_idx, const double* dir_val,
+                       const unsigned char* dir_mask)
+{
+    (void)nz;
+    double* dnx = malloc(sizeof(double) * (size_t)nx * S);
+    double* cx = malloc(sizeof(double) * (size_t)nx * S);
+    double* dny = malloc(sizeof(double) * (size_t)ny * S);
+    double* cy = malloc(sizeof(double) * (size_t)ny * S);
+    double* c1 = malloc(sizeof(double) * (size_t)S);
+    thomas_coefficients_split(nx, S, dx, dt, D, lambda, 2, dnx, cx);
+    thomas_coefficients_split(ny, S, dx, dt, D, lambda, 2, dny, cy);
+    thomas_constant1(S, dx, dt, D, c1);
+    oracle_apply_dirichlet(S, rho, n_dir, dir_idx, dir_val, dir_mask);
+    for (int j = 0; j < ny; ++j) thomas_line(rho, (size_t)j * nx, 1, nx, S, c1, dnx, cx);
+    oracle_apply_dirichlet(S, rho, n_dir, dir_idx, dir_val, dir_mask);
+    for (int i = 0; i < nx; ++i) thomas_line(rho, (size_t)i, (size_t)nx, ny, S, c1, dny, cy);
+    free(dnx); free(cx); free(dny); free(cy); free(c1);
 }
 
 /*
@@ -296,7 +331,9 @@ int oracle_run(int nx, int ny, int nz, int S, double dx,
             oracle_bulk_step(nvox, S, dt, rho, bulk_supply, bulk_target, bulk_uptake);
         if ((phases & ORACLE_CELLS) && n_agents > 0)
             oracle_agent_step(S, rho, n_agents, voxel, t1, t2);
-        if (phases & ORACLE_DIFFUSION)
+        if ((phases & ORACLE_DIFFUSION) && (phases & ORACLE_LOD_2D))
+            oracle_lod2d_step(nx, ny, nz, S, dx, dt, D, lambda, rho, n_dir, dir_idx, dir_val, dir_mask);
+        else if (phases & ORACLE_DIFFUSION)
             oracle_lod_step(nx, ny, nz, S, dx, dt, D, lambda, rho, n_dir, dir_idx, dir_val, dir_mask);
     }
     free(voxel); free(t1); free(t2);

@@ -53,7 +53,7 @@ void uptake_cb(Microenvironment*, int voxel, std::vector<double>* dst) { copy_fi
 extern "C" {
 
 /* Flags for ref_run's `phases`: which of the three step methods run each step. */
-enum { REF_BULK = 1, REF_CELLS = 2, REF_DIFFUSION = 4 };
+enum { REF_BULK = 1, REF_CELLS = 2, REF_DIFFUSION = 4, REF_LOD_2D = 8 };
 
 /*
  * Runs `steps` steps of the reference on an nx*ny*nz grid of S substrates.
@@ -86,7 +86,8 @@ int ref_run(int nx, int ny, int nz, int S, double dx,
         delete M;
         return -1;
     }
-    M->diffusion_decay_solver = diffusion_decay_solver__constant_coefficients_LOD_3D;
+    M->diffusion_decay_solver = (phases & REF_LOD_2D) ? diffusion_decay_solver__constant_coefficients_LOD_2D
+                                                      : diffusion_decay_solver__constant_coefficients_LOD_3D;
     for (int s = 0; s < S; ++s) {
         M->diffusion_coefficients[s] = D[s];
         M->decay_rates[s] = lambda[s];

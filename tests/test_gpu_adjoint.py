@@ -161,3 +161,22 @@ def test_tape_errors(small):
         with pytest.raises(BfvmError):
             m.tape_record(case["dt"] * 2, 2, case["phases"], 1)  # dt differs from the agents' dt
         m.tape_free()
+
+
+def test_lod2d_gradients_against_oracle_central_differences():
+    rng = np.random.default_rng(21)
+    nx, ny, S = 10, 8, 2
+    case = W.make_case(nx, ny, 1, S, 20.0, [2e4, 5e2], [0.2, 0.05], 0.01, 5, ALL, rng.uniform(0, 1, nx * ny * S))
+    case.update(W.random_bulk(rng, nx * ny, S))
+    case.update(W.random_agents(rng, 15, S, nx, ny, 1, 20.0, share_voxel=3))
+    case["lod_dims"] = 2
+    target = oracle.oracle_run(case) + rng.normal(0, 0.05, nx * ny * S)
+    loss, g = adjoint_grads(case, target, FP64)
+    assert abs(loss - loss_of(oracle.oracle_run(case), target)) <= 1e-10 * loss
+    for key, gname, ckey in GROUPS:
+        p = np.asarray(case[ckey], dtype=np.float64)
+        d = rng.normal(size=p.shape) * p
+        fd = (loss_of(oracle.oracle_run(with_params(case, **{key: p + H * d})), target)
+              - loss_of(oracle.oracle_run(with_params(case, **{key: p - H * d})), target)) / (2 * H)
+        ad = float(np.sum(g[gname].reshape(-1) * d.reshape(-1)))
+        assert abs(ad - fd) <= 1e-6 * abs(fd) + 1e-12, (key, ad, fd)

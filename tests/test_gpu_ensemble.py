@@ -172,3 +172,22 @@ def test_ensemble_errors():
     finally:
         for m in (a, b, c):
             m.close()
+
+
+def test_ensemble_with_lod2d_members():
+    rng = np.random.default_rng(31)
+    cases = W.ensemble_mixed(seed=9, steps=2)[:3]
+    for nx, ny, S in [(24, 16, 2), (1, 30, 1)]:
+        c = W.make_case(nx, ny, 1, S, 20.0, 10.0 ** rng.uniform(2, 5, S), 10.0 ** rng.uniform(-3, 0, S), 0.01, 2,
+                        ALL, rng.uniform(0, 1, nx * ny * S))
+        c.update(W.random_agents(rng, 20, S, nx, ny, 1, 20.0))
+        c["lod_dims"] = 2
+        cases.append(c)
+    members = _members(cases, FP64)
+    ens = Ensemble(members)
+    try:
+        ens.run(0.01, 2, ALL, graph=True)
+        for i, (m, c) in enumerate(zip(members, cases)):
+            assert rel_err(m.download(), oracle.oracle_run(c)) <= TOL[FP64], i
+    finally:
+        _close(ens, members)

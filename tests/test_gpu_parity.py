@@ -260,3 +260,69 @@ def test_gradients_against_oracle(precision):
             np.testing.assert_array_equal(g, ref, err_msg=str(dims))
         else:  # FP32 output of the FP64 difference of FP32 values
             assert np.max(np.abs(g - ref)) <= 1e-6 * np.max(np.abs(ref)) + 1e-12, dims
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_lod2d_against_oracle(precision):
+    rng = np.random.default_rng(81)
+    for nx, ny, S in [(40, 33, 3), (1, 50, 2), (70, 1, 1), (300, 200, 2)]:
+        case = W.make_case(nx, ny, 1, S, 10.0, 10.0 ** rng.uniform(2, 5, S), 10.0 ** rng.uniform(-3, 0, S),
+                           0.01, 3, W.ALL, rng.uniform(0, 1, nx * ny * S))
+        case.update(W.random_bulk(rng, nx * ny, S))
+        case.update(W.random_agents(rng, 50, S, nx, ny, 1, 10.0, share_voxel=4))
+        idx = W.outer_face_voxels(nx, ny, 1)
+        case.update(dir_idx=idx, dir_val=rng.uniform(0, 1, idx.size * S))
+        case["lod_dims"] = 2
+        err = rel_err(run_case(case, precision), oracle.oracle_run(case))
+        assert err <= TOL[precision], (nx, ny, S, err)
+
+
+def test_lod2d_needs_a_flat_mesh():
+    with DeviceMicroenvironment(5, 4, 3, 1, 10.0, FP64) as m:
+        with pytest.raises(BfvmError):
+            m.set_lod_dimensions(2)
+        with pytest.raises(BfvmError):
+            m.set_lod_dimensions(1)
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_device_agent_setup_equals_host_setup(precision):
+    # voxels, constants and grouping computed on the GPU from device arrays,
+    # against the host path; then the agents move and are re-binned
+    import torch
+    rng = np.random.default_rng(91)
+    nx, ny, nz, S = 23, 17, 11, 3
+    case = W.make_case(nx, ny, nz, S, 10.0, [3e4, 1e3, 2e2], [0.1, 0.5, 0.01], 0.01, 3, W.CELLS | W.DIFFUSION,
+                       rng.uniform(0, 1, nx * ny * nz * S))
+    case.update(W.random_agents(rng, 3000, S, nx, ny, nz, 10.0, share_voxel=40))
+    n = len(case["agent_volume"])
+    dev = {k: torch.tensor(case[k], dtype=torch.float64, device="cuda")
+           for k in ("agent_pos", "agent_volume", "agent_secretion", "agent_saturation", "agent_uptake")}
+    with setup_case(case, precision) as host, setup_case(W.make_case(nx, ny, nz, S, 10.0, case["D"], case["lam"],
+                                                                     0.01, 3, case["phases"], case["rho0"]),
+                                                         precision) as m:
+        vox = torch.empty(n, dtype=torch.int32, device="cuda")
+        m.set_agent_rates_device(dev["agent_pos"], dev["agent_volume"], dev["agent_secretion"],
+                                 dev["agent_saturation"], dev["agent_uptake"], case["dt"], out_voxel=vox)
+        torch.cuda.synchronize()
+        ref_vox = oracle.oracle_run(case, return_voxels=True)[1]
+        np.testing.assert_array_equal(vox.cpu().numpy(), ref_vox)
+        host.run(case["dt"], 3, case["phases"])
+        m.run(case["dt"], 3, case["phases"])
+        np.testing.assert_array_equal(m.download(), host.download())
+        # move every agent by up to two voxels, re-bin both ways
+        moved = np.clip(case["agent_pos"] + rng.uniform(-20, 20, case["agent_pos"].size), 0.0,
+                        np.repeat([[nx * 10.0, ny * 10.0, nz * 10.0]], n, axis=0).reshape(-1))
+        host.set_agent_rates(moved, case["agent_volume"], case["agent_secretion"], case["agent_saturation"],
+                             case["agent_uptake"], case["dt"])
+        dev["agent_pos"].copy_(torch.tensor(moved, dtype=torch.float64))
+        m.set_agent_rates_device(dev["agent_pos"], dev["agent_volume"], dev["agent_secretion"],
+                                 dev["agent_saturation"], dev["agent_uptake"], case["dt"])
+        host.run(case["dt"], 2, case["phases"])
+        m.run(case["dt"], 2, case["phases"])
+        np.testing.assert_array_equal(m.download(), host.download())
+        # an agent outside the domain is refused
+        dev["agent_pos"][0] = -1.0
+        with pytest.raises(BfvmError):
+            m.set_agent_rates_device(dev["agent_pos"], dev["agent_volume"], dev["agent_secretion"],
+                                     dev["agent_saturation"], dev["agent_uptake"], case["dt"])

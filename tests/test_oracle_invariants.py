@@ -246,3 +246,43 @@ def test_gradient_matches_compiled_reference(shape):
     rho = np.random.default_rng(sum(shape)).uniform(0, 1, nx * ny * nz * S)
     np.testing.assert_array_equal(oracle.gradients(nx, ny, nz, S, 20.0, rho),
                                   oracle.ref_gradients(nx, ny, nz, S, 20.0, rho))
+
+
+# ---- LOD_2D (diffusion_decay_solver__constant_coefficients_LOD_2D) --------
+
+def _lod2d_case(nx, ny, S, steps, rho0, D, lam, phases=4):
+    from biofvm_amd import workloads as W
+    case = W.make_case(nx, ny, 1, S, 20.0, D, lam, 0.01, steps, phases, rho0)
+    case["lod_dims"] = 2
+    return case
+
+
+def test_lod2d_cosine_modes_and_mass_decay():
+    nx, ny, dx, dt, D, lam = 12, 9, 20.0, 0.01, 4e4, 0.3
+    c1, c2 = dt * D / dx ** 2, dt * lam * 0.5
+    j, i = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    mx, my = 3, 2
+    mode = (np.cos(np.pi * mx * (i + 0.5) / nx) * np.cos(np.pi * my * (j + 0.5) / ny)).reshape(-1)
+    out = oracle.oracle_run(_lod2d_case(nx, ny, 1, 3, mode, [D], [lam]))
+    f = 1.0
+    for m, n in ((mx, nx), (my, ny)):
+        f /= 1 + c2 + 4 * c1 * np.sin(np.pi * m / (2 * n)) ** 2
+    np.testing.assert_allclose(out, mode * f ** 3, rtol=1e-11, atol=1e-13)
+    rho = np.random.default_rng(1).uniform(0, 1, nx * ny)
+    out = oracle.oracle_run(_lod2d_case(nx, ny, 1, 4, rho, [D], [lam]))
+    np.testing.assert_allclose(out.sum(), rho.sum() * (1 + c2) ** -8, rtol=1e-12)
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="compiled reference not built")
+@pytest.mark.parametrize("dims", [(13, 9, 3), (1, 7, 2), (6, 1, 1)])
+def test_lod2d_matches_compiled_reference(dims):
+    from biofvm_amd import workloads as W
+    nx, ny, S = dims
+    rng = np.random.default_rng(sum(dims))
+    case = _lod2d_case(nx, ny, S, 4, rng.uniform(0, 1, nx * ny * S), 10 ** rng.uniform(2, 5, S),
+                       10 ** rng.uniform(-3, 0, S), phases=7)
+    case.update(W.random_bulk(rng, nx * ny, S))
+    case.update(W.random_agents(rng, 20, S, nx, ny, 1, 20.0, share_voxel=3))
+    idx = W.outer_face_voxels(nx, ny, 1)[::2]
+    case.update(dir_idx=idx, dir_val=rng.uniform(0, 1, idx.size * S))
+    np.testing.assert_array_equal(oracle.oracle_run(case), oracle.ref_run(case))
