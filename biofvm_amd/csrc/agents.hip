@@ -96,21 +96,24 @@ __global__ void k_agent_constants(const int* __restrict__ order, int n, int S, d
                                   double* __restrict__ t1, double* __restrict__ t2,
                                   double* __restrict__ cg, double* __restrict__ satg)
 {
+#pragma clang fp contract(off)  // the reference's separate roundings, no FMA
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int a = order[q];
-    double c = __dmul_rn(dt, vol[a]);
-    c = __ddiv_rn(c, dV);
+    double c = dt * vol[a];
+    c = c / dV;
     cg[q] = c;
     for (int s = 0; s < S; ++s) {
         const size_t i = (size_t)a * S + s, o = (size_t)q * S + s;
         double x = 0.0;
-        x = __dadd_rn(x, sec[i]);
-        x = __dmul_rn(x, sat[i]);
-        x = __dmul_rn(x, c);
+        x += sec[i];
+        x *= sat[i];
+        x *= c;
         double y = 1.0;
-        y = __dadd_rn(y, __dmul_rn(c, sec[i]));
-        y = __dadd_rn(y, __dmul_rn(c, upt[i]));
+        const double cs = c * sec[i];
+        y += cs;
+        const double cu = c * upt[i];
+        y += cu;
         t1[o] = x;
         t2[o] = y;
         satg[o] = sat[i];

@@ -8,8 +8,8 @@
 // reference writes exactly name.size() bytes), then one column per voxel:
 // centre x, y, z, volume, the S densities, all FP64.
 //
-// A kernel packs the columns on the device (centres x0 + (i + 0.5) dx with
-// round-to-nearest multiply and add, volume (dx*dy)*dz, as the mesh builds
+// A kernel packs the columns on the device (centres x0 + (i + 0.5) dx with a
+// separately rounded multiply and add, volume (dx*dy)*dz, as the mesh builds
 // them, BioFVM_mesh.cpp:817-860), the packed matrix goes to pinned host
 // memory, and the host writes the file.  The asynchronous form snapshots the
 // store at that point of the handle's stream, copies on a second stream
@@ -51,6 +51,7 @@ __global__ void k_pack_matlab(const T* __restrict__ rho, double* __restrict__ ou
                               int S, double x0, double y0, double z0, double dx, double dy, double dz,
                               double dV)
 {
+#pragma clang fp contract(off)  // the reference's separate roundings, no FMA
     const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nvox = (size_t)nx * ny * nz;
     if (v >= nvox) return;
@@ -58,9 +59,9 @@ __global__ void k_pack_matlab(const T* __restrict__ rho, double* __restrict__ ou
     const int j = (int)((v / nx) % ny);
     const int k = (int)(v / ((size_t)nx * ny));
     double* col = out + v * (size_t)(4 + S);
-    col[0] = __dadd_rn(x0, __dmul_rn((double)i + 0.5, dx));
-    col[1] = __dadd_rn(y0, __dmul_rn((double)j + 0.5, dy));
-    col[2] = __dadd_rn(z0, __dmul_rn((double)k + 0.5, dz));
+    col[0] = x0 + ((double)i + 0.5) * dx;
+    col[1] = y0 + ((double)j + 0.5) * dy;
+    col[2] = z0 + ((double)k + 0.5) * dz;
     col[3] = dV;
     for (int s = 0; s < S; ++s) col[4 + s] = (double)rho[v * S + s];
 }

@@ -26,8 +26,9 @@ __global__ void k_dirichlet(T* __restrict__ rho, int S, int n_dir, const int* __
 }
 
 // Implicit bulk terms for one dt (BioFVM_microenvironment.cpp:603-607),
-// in FP64 with explicit round-to-nearest operations so that no FMA
-// contraction changes the reference's rounding:
+// in FP64 with contraction off so that no FMA changes the reference's
+// rounding (written as plain operators: the pragma covers only expressions in
+// its own scope, not those inside the _rn helper functions):
 //   A = dt*(T*S),  B = ((U+S)*dt) + 1
 // A and B stay FP64 for either store precision: rounding them to FP32 would
 // bias every step's source term the same way.
@@ -35,12 +36,13 @@ __global__ void k_bulk_prepare(size_t nvs, double dt, const double* __restrict__
                                const double* __restrict__ tgt, const double* __restrict__ upt,
                                double* __restrict__ A, double* __restrict__ B)
 {
+#pragma clang fp contract(off)  // the reference's separate roundings, no FMA
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nvs) return;
-    double st = __dmul_rn(tgt[e], sup[e]);
-    A[e] = __dmul_rn(dt, st);
-    double us = __dadd_rn(upt[e], sup[e]);
-    B[e] = __dadd_rn(__dmul_rn(us, dt), 1.0);
+    const double st = tgt[e] * sup[e];
+    A[e] = dt * st;
+    const double us = upt[e] + sup[e];
+    B[e] = us * dt + 1.0;
 }
 
 // rho = (rho + A) / B  (BioFVM_microenvironment.cpp:604,609), evaluated in

@@ -326,3 +326,14 @@ def test_device_agent_setup_equals_host_setup(precision):
         with pytest.raises(BfvmError):
             m.set_agent_rates_device(dev["agent_pos"], dev["agent_volume"], dev["agent_secretion"],
                                      dev["agent_saturation"], dev["agent_uptake"], case["dt"])
+
+
+def test_sources_only_steps_are_bit_exact():
+    # bulk and agent updates (FP64 store) have no reassociation: bit for bit the oracle's
+    rng = np.random.default_rng(101)
+    nx, ny, nz, S = 19, 13, 7, 3
+    case = W.make_case(nx, ny, nz, S, 20.0, [1e4] * S, [0.1] * S, 0.01, 4, W.BULK | W.CELLS,
+                       rng.uniform(0, 1, nx * ny * nz * S))
+    case.update(W.random_bulk(rng, nx * ny * nz, S))
+    case.update(W.random_agents(rng, 500, S, nx, ny, nz, 20.0, share_voxel=25))
+    np.testing.assert_array_equal(run_case(case, FP64), oracle.oracle_run(case))
