@@ -43,6 +43,8 @@ SIGNATURES = {
     "bfvm_autotune": (_i, [_h, _d]),
     "bfvm_get_sweep_plan": (_i, [_h, _i, _ip, _ip, _ip]),
     "bfvm_set_sweep_plan": (_i, [_h, _i, _i, _i, _i]),
+    "bfvm_get_sweep_spread": (_i, [_h, _i, _ip]),
+    "bfvm_set_sweep_spread": (_i, [_h, _i, _i]),
     "bfvm_compute_gradients": (_i, [_h]),
     "bfvm_download_gradients": (_i, [_h, _dp]),
     "bfvm_gradient_device_ptr": (_i, [_h, ctypes.POINTER(_vp)]),

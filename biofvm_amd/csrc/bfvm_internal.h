@@ -52,14 +52,18 @@ struct SweepArgs {
     unsigned vstride;        // values between voxels of a plane's column set
     int cols;                // lines per block (a power of two)
     int tiles, planes;       // column tiles per plane, planes
+    int spread;              // tiles per XCD chunk (see xcd_tile in lod.hip)
 };
 
 // Launch shape of the segmented sweep of one axis (see lod.hip).
 struct SweepPlan {
-    int cols = 0;    // lines per block (16, 32 or 64); 0 = default
+    int cols = 0;    // lines per block (16, 32 or 64; row tiles: a multiple of S); 0 = default
     int L = 0;       // nodes per segment
-    int stream = 0;  // 1: resident blocks that load the next tile while solving one
+    int stream = 0;  // kernel: 0 one block per tile, 1 resident blocks that load the
+                     // next tile while solving one, 2 x sweep on LDS-staged row tiles
 };
+
+constexpr int kDefaultSpread = -1;
 
 struct Env {
     int nx = 0, ny = 0, nz = 0, S = 0;
@@ -84,6 +88,10 @@ struct Env {
     void* c1 = nullptr;  // dt*D/dx^2 per substrate (FP64)
     AxisCoef ax[3];
     SweepPlan plan[3];  // per-axis launch shape chosen by the autotuner
+    // per axis: how tiles are spread over the XCDs (xcd_tile in lod.hip):
+    // 0 as dispatched, k > 0 chunks of k consecutive tiles per XCD, -1 one
+    // contiguous range per XCD.  Chosen by the autotuner.
+    int spread[3] = {kDefaultSpread, kDefaultSpread, kDefaultSpread};
     // x and y sweeps run in chunks of this many z planes (x then y per chunk),
     // so the y sweep reads the x sweep's output from the last-level cache;
     // 0 = whole-store sweeps.  Chosen by the autotuner.
@@ -165,7 +173,7 @@ hipError_t launch_sweep_planes(Env& e, int axis, int k0, int k1);
 bool batch_member_ok(const Env& e, int axis, int L);
 SweepArgs batch_args(const Env& e, int axis, int L);
 size_t batch_lds(const SweepArgs& A);
-hipError_t launch_sweep_batch(int precision, int L, const SweepArgs* d_args, const int* d_first,
+hipError_t launch_sweep_batch(int precision, int L, int spread, const SweepArgs* d_args, const int* d_first,
                               int nm, int blocks, size_t lds, hipStream_t st);
 
 // Agents of several grids in one launch: one entry per member.
@@ -205,7 +213,7 @@ void free_writer(Env& e);
 
 // Valid segmented shapes for a line of n nodes and a store precision, for
 // the autotuner.
-std::vector<SweepPlan> sweep_candidates(int n, int precision);
+std::vector<SweepPlan> sweep_candidates(const Env& e, int axis);
 // The launch shape a sweep of `axis` uses now (autotuned or default).
 SweepPlan effective_plan(const Env& e, int axis);
 

@@ -730,11 +730,10 @@ int bfvm_autotune(bfvm_handle h, double dt)
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
     hipError_t st = hipSuccess;
-    const int n_axis[3] = {e.nx, e.ny, e.nz};
     for (int axis = 0; axis < 3 && st == hipSuccess; ++axis) {
         bfvm::SweepPlan best{};
         float best_ms = 1e30f;
-        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(n_axis[axis], e.precision)) {
+        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(e, axis)) {
             e.plan[axis] = cand;
             if ((st = bfvm::launch_sweep(e, axis)) != hipSuccess) break;  // warm-up
             (void)hipEventRecord(t0, e.stream);
@@ -749,6 +748,24 @@ int bfvm_autotune(bfvm_handle h, double dt)
             }
         }
         e.plan[axis] = best;
+        // then how that shape's tiles are spread over the XCDs
+        int best_spread = e.spread[axis];
+        best_ms = 1e30f;
+        for (int spread : {0, 16, 64, 256, -1}) {
+            e.spread[axis] = spread;
+            if ((st = bfvm::launch_sweep(e, axis)) != hipSuccess) break;
+            (void)hipEventRecord(t0, e.stream);
+            for (int rep = 0; rep < 3 && st == hipSuccess; ++rep) st = bfvm::launch_sweep(e, axis);
+            (void)hipEventRecord(t1, e.stream);
+            if (st != hipSuccess || (st = hipEventSynchronize(t1)) != hipSuccess) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_spread = spread;
+            }
+        }
+        e.spread[axis] = best_spread;
     }
     // plane chunking of the x and y sweeps: whole diffusion phases, timed
     if (st == hipSuccess && bfvm::sweep_chunkable(e, 0) && bfvm::sweep_chunkable(e, 1)) {
@@ -807,13 +824,30 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
         h->plan[axis] = bfvm::SweepPlan{};
         return BFVM_OK;
     }
-    const int n = axis == 0 ? h->nx : axis == 1 ? h->ny : h->nz;
-    for (const bfvm::SweepPlan& c : bfvm::sweep_candidates(n, h->precision))
-        if (c.cols == lines_per_block && c.L == nodes_per_segment && c.stream == (streaming ? 1 : 0)) {
+    for (const bfvm::SweepPlan& c : bfvm::sweep_candidates(*h, axis))
+        if (c.cols == lines_per_block && c.L == nodes_per_segment && c.stream == streaming) {
             h->plan[axis] = c;
             return BFVM_OK;
         }
     return fail(h, BFVM_E_ARG, "launch shape not valid for this axis length");
+}
+
+int bfvm_get_sweep_spread(bfvm_handle h, int axis, int* tiles_per_chunk)
+{
+    if (!h || !tiles_per_chunk) return fail(h, BFVM_E_ARG, "null argument");
+    if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
+    *tiles_per_chunk = h->spread[axis];
+    return BFVM_OK;
+}
+
+int bfvm_set_sweep_spread(bfvm_handle h, int axis, int tiles_per_chunk)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
+    if (tiles_per_chunk < -1) return fail(h, BFVM_E_ARG, "tiles_per_chunk must be -1, 0 or positive");
+    HIP_OR_FAIL(h, quiesce(*h));
+    h->spread[axis] = tiles_per_chunk;
+    return BFVM_OK;
 }
 
 int bfvm_compute_gradients(bfvm_handle h)

@@ -201,7 +201,7 @@ hipError_t enqueue(bfvm_ensemble_t& E, int flags)
                     (st = on_stream(*pm, E.stream, [&] { return bfvm::launch_dirichlet(*pm); })))
                     return st;
             if (E.nbatch[axis] > 0 &&
-                (st = bfvm::launch_sweep_batch(E.precision, E.L, E.d_args[axis], E.d_first[axis],
+                (st = bfvm::launch_sweep_batch(E.precision, E.L, bfvm::kDefaultSpread, E.d_args[axis], E.d_first[axis],
                                                E.nbatch[axis], E.blocks[axis], E.lds[axis], E.stream)))
                 return st;
             for (int i : E.solo[axis])

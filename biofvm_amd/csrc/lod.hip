@@ -85,6 +85,28 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Workgroups are dealt to the XCDs round robin (workgroup i to XCD i % 8),
+// each XCD with its own L2.  Renumbering the tiles so that one XCD runs
+// neighbouring column tiles, which share DRAM pages, at the same time is
+// worth +7 % on the y-sweep pattern and +15 % on z with 32-column tiles.
+// spread = 0: tile i as dispatched; k > 0: in each round of 8k tiles, XCD x
+// runs tiles x*k .. x*k + k-1 (the last partial round as dispatched);
+// -1: XCD x runs the x-th of 8 contiguous ranges.  A bijection of [0, N).
+constexpr unsigned kXcds = 8;
+
+__device__ __forceinline__ unsigned xcd_tile(unsigned lin, unsigned N, int spread)
+{
+    if (spread == 0) return lin;
+    if (spread < 0) {
+        const unsigned q = N / kXcds, r = N % kXcds, x = lin % kXcds, i = lin / kXcds;
+        return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+    }
+    const unsigned k = (unsigned)spread, R = k * kXcds;
+    if (lin >= N / R * R) return lin;
+    const unsigned j = lin % R;
+    return lin - j + (j % kXcds) * k + j / kXcds;
+}
+
 // Carry chain of one column across its nseg segments, run by one thread:
 // slot u*C + lane holds alpha_u on entry and the carry into segment u on
 // exit, for c' = alpha_u + aL * c walked from segment 0 up (forward) or from
@@ -319,6 +341,93 @@ __device__ __forceinline__ void seg_solve(const SweepArgs& A, int C, const SegPo
     }
 }
 
+// The same arithmetic as seg_solve (the same operations in the same order,
+// so the same bits) for a segment whose values come from and go to LDS:
+// lane = its column in the block (0..C-1, C any count), s its substrate,
+// p0 its first node, slot = g*C + lane its carry slot; gtab = the correction
+// tables and K in stage_tables' layout; sink(t, z) receives the final value
+// of node p0 + t, from the top of the segment down.
+template <typename T, int L, class Sink>
+__device__ __forceinline__ void seg_core(const SweepArgs& A, int C, int lane, int s, int p0, int slot,
+                                         bool last_writer, const T (&raw)[L], double r, double a, double* xa,
+                                         double* xb, double* wfirst, double* wlast, const double* gtab,
+                                         Sink&& sink)
+{
+    const int n = A.n, nseg = A.nseg, S = A.S, kg0 = A.kg0, kg1 = A.kg1;
+    const int tid = threadIdx.x;
+    const bool has_last = p0 + L >= n;  // holds node n-1 (and nodes past it)
+    const double* K = gtab + (kg0 + kg1 + 3 * kPad) * S;  // read after the first barrier
+
+    // forward, zero carry-in
+    double v[L];
+    double y = 0.0;
+#pragma unroll
+    for (int t = 0; t < L; ++t) v[t] = y = fma(a, y, (double)raw[t] * r);
+    double aL = a;
+#pragma unroll
+    for (int t = 1; t < L; ++t) aL *= a;
+    xa[tid] = y;
+    lds_barrier();
+    if (tid < C) chain(xa, aL, lane, C, nseg, true);
+    lds_barrier();
+    {
+        double PY = xa[slot];
+#pragma unroll
+        for (int t = 0; t < L; ++t) {
+            PY *= a;
+            v[t] += PY;
+        }
+    }
+    if (has_last && last_writer) {  // z_{n-1} = y_{n-1}
+        double wl = 0.0;
+#pragma unroll
+        for (int t = 0; t < L; ++t)
+            if (p0 + t == n - 1) wl = v[t];
+        wlast[lane] = wl;
+    }
+
+    // backward, zero carry-in from above
+    double z = 0.0;
+    if (!has_last) {
+#pragma unroll
+        for (int t = L - 1; t >= 0; --t) v[t] = z = fma(a, z, v[t]);
+    } else {
+#pragma unroll
+        for (int t = L - 1; t >= 0; --t) v[t] = z = fma(p0 + t < n - 1 ? a : 0.0, z, v[t]);
+    }
+    xb[tid] = z;
+    lds_barrier();
+    if (tid < C) wfirst[lane] = chain(xb, aL, lane, C, nseg, false);
+    lds_barrier();
+
+    // Last pass, from the top of the segment down, each node finished and
+    // stored as soon as it is final: the carry from above (a^(q-i) Z), the
+    // rank-2 correction near the line ends (mu0 g0_i + mu1 g1_i), store.
+    const double QZ0 = has_last ? 0.0 : xb[slot];  // the last segment has nothing above
+    const bool head = p0 < kg0, tail = p0 + L > n - kg1;
+    double mu0 = 0.0, mu1 = 0.0;
+    if (head || tail) {
+        const double w0 = wfirst[lane], wn = wlast[lane];
+        const double* k = K + s * 8 + 2;  // K00, K01, K10, K11
+        mu0 = head ? k[0] * w0 + k[1] * wn : 0.0;
+        mu1 = tail ? k[2] * w0 + k[3] * wn : 0.0;
+    }
+    // g0 rows from node p0 up; g1 rows from node p0 down (zero padded)
+    const double* G0 = gtab;
+    const double* G1 = gtab + (kg0 + kPad) * S;
+    const double* g0r = G0 + (head ? p0 : 0) * S + s;
+    const double* g1r = G1 + (tail ? kPad + n - 1 - p0 : 0) * S + s;
+    double QZ = QZ0;
+#pragma unroll
+    for (int t = L - 1; t >= 0; --t) {
+        QZ *= a;
+        double zf = v[t] + QZ;
+        if (head) zf = fma(mu0, g0r[t * S], zf);
+        if (tail) zf = fma(mu1, g1r[-t * S], zf);
+        sink(t, zf);
+    }
+}
+
 // Block-level body of the segmented sweep for one plane's column tile.
 template <typename T, int L>
 __device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, int plane,
@@ -342,7 +451,10 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4) k_sweep(SweepArg
     __shared__ double wfirst[COLS];  // w at node 0 of each column
     __shared__ double wlast[COLS];   // w at node n-1 of each column
     extern __shared__ double gtab[];
-    sweep_tile<T, L>(A, COLS, blockIdx.x, blockIdx.y, xa, xb, wfirst, wlast, gtab);
+    const unsigned w =
+        xcd_tile(blockIdx.x + blockIdx.y * (unsigned)A.tiles, (unsigned)(A.tiles * A.planes), A.spread);
+    const unsigned plane = w / (unsigned)A.tiles;
+    sweep_tile<T, L>(A, COLS, (int)(w - plane * (unsigned)A.tiles), (int)plane, xa, xb, wfirst, wlast, gtab);
 }
 
 // One grid, resident blocks: block b solves tiles b, b + gridDim.x, ... (tile
@@ -364,7 +476,8 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_stream(SweepArgs A, in
     const int total = A.tiles * A.planes;
     int w = blockIdx.x;  // the same in every lane, as is everything derived from it
     T raw[L];
-    seg_load<T, L, true>(A, seg_pos<T, L>(A, COLS, w % A.tiles, w / A.tiles), raw);
+    int wr = (int)xcd_tile((unsigned)w, (unsigned)total, A.spread);  // tile w, renumbered per XCD
+    seg_load<T, L, true>(A, seg_pos<T, L>(A, COLS, wr % A.tiles, wr / A.tiles), raw);
     stage_tables(A, gtab);
     lds_barrier();
     const double* K = gtab + (A.kg0 + A.kg1 + 3 * kPad) * A.S;
@@ -374,15 +487,16 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_stream(SweepArgs A, in
         // the last pass loads its own tile again rather than branch around
         // the loads: with no load behind a branch, the waits before the
         // arithmetic below count the prefetch as in flight
-        const int wl = more ? wn : w;
+        const int wl = (int)xcd_tile((unsigned)(more ? wn : w), (unsigned)total, A.spread);
         T nxt[L];
         seg_load<T, L, true>(A, seg_pos<T, L>(A, COLS, wl % A.tiles, wl / A.tiles), nxt);
-        const SegPos P = seg_pos<T, L>(A, COLS, w % A.tiles, w / A.tiles);
+        const SegPos P = seg_pos<T, L>(A, COLS, wr % A.tiles, wr / A.tiles);
         seg_solve<T, L, true>(A, COLS, P, raw, K[P.s * 8 + 0], K[P.s * 8 + 1], xa, xb, wfirst, wlast, gtab);
         if (!more) break;
 #pragma unroll
         for (int t = 0; t < L; ++t) raw[t] = nxt[t];
         w = wn;
+        wr = wl;
     }
 }
 
@@ -391,14 +505,14 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_stream(SweepArgs A, in
 // kMaxThreads threads (A[m].cols * A[m].nseg of them active).
 template <typename T, int L>
 __global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4)
-k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ first, int nm)
+k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ first, int nm, int spread)
 {
     __shared__ double xa[kMaxThreads];
     __shared__ double xb[kMaxThreads];
     __shared__ double wfirst[kMaxThreads];
     __shared__ double wlast[kMaxThreads];
     extern __shared__ double gtab[];
-    const int b = blockIdx.x;
+    const int b = (int)xcd_tile(blockIdx.x, gridDim.x, spread);
     int lo = 0, hi = nm;  // first[lo] <= b < first[hi]
     while (hi - lo > 1) {
         const int mid = (lo + hi) / 2;
@@ -408,6 +522,115 @@ k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ first,
     const SweepArgs A = args[lo];
     const int local = b - first[lo];
     sweep_tile<T, L>(A, A.cols, local % A.tiles, local / A.tiles, xa, xb, wfirst, wlast, gtab);
+}
+
+// x sweep on row tiles staged in LDS.  A row is the nx*S values of one (j, k)
+// pair, contiguous in the store; a tile is R consecutive rows (one contiguous
+// run of R*nx*S values), loaded and stored with one value per lane per
+// instruction, consecutive lanes on consecutive values, and transposed
+// through LDS so that each thread finds its (row, substrate) line segment
+// there as L consecutive values.  The solve is seg_core's (bit-identical to
+// k_sweep with the same L).
+//
+// LDS: xa, xb (one double per thread), wfirst, wlast (C each), the correction
+// tables (stage_tables), then the tile:
+// node t of segment g of line l = r*S + s at (g*C + l)*(L+1) + t.  L+1 is
+// odd, so the 64 lanes of a wave, reading node t of 64 consecutive (g, l)
+// slots, meet 64 different banks.  The nodes past n-1 of the last segment
+// are zeros (they reach no real node: the backward step into node n-1
+// multiplies them by 0).
+struct RowTile {
+    SweepArgs A;             // axis-0 arguments of the planes swept (rho at the first row)
+    size_t vals;             // values swept: rows * nx * S
+    int R, C;                // rows per tile, lines per tile (R*S)
+    int nxS;                 // values per row
+    unsigned mag_nxS, mag_S;  // ceil(2^32/d), d >= 2: e/d = umulhi(e, mag) for e*d < 2^32
+    int tile_vals;           // R*nx*S
+};
+
+// LDS slot of tile value e (row-major, substrate innermost in the store).
+template <int L>
+__device__ __forceinline__ unsigned row_tile_slot(const RowTile& X, unsigned e)
+{
+    const unsigned S = (unsigned)X.A.S;
+    const unsigned r = __umulhi(e, X.mag_nxS);
+    const unsigned w = e - r * (unsigned)X.nxS;
+    const unsigned i = S == 1 ? w : __umulhi(w, X.mag_S);
+    const unsigned s = w - i * S;
+    const unsigned g = i / L, t = i - g * L;
+    return (g * (unsigned)X.C + r * S + s) * (L + 1) + t;
+}
+
+__device__ __forceinline__ void val_store(float*, __amdgpu_buffer_rsrc_t r, float x, unsigned vo)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, vo, 0, kStorePolicy);
+}
+__device__ __forceinline__ void val_store(double*, __amdgpu_buffer_rsrc_t r, double x, unsigned vo)
+{
+    using u64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, x), r, vo, 0, kStorePolicy);
+}
+
+// Grid: one block per tile; blockDim >= C*nseg threads, L*blockDim >= R*nx*S.
+template <typename T, int L>
+__global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_rows(RowTile X)
+{
+    extern __shared__ double smem[];
+    const SweepArgs& A = X.A;
+    const int nthr = blockDim.x, tid = threadIdx.x;
+    double* xa = smem;
+    double* xb = xa + nthr;
+    double* wfirst = xb + nthr;
+    double* wlast = wfirst + X.C;
+    double* gtab = wlast + X.C;
+    T* tile = (T*)(gtab + (A.kg0 + A.kg1 + 3 * kPad + 8) * A.S);
+
+    // the buffer ends with the tile's last real value: reads past it give 0
+    // (the rows past the last are zeros) and stores past it are dropped
+    const size_t first = (size_t)xcd_tile(blockIdx.x, gridDim.x, A.spread) * X.tile_vals;
+    const unsigned range = (unsigned)min((size_t)X.tile_vals, X.vals - first) * (unsigned)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc((T*)A.rho + first, range);
+    {
+        T v[L];
+#pragma unroll
+        for (int k = 0; k < L; ++k)
+            v[k] = buf_load((T*)nullptr, rs, (unsigned)(tid + k * nthr) * (unsigned)sizeof(T), 0);
+        stage_tables(A, gtab);  // while the loads are in flight
+        // zeros past node n-1 of each line's last segment
+        const int tail = A.nseg * L - A.n;
+        T* last = tile + (size_t)(A.nseg - 1) * X.C * (L + 1) + (L - tail);
+        for (int q = tid; q < X.C * tail; q += nthr) {
+            const int l = q / tail;
+            last[l * (L + 1) + (q - l * tail)] = (T)0;
+        }
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            const unsigned e = (unsigned)(tid + k * nthr);
+            if (e < (unsigned)X.tile_vals) tile[row_tile_slot<L>(X, e)] = v[k];
+        }
+    }
+    lds_barrier();
+
+    const int lane = tid % X.C, graw = tid / X.C;
+    const int g = min(graw, A.nseg - 1);  // threads past the last segment are copies of it
+    const int s = lane % A.S;
+    T* seg = tile + (g * X.C + lane) * (L + 1);
+    T raw[L];
+#pragma unroll
+    for (int t = 0; t < L; ++t) raw[t] = seg[t];
+    const double* K = gtab + (A.kg0 + A.kg1 + 3 * kPad) * A.S;
+    const bool act = graw < A.nseg;
+    seg_core<T, L>(A, X.C, lane, s, g * L, g * X.C + lane, graw == A.nseg - 1, raw, K[s * 8 + 0], K[s * 8 + 1],
+                   xa, xb, wfirst, wlast, gtab, [&](int t, double zf) {
+                       if (act) seg[t] = (T)zf;
+                   });
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < L; ++k) {
+        const unsigned e = (unsigned)(tid + k * nthr);
+        if (e < (unsigned)X.tile_vals)
+            val_store((T*)nullptr, rs, tile[row_tile_slot<L>(X, e)], e * (unsigned)sizeof(T));
+    }
 }
 
 // One thread owns one whole (line, substrate) pair and runs the reference's
@@ -542,6 +765,7 @@ SweepArgs sweep_args(const Env& e, int axis, int cols, int nseg, int k0 = 0, int
     A.cols = cols;
     A.tiles = (gm.ncols + cols - 1) / cols;
     A.planes = k1 - k0;
+    A.spread = e.spread[axis];
     return A;
 }
 
@@ -627,10 +851,86 @@ hipError_t launch_cols_stream(Env& e, int axis, int L, int nseg, int k0, int k1)
     }
 }
 
+// Row tiles of the x sweep (k_sweep_rows): plan.cols = C = R*S lines, R rows
+// per tile.
+constexpr size_t kMaxRowTileLds = 150 * 1024;
+
+struct RowShape {
+    int R = 0, C = 0, nseg = 0, threads = 0;
+    size_t lds = 0;
+};
+
+RowShape row_shape(const Env& e, int C, int L, int elem)
+{
+    RowShape r;
+    if (C <= 0 || L <= 0 || C % e.S != 0 || e.nx < 2) return r;
+    r.C = C;
+    r.R = C / e.S;
+    r.nseg = (e.nx + L - 1) / L;
+    r.threads = (C * r.nseg + 63) / 64 * 64;
+    const size_t tables = (size_t)(e.ax[0].kg0 + e.ax[0].kg1 + 3 * kPad + 8) * e.S;
+    r.lds = sizeof(double) * (2 * r.threads + 2 * C + tables) + (size_t)elem * C * r.nseg * (L + 1);
+    return r;
+}
+
+bool rows_plan_valid(const Env& e, int C, int L)
+{
+    const RowShape r = row_shape(e, C, L, (int)e.elem);
+    const size_t nxS = (size_t)e.nx * e.S, tile_vals = (size_t)r.R * nxS;
+    // the slot arithmetic divides by umulhi: exact for numerator * divisor < 2^32
+    return r.C > 0 && r.C * r.nseg <= kMaxThreads && r.lds <= kMaxRowTileLds && tile_vals * nxS < (1ull << 32);
+}
+
+unsigned div_magic(unsigned d)
+{
+    return (unsigned)((((unsigned long long)1 << 32) + d - 1) / d);
+}
+
+template <typename T, int L>
+hipError_t launch_rows(Env& e, const SweepPlan& p, int k0, int k1)
+{
+    const RowShape sh = row_shape(e, p.cols, L, sizeof(T));
+    RowTile X;
+    X.A = sweep_args(e, 0, p.cols, sh.nseg, k0, k1);
+    const long long rows = (long long)e.ny * (k1 - k0);
+    X.R = sh.R;
+    X.C = sh.C;
+    X.nxS = e.nx * e.S;
+    X.mag_nxS = div_magic((unsigned)X.nxS);
+    X.mag_S = e.S > 1 ? div_magic((unsigned)e.S) : 0;  // S = 1 divides by nothing
+    X.tile_vals = X.R * X.nxS;
+    X.vals = (size_t)rows * X.nxS;
+    const long long grid = (rows + X.R - 1) / X.R;
+    if (grid >= (1ll << 31)) return hipErrorInvalidValue;
+    static bool lds_set = false;  // per instantiation
+    if (!lds_set) {
+        hipError_t st = hipFuncSetAttribute((const void*)k_sweep_rows<T, L>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxRowTileLds);
+        if (st != hipSuccess) return st;
+        lds_set = true;
+    }
+    k_sweep_rows<T, L><<<(unsigned)grid, sh.threads, sh.lds, e.stream>>>(X);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_rows_L(Env& e, const SweepPlan& p, int k0, int k1)
+{
+    switch (p.L) {
+    case 4: return launch_rows<T, 4>(e, p, k0, k1);
+    case 8: return launch_rows<T, 8>(e, p, k0, k1);
+    case 12: return launch_rows<T, 12>(e, p, k0, k1);
+    case 16: return launch_rows<T, 16>(e, p, k0, k1);
+    case 24: return launch_rows<T, 24>(e, p, k0, k1);
+    default: return launch_rows<T, 32>(e, p, k0, k1);
+    }
+}
+
 // Whether the segmented kernels run this axis (else the per-line fallback).
 template <typename T>
 bool segmented(const Env& e, int axis, const SweepPlan& p)
 {
+    if (p.stream == 2) return axis == 0 && rows_plan_valid(e, p.cols, p.L);
     Geometry gm = geometry(e, axis);
     // buffer offsets are 32-bit: the segmented kernel addresses at most 4 GiB
     // from a plane's start
@@ -651,6 +951,7 @@ hipError_t sweep(Env& e, int axis, int k0 = 0, int k1 = -1)
     SweepPlan p = plan_for(e, axis);
     if (!segmented<T>(e, axis, p)) return sweep_naive<T>(e, axis);
     if (k1 < 0) k1 = gm.nplanes;
+    if (p.stream == 2) return launch_rows_L<T>(e, p, k0, k1);
     int nseg = (gm.n + p.L - 1) / p.L;
     const long long tiles = ((long long)gm.ncols + p.cols - 1) / p.cols;
     if (p.stream && stream_nodes_ok(p.L, e.precision) && tiles * (k1 - k0) < (1ll << 31)) {
@@ -674,15 +975,21 @@ SweepPlan effective_plan(const Env& e, int axis)
     return plan_for(e, axis);
 }
 
-std::vector<SweepPlan> sweep_candidates(int n, int precision)
+std::vector<SweepPlan> sweep_candidates(const Env& e, int axis)
 {
+    const int n = axis == 0 ? e.nx : axis == 1 ? e.ny : e.nz;
     std::vector<SweepPlan> out;
     for (int cols : kColsSet)
         for (int L : kNodes)
             if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n)) {
                 out.push_back(SweepPlan{cols, L, 0});
-                if (stream_nodes_ok(L, precision)) out.push_back(SweepPlan{cols, L, 1});
+                if (stream_nodes_ok(L, e.precision)) out.push_back(SweepPlan{cols, L, 1});
             }
+    if (axis == 0)
+        for (int R = 1; R <= 64; R *= 2)
+            for (int L : kNodes)
+                if ((L <= 8 || L < 2 * n) && rows_plan_valid(e, R * e.S, L))
+                    out.push_back(SweepPlan{R * e.S, L, 2});
     return out;
 }
 
@@ -702,9 +1009,9 @@ bool batch_member_ok_t(const Env& e, int axis, int L)
 
 template <typename T, int L>
 hipError_t launch_batch_t(const SweepArgs* d_args, const int* d_first, int nm, int blocks,
-                          size_t lds, hipStream_t st)
+                          size_t lds, hipStream_t st, int spread)
 {
-    k_sweep_batch<T, L><<<blocks, kMaxThreads, lds, st>>>(d_args, d_first, nm);
+    k_sweep_batch<T, L><<<blocks, kMaxThreads, lds, st>>>(d_args, d_first, nm, spread);
     return hipGetLastError();
 }
 }  // namespace
@@ -726,22 +1033,22 @@ SweepArgs batch_args(const Env& e, int axis, int L)
 
 size_t batch_lds(const SweepArgs& A) { return sweep_lds(A); }
 
-hipError_t launch_sweep_batch(int precision, int L, const SweepArgs* d_args, const int* d_first,
+hipError_t launch_sweep_batch(int precision, int L, int spread, const SweepArgs* d_args, const int* d_first,
                               int nm, int blocks, size_t lds, hipStream_t st)
 {
     if (precision == BFVM_FP64) {
         switch (L) {
-        case 4: return launch_batch_t<double, 4>(d_args, d_first, nm, blocks, lds, st);
-        case 8: return launch_batch_t<double, 8>(d_args, d_first, nm, blocks, lds, st);
-        case 16: return launch_batch_t<double, 16>(d_args, d_first, nm, blocks, lds, st);
-        default: return launch_batch_t<double, 32>(d_args, d_first, nm, blocks, lds, st);
+        case 4: return launch_batch_t<double, 4>(d_args, d_first, nm, blocks, lds, st, spread);
+        case 8: return launch_batch_t<double, 8>(d_args, d_first, nm, blocks, lds, st, spread);
+        case 16: return launch_batch_t<double, 16>(d_args, d_first, nm, blocks, lds, st, spread);
+        default: return launch_batch_t<double, 32>(d_args, d_first, nm, blocks, lds, st, spread);
         }
     }
     switch (L) {
-    case 4: return launch_batch_t<float, 4>(d_args, d_first, nm, blocks, lds, st);
-    case 8: return launch_batch_t<float, 8>(d_args, d_first, nm, blocks, lds, st);
-    case 16: return launch_batch_t<float, 16>(d_args, d_first, nm, blocks, lds, st);
-    default: return launch_batch_t<float, 32>(d_args, d_first, nm, blocks, lds, st);
+    case 4: return launch_batch_t<float, 4>(d_args, d_first, nm, blocks, lds, st, spread);
+    case 8: return launch_batch_t<float, 8>(d_args, d_first, nm, blocks, lds, st, spread);
+    case 16: return launch_batch_t<float, 16>(d_args, d_first, nm, blocks, lds, st, spread);
+    default: return launch_batch_t<float, 32>(d_args, d_first, nm, blocks, lds, st, spread);
     }
 }
 

@@ -132,9 +132,20 @@ class DeviceMicroenvironment:
         """Picks the fastest launch shape of each sweep (on a scratch copy)."""
         self._check(self.lib.bfvm_autotune(self.h, float(dt)))
 
-    def set_sweep_plan(self, axis, lines_per_block, nodes_per_segment, streaming=False):
+    def set_sweep_plan(self, axis, lines_per_block, nodes_per_segment, streaming=0):
+        """streaming (the kernel): 0 one block per column tile, 1 resident
+        streaming blocks, 2 row tiles through LDS (x only; lines = rows x S)."""
         self._check(self.lib.bfvm_set_sweep_plan(self.h, int(axis), int(lines_per_block),
-                                                 int(nodes_per_segment), int(bool(streaming))))
+                                                 int(nodes_per_segment), int(streaming)))
+
+    def sweep_spread(self, axis):
+        """Tiles per XCD chunk of one axis (0 as dispatched, -1 contiguous eighths)."""
+        v = ctypes.c_int()
+        self._check(self.lib.bfvm_get_sweep_spread(self.h, int(axis), ctypes.byref(v)))
+        return v.value
+
+    def set_sweep_spread(self, axis, tiles_per_chunk):
+        self._check(self.lib.bfvm_set_sweep_spread(self.h, int(axis), int(tiles_per_chunk)))
 
     def set_lod_dimensions(self, dims):
         """3 = LOD_3D (default), 2 = LOD_2D (nz must be 1)."""

@@ -178,24 +178,37 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
  * Autotuner: times every valid launch shape of the three segmented sweeps
  * (lines per block x nodes per segment x kernel) for this grid, substrate
  * count and precision on a scratch copy of the store, keeps the fastest per
- * axis, then times the x/y plane chunk sizes (bfvm_set_sweep_chunk).
+ * axis, then its XCD spread (bfvm_set_sweep_spread), then times the x/y
+ * plane chunk sizes (bfvm_set_sweep_chunk).
  * The densities are left unchanged.  No counterpart in the reference (its
  * OpenMP loops have no launch shape).
  */
 int bfvm_autotune(bfvm_handle h, double dt);
 
 /* The launch shape in use for one axis (0 = x, 1 = y, 2 = z):
- * lines per block, nodes per segment, and streaming (1: resident blocks
- * that load the next tile while solving one; 0: one block per tile);
+ * lines per block, nodes per segment, and the kernel ("streaming"): 0 one
+ * block per column tile, 1 resident blocks that load the next tile while
+ * solving one, 2 (x only) tiles of whole rows loaded contiguously and
+ * transposed through LDS, lines_per_block = rows per tile x substrates;
  * *lines_per_block = 0 means the one-thread-per-line fallback.  Any output
  * pointer may be NULL. */
 int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment,
                         int* streaming);
 
 /* Sets the launch shape of one axis by hand (lines_per_block = 0 restores
- * the default).  Returns BFVM_E_ARG for a shape the kernel cannot run. */
+ * the default).  Returns BFVM_E_ARG for a shape the kernel cannot run
+ * (for kernel 2, the shapes bfvm_autotune would try). */
 int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment,
                         int streaming);
+
+/* How the column tiles of one axis's sweep are spread over the GPU's 8 XCDs
+ * (each with its own L2; workgroups are dealt to them round robin):
+ * 0 = as dispatched, k > 0 = each XCD runs chunks of k consecutive tiles,
+ * -1 = each XCD runs one contiguous eighth of the tiles (the default).  It
+ * changes which workgroup computes which tile, never the result.  Set by
+ * bfvm_autotune; no counterpart in the reference. */
+int bfvm_get_sweep_spread(bfvm_handle h, int axis, int* tiles_per_chunk);
+int bfvm_set_sweep_spread(bfvm_handle h, int axis, int tiles_per_chunk);
 
 /*
  * Gradient vectors of the current densities (Microenvironment::

@@ -164,8 +164,12 @@ def test_autotune_keeps_state_and_parity(precision):
         before = m.download()
         m.autotune(case["dt"])
         np.testing.assert_array_equal(m.download(), before)
-        for cols, L, streaming in m.sweep_plans():
-            assert cols in (16, 32, 64) and 4 <= L <= 32 and streaming in (0, 1)
+        for axis, (cols, L, streaming) in enumerate(m.sweep_plans()):
+            assert 4 <= L <= 32 and streaming in (0, 1, 2)
+            if streaming == 2:
+                assert axis == 0 and cols % case["S"] == 0
+            else:
+                assert cols in (16, 32, 64)
         m.run(case["dt"], case["steps"], case["phases"])
         assert rel_err(m.download(), expected) <= TOL[precision]
 
@@ -208,6 +212,68 @@ def test_every_launch_shape_against_oracle(plan_case, precision):
             m.set_sweep_plan(axis, 0, 0)
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_xcd_spread_never_changes_results(plan_case, precision):
+    # the spread only renumbers which workgroup solves which tile
+    case, ref = plan_case
+    with setup_case(case, precision) as m:
+        outs = []
+        for spread in (0, 1, 3, 16, 64, 1000, -1):
+            for axis in range(3):
+                m.set_sweep_spread(axis, spread)
+                assert m.sweep_spread(axis) == spread
+            m.upload(case["rho0"])
+            m.run(case["dt"], case["steps"], case["phases"])
+            outs.append(m.download())
+        for o in outs[1:]:
+            np.testing.assert_array_equal(o, outs[0])
+        assert rel_err(outs[0], ref) <= TOL[precision]
+        with pytest.raises(BfvmError):
+            m.set_sweep_spread(0, -2)
+
+
+ROW_SHAPES = [(112, 128, 80, 3), (37, 21, 5, 2), (400, 6, 3, 8), (2, 9, 7, 1), (70, 3, 2, 5)]
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_row_tile_x_sweep_equals_column_tiles(precision):
+    # the LDS row-tile x sweep runs the same arithmetic as the column-tile
+    # kernel with the same segment length: bit-identical stores, for every
+    # row-tile shape the library accepts (partial last tiles and segments)
+    rng = np.random.default_rng(91)
+    for nx, ny, nz, S in ROW_SHAPES:
+        case = W.make_case(nx, ny, nz, S, 10.0, 10.0 ** rng.uniform(2, 5, S), 10.0 ** rng.uniform(-3, 0, S),
+                           0.01, 2, W.DIFFUSION, rng.uniform(0, 1, nx * ny * nz * S))
+        ref = oracle.oracle_run(case)
+        tried = 0
+        with setup_case(case, precision) as m:
+            for L in (4, 8, 12, 16, 24, 32):
+                base = None
+                for cols in (16, 32, 64):
+                    try:
+                        m.set_sweep_plan(0, cols, L, 0)
+                    except BfvmError:
+                        continue
+                    m.upload(case["rho0"])
+                    m.run(case["dt"], case["steps"], case["phases"])
+                    base = m.download()
+                    break
+                for R in (1, 2, 4, 8, 16, 32, 64):
+                    try:
+                        m.set_sweep_plan(0, R * S, L, 2)
+                    except BfvmError:
+                        continue
+                    assert m.sweep_plans()[0] == (R * S, L, 2)
+                    m.upload(case["rho0"])
+                    m.run(case["dt"], case["steps"], case["phases"])
+                    got = m.download()
+                    tried += 1
+                    assert rel_err(got, ref) <= TOL[precision], (nx, ny, nz, S, R, L)
+                    if base is not None:
+                        np.testing.assert_array_equal(got, base, err_msg=str((nx, ny, nz, S, R, L)))
+        assert tried > 0, (nx, ny, nz, S)
+
+
 def test_unrunnable_launch_shapes_are_refused():
     with DeviceMicroenvironment(40, 40, 300, 2, 10.0, FP64) as m:
         with pytest.raises(BfvmError):
@@ -216,6 +282,10 @@ def test_unrunnable_launch_shapes_are_refused():
             m.set_sweep_plan(0, 32, 24, True)  # no FP64 streaming kernel with 24 nodes
         with pytest.raises(BfvmError):
             m.set_sweep_plan(0, 48, 8)
+        with pytest.raises(BfvmError):
+            m.set_sweep_plan(1, 2, 8, 2)  # row tiles are for the x sweep only
+        with pytest.raises(BfvmError):
+            m.set_sweep_plan(0, 3, 8, 2)  # lines per row tile must be a multiple of S
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
