@@ -54,10 +54,25 @@ def algorithmic_bytes(case, w):
     return b
 
 
-def kernel_bytes(case, kernel, w):
-    """Algorithmic bytes of one step's launches of `kernel` (for the sweeps: one
-    whole sweep, whether it runs as one launch or plane chunk by chunk)."""
+def source_bytes(case, w):
+    """Algorithmic bytes of the sources the fused x sweep applies: bulk A/B
+    (FP64) and two Dirichlet overwrites (the agents keep their own kernel)."""
     nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    S, ph, b = case["S"], case["phases"], 0
+    if ph & 1 and case.get("bulk_supply") is not None:
+        b += nvs * 2 * 8
+    if ph & 4 and case.get("dir_idx") is not None:
+        b += 2 * len(case["dir_idx"]) * (4 + S * w)
+    return b
+
+
+def kernel_bytes(case, kernel, w, fused=False):
+    """Algorithmic bytes of one step's launches of `kernel` (for the sweeps: one
+    whole sweep, whether it runs as one launch or plane chunk by chunk; the x
+    sweep with the sources fused in adds theirs)."""
+    nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    if kernel == "sweep_x" and fused:
+        return 2 * nvs * w + source_bytes(case, w)
     if kernel.startswith("sweep"):
         return 2 * nvs * w
     if kernel == "bulk":
@@ -87,10 +102,11 @@ def lod_only(per, nvs, w):
             "hbm_frac": 6 * nvs * w / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
-def time_precision(case, precision, steps, warmup):
+def time_precision(case, precision, steps, warmup, fused=True):
     from biofvm_amd import setup_case
     m = setup_case(case, precision)
     dt, ph = case["dt"], case["phases"]
+    m.fused_sources = fused
     m.autotune(dt)  # launch shapes, picked on a scratch copy outside the timed region
     m.run(dt, warmup, ph)
     m.sync()
@@ -396,15 +412,23 @@ def main():
     plans = m32.sweep_plans()
     value = nvs * steps / wall
     step_ms = 1e3 * wall / steps
+    ph = case["phases"]
+    has_bulk = bool(ph & 1 and case.get("bulk_supply") is not None)
+    has_dir = bool(ph & 4 and case.get("dir_idx") is not None and len(case["dir_idx"]) > 0)
+    # the bulk update and the first two Dirichlet overwrites ran inside the x
+    # sweep: no bulk launch, one Dirichlet launch per step (before z)
+    fused = bool((has_bulk or has_dir) and per.get("bulk", (0, 0))[1] == 0
+                 and (not has_dir or per.get("dirichlet", (0, 0))[1] < 2))
     dom = max(per, key=lambda k: per[k][0])
     dom_ms, dom_n = per[dom]  # per step
     dom_avg = dom_ms / dom_n  # per launch
-    dom_bytes = kernel_bytes(case, dom, w_main)
+    dom_bytes = kernel_bytes(case, dom, w_main, fused)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9  # one sweep's bytes over its time per step
     traffic = None
     if dom.startswith("sweep"):
         cols, L, streaming = plans["xyz".index(dom[-1])]
-        kname = "k_sweep_stream" if streaming else "k_sweep"
+        kname = ("k_sweep_x_src" if dom == "sweep_x" and fused else
+                 "k_sweep_rows" if streaming == 2 else "k_sweep_stream" if streaming else "k_sweep")
         traffic = measured_traffic(args.workload,
                                    f"{kname}<{'float' if main_prec == FP32 else 'double'}, {L}, {cols}>")
     step_bytes = algorithmic_bytes(case, w_main)
@@ -429,6 +453,13 @@ def main():
             "l2_rel": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
         del a, b
     m32.close()
+    if fused:
+        # the same step with bulk, agents and Dirichlet as kernels of their own
+        msep, wsep, _, persep, _ = time_precision(case, main_prec, max(1, steps // 2), warmup, fused=False)
+        msep.close()
+        out_extra["separate_source_kernels"] = {
+            "ms_per_step": 1e3 * wsep / max(1, steps // 2),
+            "kernels_ms_per_step": {k: v[0] for k, v in persep.items()}}
     if main_prec == FP32 and not args.no_fp64 and args.error_steps > 0:
         out_extra["error_vs_fp64_curve"] = error_curve(case, args.error_steps, 100)
 
@@ -455,6 +486,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_step": dom_bytes, "ms_per_step": dom_ms, "launches_per_step": dom_n,
                      "avg_launch_ms": dom_avg},
+        "sources_fused_in_x_sweep": fused,
         "kernels_ms_per_step": {k: v[0] for k, v in per.items()},
         "kernel_launches_per_step": {k: v[1] for k, v in per.items()},
         "sweep_xy_chunk_planes": m32_chunk,

@@ -145,6 +145,7 @@ int bfvm_set_agent_rates_device(bfvm_handle h, int n, const double* position, co
     // drop the current agents (the arrays are rebuilt)
     release(e.agent_seg_voxel);
     release(e.agent_seg_start);
+    release(e.agent_bits);
     release(e.agent_t1);
     release(e.agent_t2);
     release(e.agent_c);
@@ -245,6 +246,7 @@ int bfvm_set_agent_rates_device(bfvm_handle h, int n, const double* position, co
     e.agent_order = std::move(order);
     e.agent_rates = true;
     e.agent_dt = dt;
+    GHIP(e, bfvm::build_agent_bits(e));
     return BFVM_OK;
 }
 

@@ -88,6 +88,7 @@ struct Env {
     void* c1 = nullptr;  // dt*D/dx^2 per substrate (FP64)
     AxisCoef ax[3];
     SweepPlan plan[3];  // per-axis launch shape chosen by the autotuner
+    SweepPlan fused_plan;  // the fastest one-block-per-tile x shape (for the fused x sweep)
     // per axis: how tiles are spread over the XCDs (xcd_tile in lod.hip):
     // 0 as dispatched, k > 0 chunks of k consecutive tiles per XCD, -1 one
     // contiguous range per XCD.  Chosen by the autotuner.
@@ -100,12 +101,22 @@ struct Env {
     // decay split in thirds); 2: ..._LOD_2D (BioFVM_solvers.cpp:278-427: x and
     // y sweeps, decay split in halves; nz must be 1).
     int lod_dims = 3;
+    // bulk, agents and the Dirichlet overwrites before the x and y sweeps run
+    // inside the x sweep (load prologue and store epilogue) when it can
+    bool fused_sources = true;
 
-    // Dirichlet nodes.
+    // Dirichlet nodes, one per voxel (the last of repeated entries, as the
+    // reference's in-order overwrite leaves it), sorted by voxel index.
     int n_dir = 0;
     int* dir_idx = nullptr;
     void* dir_val = nullptr;          // n_dir*S, device precision
     unsigned char* dir_mask = nullptr;  // S bytes
+    // The same nodes as a bitmap over voxels (bit v%32 of word v/32, two
+    // spare zero words at the end) and, per word, the number of nodes in the
+    // words before it: node of voxel v = dir_rank[v/32] + popcount of the
+    // lower bits of its word.  Read by the fused x sweep.
+    unsigned* dir_bits = nullptr;
+    int* dir_rank = nullptr;
     unsigned source_epoch = 0;  // bumped whenever Dirichlet nodes or bulk fields are set
 
     // Bulk fields and the implicit-step terms built from them for one dt:
@@ -125,6 +136,7 @@ struct Env {
     unsigned agent_epoch = 0;  // bumped whenever the agent arrays change
     int* agent_seg_voxel = nullptr;  // n_agent_voxels
     int* agent_seg_start = nullptr;  // n_agent_voxels + 1
+    unsigned* agent_bits = nullptr;  // voxels with agents: bit v%32 of word v/32 (+2 spare words)
     void* agent_t1 = nullptr;        // n_agents*S, grouped order, FP64
     void* agent_t2 = nullptr;
     // Agents set from rates, for the adjoint: c = dt*V/V_voxel per agent and
@@ -159,10 +171,23 @@ struct Env {
 hipError_t launch_dirichlet(Env& e);
 hipError_t launch_bulk_prepare(Env& e, double dt);
 hipError_t launch_bulk(Env& e);
-hipError_t launch_agents(Env& e);
+// with_bulk: the bulk update of each agent voxel first (the fused x sweep
+// then applies it to the voxels without agents only)
+hipError_t launch_agents(Env& e, bool with_bulk = false);
 // Gradient vectors of the store into out (nvs*3 values, store precision).
 hipError_t launch_gradient(Env& e, void* out);
 hipError_t launch_sweep(Env& e, int axis);
+// Whether this grid's x sweep can run with sources fused in (a segmented
+// one-block-per-tile shape: the autotuner's fastest, or the default); if so,
+// launch_sweep_x_fused runs it with, in its load prologue, the bulk update
+// (if `bulk`; skipping voxels with agents if `skip_agent_voxels`, whose bulk
+// update the agent kernel applied) and the Dirichlet overwrite before the
+// sweep, and in its store epilogue the Dirichlet overwrite of the next sweep
+// (if `dir_after`).
+bool sweep_x_fusable(const Env& e);
+hipError_t launch_sweep_x_fused(Env& e, bool bulk, bool skip_agent_voxels, bool dir_after);
+// agent_bits from the grouped agents (after each agent set-up).
+hipError_t build_agent_bits(Env& e);
 // x or y sweep of planes (z index) [k0, k1) only, for plane-chunked steps;
 // valid when sweep_chunkable(e, axis) (the segmented kernels run the axis).
 bool sweep_chunkable(const Env& e, int axis);

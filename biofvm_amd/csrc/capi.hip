@@ -278,26 +278,49 @@ int run_kernel(Env& e, int kind, F&& launch)
 int step_impl(Env& e, double dt, int flags)
 {
     int rc;
-    if ((flags & BFVM_BULK) && e.have_bulk) {
-        HIP_OR_FAIL(&e, bfvm::ensure_bulk(e, dt));
-        if ((rc = run_kernel(e, BFVM_K_BULK, [&] { return bfvm::launch_bulk(e); }))) return rc;
-    }
-    if ((flags & BFVM_CELLS) && e.n_agents > 0) {
-        if ((rc = run_kernel(e, BFVM_K_AGENTS, [&] { return bfvm::launch_agents(e); }))) return rc;
-    }
-    if (flags & BFVM_DIFFUSION) {
+    const bool bulk = (flags & BFVM_BULK) && e.have_bulk;
+    const bool cells = (flags & BFVM_CELLS) && e.n_agents > 0;
+    const bool diffusion = (flags & BFVM_DIFFUSION) != 0;
+    if (diffusion) {
         if (!e.uniform)
             return fail(&e, BFVM_E_MESH,
                         "This algorithm is written for uniform Cartesian meshes (dx == dy == dz).");
         if (e.D.empty()) return fail(&e, BFVM_E_STATE, "coefficients not set (bfvm_set_coefficients)");
         if ((rc = ensure_coefficients_impl(e, dt))) return rc;
+    }
+    if (bulk) HIP_OR_FAIL(&e, bfvm::ensure_bulk(e, dt));
+    const int chunk = e.xy_chunk;
+    const bool chunked = diffusion && chunk > 0 && chunk < e.nz && bfvm::sweep_chunkable(e, 0) &&
+                         bfvm::sweep_chunkable(e, 1);
+    // the bulk update and the first two Dirichlet overwrites inside the x sweep
+    // (the agent kernel applies the bulk update of the voxels with agents)
+    const bool fused = diffusion && e.fused_sources && !chunked && (bulk || e.n_dir > 0) &&
+                       bfvm::sweep_x_fusable(e);
+    if (bulk && !fused) {
+        if ((rc = run_kernel(e, BFVM_K_BULK, [&] { return bfvm::launch_bulk(e); }))) return rc;
+    }
+    if (cells) {
+        if ((rc = run_kernel(e, BFVM_K_AGENTS, [&] { return bfvm::launch_agents(e, bulk && fused); }))) return rc;
+    }
+    if (diffusion) {
         auto dirichlet = [&] {
             return e.n_dir > 0 ? run_kernel(e, BFVM_K_DIRICHLET, [&] { return bfvm::launch_dirichlet(e); })
                                : BFVM_OK;
         };
-        const int chunk = e.xy_chunk;
+        if (fused) {
+            const bool dir_after = e.lod_dims >= 2;
+            if ((rc = run_kernel(e, BFVM_K_SWEEP_X,
+                                 [&] { return bfvm::launch_sweep_x_fused(e, bulk, cells, dir_after); })))
+                return rc;
+            for (int axis = 1; axis < e.lod_dims; ++axis) {
+                if (axis > 1 && (rc = dirichlet())) return rc;
+                if ((rc = run_kernel(e, BFVM_K_SWEEP_X + axis, [&] { return bfvm::launch_sweep(e, axis); })))
+                    return rc;
+            }
+            return BFVM_OK;
+        }
         int axis0 = 0;
-        if (chunk > 0 && chunk < e.nz && bfvm::sweep_chunkable(e, 0) && bfvm::sweep_chunkable(e, 1)) {
+        if (chunked) {
             // x then y per chunk of z planes.  Every Dirichlet launch writes all
             // nodes; the overwrite is idempotent, so each plane still sees it
             // before its x, y and z sweeps and never in the middle of one.
@@ -325,6 +348,7 @@ void free_agents(Env& e)
 {
     dfree_t(e.agent_seg_voxel);
     dfree_t(e.agent_seg_start);
+    dfree_t(e.agent_bits);
     dfree(e.agent_t1);
     dfree(e.agent_t2);
     dfree_t(e.agent_c);
@@ -389,6 +413,7 @@ int set_agents_impl(Env& e, int n, const int* voxel, const double* t1, const dou
     }
     e.n_agents = n;
     e.n_agent_voxels = ng;
+    HIP_OR_FAIL(&e, bfvm::build_agent_bits(e));
     return BFVM_OK;
 }
 
@@ -487,6 +512,9 @@ int bfvm_destroy(bfvm_handle h)
     dfree_t(e->dir_idx);
     dfree(e->dir_val);
     dfree_t(e->dir_mask);
+    dfree_t(e->dir_bits);
+    dfree_t(e->dir_rank);
+    dfree_t(e->agent_bits);
     dfree_t(e->bulk_supply);
     dfree_t(e->bulk_target);
     dfree_t(e->bulk_uptake);
@@ -543,6 +571,8 @@ int bfvm_set_dirichlet(bfvm_handle h, int n, const int* voxel_index, const doubl
     ++h->source_epoch;
     dfree_t(h->dir_idx);
     dfree(h->dir_val);
+    dfree_t(h->dir_bits);
+    dfree_t(h->dir_rank);
     h->n_dir = 0;
     if (n == 0) return BFVM_OK;
     if (!voxel_index || !values) return fail(h, BFVM_E_ARG, "voxel_index and values must be non-NULL");
@@ -552,12 +582,35 @@ int bfvm_set_dirichlet(bfvm_handle h, int n, const int* voxel_index, const doubl
     std::vector<unsigned char> m(h->S, 1);
     if (mask)
         for (int s = 0; s < h->S; ++s) m[s] = mask[s] ? 1 : 0;
-    HIP_OR_FAIL(h, hipMalloc((void**)&h->dir_idx, sizeof(int) * n));
-    HIP_OR_FAIL(h, alloc_values(*h, &h->dir_val, (size_t)n * h->S));
-    HIP_OR_FAIL(h, hipMemcpy(h->dir_idx, voxel_index, sizeof(int) * n, hipMemcpyHostToDevice));
-    HIP_OR_FAIL(h, upload_values(*h, h->dir_val, values, (size_t)n * h->S));
+    // One node per voxel, sorted by voxel: where a voxel is listed more than
+    // once its last entry wins, as the reference's in-order overwrite leaves it.
+    const int S = h->S;
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return voxel_index[a] < voxel_index[b]; });
+    std::vector<int> idx;
+    std::vector<double> val;
+    for (int q = 0; q < n; ++q) {
+        if (q + 1 < n && voxel_index[ord[q + 1]] == voxel_index[ord[q]]) continue;  // a later entry follows
+        idx.push_back(voxel_index[ord[q]]);
+        val.insert(val.end(), values + (size_t)ord[q] * S, values + (size_t)ord[q] * S + S);
+    }
+    const int u = (int)idx.size();
+    const size_t words = h->nvox / 32 + 2;
+    std::vector<unsigned> bits(words, 0u);
+    std::vector<int> rank(words, 0);
+    for (int v : idx) bits[(size_t)v / 32] |= 1u << (v % 32);
+    for (size_t w = 1; w < words; ++w) rank[w] = rank[w - 1] + __builtin_popcount(bits[w - 1]);
+    HIP_OR_FAIL(h, hipMalloc((void**)&h->dir_idx, sizeof(int) * u));
+    HIP_OR_FAIL(h, alloc_values(*h, &h->dir_val, (size_t)u * S));
+    HIP_OR_FAIL(h, hipMalloc((void**)&h->dir_bits, sizeof(unsigned) * words));
+    HIP_OR_FAIL(h, hipMalloc((void**)&h->dir_rank, sizeof(int) * words));
+    HIP_OR_FAIL(h, hipMemcpy(h->dir_idx, idx.data(), sizeof(int) * u, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(h, upload_values(*h, h->dir_val, val.data(), (size_t)u * S));
+    HIP_OR_FAIL(h, hipMemcpy(h->dir_bits, bits.data(), sizeof(unsigned) * words, hipMemcpyHostToDevice));
+    HIP_OR_FAIL(h, hipMemcpy(h->dir_rank, rank.data(), sizeof(int) * words, hipMemcpyHostToDevice));
     HIP_OR_FAIL(h, hipMemcpy(h->dir_mask, m.data(), h->S, hipMemcpyHostToDevice));
-    h->n_dir = n;
+    h->n_dir = u;
     return BFVM_OK;
 }
 
@@ -733,6 +786,7 @@ int bfvm_autotune(bfvm_handle h, double dt)
     for (int axis = 0; axis < 3 && st == hipSuccess; ++axis) {
         bfvm::SweepPlan best{};
         float best_ms = 1e30f;
+        float best_block_ms = 1e30f;  // the fastest one-block-per-tile x shape, for the fused x sweep
         for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(e, axis)) {
             e.plan[axis] = cand;
             if ((st = bfvm::launch_sweep(e, axis)) != hipSuccess) break;  // warm-up
@@ -745,6 +799,10 @@ int bfvm_autotune(bfvm_handle h, double dt)
             if (ms < best_ms) {
                 best_ms = ms;
                 best = cand;
+            }
+            if (axis == 0 && cand.stream == 0 && ms < best_block_ms) {
+                best_block_ms = ms;
+                e.fused_plan = cand;
             }
         }
         e.plan[axis] = best;
@@ -798,6 +856,7 @@ int bfvm_autotune(bfvm_handle h, double dt)
     (void)hipEventDestroy(t1);
     if (st != hipSuccess) {
         for (auto& p : e.plan) p = bfvm::SweepPlan{};
+        e.fused_plan = bfvm::SweepPlan{};
         return hip_fail(&e, st, "bfvm_autotune");
     }
     return BFVM_OK;
@@ -820,6 +879,7 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
+    if (axis == 0) h->fused_plan = bfvm::SweepPlan{};  // the fused x sweep follows the x shape set here
     if (lines_per_block == 0) {
         h->plan[axis] = bfvm::SweepPlan{};
         return BFVM_OK;
@@ -830,6 +890,21 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
             return BFVM_OK;
         }
     return fail(h, BFVM_E_ARG, "launch shape not valid for this axis length");
+}
+
+int bfvm_set_fused_sources(bfvm_handle h, int on)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    HIP_OR_FAIL(h, quiesce(*h));
+    h->fused_sources = on != 0;
+    return BFVM_OK;
+}
+
+int bfvm_get_fused_sources(bfvm_handle h, int* on)
+{
+    if (!h || !on) return fail(h, BFVM_E_ARG, "null argument");
+    *on = h->fused_sources ? 1 : 0;
+    return BFVM_OK;
 }
 
 int bfvm_get_sweep_spread(bfvm_handle h, int axis, int* tiles_per_chunk)

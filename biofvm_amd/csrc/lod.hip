@@ -342,16 +342,22 @@ __device__ __forceinline__ void seg_solve(const SweepArgs& A, int C, const SegPo
 }
 
 // The same arithmetic as seg_solve (the same operations in the same order,
-// so the same bits) for a segment whose values come from and go to LDS:
-// lane = its column in the block (0..C-1, C any count), s its substrate,
-// p0 its first node, slot = g*C + lane its carry slot; gtab = the correction
-// tables and K in stage_tables' layout; sink(t, z) receives the final value
-// of node p0 + t, from the top of the segment down.
-template <typename T, int L, class Sink>
+// so the same bits) with the values' source and destination left to the
+// caller: lane = the segment's column in the block (0..C-1, C any count), s
+// its substrate, p0 its first node, slot = g*C + lane its carry slot; gtab =
+// the correction tables and K in stage_tables' layout; src(t, x) turns raw
+// value t (widened to FP64) into the value the sweep solves for (sources
+// applied before the sweep), in node order; sink(t, z) receives the final
+// value of node p0 + t, from the top of the segment down.
+struct NoSource {
+    __device__ __forceinline__ double operator()(int, double x) const { return x; }
+};
+
+template <typename T, int L, class Src, class Sink>
 __device__ __forceinline__ void seg_core(const SweepArgs& A, int C, int lane, int s, int p0, int slot,
                                          bool last_writer, const T (&raw)[L], double r, double a, double* xa,
                                          double* xb, double* wfirst, double* wlast, const double* gtab,
-                                         Sink&& sink)
+                                         Src&& src, Sink&& sink)
 {
     const int n = A.n, nseg = A.nseg, S = A.S, kg0 = A.kg0, kg1 = A.kg1;
     const int tid = threadIdx.x;
@@ -362,7 +368,7 @@ __device__ __forceinline__ void seg_core(const SweepArgs& A, int C, int lane, in
     double v[L];
     double y = 0.0;
 #pragma unroll
-    for (int t = 0; t < L; ++t) v[t] = y = fma(a, y, (double)raw[t] * r);
+    for (int t = 0; t < L; ++t) v[t] = y = fma(a, y, src(t, (double)raw[t]) * r);
     double aL = a;
 #pragma unroll
     for (int t = 1; t < L; ++t) aL *= a;
@@ -524,6 +530,109 @@ k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ first,
     sweep_tile<T, L>(A, A.cols, local % A.tiles, local / A.tiles, xa, xb, wfirst, wlast, gtab);
 }
 
+// The x sweep with sources fused in (launch_sweep_x_fused).  The reference's
+// order per step is bulk, agents, Dirichlet, x sweep, Dirichlet, y sweep ...
+// (BioFVM_microenvironment.cpp:584-633, BioFVM_solvers.cpp:177-241).  An x
+// line holds consecutive voxels, so the thread that loads a node for the x
+// sweep can apply the updates that precede the sweep, and the store can write
+// the Dirichlet value the y sweep must see:
+//   bulk      x = (x + A)/B (FP64, then the store's precision, as k_bulk
+//             leaves it), except on voxels with agents when the agent kernel
+//             already applied it there (their bits in agent_bits);
+//   Dirichlet the node's value where the voxel's bit is set and the substrate
+//             is masked in: the bits of the segment's L voxels come from two
+//             bitmap words, the value's index from the word's rank.
+// Agents stay a kernel of their own (one thread per voxel group and
+// substrate), run before this one: a voxel group's agents form a chain of
+// dependent loads that one sweep thread, which owns up to L voxels, would run
+// serially.  The same operations and roundings as the separate kernels: the
+// step's bits do not depend on fusing.
+struct FusedSources {
+    const double* bulk_a;         // A, B per value, or null
+    const double* bulk_b;
+    const unsigned* agent_bits;   // voxels whose bulk update is done already, or null
+    const unsigned* dbits;        // Dirichlet bitmap, or null
+    const int* drank;
+    const void* dval;             // node values (store precision), node-major
+    const unsigned char* dmask;   // per substrate
+    int ny;
+    int dir_after;                // also overwrite Dirichlet nodes at the stores
+};
+
+// Bits of voxels v0 .. v0+63 (bit t = voxel v0 + t) from a bitmap with a
+// spare word past its last voxel.
+__device__ __forceinline__ unsigned long long bit_window(const unsigned* bits, int v0)
+{
+    const int w0 = v0 >> 5, b0 = v0 & 31;
+    return (((unsigned long long)bits[w0 + 1] << 32) | bits[w0]) >> b0;
+}
+
+template <typename T, int L, int COLS>
+__global__ void __launch_bounds__(kMaxThreads, L <= 8 ? 8 : 4) k_sweep_x_src(SweepArgs A, FusedSources F)
+{
+    __shared__ double xa[kMaxThreads];
+    __shared__ double xb[kMaxThreads];
+    __shared__ double wfirst[COLS];
+    __shared__ double wlast[COLS];
+    extern __shared__ double gtab[];
+    const unsigned w =
+        xcd_tile(blockIdx.x + blockIdx.y * (unsigned)A.tiles, (unsigned)(A.tiles * A.planes), A.spread);
+    const unsigned plane = w / (unsigned)A.tiles;
+    const int tile = (int)(w - plane * (unsigned)A.tiles);
+    const SegPos P = seg_pos<T, L>(A, COLS, tile, (int)plane);
+    const double r = A.wb[P.s * 8 + 0], a = A.wb[P.s * 8 + 1];
+    T raw[L];
+    seg_load<T, L, false>(A, P, raw);
+    stage_tables(A, gtab);  // while the loads are in flight
+
+    const int S = A.S, s = P.s, p0 = P.p0;
+    const int cnt = min(L, A.n - p0);  // nodes past n-1 use node n-1's sources
+    const int col = min(tile * COLS + (int)(threadIdx.x & (COLS - 1)), A.cols_per_plane - 1);
+    const int v0 = ((int)plane * F.ny + col / S) * A.n + p0;  // voxel of node p0
+
+    if (F.bulk_a) {
+        const unsigned long long done = F.agent_bits ? bit_window(F.agent_bits, v0) : 0ull;
+        // eight nodes at a time, so that at most 8 pairs (A, B) wait in registers
+#pragma unroll
+        for (int t0 = 0; t0 < L; t0 += 8) {
+            double a8[8], b8[8];
+#pragma unroll
+            for (int t = t0; t < t0 + 8 && t < L; ++t) {
+                const size_t e = (size_t)(v0 + min(t, cnt - 1)) * S + s;
+                a8[t - t0] = F.bulk_a[e];
+                b8[t - t0] = F.bulk_b[e];
+            }
+#pragma unroll
+            for (int t = t0; t < t0 + 8 && t < L; ++t)
+                if (!((done >> t) & 1ull)) raw[t] = (T)(((double)raw[t] + a8[t - t0]) / b8[t - t0]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // Dirichlet bits of voxels v0 .. v0+L-1 and the node index of the first
+    unsigned long long dwin = 0;
+    int dbase = 0;
+    const bool dir_on = F.dbits && F.dmask[s];
+    if (dir_on) {
+        dwin = bit_window(F.dbits, v0);
+        dbase = F.drank[v0 >> 5] + __builtin_popcount(F.dbits[v0 >> 5] & ((1u << (v0 & 31)) - 1u));
+    }
+    const T* dval = (const T*)F.dval;
+    auto dir_value = [&](int t) {
+        return (double)dval[(size_t)(dbase + __builtin_popcountll(dwin & ((1ull << t) - 1ull))) * S + s];
+    };
+    auto src = [&](int t, double x) { return dir_on && ((dwin >> t) & 1ull) ? dir_value(t) : x; };
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc<T>(A, P);
+    const unsigned step = A.line_stride * (unsigned)sizeof(T);
+    const bool dir_after = dir_on && F.dir_after;
+    seg_core<T, L>(A, COLS, (int)(threadIdx.x & (COLS - 1)), s, p0, P.slot, P.last_writer, raw, r, a, xa, xb,
+                   wfirst, wlast, gtab, src, [&](int t, double zf) {
+                       if (P.act && t < cnt) {
+                           if (dir_after && ((dwin >> t) & 1ull)) zf = dir_value(t);
+                           buf_store((T*)nullptr, rs, zf, P.vo, t * step);
+                       }
+                   });
+}
+
 // x sweep on row tiles staged in LDS.  A row is the nx*S values of one (j, k)
 // pair, contiguous in the store; a tile is R consecutive rows (one contiguous
 // run of R*nx*S values), loaded and stored with one value per lane per
@@ -621,7 +730,7 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_rows(RowTile X)
     const double* K = gtab + (A.kg0 + A.kg1 + 3 * kPad) * A.S;
     const bool act = graw < A.nseg;
     seg_core<T, L>(A, X.C, lane, s, g * L, g * X.C + lane, graw == A.nseg - 1, raw, K[s * 8 + 0], K[s * 8 + 1],
-                   xa, xb, wfirst, wlast, gtab, [&](int t, double zf) {
+                   xa, xb, wfirst, wlast, gtab, NoSource{}, [&](int t, double zf) {
                        if (act) seg[t] = (T)zf;
                    });
     lds_barrier();
@@ -968,7 +1077,80 @@ hipError_t sweep(Env& e, int axis, int k0 = 0, int k1 = -1)
     }
 }
 
+template <typename T, int L, int COLS>
+hipError_t launch_x_src(Env& e, int nseg, const FusedSources& F)
+{
+    const SweepArgs A = sweep_args(e, 0, COLS, nseg);
+    dim3 grid(A.tiles, A.planes);
+    const int threads = (COLS * nseg + 63) / 64 * 64;
+    k_sweep_x_src<T, L, COLS><<<grid, threads, sweep_lds(A), e.stream>>>(A, F);
+    return hipGetLastError();
+}
+
+template <typename T, int COLS>
+hipError_t launch_x_src_L(Env& e, int L, int nseg, const FusedSources& F)
+{
+    switch (L) {
+    case 4: return launch_x_src<T, 4, COLS>(e, nseg, F);
+    case 8: return launch_x_src<T, 8, COLS>(e, nseg, F);
+    case 12: return launch_x_src<T, 12, COLS>(e, nseg, F);
+    case 16: return launch_x_src<T, 16, COLS>(e, nseg, F);
+    case 24: return launch_x_src<T, 24, COLS>(e, nseg, F);
+    default: return launch_x_src<T, 32, COLS>(e, nseg, F);
+    }
+}
+
+template <typename T>
+hipError_t sweep_x_fused(Env& e, const SweepPlan& p, const FusedSources& F)
+{
+    const int nseg = (e.nx + p.L - 1) / p.L;
+    switch (p.cols) {
+    case 16: return launch_x_src_L<T, 16>(e, p.L, nseg, F);
+    case 32: return launch_x_src_L<T, 32>(e, p.L, nseg, F);
+    default: return launch_x_src_L<T, 64>(e, p.L, nseg, F);
+    }
+}
+
 }  // namespace
+
+// The x shape of the fused sweep: the autotuner's fastest one-block-per-tile
+// shape, else the tuned shape if it is one, else the default.
+SweepPlan fused_x_plan(const Env& e)
+{
+    if (e.fused_plan.cols > 0) return e.fused_plan;
+    const SweepPlan p = plan_for(e, 0);
+    return p.stream == 0 ? p : default_plan(e.nx);
+}
+
+bool sweep_x_fusable(const Env& e)
+{
+    const SweepPlan p = fused_x_plan(e);
+    const bool seg = e.precision == BFVM_FP64 ? segmented<double>(e, 0, p) : segmented<float>(e, 0, p);
+    return seg && p.stream == 0 && e.nvox < (1ull << 31);
+}
+
+hipError_t launch_sweep_x_fused(Env& e, bool bulk, bool skip_agent_voxels, bool dir_after)
+{
+    FusedSources F{};
+    if (bulk) {
+        F.bulk_a = (const double*)e.bulk_A;
+        F.bulk_b = (const double*)e.bulk_B;
+        if (skip_agent_voxels && e.n_agent_voxels > 0) {
+            if (!e.agent_bits) return hipErrorInvalidValue;
+            F.agent_bits = e.agent_bits;
+        }
+    }
+    if (e.n_dir > 0) {
+        F.dbits = e.dir_bits;
+        F.drank = e.dir_rank;
+        F.dval = e.dir_val;
+        F.dmask = e.dir_mask;
+    }
+    F.ny = e.ny;
+    F.dir_after = dir_after ? 1 : 0;
+    const SweepPlan p = fused_x_plan(e);
+    return e.precision == BFVM_FP64 ? sweep_x_fused<double>(e, p, F) : sweep_x_fused<float>(e, p, F);
+}
 
 SweepPlan effective_plan(const Env& e, int axis)
 {

@@ -68,14 +68,19 @@ __global__ void k_bulk(T* __restrict__ rho, const double* __restrict__ A,
 template <typename T>
 __global__ void k_agents(T* __restrict__ rho, int S, int n_groups, const int* __restrict__ gvox,
                          const int* __restrict__ gstart, const double* __restrict__ t1,
-                         const double* __restrict__ t2)
+                         const double* __restrict__ t2, const double* __restrict__ bulk_a,
+                         const double* __restrict__ bulk_b)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)n_groups * S) return;
     int g = (int)(t / S);
     int s = (int)(t - (size_t)g * S);
-    T* r = rho + (size_t)gvox[g] * S + s;
+    const size_t i = (size_t)gvox[g] * S + s;
+    T* r = rho + i;
     double v = (double)*r;
+    // the bulk update of this voxel first, as k_bulk would have left it (the
+    // x sweep applies it to the voxels without agents)
+    if (bulk_a) v = (double)(T)((v + bulk_a[i]) / bulk_b[i]);
     int a1 = gstart[g + 1];
     for (int a = gstart[g]; a < a1; ++a) {
         size_t q = (size_t)a * S + s;
@@ -138,7 +143,30 @@ __global__ void k_gradient(const T* __restrict__ rho, T* __restrict__ out, int n
     out[t * 3 + 2] = (T)gz;
 }
 
+// Bit v%32 of word v/32 for every voxel v that holds agents.
+__global__ void k_agent_bits(const int* __restrict__ gvox, int ng, unsigned* __restrict__ bits)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int v = gvox[g];
+    atomicOr(bits + (v >> 5), 1u << (v & 31));
+}
+
 }  // namespace
+
+hipError_t build_agent_bits(Env& e)
+{
+    if (e.agent_bits) (void)hipFree(e.agent_bits);
+    e.agent_bits = nullptr;
+    if (e.n_agent_voxels <= 0) return hipSuccess;
+    const size_t words = e.nvox / 32 + 2;
+    hipError_t st = hipMalloc((void**)&e.agent_bits, sizeof(unsigned) * words);
+    if (st == hipSuccess) st = hipMemsetAsync(e.agent_bits, 0, sizeof(unsigned) * words, e.stream);
+    if (st != hipSuccess) return st;
+    k_agent_bits<<<grid_for((size_t)e.n_agent_voxels), kBlock, 0, e.stream>>>(e.agent_seg_voxel,
+                                                                             e.n_agent_voxels, e.agent_bits);
+    return hipGetLastError();
+}
 
 hipError_t launch_gradient(Env& e, void* out)
 {
@@ -199,18 +227,20 @@ hipError_t launch_bulk(Env& e)
     return hipGetLastError();
 }
 
-hipError_t launch_agents(Env& e)
+hipError_t launch_agents(Env& e, bool with_bulk)
 {
     if (e.n_agent_voxels == 0) return hipSuccess;
     size_t n = (size_t)e.n_agent_voxels * e.S;
+    const double* ba = with_bulk ? (const double*)e.bulk_A : nullptr;
+    const double* bb = with_bulk ? (const double*)e.bulk_B : nullptr;
     if (e.precision == BFVM_FP64)
         k_agents<double><<<grid_for(n), kBlock, 0, e.stream>>>(
             (double*)e.rho, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
-            (const double*)e.agent_t1, (const double*)e.agent_t2);
+            (const double*)e.agent_t1, (const double*)e.agent_t2, ba, bb);
     else
         k_agents<float><<<grid_for(n), kBlock, 0, e.stream>>>(
             (float*)e.rho, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
-            (const double*)e.agent_t1, (const double*)e.agent_t2);
+            (const double*)e.agent_t1, (const double*)e.agent_t2, ba, bb);
     return hipGetLastError();
 }
 

@@ -138,6 +138,17 @@ class DeviceMicroenvironment:
         self._check(self.lib.bfvm_set_sweep_plan(self.h, int(axis), int(lines_per_block),
                                                  int(nodes_per_segment), int(streaming)))
 
+    @property
+    def fused_sources(self):
+        """Bulk, agents and the first two Dirichlet overwrites inside the x sweep."""
+        v = ctypes.c_int()
+        self._check(self.lib.bfvm_get_fused_sources(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
+    @fused_sources.setter
+    def fused_sources(self, on):
+        self._check(self.lib.bfvm_set_fused_sources(self.h, int(bool(on))))
+
     def sweep_spread(self, axis):
         """Tiles per XCD chunk of one axis (0 as dispatched, -1 contiguous eighths)."""
         v = ctypes.c_int()

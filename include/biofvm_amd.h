@@ -201,6 +201,21 @@ int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* node
 int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment,
                         int streaming);
 
+/* Whether a step runs its bulk update and the Dirichlet overwrites before the
+ * x and y sweeps inside the x sweep (on = 1, the default) or as kernels of
+ * their own (0).  Fused, the thread that loads a node for the x sweep applies
+ * the bulk update and the Dirichlet value, in the reference's order (bulk,
+ * agents, Dirichlet: BioFVM_microenvironment.cpp:584-633, BioFVM_solvers.cpp:
+ * 177-207), and the store writes the Dirichlet value the y sweep must see;
+ * the agent kernel, which runs before the x sweep, applies the bulk update of
+ * the voxels holding agents.  The store pass of the bulk kernel and two
+ * Dirichlet launches are saved.  Same operations and roundings either way:
+ * the same bits for either store precision.  Used when the x sweep can run
+ * a one-block-per-tile shape (the fastest the autotuner timed) and no x/y
+ * plane chunking is set. */
+int bfvm_set_fused_sources(bfvm_handle h, int on);
+int bfvm_get_fused_sources(bfvm_handle h, int* on);
+
 /* How the column tiles of one axis's sweep are spread over the GPU's 8 XCDs
  * (each with its own L2; workgroups are dealt to them round robin):
  * 0 = as dispatched, k > 0 = each XCD runs chunks of k consecutive tiles,

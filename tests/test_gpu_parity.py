@@ -152,9 +152,16 @@ def test_step_by_step_equals_run_and_timed_run():
     with setup_case(case, FP64) as m:
         ms, per = m.timed_run(case["dt"], case["steps"], case["phases"], per_kernel=True)
         b = m.download()
-    assert ms > 0 and per["sweep_x"][1] == case["steps"] and per["dirichlet"][1] == 3 * case["steps"]
+    # the bulk update and the Dirichlet overwrites before the x and y sweeps run inside the x sweep
+    assert ms > 0 and per["sweep_x"][1] == case["steps"] and per["dirichlet"][1] == case["steps"]
+    assert per["bulk"][1] == 0 and per["agents"][1] == case["steps"]
     np.testing.assert_array_equal(a, b)
     assert rel_err(a, expected) <= TOL[FP64]
+    with setup_case(case, FP64) as m:
+        m.fused_sources = False
+        ms, per = m.timed_run(case["dt"], case["steps"], case["phases"], per_kernel=True)
+        assert per["dirichlet"][1] == 3 * case["steps"] and per["bulk"][1] == case["steps"]
+        np.testing.assert_array_equal(m.download(), a)
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
@@ -301,6 +308,7 @@ def test_plane_chunked_xy_sweeps_are_bit_identical(precision):
     case.update(dir_idx=idx, dir_val=rng.uniform(0, 2, idx.size * S), dir_mask=np.array([1, 0, 1], np.uint8))
     ref = oracle.oracle_run(case)
     with setup_case(case, precision) as m:
+        m.fused_sources = False  # chunked sweeps run the separate source kernels
         m.run(case["dt"], case["steps"], case["phases"])
         whole = m.download()
     assert rel_err(whole, ref) <= TOL[precision]
@@ -310,6 +318,51 @@ def test_plane_chunked_xy_sweeps_are_bit_identical(precision):
             assert m.sweep_chunk == chunk
             m.run(case["dt"], case["steps"], case["phases"])
             np.testing.assert_array_equal(m.download(), whole, err_msg=f"chunk {chunk}")
+
+
+FUSED_SHAPES = [(30, 20, 17, 3, 3), (37, 21, 5, 2, 3), (64, 9, 7, 1, 3), (70, 3, 2, 5, 3), (45, 33, 1, 2, 2),
+                (2, 9, 7, 1, 3)]
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_fused_sources_equal_separate_kernels(precision):
+    # bulk, agents (some stacked on one voxel) and masked Dirichlet faces, with
+    # a voxel listed twice (its last entry wins): the x sweep with the bulk
+    # update and Dirichlet overwrites fused in gives the bits of the separate
+    # kernels, for every x launch shape
+    rng = np.random.default_rng(303)
+    for nx, ny, nz, S, dims in FUSED_SHAPES:
+        nvox = nx * ny * nz
+        case = W.make_case(nx, ny, nz, S, 10.0, 10.0 ** rng.uniform(2, 5, S), 10.0 ** rng.uniform(-3, 0, S), 0.01,
+                           3, W.ALL, rng.uniform(0, 1, nvox * S))
+        case.update(W.random_bulk(rng, nvox, S))
+        case.update(W.random_agents(rng, 3 * nvox // 10 + 5, S, nx, ny, nz, 10.0, share_voxel=4))
+        idx = np.concatenate([W.outer_face_voxels(nx, ny, nz), [nvox // 2, 0]]).astype(np.int32)
+        mask = (np.arange(S) % 2 == 0).astype(np.uint8)
+        case.update(dir_idx=idx, dir_val=rng.uniform(0, 2, idx.size * S), dir_mask=mask)
+        if dims == 2:
+            case["lod_dims"] = 2
+        ref = oracle.oracle_run(case)
+        for cols, L in [(0, 0), (16, 4), (32, 8), (16, 12), (64, 16), (32, 24), (32, 32)]:
+            out = {}
+            for fused in (False, True):
+                with setup_case(case, precision) as m:
+                    if cols:
+                        try:
+                            m.set_sweep_plan(0, cols, L, 0)
+                        except BfvmError:
+                            break
+                    m.fused_sources = fused
+                    ms, per = m.timed_run(case["dt"], case["steps"], case["phases"], per_kernel=True)
+                    if fused:  # bulk and two of the Dirichlet overwrites inside the x sweep
+                        steps = case["steps"]
+                        assert per["bulk"][1] == 0 and per["agents"][1] == steps, (nx, ny, nz, S, cols, L)
+                        assert per["dirichlet"][1] == (steps if dims == 3 else 0), (nx, ny, nz, S, cols, L)
+                    out[fused] = m.download()
+            if not out:
+                continue
+            assert rel_err(out[False], ref) <= TOL[precision], (nx, ny, nz, S, cols, L)
+            np.testing.assert_array_equal(out[True], out[False], err_msg=str((nx, ny, nz, S, dims, cols, L)))
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
