@@ -92,7 +92,8 @@ def measured_traffic(workload, kernel_name):
         return None
     with open(path) as f:
         t = json.load(f)
-    return t["traffic_bytes_per_launch"] if t.get("kernel") == kernel_name else None
+    k = t.get("kernels", {}).get(kernel_name)
+    return k["traffic_bytes_per_launch"] if k else None
 
 
 def lod_only(per, nvs, w):
