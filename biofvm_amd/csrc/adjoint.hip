@@ -202,17 +202,34 @@ __global__ void __launch_bounds__(kBlock) k_axis_grad(const double* __restrict__
 }
 
 // gc[s] -= sum of the blocks' edge sums, gc[S + s] -= sum of the l.y sums.
-__global__ void k_axis_grad_finish(const double* __restrict__ partial, int nblocks, int S, int s0, int sc,
-                                   double* __restrict__ gc)
+__global__ void __launch_bounds__(kBlock) k_axis_grad_finish(const double* __restrict__ partial, int nblocks,
+                                                              int S, int s0, int sc, double* __restrict__ gc)
 {
-    const int m = threadIdx.x;
-    if (m >= 2 * kChunk) return;
-    const int s = m % kChunk;
+    // thread t adds blocks t, t + blockDim, ... in order, then a fixed tree in
+    // LDS: the same order on every run (deterministic), log-depth (a serial
+    // loop over the blocks waited for one load after another)
+    __shared__ double red[kBlock][2 * kChunk + 1];
+    const int t = threadIdx.x;
+    double v[2 * kChunk];
+#pragma unroll
+    for (int m = 0; m < 2 * kChunk; ++m) v[m] = 0.0;
+    for (int b = t; b < nblocks; b += blockDim.x)
+#pragma unroll
+        for (int m = 0; m < 2 * kChunk; ++m) v[m] += partial[(size_t)b * 2 * kChunk + m];
+#pragma unroll
+    for (int m = 0; m < 2 * kChunk; ++m) red[t][m] = v[m];
+    __syncthreads();
+    for (int half = blockDim.x / 2; half > 0; half >>= 1) {
+        if (t < half)
+#pragma unroll
+            for (int m = 0; m < 2 * kChunk; ++m) red[t][m] += red[t + half][m];
+        __syncthreads();
+    }
+    if (t >= 2 * kChunk) return;
+    const int s = t % kChunk;
     if (s >= sc) return;
-    double v = 0.0;
-    for (int b = 0; b < nblocks; ++b) v += partial[(size_t)b * 2 * kChunk + m];
-    if (m < kChunk) gc[s0 + s] -= v;
-    else gc[S + s0 + s] -= v;
+    if (t < kChunk) gc[s0 + s] -= red[0][t];
+    else gc[S + s0 + s] -= red[0][t];
 }
 
 // lam = rho - target; partial sums of (rho - target)^2 / 2.
@@ -258,7 +275,7 @@ hipError_t axis_grad(Env& e, Tape& tp, const void* y, int axis)
                                                            e.S, axis, s0, sc, tp.partial);
         hipError_t st = hipGetLastError();
         if (st != hipSuccess) return st;
-        k_axis_grad_finish<<<1, 64, 0, e.stream>>>(tp.partial, kRedBlocks, e.S, s0, sc, tp.gc);
+        k_axis_grad_finish<<<1, kBlock, 0, e.stream>>>(tp.partial, kRedBlocks, e.S, s0, sc, tp.gc);
         if ((st = hipGetLastError()) != hipSuccess) return st;
     }
     return hipSuccess;

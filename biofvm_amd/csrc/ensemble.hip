@@ -24,6 +24,7 @@ struct bfvm_ensemble_t {
     int precision = BFVM_FP64;
     hipStream_t stream = nullptr;
     int L = 8;  // nodes per segment of the batched sweep
+    int spread = 0;  // XCD spread of the batched sweeps' blocks (xcd_tile in lod.hip)
 
     // tables, valid for (dt, L, member coefficient/agent epochs)
     bool tables_valid = false;
@@ -201,7 +202,7 @@ hipError_t enqueue(bfvm_ensemble_t& E, int flags)
                     (st = on_stream(*pm, E.stream, [&] { return bfvm::launch_dirichlet(*pm); })))
                     return st;
             if (E.nbatch[axis] > 0 &&
-                (st = bfvm::launch_sweep_batch(E.precision, E.L, bfvm::kDefaultSpread, E.d_args[axis], E.d_first[axis],
+                (st = bfvm::launch_sweep_batch(E.precision, E.L, E.spread, E.d_args[axis], E.d_first[axis],
                                                E.nbatch[axis], E.blocks[axis], E.lds[axis], E.stream)))
                 return st;
             for (int i : E.solo[axis])
@@ -358,18 +359,23 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
         e->m[i]->rho = scratch[i];
         ++e->m[i]->coef_epoch;  // launch tables point at the scratch stores
     }
-    int best_L = e->L;
+    int best_L = e->L, best_spread = e->spread;
     double best = 1e300;
     int rc = BFVM_OK;
     for (int L : {4, 8, 16, 32}) {
-        e->L = L;
-        double ms = 0;
-        if ((rc = bfvm_ensemble_timed_run(e, dt, 1, BFVM_DIFFUSION, 0, &ms))) break;
-        if ((rc = bfvm_ensemble_timed_run(e, dt, 3, BFVM_DIFFUSION, 0, &ms))) break;
-        if (ms < best) {
-            best = ms;
-            best_L = L;
+        for (int spread : {0, 64, -1}) {
+            e->L = L;
+            e->spread = spread;
+            double ms = 0;
+            if ((rc = bfvm_ensemble_timed_run(e, dt, 1, BFVM_DIFFUSION, 0, &ms))) break;
+            if ((rc = bfvm_ensemble_timed_run(e, dt, 3, BFVM_DIFFUSION, 0, &ms))) break;
+            if (ms < best) {
+                best = ms;
+                best_L = L;
+                best_spread = spread;
+            }
         }
+        if (rc) break;
     }
     (void)hipStreamSynchronize(e->stream);
     for (size_t i = 0; i < e->m.size(); ++i) {
@@ -378,6 +384,7 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
         (void)hipFree(scratch[i]);
     }
     e->L = best_L;
+    e->spread = best_spread;
     e->tables_valid = false;
     drop_graph(*e);
     return rc;

@@ -313,8 +313,10 @@ int bfvm_ensemble_sync(bfvm_ensemble e);
 int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph,
                             double* ms_total);
 
-/* Times the batched sweeps for each segment length (4, 8, 16, 32 nodes) on
- * scratch copies of the stores and keeps the fastest.  Densities unchanged. */
+/* Times the batched sweeps for each segment length (4, 8, 16, 32 nodes) and
+ * spread of their blocks over the XCDs (as dispatched, chunks of 64 tiles,
+ * contiguous eighths) on scratch copies of the stores and keeps the fastest.
+ * Densities unchanged. */
 int bfvm_ensemble_autotune(bfvm_ensemble e, double dt);
 int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment);
 int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment);
