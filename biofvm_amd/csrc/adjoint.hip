@@ -177,18 +177,20 @@ __global__ void __launch_bounds__(kBlock) k_axis_grad(const double* __restrict__
     double acc[2 * kChunk];
 #pragma unroll
     for (int m = 0; m < 2 * kChunk; ++m) acc[m] = 0.0;
-    const size_t nvox = (size_t)nx * ny * nz;
-    const size_t stride = axis == 0 ? 1 : axis == 1 ? (size_t)nx : (size_t)nx * ny;
-    const int n = axis == 0 ? nx : axis == 1 ? ny : nz;
-    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
-         v += (size_t)gridDim.x * blockDim.x) {
-        const int coord = axis == 0 ? (int)(v % nx) : axis == 1 ? (int)((v / nx) % ny) : (int)(v / ((size_t)nx * ny));
-        const bool edge = coord < n - 1;
-        const size_t w = v + stride;
+    // voxel indices in 32 bits (the handle has fewer than 2^31 voxels): the
+    // coordinate along `axis` costs a 32-bit division, not a 64-bit one
+    const unsigned nxu = (unsigned)nx, nxy = (unsigned)nx * (unsigned)ny;
+    const unsigned nvox = nxy * (unsigned)nz;
+    const unsigned stride = axis == 0 ? 1u : axis == 1 ? nxu : nxy;
+    const unsigned n = (unsigned)(axis == 0 ? nx : axis == 1 ? ny : nz);
+    for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += gridDim.x * blockDim.x) {
+        const unsigned coord = axis == 0 ? v % nxu : axis == 1 ? (v / nxu) % (unsigned)ny : v / nxy;
+        const bool edge = coord + 1 < n;
+        const size_t w = (size_t)v + stride;
 #pragma unroll
         for (int m = 0; m < kChunk; ++m) {
             if (m < sc) {
-                const size_t q = v * S + s0 + m;
+                const size_t q = (size_t)v * S + s0 + m;
                 const double l = (double)lam[q], yv = (double)y[q];
                 acc[kChunk + m] += l * yv;
                 if (edge) {
