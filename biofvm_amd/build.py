@@ -62,6 +62,7 @@ def build_facade(verbose=False):
     tmp = FACADE_LIB + ".tmp"
     cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
            f"-I{INCLUDE}", f"-I{FACADE_DIR}", os.path.join(FACADE_DIR, "BioFVM_gpu.cpp"),
+           os.path.join(FACADE_DIR, "BioFVM_MultiCellDS_gpu.cpp"),
            f"-L{HERE}", "-lbiofvm_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp]
     if verbose:
         print(" ".join(shlex.quote(c) for c in cmd), flush=True)

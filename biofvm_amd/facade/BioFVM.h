@@ -24,8 +24,11 @@
  * down if it is newer, and marks the mirror as the newer copy, so the next
  * simulate_* call uploads it.  Also here: the gradient vectors
  * (compute_all_gradient_vectors on the GPU, BioFVM_microenvironment.cpp:658-842)
- * and write_to_matlab (:557-580, packed on the GPU).  MultiCellDS output and
- * the general (non-Cartesian) mesh are not part of this façade.
+ * and write_to_matlab (:557-580, packed on the GPU); MultiCellDS snapshots
+ * (save_microenvironment_to_MultiCellDS_xml_pugi, BioFVM_MultiCellDS.cpp:
+ * 394-720) with the densities' MAT file packed on the GPU.  The general
+ * (non-Cartesian) mesh and reading MultiCellDS files are not part of this
+ * façade.
  */
 #ifndef BIOFVM_AMD_FACADE_H
 #define BIOFVM_AMD_FACADE_H
@@ -58,6 +61,9 @@ class Cartesian_Mesh {
     std::vector<Voxel> voxels;
     double dx, dy, dz, dV, dS, dS_xy, dS_yz, dS_xz;
     bool Cartesian_mesh, uniform_mesh, regular_mesh, use_voxel_faces;
+    // MAT v4: a 4 x voxels matrix "mesh", one column (centre x, y, z, volume)
+    // per voxel (General_Mesh::write_to_matlab, BioFVM_mesh.cpp:350-367)
+    void write_to_matlab(std::string filename);
     std::string units;
 
     int voxel_index(int i, int j, int k);
@@ -178,6 +184,10 @@ class Microenvironment {
     // internal: the device handle (created on first use)
     DeviceState* device(void);
     friend void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, double dt);
+    friend void diffusion_decay_solver__constant_coefficients_LOD_2D(Microenvironment& M, double dt);
+/* The 2-D LOD solver (BioFVM_solvers.h:40, BioFVM_solvers.cpp:278-427) on the
+ * GPU, for meshes of one z layer. */
+void diffusion_decay_solver__constant_coefficients_LOD_2D(Microenvironment& M, double dt);
 
  private:
     std::vector<std::vector<double> > densities;  // host mirror
@@ -236,12 +246,42 @@ Basic_Agent* create_basic_agent(void);
 void delete_basic_agent(int);
 void delete_basic_agent(Basic_Agent*);
 
+/* MultiCellDS metadata (BioFVM_MultiCellDS.h:55-77, defaults
+ * BioFVM_MultiCellDS.cpp:58-74). */
+class MultiCellDS_Metadata {
+ public:
+    std::string MultiCellDS_type;
+    std::string user;
+    std::string program;
+    std::string program_version;
+    std::vector<double> bounding_box;
+    std::string spatial_units;
+    std::string time_units;
+    std::string runtime_time_units;
+    double current_time;
+    double current_runtime;
+    MultiCellDS_Metadata();
+};
+extern MultiCellDS_Metadata default_metadata;
+
+/* MultiCellDS snapshot: filename_base.xml, the data as
+ * filename_base_microenvironment0.mat (default) or as CSV lines, the voxels
+ * as XML (default) or filename_base_mesh0.mat (BioFVM_MultiCellDS.cpp:79-83,
+ * 394-720).  One document per process: later calls update the time, the run
+ * time and the data only. */
+void set_save_biovm_mesh_as_matlab(bool newvalue);
+void set_save_biovm_data_as_matlab(bool newvalue);
+void save_microenvironment_to_MultiCellDS_xml_pugi(std::string filename_base, Microenvironment& M);
+
 void set_default_microenvironment(Microenvironment* M);
 Microenvironment* get_default_microenvironment(void);
 
 /* The GPU LOD solver: same signature as BioFVM_solvers.h:38, assignable to
  * Microenvironment::diffusion_decay_solver. */
 void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, double dt);
+/* The 2-D LOD solver (BioFVM_solvers.h:40, BioFVM_solvers.cpp:278-427) on the
+ * GPU, for meshes of one z layer. */
+void diffusion_decay_solver__constant_coefficients_LOD_2D(Microenvironment& M, double dt);
 
 void zero_function(Microenvironment* pMicroenvironment, int voxel_index,
                    std::vector<double>* write_destination);

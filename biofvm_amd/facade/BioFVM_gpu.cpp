@@ -219,6 +219,7 @@ void Cartesian_Mesh::display_information(std::ostream& os)
 
 struct DeviceState {
     bfvm_handle h = nullptr;
+    int lod_dims = 3;           // the LOD solver the handle is set for
     bool host_newer = true;     // the mirror holds changes the device lacks
     bool device_newer = false;  // the device holds changes the mirror lacks
     bool dirichlet_dirty = true;
@@ -742,9 +743,37 @@ void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, d
         return;
     }
     int rc = M.push_to_device();
+    if (!rc && M.dev->lod_dims != 3) {
+        rc = bfvm_set_lod_dimensions(M.dev->h, 3);
+        if (!rc) M.dev->lod_dims = 3;
+    }
     if (!rc) rc = bfvm_step(M.dev->h, dt, BFVM_DIFFUSION);
     if (rc) {
         report("diffusion_decay_solver__constant_coefficients_LOD_3D", rc, M.dev ? M.dev->h : nullptr);
+        return;
+    }
+    M.dev->device_newer = true;
+}
+
+// BioFVM_solvers.cpp:278-427: x and y sweeps, decay split in halves.  The
+// device solver needs a single z layer (nz = 1); the reference's would sweep
+// the x and y lines of its first layer only.
+void diffusion_decay_solver__constant_coefficients_LOD_2D(Microenvironment& M, double dt)
+{
+    if (M.mesh.uniform_mesh == false) {
+        std::cout << "Error: This algorithm is written for uniform Cartesian meshes. Try: something else."
+                  << std::endl
+                  << std::endl;
+        return;
+    }
+    int rc = M.push_to_device();
+    if (!rc && M.dev->lod_dims != 2) {
+        rc = bfvm_set_lod_dimensions(M.dev->h, 2);
+        if (!rc) M.dev->lod_dims = 2;
+    }
+    if (!rc) rc = bfvm_step(M.dev->h, dt, BFVM_DIFFUSION);
+    if (rc) {
+        report("diffusion_decay_solver__constant_coefficients_LOD_2D", rc, M.dev ? M.dev->h : nullptr);
         return;
     }
     M.dev->device_newer = true;

@@ -205,6 +205,22 @@ def ref_gradients(nx, ny, nz, S, dx, rho):
     return out.reshape(nx * ny * nz, S, 3)
 
 
+def ref_write_multicellds(nx, ny, nz, S, dx, D, lam, rho, filename_base, current_time=0.0,
+                          current_runtime=0.0, mesh_as_matlab=False, data_as_matlab=True, fresh=True):
+    """The compiled reference's save_microenvironment_to_MultiCellDS_xml_pugi
+    (writes filename_base.xml and the MAT files it names); fresh=False
+    updates the document of an earlier call, as the reference's later calls do."""
+    lib = _load_ref()
+    rho, D, lam = _f64(rho), _f64(D), _f64(lam)
+    lib.ref_write_multicellds.restype = ctypes.c_int
+    rc = lib.ref_write_multicellds(nx, ny, nz, S, ctypes.c_double(dx), _ptr(D), _ptr(lam), _ptr(rho),
+                                   str(filename_base).encode(), ctypes.c_double(current_time),
+                                   ctypes.c_double(current_runtime), int(bool(mesh_as_matlab)),
+                                   int(bool(data_as_matlab)), int(bool(fresh)))
+    if rc != 0:
+        raise RuntimeError(f"ref_write_multicellds failed ({rc})")
+
+
 def ref_write_matlab(nx, ny, nz, S, dx, rho, filename):
     """The compiled reference's write_to_matlab of densities rho (origin 0)."""
     lib = _load_ref()

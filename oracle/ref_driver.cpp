@@ -229,4 +229,48 @@ int ref_write_matlab(int nx, int ny, int nz, int S, double dx, const double* rho
     return rc;
 }
 
+/*
+ * MultiCellDS snapshot (save_microenvironment_to_MultiCellDS_xml_pugi,
+ * BioFVM_MultiCellDS.cpp:394-720) of densities rho with coefficients D,
+ * lambda on a uniform grid with origin 0: writes filename_base.xml and the
+ * MAT files it names.  The reference keeps one document per process and only
+ * updates it on later calls; `fresh` != 0 starts a new one (as a first call).
+ */
+int ref_write_multicellds(int nx, int ny, int nz, int S, double dx, const double* D, const double* lam,
+                          const double* rho, const char* filename_base, double current_time,
+                          double current_runtime, int mesh_as_matlab, int data_as_matlab, int fresh)
+{
+    Microenvironment* M = new Microenvironment;
+    set_default_microenvironment(M);
+    M->name = "microenvironment";
+    M->set_density(0, "s0", "dimensionless");
+    for (int s = 1; s < S; ++s) M->add_density("s" + std::to_string(s), "mmHg");
+    M->resize_space_uniform(0.0, nx * dx, 0.0, ny * dx, 0.0, nz * dx, dx);
+    M->spatial_units = "micron";
+    M->mesh.units = "micron";
+    const int nvox = nx * ny * nz;
+    int rc = -1;
+    if ((int)M->mesh.x_coordinates.size() == nx && (int)M->mesh.y_coordinates.size() == ny &&
+        (int)M->mesh.z_coordinates.size() == nz && M->number_of_densities() == S) {
+        for (int s = 0; s < S; ++s) {
+            M->diffusion_coefficients[s] = D[s];
+            M->decay_rates[s] = lam[s];
+        }
+        for (int v = 0; v < nvox; ++v) {
+            std::vector<double>& d = M->density_vector(v);
+            for (int s = 0; s < S; ++s) d[s] = rho[(size_t)v * S + s];
+        }
+        if (fresh) biofvm_doc.reset();
+        default_metadata.current_time = current_time;
+        default_metadata.current_runtime = current_runtime;
+        set_save_biovm_mesh_as_matlab(mesh_as_matlab != 0);
+        set_save_biovm_data_as_matlab(data_as_matlab != 0);
+        save_microenvironment_to_MultiCellDS_xml_pugi(filename_base, *M);
+        rc = 0;
+    }
+    set_default_microenvironment(nullptr);
+    delete M;
+    return rc;
+}
+
 }  // extern "C"

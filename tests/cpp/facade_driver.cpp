@@ -17,6 +17,7 @@
 // also covers the mirror's download/upload path.
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "BioFVM.h"
@@ -42,8 +43,9 @@ void rd(FILE* f, T* p, size_t n)
 
 int main(int argc, char** argv)
 {
-    if (argc != 3 && argc != 5) {
-        fprintf(stderr, "usage: facade_driver case.bin out.bin [grad.bin snapshot.mat]\n");
+    if (argc != 3 && argc != 5 && argc != 7) {
+        fprintf(stderr, "usage: facade_driver case.bin out.bin [grad.bin snapshot.mat [mcds_base flags]]\n"
+                        "  flags: m = mesh as MAT, x = data as CSV lines, - = neither\n");
         return 2;
     }
     FILE* f = fopen(argv[1], "rb");
@@ -81,10 +83,15 @@ int main(int argc, char** argv)
 
     Microenvironment M;
     M.device_precision = precision;
-    M.set_density(0, "substrate0", "dimensionless");
-    for (int s = 1; s < S; ++s) M.add_density("substrate", "dimensionless");
+    M.name = "microenvironment";
+    M.set_density(0, "s0", "dimensionless");
+    for (int s = 1; s < S; ++s) M.add_density("s" + std::to_string(s), "mmHg");
     M.resize_space_uniform(0.0, nx * dx, 0.0, ny * dx, 0.0, nz * dx, dx);
-    M.diffusion_decay_solver = diffusion_decay_solver__constant_coefficients_LOD_3D;
+    M.spatial_units = "micron";
+    M.mesh.units = "micron";
+    // phase bit 8: the 2-D LOD solver (a one-layer mesh)
+    M.diffusion_decay_solver = (phases & 8) ? diffusion_decay_solver__constant_coefficients_LOD_2D
+                                            : diffusion_decay_solver__constant_coefficients_LOD_3D;
     for (int s = 0; s < S; ++s) {
         M.diffusion_coefficients[s] = D[s];
         M.decay_rates[s] = lam[s];
@@ -138,6 +145,17 @@ int main(int argc, char** argv)
             for (int s = 0; s < S; ++s) fwrite(M.gradient_vector((int)v)[s].data(), sizeof(double), 3, g);
         fclose(g);
         M.write_to_matlab(argv[4]);
+    }
+    if (argc == 7) {  // two MultiCellDS snapshots: the document, then its update
+        const std::string base = argv[5], flags = argv[6];
+        set_save_biovm_mesh_as_matlab(flags.find('m') != std::string::npos);
+        set_save_biovm_data_as_matlab(flags.find('x') == std::string::npos);
+        default_metadata.current_time = steps * dt;
+        default_metadata.current_runtime = 1.25;
+        save_microenvironment_to_MultiCellDS_xml_pugi(base + "_a", M);
+        default_metadata.current_time += 1.0;
+        default_metadata.current_runtime = 2.5;
+        save_microenvironment_to_MultiCellDS_xml_pugi(base + "_b", M);
     }
     while (!all_basic_agents.empty()) delete_basic_agent((int)all_basic_agents.size() - 1);
     return 0;
