@@ -103,12 +103,14 @@ def lod_only(per, nvs, w):
             "hbm_frac": 6 * nvs * w / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
 
 
-def time_precision(case, precision, steps, warmup, fused=True):
-    from biofvm_amd import setup_case
+def time_precision(case, precision, steps, warmup, fused=True, plan_x=""):
+    from biofvm_amd import FP32, setup_case
     m = setup_case(case, precision)
     dt, ph = case["dt"], case["phases"]
     m.fused_sources = fused
     m.autotune(dt)  # launch shapes, picked on a scratch copy outside the timed region
+    if plan_x and precision == FP32:  # a fixed x shape (for counter runs of one kernel)
+        m.set_sweep_plan(0, *[int(v) for v in plan_x.split(",")])
     m.run(dt, warmup, ph)
     m.sync()
     t0 = time.perf_counter()
@@ -177,6 +179,7 @@ def cpu_baseline(case, workload):
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     nthreads = min(nthreads, os.cpu_count() or nthreads)
     nvs = nx * ny * nz_s * S
+    one_core = None
     if oracle.have_ref():
         slab["steps"] = 2
         _, secs = oracle.ref_run(slab, nthreads=nthreads, return_seconds=True)
@@ -184,6 +187,12 @@ def cpu_baseline(case, workload):
         slab["steps"] = steps
         _, secs = oracle.ref_run(slab, nthreads=nthreads, return_seconds=True)
         kind, cores = "reference", nthreads
+        if nthreads > 1:  # the same slab on one core, about as long
+            steps1 = max(2, steps // nthreads)
+            s1 = dict(slab, steps=steps1)
+            _, secs1 = oracle.ref_run(s1, nthreads=1, return_seconds=True)
+            one_core = {"value": nvs * steps1 / secs1, "unit": "voxel-substrate updates/s", "cores": 1,
+                        "kind": "reference", "sample": f"{desc}, {steps1} steps, {secs1:.2f} s wall"}
     else:
         slab["steps"] = 2
         t0 = time.perf_counter()
@@ -191,8 +200,9 @@ def cpu_baseline(case, workload):
         secs = time.perf_counter() - t0
         steps = 2
         kind, cores = "port", 1
-    return {"value": nvs * steps / secs, "unit": "voxel-substrate updates/s", "cores": cores,
-            "kind": kind, "sample": f"{desc}, {steps} steps, {secs:.2f} s wall"}
+    out = {"value": nvs * steps / secs, "unit": "voxel-substrate updates/s", "cores": cores,
+           "kind": kind, "sample": f"{desc}, {steps} steps, {secs:.2f} s wall"}
+    return out, one_core
 
 
 def calibrate_copy(mib):
@@ -377,6 +387,8 @@ def main():
                              "differentiation_128"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plan-x", default="", metavar="LINES,NODES,KERNEL",
+                    help="after autotuning, run the FP32 x sweep with this shape (profiling aid)")
     ap.add_argument("--error-steps", type=int, default=1000,
                     help="FP32-vs-FP64 error curve every 100 steps up to this step (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -408,7 +420,7 @@ def main():
     main_prec = FP32 if args.workload == "precision_400" else FP64
     w_main = 4 if main_prec == FP32 else 8
 
-    m32, wall, ev_ms, per, n_done = time_precision(case, main_prec, steps, warmup)
+    m32, wall, ev_ms, per, n_done = time_precision(case, main_prec, steps, warmup, plan_x=args.plan_x)
     m32_chunk = m32.sweep_chunk
     plans = m32.sweep_plans()
     value = nvs * steps / wall
@@ -496,7 +508,9 @@ def main():
     }
     line.update(out_extra)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(case, args.workload)
+        line["cpu_baseline"], one_core = cpu_baseline(case, args.workload)
+        if one_core:
+            line["cpu_baseline_1core"] = one_core
     print(json.dumps(line), flush=True)
 
 
