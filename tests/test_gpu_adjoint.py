@@ -10,7 +10,8 @@ The adjoint has no counterpart in the reference; it is pinned by
     <M x, y> = <x, M^T y>  (FP64, 1e-12);
   - central differences of the FP64 GPU forward at 32^3 x 2, 40 steps, for
     FP32 adjoint gradients (1e-4 of sum_i |g_i d_i|);
-  - torch.autograd.gradcheck of the autograd bridge on a tiny case.
+  - torch.autograd.gradcheck of the autograd bridge on a tiny case;
+  - FP32-store gradients against the FP64-store adjoint's, normwise 1e-4.
 """
 import numpy as np
 import pytest
@@ -180,3 +181,18 @@ def test_lod2d_gradients_against_oracle_central_differences():
               - loss_of(oracle.oracle_run(with_params(case, **{key: p - H * d})), target)) / (2 * H)
         ad = float(np.sum(g[gname].reshape(-1) * d.reshape(-1)))
         assert abs(ad - fd) <= 1e-6 * abs(fd) + 1e-12, (key, ad, fd)
+
+
+def test_fp32_adjoint_against_fp64_adjoint():
+    # the FP32-store gradients against the FP64-store adjoint's, parameter group
+    # by group, normwise (the FP64 adjoint is itself pinned by central differences)
+    case = W.differentiation_128(steps=40, n=32, n_agents=300)
+    rng = np.random.default_rng(12)
+    with setup_case(case, FP64) as m:
+        m.run(case["dt"], case["steps"], case["phases"])
+        target = m.download() * (1 + 0.05 * rng.normal(size=case["rho0"].size))
+    _, g64 = adjoint_grads(case, target, FP64, checkpoint_every=10)
+    _, g32 = adjoint_grads(case, target, FP32, checkpoint_every=10)
+    for name in g64:
+        a, b = np.asarray(g32[name]).ravel(), np.asarray(g64[name]).ravel()
+        assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b), (name, np.linalg.norm(a - b) / np.linalg.norm(b))
