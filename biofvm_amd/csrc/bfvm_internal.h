@@ -55,6 +55,17 @@ struct SweepArgs {
     int spread;              // tiles per XCD chunk (see xcd_tile in lod.hip)
 };
 
+// Arguments of the x sweep on LDS-staged row tiles (k_sweep_rows in lod.hip):
+// a tile is R consecutive x rows, C = R*S lines.
+struct RowTile {
+    SweepArgs A;             // axis-0 arguments of the planes swept (rho at the first row)
+    size_t vals;             // values swept: rows * nx * S
+    int R, C;                // rows per tile, lines per tile (R*S)
+    int nxS;                 // values per row
+    unsigned mag_nxS, mag_S;  // ceil(2^32/d), d >= 2: e/d = umulhi(e, mag) for e*d < 2^32
+    int tile_vals;           // R*nx*S
+};
+
 // Launch shape of the segmented sweep of one axis (see lod.hip).
 struct SweepPlan {
     int cols = 0;    // lines per block (16, 32 or 64; row tiles: a multiple of S); 0 = default
@@ -198,7 +209,12 @@ hipError_t launch_sweep_planes(Env& e, int axis, int k0, int k1);
 bool batch_member_ok(const Env& e, int axis, int L);
 SweepArgs batch_args(const Env& e, int axis, int L);
 size_t batch_lds(const SweepArgs& A);
-hipError_t launch_sweep_batch(int precision, int L, int spread, const SweepArgs* d_args, const int* d_first,
+// Row-tile arguments of a grid's whole x sweep for an ensemble's batched
+// row-tile launch (false: the grid cannot use row tiles of L-node segments).
+bool batch_rows_args(const Env& e, int L, RowTile* X, int* threads, size_t* lds, int* tiles);
+hipError_t launch_rows_batch(int precision, int L, int spread, const RowTile* d_tabs, const int* d_map,
+                             int blocks, int threads, size_t lds, hipStream_t st);
+hipError_t launch_sweep_batch(int precision, int L, int spread, const SweepArgs* d_args, const int* d_map,
                               int nm, int blocks, size_t lds, hipStream_t st);
 
 // Agents of several grids in one launch: one entry per member.

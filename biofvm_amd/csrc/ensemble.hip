@@ -25,6 +25,7 @@ struct bfvm_ensemble_t {
     hipStream_t stream = nullptr;
     int L = 8;  // nodes per segment of the batched sweep
     int spread = 0;  // XCD spread of the batched sweeps' blocks (xcd_tile in lod.hip)
+    int rows_x = 1;  // x sweeps on LDS-staged row tiles (k_sweep_rows_batch) where a member can
 
     // tables, valid for (dt, L, member coefficient/agent epochs)
     bool tables_valid = false;
@@ -32,11 +33,17 @@ struct bfvm_ensemble_t {
     int tab_L = 0;
     std::vector<unsigned> coef_epoch, agent_epoch, source_epoch;
     bfvm::SweepArgs* d_args[3] = {nullptr, nullptr, nullptr};
-    int* d_first[3] = {nullptr, nullptr, nullptr};
+    int* d_first[3] = {nullptr, nullptr, nullptr};  // per block: member, block within the member
     int nbatch[3] = {0, 0, 0};       // members in the batched launch
     int blocks[3] = {0, 0, 0};
     size_t lds[3] = {0, 0, 0};
     std::vector<int> solo[3];        // members sweeping on their own
+    // x sweeps on row tiles: per-member arguments, per-block (member, tile)
+    bfvm::RowTile* d_rows = nullptr;
+    int* d_rmap = nullptr;
+    int rblocks = 0, rthreads = 0;
+    size_t rlds = 0;
+    int tab_rows_x = -1;
     bfvm::AgentBatch* d_agents = nullptr;
     int n_agent_members = 0;
     long long agent_threads = 0;
@@ -87,6 +94,11 @@ void free_tables(bfvm_ensemble_t& E)
     }
     if (E.d_agents) (void)hipFree(E.d_agents);
     E.d_agents = nullptr;
+    if (E.d_rows) (void)hipFree(E.d_rows);
+    if (E.d_rmap) (void)hipFree(E.d_rmap);
+    E.d_rows = nullptr;
+    E.d_rmap = nullptr;
+    E.rblocks = 0;
     E.tables_valid = false;
 }
 
@@ -104,7 +116,7 @@ void drop_graph(bfvm_ensemble_t& E)
 // here except bulk preparation, so a step captured after this is launches only.
 int prepare(bfvm_ensemble_t& E, double dt, int flags)
 {
-    bool changed = !E.tables_valid || E.tab_dt != dt || E.tab_L != E.L;
+    bool changed = !E.tables_valid || E.tab_dt != dt || E.tab_L != E.L || E.tab_rows_x != E.rows_x;
     for (size_t i = 0; i < E.m.size(); ++i) {
         Env& m = *E.m[i];
         if (flags & BFVM_DIFFUSION) {
@@ -121,6 +133,43 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
     if (!changed) return BFVM_OK;
     drop_graph(E);
     free_tables(E);
+    std::vector<bool> on_rows(E.m.size(), false);
+    if (E.rows_x) {
+        // x sweeps on row tiles, one launch for every member that can
+        std::vector<bfvm::RowTile> tabs;
+        std::vector<int> map;
+        size_t rest = 0;  // LDS beyond the two per-thread carry arrays
+        E.rthreads = 0;
+        for (size_t i = 0; i < E.m.size(); ++i) {
+            Env& m = *E.m[i];
+            bfvm::RowTile X;
+            int threads = 0, tiles = 0;
+            size_t lds = 0;
+            if (!m.coef_valid || !bfvm::batch_rows_args(m, E.L, &X, &threads, &lds, &tiles)) continue;
+            on_rows[i] = true;
+            for (int t = 0; t < tiles; ++t) {
+                map.push_back((int)tabs.size());
+                map.push_back(t);
+            }
+            tabs.push_back(X);
+            E.rthreads = std::max(E.rthreads, threads);
+            rest = std::max(rest, lds - sizeof(double) * 2 * (size_t)threads);
+        }
+        // every block has the largest member's thread count, and its carry
+        // arrays sized for it; beyond the LDS a block may use, no row tiles
+        E.rlds = rest + sizeof(double) * 2 * (size_t)E.rthreads;
+        if (E.rlds > 150 * 1024) {
+            tabs.clear();
+            on_rows.assign(E.m.size(), false);
+        }
+        if (!tabs.empty()) {
+            E.rblocks = (int)(map.size() / 2);
+            EHIP(&E, hipMalloc((void**)&E.d_rows, sizeof(bfvm::RowTile) * tabs.size()));
+            EHIP(&E, hipMalloc((void**)&E.d_rmap, sizeof(int) * map.size()));
+            EHIP(&E, hipMemcpy(E.d_rows, tabs.data(), sizeof(bfvm::RowTile) * tabs.size(), hipMemcpyHostToDevice));
+            EHIP(&E, hipMemcpy(E.d_rmap, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
+        }
+    }
     for (int axis = 0; axis < 3; ++axis) {
         std::vector<bfvm::SweepArgs> args;
         std::vector<int> first{0};
@@ -128,6 +177,7 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
         for (size_t i = 0; i < E.m.size(); ++i) {
             Env& m = *E.m[i];
             if (axis >= m.lod_dims) continue;  // LOD_2D members have no z sweep
+            if (axis == 0 && on_rows[i]) continue;
             if (!m.coef_valid || !bfvm::batch_member_ok(m, axis, E.L)) {
                 E.solo[axis].push_back((int)i);
                 continue;
@@ -142,11 +192,18 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
         E.lds[axis] = lds;
         if (!args.empty()) {
             EHIP(&E, hipMalloc((void**)&E.d_args[axis], sizeof(bfvm::SweepArgs) * args.size()));
-            EHIP(&E, hipMalloc((void**)&E.d_first[axis], sizeof(int) * first.size()));
+            // per block: its member and its block index within the member (one
+            // load at the block's start instead of a search over the members)
+            std::vector<int> map(2 * (size_t)first.back());
+            for (size_t i = 0; i + 1 < first.size(); ++i)
+                for (int b = first[i]; b < first[i + 1]; ++b) {
+                    map[2 * (size_t)b] = (int)i;
+                    map[2 * (size_t)b + 1] = b - first[i];
+                }
+            EHIP(&E, hipMalloc((void**)&E.d_first[axis], sizeof(int) * map.size()));
             EHIP(&E, hipMemcpy(E.d_args[axis], args.data(), sizeof(bfvm::SweepArgs) * args.size(),
                                hipMemcpyHostToDevice));
-            EHIP(&E, hipMemcpy(E.d_first[axis], first.data(), sizeof(int) * first.size(),
-                               hipMemcpyHostToDevice));
+            EHIP(&E, hipMemcpy(E.d_first[axis], map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
         }
     }
     std::vector<bfvm::AgentBatch> ag;
@@ -179,6 +236,7 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
     }
     E.tab_dt = dt;
     E.tab_L = E.L;
+    E.tab_rows_x = E.rows_x;
     E.tables_valid = true;
     return BFVM_OK;
 }
@@ -201,6 +259,10 @@ hipError_t enqueue(bfvm_ensemble_t& E, int flags)
                 if (pm->n_dir > 0 && axis < pm->lod_dims &&
                     (st = on_stream(*pm, E.stream, [&] { return bfvm::launch_dirichlet(*pm); })))
                     return st;
+            if (axis == 0 && E.rblocks > 0 &&
+                (st = bfvm::launch_rows_batch(E.precision, E.L, E.spread, E.d_rows, E.d_rmap, E.rblocks,
+                                              E.rthreads, E.rlds, E.stream)))
+                return st;
             if (E.nbatch[axis] > 0 &&
                 (st = bfvm::launch_sweep_batch(E.precision, E.L, E.spread, E.d_args[axis], E.d_first[axis],
                                                E.nbatch[axis], E.blocks[axis], E.lds[axis], E.stream)))
@@ -359,21 +421,26 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
         e->m[i]->rho = scratch[i];
         ++e->m[i]->coef_epoch;  // launch tables point at the scratch stores
     }
-    int best_L = e->L, best_spread = e->spread;
+    int best_L = e->L, best_spread = e->spread, best_rows = e->rows_x;
     double best = 1e300;
     int rc = BFVM_OK;
     for (int L : {4, 8, 16, 32}) {
         for (int spread : {0, 64, -1}) {
-            e->L = L;
-            e->spread = spread;
-            double ms = 0;
-            if ((rc = bfvm_ensemble_timed_run(e, dt, 1, BFVM_DIFFUSION, 0, &ms))) break;
-            if ((rc = bfvm_ensemble_timed_run(e, dt, 3, BFVM_DIFFUSION, 0, &ms))) break;
-            if (ms < best) {
-                best = ms;
-                best_L = L;
-                best_spread = spread;
+            for (int rows : {0, 1}) {
+                e->L = L;
+                e->spread = spread;
+                e->rows_x = rows;
+                double ms = 0;
+                if ((rc = bfvm_ensemble_timed_run(e, dt, 1, BFVM_DIFFUSION, 0, &ms))) break;
+                if ((rc = bfvm_ensemble_timed_run(e, dt, 3, BFVM_DIFFUSION, 0, &ms))) break;
+                if (ms < best) {
+                    best = ms;
+                    best_L = L;
+                    best_spread = spread;
+                    best_rows = rows;
+                }
             }
+            if (rc) break;
         }
         if (rc) break;
     }
@@ -385,6 +452,7 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
     }
     e->L = best_L;
     e->spread = best_spread;
+    e->rows_x = best_rows;
     e->tables_valid = false;
     drop_graph(*e);
     return rc;
@@ -406,6 +474,22 @@ int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment)
     e->L = nodes_per_segment;
     e->tables_valid = false;
     drop_graph(*e);
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_set_row_tiles(bfvm_ensemble e, int on)
+{
+    if (!e) return BFVM_E_ARG;
+    e->rows_x = on ? 1 : 0;
+    e->tables_valid = false;
+    drop_graph(*e);
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_get_row_tiles(bfvm_ensemble e, int* on)
+{
+    if (!e || !on) return BFVM_E_ARG;
+    *on = e->rows_x;
     return BFVM_OK;
 }
 

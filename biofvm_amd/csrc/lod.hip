@@ -506,27 +506,23 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_stream(SweepArgs A, in
     }
 }
 
-// An ensemble: member m owns blocks [first[m], first[m+1]) of a 1-D grid,
-// laid out as planes x column tiles of A[m].cols lines; every block has
-// kMaxThreads threads (A[m].cols * A[m].nseg of them active).
+// An ensemble: member m owns a contiguous range of blocks of a 1-D grid, laid
+// out as planes x column tiles of A[m].cols lines; map[2b], map[2b+1] = the
+// member of block b and b's index within it.  Every block has kMaxThreads
+// threads (A[m].cols * A[m].nseg of them active).
 template <typename T, int L>
 __global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4)
-k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ first, int nm, int spread)
+k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ map, int nm, int spread)
 {
     __shared__ double xa[kMaxThreads];
     __shared__ double xb[kMaxThreads];
     __shared__ double wfirst[kMaxThreads];
     __shared__ double wlast[kMaxThreads];
     extern __shared__ double gtab[];
-    const int b = (int)xcd_tile(blockIdx.x, gridDim.x, spread);
-    int lo = 0, hi = nm;  // first[lo] <= b < first[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) / 2;
-        if (first[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    const SweepArgs A = args[lo];
-    const int local = b - first[lo];
+    const unsigned b = xcd_tile(blockIdx.x, gridDim.x, spread);
+    const int member = __builtin_amdgcn_readfirstlane(map[2 * b]);
+    const int local = __builtin_amdgcn_readfirstlane(map[2 * b + 1]);
+    const SweepArgs A = args[member];
     sweep_tile<T, L>(A, A.cols, local % A.tiles, local / A.tiles, xa, xb, wfirst, wlast, gtab);
 }
 
@@ -648,14 +644,7 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 8 ? 8 : 4) k_sweep_x_src(Swe
 // slots, meet 64 different banks.  The nodes past n-1 of the last segment
 // are zeros (they reach no real node: the backward step into node n-1
 // multiplies them by 0).
-struct RowTile {
-    SweepArgs A;             // axis-0 arguments of the planes swept (rho at the first row)
-    size_t vals;             // values swept: rows * nx * S
-    int R, C;                // rows per tile, lines per tile (R*S)
-    int nxS;                 // values per row
-    unsigned mag_nxS, mag_S;  // ceil(2^32/d), d >= 2: e/d = umulhi(e, mag) for e*d < 2^32
-    int tile_vals;           // R*nx*S
-};
+
 
 // LDS slot of tile value e (row-major, substrate innermost in the store).
 template <int L>
@@ -680,9 +669,9 @@ __device__ __forceinline__ void val_store(double*, __amdgpu_buffer_rsrc_t r, dou
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, x), r, vo, 0, kStorePolicy);
 }
 
-// Grid: one block per tile; blockDim >= C*nseg threads, L*blockDim >= R*nx*S.
+// Row tile `tile` of X (blockDim >= C*nseg threads, L*blockDim >= R*nx*S).
 template <typename T, int L>
-__global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_rows(RowTile X)
+__device__ __forceinline__ void rows_tile(const RowTile& X, unsigned tile_index)
 {
     extern __shared__ double smem[];
     const SweepArgs& A = X.A;
@@ -696,7 +685,7 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_rows(RowTile X)
 
     // the buffer ends with the tile's last real value: reads past it give 0
     // (the rows past the last are zeros) and stores past it are dropped
-    const size_t first = (size_t)xcd_tile(blockIdx.x, gridDim.x, A.spread) * X.tile_vals;
+    const size_t first = (size_t)tile_index * X.tile_vals;
     const unsigned range = (unsigned)min((size_t)X.tile_vals, X.vals - first) * (unsigned)sizeof(T);
     const __amdgpu_buffer_rsrc_t rs = make_rsrc((T*)A.rho + first, range);
     {
@@ -740,6 +729,26 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_rows(RowTile X)
         if (e < (unsigned)X.tile_vals)
             val_store((T*)nullptr, rs, tile[row_tile_slot<L>(X, e)], e * (unsigned)sizeof(T));
     }
+}
+
+// Grid: one block per row tile.
+template <typename T, int L>
+__global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_rows(RowTile X)
+{
+    rows_tile<T, L>(X, xcd_tile(blockIdx.x, gridDim.x, X.A.spread));
+}
+
+// Row tiles of several grids' x sweeps: map[2b], map[2b+1] = the grid of block
+// b and b's tile within it; blockDim and the LDS fit every grid's tiles.
+template <typename T, int L>
+__global__ void __launch_bounds__(kMaxThreads, 4)
+k_sweep_rows_batch(const RowTile* __restrict__ tabs, const int* __restrict__ map, int spread)
+{
+    const unsigned b = xcd_tile(blockIdx.x, gridDim.x, spread);
+    const int member = __builtin_amdgcn_readfirstlane(map[2 * b]);
+    const int local = __builtin_amdgcn_readfirstlane(map[2 * b + 1]);
+    const RowTile X = tabs[member];
+    rows_tile<T, L>(X, (unsigned)local);
 }
 
 // One thread owns one whole (line, substrate) pair and runs the reference's
@@ -995,12 +1004,11 @@ unsigned div_magic(unsigned d)
     return (unsigned)((((unsigned long long)1 << 32) + d - 1) / d);
 }
 
-template <typename T, int L>
-hipError_t launch_rows(Env& e, const SweepPlan& p, int k0, int k1)
+// Row-tile arguments of planes [k0, k1) of e's x sweep, C lines per tile.
+RowTile row_tile_args(const Env& e, const RowShape& sh, int k0, int k1)
 {
-    const RowShape sh = row_shape(e, p.cols, L, sizeof(T));
     RowTile X;
-    X.A = sweep_args(e, 0, p.cols, sh.nseg, k0, k1);
+    X.A = sweep_args(e, 0, sh.C, sh.nseg, k0, k1);
     const long long rows = (long long)e.ny * (k1 - k0);
     X.R = sh.R;
     X.C = sh.C;
@@ -1009,6 +1017,15 @@ hipError_t launch_rows(Env& e, const SweepPlan& p, int k0, int k1)
     X.mag_S = e.S > 1 ? div_magic((unsigned)e.S) : 0;  // S = 1 divides by nothing
     X.tile_vals = X.R * X.nxS;
     X.vals = (size_t)rows * X.nxS;
+    return X;
+}
+
+template <typename T, int L>
+hipError_t launch_rows(Env& e, const SweepPlan& p, int k0, int k1)
+{
+    const RowShape sh = row_shape(e, p.cols, L, sizeof(T));
+    const RowTile X = row_tile_args(e, sh, k0, k1);
+    const long long rows = (long long)e.ny * (k1 - k0);
     const long long grid = (rows + X.R - 1) / X.R;
     if (grid >= (1ll << 31)) return hipErrorInvalidValue;
     static bool lds_set = false;  // per instantiation
@@ -1231,6 +1248,61 @@ hipError_t launch_sweep_batch(int precision, int L, int spread, const SweepArgs*
     case 8: return launch_batch_t<float, 8>(d_args, d_first, nm, blocks, lds, st, spread);
     case 16: return launch_batch_t<float, 16>(d_args, d_first, nm, blocks, lds, st, spread);
     default: return launch_batch_t<float, 32>(d_args, d_first, nm, blocks, lds, st, spread);
+    }
+}
+
+// Row tiles of an ensemble's x sweeps (k_sweep_rows_batch): a member joins
+// with C = the most lines (a multiple of S, at most 64) whose tile fits the
+// thread and LDS limits for segments of L nodes.
+bool batch_rows_args(const Env& e, int L, RowTile* X, int* threads, size_t* lds, int* tiles)
+{
+    if (e.nx < 2 || e.nvox >= (1ull << 31)) return false;
+    for (int C = 64 / e.S * e.S; C >= e.S; C -= e.S) {
+        if (!rows_plan_valid(e, C, L)) continue;
+        const RowShape sh = row_shape(e, C, L, (int)e.elem);
+        *X = row_tile_args(e, sh, 0, e.nz);
+        *threads = sh.threads;
+        *lds = sh.lds;
+        *tiles = (int)(((long long)e.ny * e.nz + sh.R - 1) / sh.R);
+        return true;
+    }
+    return false;
+}
+
+namespace {
+template <typename T, int L>
+hipError_t launch_rows_batch_t(const RowTile* d_tabs, const int* d_map, int blocks, int threads, size_t lds,
+                               int spread, hipStream_t st)
+{
+    static bool lds_set = false;  // per instantiation
+    if (!lds_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_sweep_rows_batch<T, L>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxRowTileLds);
+        if (e != hipSuccess) return e;
+        lds_set = true;
+    }
+    k_sweep_rows_batch<T, L><<<blocks, threads, lds, st>>>(d_tabs, d_map, spread);
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_rows_batch(int precision, int L, int spread, const RowTile* d_tabs, const int* d_map,
+                             int blocks, int threads, size_t lds, hipStream_t st)
+{
+    if (blocks == 0) return hipSuccess;
+    if (precision == BFVM_FP64) {
+        switch (L) {
+        case 4: return launch_rows_batch_t<double, 4>(d_tabs, d_map, blocks, threads, lds, spread, st);
+        case 8: return launch_rows_batch_t<double, 8>(d_tabs, d_map, blocks, threads, lds, spread, st);
+        case 16: return launch_rows_batch_t<double, 16>(d_tabs, d_map, blocks, threads, lds, spread, st);
+        default: return launch_rows_batch_t<double, 32>(d_tabs, d_map, blocks, threads, lds, spread, st);
+        }
+    }
+    switch (L) {
+    case 4: return launch_rows_batch_t<float, 4>(d_tabs, d_map, blocks, threads, lds, spread, st);
+    case 8: return launch_rows_batch_t<float, 8>(d_tabs, d_map, blocks, threads, lds, spread, st);
+    case 16: return launch_rows_batch_t<float, 16>(d_tabs, d_map, blocks, threads, lds, spread, st);
+    default: return launch_rows_batch_t<float, 32>(d_tabs, d_map, blocks, threads, lds, spread, st);
     }
 }
 

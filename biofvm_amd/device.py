@@ -338,3 +338,14 @@ class Ensemble:
     @segment.setter
     def segment(self, nodes):
         self._check(self.lib.bfvm_ensemble_set_segment(self.e, int(nodes)))
+
+    @property
+    def row_tiles(self):
+        """x sweeps on LDS-staged row tiles (True) or column tiles (False)."""
+        v = ctypes.c_int(0)
+        self._check(self.lib.bfvm_ensemble_get_row_tiles(self.e, ctypes.byref(v)))
+        return bool(v.value)
+
+    @row_tiles.setter
+    def row_tiles(self, on):
+        self._check(self.lib.bfvm_ensemble_set_row_tiles(self.e, int(bool(on))))

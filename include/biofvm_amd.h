@@ -315,11 +315,19 @@ int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, in
 
 /* Times the batched sweeps for each segment length (4, 8, 16, 32 nodes) and
  * spread of their blocks over the XCDs (as dispatched, chunks of 64 tiles,
- * contiguous eighths) on scratch copies of the stores and keeps the fastest.
+ * contiguous eighths) and x tiles (rows or columns) on scratch copies of the
+ * stores and keeps the fastest.
  * Densities unchanged. */
 int bfvm_ensemble_autotune(bfvm_ensemble e, double dt);
 int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment);
 int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment);
+/* Whether the members' x sweeps run on LDS-staged row tiles (on = 1, the
+ * default; members that cannot take part run with column tiles) or all on
+ * column tiles (0).  Row tiles load whole x rows contiguously, which matters
+ * for few substrates per voxel; the result is the same bit for bit.  Set by
+ * bfvm_ensemble_autotune. */
+int bfvm_ensemble_set_row_tiles(bfvm_ensemble e, int on);
+int bfvm_ensemble_get_row_tiles(bfvm_ensemble e, int* on);
 
 const char* bfvm_ensemble_last_error(bfvm_ensemble e);
 

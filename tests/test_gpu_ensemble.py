@@ -191,3 +191,26 @@ def test_ensemble_with_lod2d_members():
             assert rel_err(m.download(), oracle.oracle_run(c)) <= TOL[FP64], i
     finally:
         _close(ens, members)
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("L", [8, 16])
+def test_row_tile_x_sweeps_equal_column_tiles(mixed, precision, L):
+    # members' x sweeps on row tiles (one batched launch) give the bits of the
+    # column-tile batch with the same segment length
+    cases, refs = mixed
+    out = {}
+    for rows in (False, True):
+        members = _members(cases, precision)
+        ens = Ensemble(members)
+        try:
+            ens.segment = L
+            ens.row_tiles = rows
+            assert ens.row_tiles == rows
+            ens.run(cases[0]["dt"], cases[0]["steps"], ALL, graph=rows)
+            out[rows] = [m.download() for m in members]
+        finally:
+            _close(ens, members)
+    for i, (a, b, ref) in enumerate(zip(out[True], out[False], refs)):
+        np.testing.assert_array_equal(a, b, err_msg=f"member {i}")
+        assert rel_err(a, ref) <= TOL[precision], i
