@@ -138,7 +138,7 @@ __device__ __forceinline__ double chain(double* x, double aL, int lane, int C, i
 // column is one (voxel of a plane, substrate) pair, col = q*S + s, whose line
 // starts at rho[plane*plane_stride + q*vstride + s] and advances line_stride
 // per node.
-//   x: plane = k, q = j, vstride = nx*S, line_stride = S
+//   x: one plane (a run of z planes), q = k*ny + j, vstride = nx*S, line_stride = S
 //   y: plane = k, q = i, vstride = S,    line_stride = nx*S
 //   z: one plane, q = j*nx + i, vstride = S, line_stride = nx*ny*S
 // Block = COLS columns x nseg segments of L nodes; thread tid -> column
@@ -788,10 +788,15 @@ struct Geometry {
 };
 
 // Lines of one sweep as columns of planes (see k_sweep).
+// x: every (j, k) row's start is nx*S after the previous row's, across planes
+// too, so the x lines of the whole store (or of a run of z planes) form one
+// plane of ny*nz*S columns: a tile never straddles planes, and small planes
+// fill whole blocks.
 Geometry geometry(const Env& e, int axis)
 {
     const size_t S = e.S, nx = e.nx, ny = e.ny;
-    if (axis == 0) return {e.nx, (int)(ny * S), e.nz, S, nx * ny * S, nx * S};
+    // (a store whose rows overflow an int sweeps x in runs of planes: sweep())
+    if (axis == 0) return {e.nx, (int)std::min<size_t>(ny * e.nz * S, 0x7fffffff), 1, S, 0, nx * S};
     if (axis == 1) return {e.ny, (int)(nx * S), e.nz, nx * S, nx * ny * S, S};
     return {e.nz, (int)(nx * ny * S), 1, nx * ny * S, 0, S};
 }
@@ -864,10 +869,21 @@ SweepArgs sweep_args(const Env& e, int axis, int cols, int nseg, int k0 = 0, int
 {
     Geometry gm = geometry(e, axis);
     const AxisCoef& a = e.ax[axis];
-    if (k1 < 0) k1 = gm.nplanes;
+    size_t first = 0;  // first value swept
+    if (axis == 0) {
+        // z planes [k0, k1) as one plane of their rows
+        if (k1 < 0) k1 = e.nz;
+        first = (size_t)k0 * e.nx * e.ny * e.S;
+        gm.ncols = (k1 - k0) * e.ny * e.S;
+        k0 = 0;
+        k1 = 1;
+    } else {
+        if (k1 < 0) k1 = gm.nplanes;
+        first = (size_t)k0 * gm.plane_stride;
+    }
     SweepArgs A;
-    A.rho = (char*)e.rho + (size_t)k0 * gm.plane_stride * e.elem;
-    A.nvs = e.nvs - (size_t)k0 * gm.plane_stride;
+    A.rho = (char*)e.rho + first * e.elem;
+    A.nvs = e.nvs - first;
     A.plane_stride = gm.plane_stride;
     A.wb = (const double*)a.wb;
     A.g0 = (const double*)a.g0;
@@ -1059,7 +1075,8 @@ bool segmented(const Env& e, int axis, const SweepPlan& p)
     if (p.stream == 2) return axis == 0 && rows_plan_valid(e, p.cols, p.L);
     Geometry gm = geometry(e, axis);
     // buffer offsets are 32-bit: the segmented kernel addresses at most 4 GiB
-    // from a plane's start
+    // from a plane's start (x: from the first of a run of z planes, at least
+    // one)
     const size_t span = (axis == 2 ? e.nvs : (size_t)e.nx * e.ny * e.S) * sizeof(T);
     // the end-correction tables live in LDS beside the 16 KiB carry arrays
     const size_t gbytes =
@@ -1076,11 +1093,26 @@ hipError_t sweep(Env& e, int axis, int k0 = 0, int k1 = -1)
     Geometry gm = geometry(e, axis);
     SweepPlan p = plan_for(e, axis);
     if (!segmented<T>(e, axis, p)) return sweep_naive<T>(e, axis);
-    if (k1 < 0) k1 = gm.nplanes;
+    if (k1 < 0) k1 = axis == 0 ? e.nz : gm.nplanes;
     if (p.stream == 2) return launch_rows_L<T>(e, p, k0, k1);
+    int planes = k1 - k0;
+    if (axis == 0) {
+        // x buffer offsets are 32-bit from the first value swept: runs of
+        // planes that span less than 4 GiB (segmented() checked one fits)
+        const int run = (int)std::min<size_t>(e.nz, 0xffffffffull / ((size_t)e.nx * e.ny * e.S * sizeof(T)));
+        if (k1 - k0 > run) {
+            for (int k = k0; k < k1; k += run) {
+                const hipError_t st = sweep<T>(e, 0, k, std::min(k1, k + run));
+                if (st != hipSuccess) return st;
+            }
+            return hipSuccess;
+        }
+        gm.ncols = (k1 - k0) * e.ny * e.S;
+        planes = 1;
+    }
     int nseg = (gm.n + p.L - 1) / p.L;
     const long long tiles = ((long long)gm.ncols + p.cols - 1) / p.cols;
-    if (p.stream && stream_nodes_ok(p.L, e.precision) && tiles * (k1 - k0) < (1ll << 31)) {
+    if (p.stream && stream_nodes_ok(p.L, e.precision) && tiles * planes < (1ll << 31)) {
         switch (p.cols) {
         case 16: return launch_cols_stream<T, 16>(e, axis, p.L, nseg, k0, k1);
         case 32: return launch_cols_stream<T, 32>(e, axis, p.L, nseg, k0, k1);
@@ -1143,7 +1175,8 @@ bool sweep_x_fusable(const Env& e)
 {
     const SweepPlan p = fused_x_plan(e);
     const bool seg = e.precision == BFVM_FP64 ? segmented<double>(e, 0, p) : segmented<float>(e, 0, p);
-    return seg && p.stream == 0 && e.nvox < (1ull << 31);
+    // one launch over all planes: 32-bit offsets from the store's start
+    return seg && p.stream == 0 && e.nvox < (1ull << 31) && e.nvs * e.elem < 0xffffffffull;
 }
 
 hipError_t launch_sweep_x_fused(Env& e, bool bulk, bool skip_agent_voxels, bool dir_after)
@@ -1199,7 +1232,8 @@ template <typename T>
 bool batch_member_ok_t(const Env& e, int axis, int L)
 {
     Geometry gm = geometry(e, axis);
-    const size_t span = (axis == 2 ? e.nvs : (size_t)e.nx * e.ny * e.S) * sizeof(T);
+    // a member's x lines are one plane of all its rows
+    const size_t span = (axis == 1 ? (size_t)e.nx * e.ny * e.S : e.nvs) * sizeof(T);
     const size_t gbytes =
         sizeof(double) * (size_t)(e.ax[axis].kg0 + e.ax[axis].kg1 + 3 * kPad + 8) * e.S;
     const int nseg = (gm.n + L - 1) / L;
@@ -1252,14 +1286,18 @@ hipError_t launch_sweep_batch(int precision, int L, int spread, const SweepArgs*
 }
 
 // Row tiles of an ensemble's x sweeps (k_sweep_rows_batch): a member joins
-// with C = the most lines (a multiple of S, at most 64) whose tile fits the
-// thread and LDS limits for segments of L nodes.
+// with C = the most lines (a multiple of S) whose tile fills at most 1024
+// threads for segments of L nodes and fits the LDS limit, so that the blocks
+// of short-lined members are as full as those of long-lined ones.
 bool batch_rows_args(const Env& e, int L, RowTile* X, int* threads, size_t* lds, int* tiles)
 {
     if (e.nx < 2 || e.nvox >= (1ull << 31)) return false;
-    for (int C = 64 / e.S * e.S; C >= e.S; C -= e.S) {
+    const int nseg = (e.nx + L - 1) / L;
+    for (int C = kMaxThreads / nseg / e.S * e.S; C >= e.S; C -= e.S) {
         if (!rows_plan_valid(e, C, L)) continue;
         const RowShape sh = row_shape(e, C, L, (int)e.elem);
+        // the batch's blocks may have up to kMaxThreads threads' carry arrays
+        if (sh.lds - sizeof(double) * 2 * sh.threads + sizeof(double) * 2 * kMaxThreads > kMaxRowTileLds) continue;
         *X = row_tile_args(e, sh, 0, e.nz);
         *threads = sh.threads;
         *lds = sh.lds;
