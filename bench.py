@@ -320,23 +320,34 @@ def differentiation_bench(args):
     case["lam"] = hidden["lam"] * np.exp(rng.uniform(-0.3, 0.3, hidden["S"]))
     case["agent_secretion"] = hidden["agent_secretion"] * np.exp(
         rng.uniform(-0.3, 0.3, hidden["agent_secretion"].shape))
-    m = setup_case(case, FP32)
-    m.autotune(dt)
-    m.sync()
-    t0 = time.perf_counter()
-    m.tape_record(dt, steps, ph, 50)
-    m.sync()
-    t_fwd = time.perf_counter() - t0
-    loss = m.tape_seed_l2(target)
-    t0 = time.perf_counter()
-    g = m.tape_backward(n_agents)
-    t_bwd = time.perf_counter() - t0
-    m.close()
+    def adjoint(precision):
+        m = setup_case(case, precision)
+        m.autotune(dt)
+        m.sync()
+        t0 = time.perf_counter()
+        m.tape_record(dt, steps, ph, 50)
+        m.sync()
+        t_fwd = time.perf_counter() - t0
+        loss = m.tape_seed_l2(target)
+        t0 = time.perf_counter()
+        g = m.tape_backward(n_agents)
+        t_bwd = time.perf_counter() - t0
+        m.close()
+        return g, t_fwd, t_bwd, loss
+
+    g, t_fwd, t_bwd, loss = adjoint(FP32)
     out = {"forward_s": t_fwd, "backward_s": t_bwd, "loss": loss}
     if not args.no_fp64:
         keys = ["rho0", "D", "lam", "agent_secretion", "agent_uptake"]
         gk = {"rho0": "grad_rho0", "D": "grad_D", "lam": "grad_lambda", "agent_secretion": "grad_secretion",
               "agent_uptake": "grad_uptake"}
+        # the FP64 adjoint: the same transposes on the FP64 store's trajectory
+        g64, _, t_bwd64, _ = adjoint(FP64)
+        out["fp64_backward_s"] = t_bwd64
+        out["fp32_vs_fp64_adjoint"] = {
+            k: {"max_abs_over_max": float(np.abs(g[v] - g64[v]).max() / np.abs(g64[v]).max()),
+                "l2_rel": float(np.linalg.norm(g[v] - g64[v]) / np.linalg.norm(g64[v]))}
+            for k, v in gk.items()}
 
         def J(ds, h):
             c = dict(case)
@@ -347,18 +358,26 @@ def differentiation_bench(args):
                 r = mm.download() - target
                 return 0.5 * float(r @ r)
 
-        def check(ks, h):
+        def check(ks, h, grads=(g,)):
+            """Per gradient in `grads`: |ad - fd| / |fd| and |ad - fd| / sum |terms|."""
             ds = {k: rng.normal(size=np.shape(case[k])) * np.asarray(case[k]) for k in ks}
             fd = (J(ds, h) - J(ds, -h)) / (2 * h)
-            terms = np.concatenate([g[gk[k]].reshape(-1) * ds[k].reshape(-1) for k in ks])
-            ad = float(terms.sum())
-            return abs(ad - fd) / abs(fd), abs(ad - fd) / float(np.abs(terms).sum())
+            res = []
+            for gr in grads:
+                terms = np.concatenate([gr[gk[k]].reshape(-1) * ds[k].reshape(-1) for k in ks])
+                ad = float(terms.sum())
+                res.append((abs(ad - fd) / abs(fd), abs(ad - fd) / float(np.abs(terms).sum())))
+            return res
 
-        joint = [check(keys, 1e-6) for _ in range(32)]
+        joint = [check(keys, 1e-6, (g, g64)) for _ in range(32)]
         out["fd_check"] = {"directions": 32, "h": 1e-6, "space": "joint (rho0, D, lambda, secretion, uptake)",
-                           "max_rel_err": max(a for a, _ in joint),
-                           "max_rel_err_vs_term_scale": max(b for _, b in joint)}
-        dl = {h: [check(["D", "lam"], h) for _ in range(4)] for h in (1e-6, 1e-4)}
+                           "max_rel_err": max(r[0][0] for r in joint),
+                           "max_rel_err_vs_term_scale": max(r[0][1] for r in joint),
+                           "directions_within_1e-4": sum(r[0][0] <= 1e-4 for r in joint),
+                           # the same directions with the FP64 adjoint: the differences' own error
+                           "fp64_adjoint_max_rel_err": max(r[1][0] for r in joint),
+                           "fp64_adjoint_directions_within_1e-4": sum(r[1][0] <= 1e-4 for r in joint)}
+        dl = {h: [check(["D", "lam"], h)[0] for _ in range(4)] for h in (1e-6, 1e-4)}
         out["fd_check_D_lambda"] = {str(h): {"max_rel_err": max(a for a, _ in v),
                                              "max_rel_err_vs_term_scale": max(b for _, b in v)}
                                     for h, v in dl.items()}
