@@ -824,8 +824,15 @@ bool plan_valid(int n, int cols, int L)
     return cols * ((n + L - 1) / L) <= kMaxThreads;
 }
 
+// The shape of a handle the autotuner has not seen: 32 lines of 16-node
+// segments where the line fits (the tuned shapes on the grids measured have
+// 16 or 32 nodes; 4-node segments ran 30-45 % slower at 128^3 x 2), longer
+// segments for longer lines, shorter ones for short lines.
 SweepPlan default_plan(int n)
 {
+    for (int cols : {32, 16})
+        for (int L : {16, 32, 8, 24, 12, 4})
+            if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n)) return SweepPlan{cols, L};
     for (int cols : {32, 16})
         for (int L : kNodes)
             if (plan_valid(n, cols, L)) return SweepPlan{cols, L};

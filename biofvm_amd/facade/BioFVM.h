@@ -29,6 +29,11 @@
  * 394-720) with the densities' MAT file packed on the GPU.  The general
  * (non-Cartesian) mesh and reading MultiCellDS files are not part of this
  * façade.
+ *
+ * Launch shapes: a microenvironment's first LOD solver call picks the sweep
+ * shapes for its grid (bfvm_autotune: each candidate timed on a scratch copy
+ * of the store), as the reference's solver sets itself up on its first call;
+ * BIOFVM_AMD_AUTOTUNE=0 in the environment keeps the default shapes.
  */
 #ifndef BIOFVM_AMD_FACADE_H
 #define BIOFVM_AMD_FACADE_H

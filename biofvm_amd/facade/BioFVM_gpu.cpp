@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "biofvm_amd.h"
@@ -225,6 +226,7 @@ struct DeviceState {
     bool dirichlet_dirty = true;
     unsigned long long agents_seen = 0;  // sum of agent revisions last uploaded
     size_t agents_count = (size_t)-1;
+    bool tuned = false;  // launch shapes picked (bfvm_autotune) for this handle
     std::vector<double> flat;  // staging for host<->device copies
 };
 
@@ -452,6 +454,7 @@ DeviceState* Microenvironment::device(void)
         dev->device_newer = false;
         dev->dirichlet_dirty = true;
         dev->agents_count = (size_t)-1;
+        dev->tuned = false;
     }
     return dev;
 }
@@ -732,6 +735,20 @@ void Microenvironment::display_information(std::ostream& os)
        << std::endl;
 }
 
+// The first diffusion step of a handle picks its launch shapes (bfvm_autotune:
+// every candidate timed on a scratch copy of the store, results unchanged but
+// for the last bits of the segmented solve's rounding), as the reference's
+// solver sets itself up on its first call (BioFVM_solvers.cpp:87-173).  The
+// environment variable BIOFVM_AMD_AUTOTUNE=0 keeps the default shapes.
+static int tune_once(DeviceState* d, double dt)
+{
+    if (d->tuned) return 0;
+    d->tuned = true;
+    const char* env = std::getenv("BIOFVM_AMD_AUTOTUNE");
+    if (env && std::string(env) == "0") return 0;
+    return bfvm_autotune(d->h, dt);
+}
+
 // The GPU LOD_3D solver (replaces BioFVM_solvers.cpp:77-276).
 void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, double dt)
 {
@@ -747,6 +764,7 @@ void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, d
         rc = bfvm_set_lod_dimensions(M.dev->h, 3);
         if (!rc) M.dev->lod_dims = 3;
     }
+    if (!rc) rc = tune_once(M.dev, dt);
     if (!rc) rc = bfvm_step(M.dev->h, dt, BFVM_DIFFUSION);
     if (rc) {
         report("diffusion_decay_solver__constant_coefficients_LOD_3D", rc, M.dev ? M.dev->h : nullptr);
@@ -771,6 +789,7 @@ void diffusion_decay_solver__constant_coefficients_LOD_2D(Microenvironment& M, d
         rc = bfvm_set_lod_dimensions(M.dev->h, 2);
         if (!rc) M.dev->lod_dims = 2;
     }
+    if (!rc) rc = tune_once(M.dev, dt);
     if (!rc) rc = bfvm_step(M.dev->h, dt, BFVM_DIFFUSION);
     if (rc) {
         report("diffusion_decay_solver__constant_coefficients_LOD_2D", rc, M.dev ? M.dev->h : nullptr);
