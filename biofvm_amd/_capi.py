@@ -92,6 +92,15 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m biofvm_amd.build`")
+        # One HIP runtime per process: PyTorch's HIP libraries ask for
+        # libamdhip64 under another name than this library does, so if this
+        # library loaded the system runtime first, torch would load its own
+        # copy beside it and find no GPU.  Loaded after torch, this library
+        # binds to the runtime torch brought in (same soname).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

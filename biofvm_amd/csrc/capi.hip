@@ -825,6 +825,35 @@ int bfvm_autotune(bfvm_handle h, double dt)
         }
         e.spread[axis] = best_spread;
     }
+    // the x sweep with the sources fused in loads the bulk fields beside the
+    // store, so its best shape is timed on its own (on the scratch store,
+    // with the handle's bulk fields, agents and Dirichlet nodes)
+    if (st == hipSuccess && e.fused_sources && (e.have_bulk || e.n_dir > 0)) {
+        if (e.have_bulk) st = bfvm::ensure_bulk(e, dt);
+        bfvm::SweepPlan best_fused = e.fused_plan;
+        float best_ms = 1e30f;
+        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(e, 0)) {
+            if (st != hipSuccess) break;
+            if (cand.stream != 0) continue;
+            e.fused_plan = cand;
+            if (!bfvm::sweep_x_fusable(e)) continue;
+            auto fused_x = [&] {
+                return bfvm::launch_sweep_x_fused(e, e.have_bulk, e.n_agents > 0, e.lod_dims >= 2);
+            };
+            if ((st = fused_x()) != hipSuccess) break;  // warm-up
+            (void)hipEventRecord(t0, e.stream);
+            for (int rep = 0; rep < 3 && st == hipSuccess; ++rep) st = fused_x();
+            (void)hipEventRecord(t1, e.stream);
+            if (st != hipSuccess || (st = hipEventSynchronize(t1)) != hipSuccess) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_fused = cand;
+            }
+        }
+        e.fused_plan = best_fused;
+    }
     // plane chunking of the x and y sweeps: whole diffusion phases, timed
     if (st == hipSuccess && bfvm::sweep_chunkable(e, 0) && bfvm::sweep_chunkable(e, 1)) {
         int best_chunk = 0;

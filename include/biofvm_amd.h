@@ -178,8 +178,10 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
  * Autotuner: times every valid launch shape of the three segmented sweeps
  * (lines per block x nodes per segment x kernel) for this grid, substrate
  * count and precision on a scratch copy of the store, keeps the fastest per
- * axis, then its XCD spread (bfvm_set_sweep_spread), then times the x/y
- * plane chunk sizes (bfvm_set_sweep_chunk).
+ * axis, then its XCD spread (bfvm_set_sweep_spread), then (with bulk fields
+ * or Dirichlet nodes set and fused sources on) the one-block-per-tile shapes
+ * of the x sweep with the sources fused in, then times the x/y plane chunk
+ * sizes (bfvm_set_sweep_chunk).
  * The densities are left unchanged.  No counterpart in the reference (its
  * OpenMP loops have no launch shape).
  */
@@ -211,8 +213,8 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
  * the voxels holding agents.  The store pass of the bulk kernel and two
  * Dirichlet launches are saved.  Same operations and roundings either way:
  * the same bits for either store precision.  Used when the x sweep can run
- * a one-block-per-tile shape (the fastest the autotuner timed) and no x/y
- * plane chunking is set. */
+ * a one-block-per-tile shape (the autotuner's fastest for the fused sweep,
+ * timed with the handle's sources) and no x/y plane chunking is set. */
 int bfvm_set_fused_sources(bfvm_handle h, int on);
 int bfvm_get_fused_sources(bfvm_handle h, int* on);
 
