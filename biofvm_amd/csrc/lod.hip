@@ -546,6 +546,7 @@ k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ map, i
 struct FusedSources {
     const double* bulk_a;         // A, B per value, or null
     const double* bulk_b;
+    size_t bulk_bytes;            // bytes of each (below 4 GiB)
     const unsigned* agent_bits;   // voxels whose bulk update is done already, or null
     const unsigned* dbits;        // Dirichlet bitmap, or null
     const int* drank;
@@ -588,19 +589,27 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 8 ? 8 : 4) k_sweep_x_src(Swe
 
     if (F.bulk_a) {
         const unsigned long long done = F.agent_bits ? bit_window(F.agent_bits, v0) : 0ull;
+        // buffer loads: node p0's byte offset per lane, then one uniform
+        // offset per node (no per-node address arithmetic).  Nodes past n-1
+        // keep their clamped raw value (finite, and they reach no stored
+        // node), so their (A, B), which may lie past the buffer and read 0,
+        // are not used.
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(F.bulk_a, F.bulk_bytes);
+        const __amdgpu_buffer_rsrc_t rb = make_rsrc(F.bulk_b, F.bulk_bytes);
+        const unsigned vb = ((unsigned)v0 * (unsigned)S + (unsigned)s) * (unsigned)sizeof(double);
+        const unsigned stepb = (unsigned)S * (unsigned)sizeof(double);
         // eight nodes at a time, so that at most 8 pairs (A, B) wait in registers
 #pragma unroll
         for (int t0 = 0; t0 < L; t0 += 8) {
             double a8[8], b8[8];
 #pragma unroll
             for (int t = t0; t < t0 + 8 && t < L; ++t) {
-                const size_t e = (size_t)(v0 + min(t, cnt - 1)) * S + s;
-                a8[t - t0] = F.bulk_a[e];
-                b8[t - t0] = F.bulk_b[e];
+                a8[t - t0] = buf_load((double*)nullptr, ra, vb, (unsigned)t * stepb);
+                b8[t - t0] = buf_load((double*)nullptr, rb, vb, (unsigned)t * stepb);
             }
 #pragma unroll
             for (int t = t0; t < t0 + 8 && t < L; ++t)
-                if (!((done >> t) & 1ull)) raw[t] = (T)(((double)raw[t] + a8[t - t0]) / b8[t - t0]);
+                if (t < cnt && !((done >> t) & 1ull)) raw[t] = (T)(((double)raw[t] + a8[t - t0]) / b8[t - t0]);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
@@ -1182,8 +1191,10 @@ bool sweep_x_fusable(const Env& e)
 {
     const SweepPlan p = fused_x_plan(e);
     const bool seg = e.precision == BFVM_FP64 ? segmented<double>(e, 0, p) : segmented<float>(e, 0, p);
-    // one launch over all planes: 32-bit offsets from the store's start
-    return seg && p.stream == 0 && e.nvox < (1ull << 31) && e.nvs * e.elem < 0xffffffffull;
+    // one launch over all planes: 32-bit offsets from the store's start and
+    // into the FP64 bulk fields
+    return seg && p.stream == 0 && e.nvox < (1ull << 31) && e.nvs * e.elem < 0xffffffffull &&
+           e.nvs * sizeof(double) < 0xffffffffull;
 }
 
 hipError_t launch_sweep_x_fused(Env& e, bool bulk, bool skip_agent_voxels, bool dir_after)
@@ -1192,6 +1203,7 @@ hipError_t launch_sweep_x_fused(Env& e, bool bulk, bool skip_agent_voxels, bool 
     if (bulk) {
         F.bulk_a = (const double*)e.bulk_A;
         F.bulk_b = (const double*)e.bulk_B;
+        F.bulk_bytes = e.nvs * sizeof(double);
         if (skip_agent_voxels && e.n_agent_voxels > 0) {
             if (!e.agent_bits) return hipErrorInvalidValue;
             F.agent_bits = e.agent_bits;
