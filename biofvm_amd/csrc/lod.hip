@@ -564,6 +564,8 @@ __device__ __forceinline__ unsigned long long bit_window(const unsigned* bits, i
     return (((unsigned long long)bits[w0 + 1] << 32) | bits[w0]) >> b0;
 }
 
+constexpr int kBulkChunk = 8;  // 16 measured no faster (drop_in_200: 246 vs 242 us)
+
 template <typename T, int L, int COLS>
 __global__ void __launch_bounds__(kMaxThreads, L <= 8 ? 8 : 4) k_sweep_x_src(SweepArgs A, FusedSources F)
 {
@@ -598,17 +600,18 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 8 ? 8 : 4) k_sweep_x_src(Swe
         const __amdgpu_buffer_rsrc_t rb = make_rsrc(F.bulk_b, F.bulk_bytes);
         const unsigned vb = ((unsigned)v0 * (unsigned)S + (unsigned)s) * (unsigned)sizeof(double);
         const unsigned stepb = (unsigned)S * (unsigned)sizeof(double);
-        // eight nodes at a time, so that at most 8 pairs (A, B) wait in registers
+        // kBulkChunk nodes at a time, so that at most kBulkChunk pairs (A, B)
+        // wait in registers
 #pragma unroll
-        for (int t0 = 0; t0 < L; t0 += 8) {
-            double a8[8], b8[8];
+        for (int t0 = 0; t0 < L; t0 += kBulkChunk) {
+            double a8[kBulkChunk], b8[kBulkChunk];
 #pragma unroll
-            for (int t = t0; t < t0 + 8 && t < L; ++t) {
+            for (int t = t0; t < t0 + kBulkChunk && t < L; ++t) {
                 a8[t - t0] = buf_load((double*)nullptr, ra, vb, (unsigned)t * stepb);
                 b8[t - t0] = buf_load((double*)nullptr, rb, vb, (unsigned)t * stepb);
             }
 #pragma unroll
-            for (int t = t0; t < t0 + 8 && t < L; ++t)
+            for (int t = t0; t < t0 + kBulkChunk && t < L; ++t)
                 if (t < cnt && !((done >> t) & 1ull)) raw[t] = (T)(((double)raw[t] + a8[t - t0]) / b8[t - t0]);
             __builtin_amdgcn_sched_barrier(0);
         }
