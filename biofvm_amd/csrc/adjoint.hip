@@ -205,6 +205,60 @@ __global__ void __launch_bounds__(kBlock) k_axis_grad(const double* __restrict__
     block_sums<2 * kChunk>(acc, partial + (size_t)blockIdx.x * 2 * kChunk);
 }
 
+// The same sums for the y or z axis, whose neighbour is nx or nx*ny voxels
+// away: a work item is a run of kRun consecutive nodes of one line (all sc
+// substrates), whose values are loaded once (plus the next run's first node
+// for the last edge), all loads of a substrate in flight together; items are
+// (run, column) with the column fastest, so a wave's loads are contiguous.
+constexpr int kRun = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_axis_grad_runs(const double* __restrict__ lam,
+                                                           const T* __restrict__ y, int nx, int ny, int nz,
+                                                           int S, int axis, int s0, int sc,
+                                                           double* __restrict__ partial)
+{
+    double acc[2 * kChunk];
+#pragma unroll
+    for (int m = 0; m < 2 * kChunk; ++m) acc[m] = 0.0;
+    const unsigned nxu = (unsigned)nx, nxy = (unsigned)nx * (unsigned)ny;
+    const unsigned n = (unsigned)(axis == 1 ? ny : nz);
+    const unsigned stride = axis == 1 ? nxu : nxy;            // voxels between nodes of a line
+    const unsigned ncol = axis == 1 ? nxu * (unsigned)nz : nxy;  // lines
+    const unsigned nrun = (n + kRun - 1) / kRun;
+    const unsigned items = ncol * nrun;
+    for (unsigned it = blockIdx.x * blockDim.x + threadIdx.x; it < items; it += gridDim.x * blockDim.x) {
+        const unsigned run = it / ncol, c = it - run * ncol;
+        // line c's node 0: z lines start in plane 0; y line (i, k) at (k, 0, i)
+        const unsigned base = axis == 1 ? (c / nxu) * nxy + c % nxu : c;
+        const unsigned p0 = run * kRun;
+        const int cnt = (int)min((unsigned)kRun, n - p0);         // nodes of this run
+        const bool more = p0 + kRun < n;                          // the next run's node 0 exists
+#pragma unroll
+        for (int m = 0; m < kChunk; ++m) {
+            if (m < sc) {
+                double l[kRun + 1], yv[kRun + 1];
+#pragma unroll
+                for (int u = 0; u <= kRun; ++u) {
+                    const bool in = u < cnt || (u == kRun && more);
+                    const size_t q = (size_t)(base + (p0 + (unsigned)u) * stride) * S + s0 + m;
+                    l[u] = in ? lam[q] : 0.0;
+                    yv[u] = in ? (double)y[q] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kRun; ++u) {
+                    if (u < cnt) {
+                        acc[kChunk + m] += l[u] * yv[u];
+                        if (u + 1 < cnt || (u + 1 == kRun && more))
+                            acc[m] += (l[u] - l[u + 1]) * (yv[u] - yv[u + 1]);
+                    }
+                }
+            }
+        }
+    }
+    block_sums<2 * kChunk>(acc, partial + (size_t)blockIdx.x * 2 * kChunk);
+}
+
 // gc[s] -= sum of the blocks' edge sums, gc[S + s] -= sum of the l.y sums.
 __global__ void __launch_bounds__(kBlock) k_axis_grad_finish(const double* __restrict__ partial, int nblocks,
                                                               int S, int s0, int sc, double* __restrict__ gc)
@@ -275,8 +329,12 @@ hipError_t axis_grad(Env& e, Tape& tp, const void* y, int axis)
 {
     for (int s0 = 0; s0 < e.S; s0 += kChunk) {
         const int sc = std::min(kChunk, e.S - s0);
-        k_axis_grad<T><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny, e.nz,
-                                                           e.S, axis, s0, sc, tp.partial);
+        if (axis == 0)  // the neighbour is the next voxel: the same or the next cache line
+            k_axis_grad<T><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny, e.nz,
+                                                               e.S, axis, s0, sc, tp.partial);
+        else
+            k_axis_grad_runs<T><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny,
+                                                                    e.nz, e.S, axis, s0, sc, tp.partial);
         hipError_t st = hipGetLastError();
         if (st != hipSuccess) return st;
         k_axis_grad_finish<<<1, kBlock, 0, e.stream>>>(tp.partial, kRedBlocks, e.S, s0, sc, tp.gc);
