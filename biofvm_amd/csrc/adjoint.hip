@@ -57,6 +57,10 @@ struct Tape {
     double* gupt = nullptr;    // agents: uptake gradient (grouped order)
     double* gc = nullptr;      // 2*S: dJ/dc1, dJ/dc2
     double* partial = nullptr; // reduction partials
+    // the D / lambda sums' block partials, per chunk of kChunk substrates,
+    // added up over the whole backward pass and reduced once at its end
+    double* gpart = nullptr;
+    size_t gpart_n = 0;
 };
 
 }  // namespace bfvm
@@ -147,10 +151,11 @@ __global__ void k_agents_adjoint(double* __restrict__ lam, const T* __restrict__
     lam[v] = l;
 }
 
-// out[m] = sum over the block's threads of acc[m], m < M, in a fixed order
-// (wave shuffles, then the waves in index order).
+// out[m] = (or, with accumulate, +=) sum over the block's threads of
+// acc[m], m < M, in a fixed order (wave shuffles, then the waves in index
+// order).
 template <int M>
-__device__ void block_sums(double (&acc)[M], double* __restrict__ out)
+__device__ void block_sums(double (&acc)[M], double* __restrict__ out, bool accumulate = false)
 {
     __shared__ double red[kBlock / 64][M];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -164,13 +169,14 @@ __device__ void block_sums(double (&acc)[M], double* __restrict__ out)
     if (threadIdx.x < M) {
         double v = 0.0;
         for (int w = 0; w < kBlock / 64; ++w) v += red[w][threadIdx.x];
-        out[threadIdx.x] = v;
+        out[threadIdx.x] = accumulate ? out[threadIdx.x] + v : v;
     }
 }
 
 // Per block: for substrates s0 .. s0+sc-1 (sc <= kChunk), the sums over the
 // block's voxels of (l_v - l_w)(y_v - y_w) over edges v -> w = v + step along
-// `axis`, and of l_v y_v.  partial[block][0..kChunk) and [kChunk..2 kChunk).
+// `axis`, and of l_v y_v, added to partial[block][0..kChunk) and
+// [kChunk..2 kChunk).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_axis_grad(const double* __restrict__ lam, const T* __restrict__ y,
                                                       int nx, int ny, int nz, int S, int axis, int s0,
@@ -202,7 +208,7 @@ __global__ void __launch_bounds__(kBlock) k_axis_grad(const double* __restrict__
             }
         }
     }
-    block_sums<2 * kChunk>(acc, partial + (size_t)blockIdx.x * 2 * kChunk);
+    block_sums<2 * kChunk>(acc, partial + (size_t)blockIdx.x * 2 * kChunk, true);
 }
 
 // The same sums for the y or z axis, whose neighbour is nx or nx*ny voxels
@@ -256,7 +262,7 @@ __global__ void __launch_bounds__(kBlock) k_axis_grad_runs(const double* __restr
             }
         }
     }
-    block_sums<2 * kChunk>(acc, partial + (size_t)blockIdx.x * 2 * kChunk);
+    block_sums<2 * kChunk>(acc, partial + (size_t)blockIdx.x * 2 * kChunk, true);
 }
 
 // gc[s] -= sum of the blocks' edge sums, gc[S + s] -= sum of the l.y sums.
@@ -324,21 +330,35 @@ hipError_t dirichlet_zero(Env& e, double* lam)
     return hipGetLastError();
 }
 
+// The axis's D / lambda sums, added to the block partials of tp.gpart (one
+// kRedBlocks x 2 kChunk slab per chunk of substrates); axis_grad_finish
+// reduces them into gc once the backward pass is done.
 template <typename T>
 hipError_t axis_grad(Env& e, Tape& tp, const void* y, int axis)
 {
     for (int s0 = 0; s0 < e.S; s0 += kChunk) {
         const int sc = std::min(kChunk, e.S - s0);
+        double* part = tp.gpart + (size_t)(s0 / kChunk) * kRedBlocks * 2 * kChunk;
         if (axis == 0)  // the neighbour is the next voxel: the same or the next cache line
             k_axis_grad<T><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny, e.nz,
-                                                               e.S, axis, s0, sc, tp.partial);
+                                                               e.S, axis, s0, sc, part);
         else
             k_axis_grad_runs<T><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny,
-                                                                    e.nz, e.S, axis, s0, sc, tp.partial);
+                                                                    e.nz, e.S, axis, s0, sc, part);
         hipError_t st = hipGetLastError();
         if (st != hipSuccess) return st;
-        k_axis_grad_finish<<<1, kBlock, 0, e.stream>>>(tp.partial, kRedBlocks, e.S, s0, sc, tp.gc);
-        if ((st = hipGetLastError()) != hipSuccess) return st;
+    }
+    return hipSuccess;
+}
+
+hipError_t axis_grad_finish(Env& e, Tape& tp)
+{
+    for (int s0 = 0; s0 < e.S; s0 += kChunk) {
+        const int sc = std::min(kChunk, e.S - s0);
+        const double* part = tp.gpart + (size_t)(s0 / kChunk) * kRedBlocks * 2 * kChunk;
+        k_axis_grad_finish<<<1, kBlock, 0, e.stream>>>(part, kRedBlocks, e.S, s0, sc, tp.gc);
+        hipError_t st = hipGetLastError();
+        if (st != hipSuccess) return st;
     }
     return hipSuccess;
 }
@@ -497,6 +517,7 @@ void free_tape(Env& e)
     free_ptr(tp->gupt);
     free_ptr(tp->gc);
     free_ptr(tp->partial);
+    free_ptr(tp->gpart);
     delete tp;
     e.tape = nullptr;
 }
@@ -619,6 +640,13 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
     if (!tp->partial) AHIP(e, hipMalloc((void**)&tp->partial, sizeof(double) * kRedBlocks * 2 * kChunk));
     if (!tp->gc) AHIP(e, hipMalloc((void**)&tp->gc, sizeof(double) * 2 * S));
     AHIP(e, hipMemsetAsync(tp->gc, 0, sizeof(double) * 2 * S, e.stream));
+    const size_t gpart_n = (size_t)((S + kChunk - 1) / kChunk) * kRedBlocks * 2 * kChunk;
+    if (tp->gpart_n != gpart_n) {
+        free_ptr(tp->gpart);
+        AHIP(e, hipMalloc((void**)&tp->gpart, sizeof(double) * gpart_n));
+        tp->gpart_n = gpart_n;
+    }
+    AHIP(e, hipMemsetAsync(tp->gpart, 0, sizeof(double) * gpart_n, e.stream));
     const size_t na = (size_t)e.n_agents * S;
     if (na > 0) {
         free_ptr(tp->rk);
@@ -644,6 +672,7 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
             AHIP(e, e.precision == BFVM_FP64 ? adjoint_step<double>(e, *tp) : adjoint_step<float>(e, *tp));
         }
     }
+    if (tp->flags & BFVM_DIFFUSION) AHIP(e, axis_grad_finish(e, *tp));
     tp->seeded = false;  // the adjoint state now holds dJ/drho_0
     // results
     std::vector<double> gc(2 * S);
