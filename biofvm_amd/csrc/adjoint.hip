@@ -47,6 +47,7 @@ struct Tape {
     std::vector<void*> cps;   // state at steps 0, K, 2K, ...
     std::vector<void*> ring;  // starts of the steps of one segment
     void* s1 = nullptr;       // after bulk (the agents' input)
+    const void* after_bulk = nullptr;  // s1, or the step's start when no bulk update ran
     void* s4 = nullptr;       // after the x sweep
     void* s6 = nullptr;       // after the y sweep
     void* s8 = nullptr;       // after the z sweep (the step's result)
@@ -411,11 +412,16 @@ hipError_t forward_step(Env& e, const Tape& tp, void* buf)
 hipError_t forward_step_keep(Env& e, Tape& tp, const void* start)
 {
     hipError_t st;
-    if ((st = copy_store(e, tp.s1, start))) return st;
-    if ((tp.flags & BFVM_BULK) && e.have_bulk &&
-        (st = on_buffer(e, tp.s1, [&] { return bfvm::launch_bulk(e); })))
-        return st;
-    if ((st = copy_store(e, tp.s4, tp.s1))) return st;
+    // without a bulk update the state after it is the start itself (the
+    // ring slot stays untouched until the step's transposes have run)
+    if ((tp.flags & BFVM_BULK) && e.have_bulk) {
+        if ((st = copy_store(e, tp.s1, start))) return st;
+        if ((st = on_buffer(e, tp.s1, [&] { return bfvm::launch_bulk(e); }))) return st;
+        tp.after_bulk = tp.s1;
+    } else {
+        tp.after_bulk = start;
+    }
+    if ((st = copy_store(e, tp.s4, tp.after_bulk))) return st;
     if ((tp.flags & BFVM_CELLS) && e.n_agents > 0 &&
         (st = on_buffer(e, tp.s4, [&] { return bfvm::launch_agents(e); })))
         return st;
@@ -451,7 +457,7 @@ hipError_t adjoint_step(Env& e, Tape& tp)
     if ((tp.flags & BFVM_CELLS) && e.n_agent_voxels > 0) {
         const size_t n = (size_t)e.n_agent_voxels * e.S;
         k_agents_adjoint<T><<<grid_for(n), kBlock, 0, e.stream>>>(
-            tp.lam, (const T*)tp.s1, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
+            tp.lam, (const T*)tp.after_bulk, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
             (const double*)e.agent_t1, (const double*)e.agent_t2, e.agent_rates ? e.agent_c : nullptr,
             e.agent_sat, tp.rk, tp.gsec, tp.gupt);
         if ((st = hipGetLastError())) return st;
