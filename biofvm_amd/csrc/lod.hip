@@ -36,7 +36,12 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxThreads = 1024;  // block = COLS columns x nseg segments
-constexpr size_t kMaxCorrectionLds = 48 * 1024;  // g0/g1 rows staged per block
+// g0/g1 rows and K staged per block (dynamic LDS beside the kernels' static
+// 16-17 KiB, or 32 KiB for the ensemble kernel): up to 128 KiB, so grids
+// with many substrates keep the segmented kernels (the tables grow with S
+// times the longest correction, ~130 rows for the fastest diffusion here)
+constexpr size_t kMaxCorrectionLds = 128 * 1024;
+constexpr size_t kMaxBatchCorrectionLds = 112 * 1024;
 constexpr int kPad = 32;  // zero rows around the correction tables (>= the largest L)
 
 // Raw buffer access: a per-lane byte offset plus a uniform (scalar) one, no
@@ -927,12 +932,28 @@ size_t sweep_lds(const SweepArgs& A)
     return sizeof(double) * (size_t)(A.kg0 + A.kg1 + 3 * kPad + 8) * A.S;
 }
 
+// Lets `kernel` take more than 64 KiB of dynamic LDS (the correction tables
+// of a grid with many substrates): set once per kernel.
+hipError_t allow_table_lds(const void* kernel, size_t lds, size_t cap = kMaxCorrectionLds)
+{
+    if (lds <= 64 * 1024) return hipSuccess;
+    static std::mutex mu;
+    static std::map<const void*, bool> done;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[kernel]) return hipSuccess;
+    const hipError_t st = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
+    if (st == hipSuccess) done[kernel] = true;
+    return st;
+}
+
 template <typename T, int L, int COLS>
 hipError_t launch_seg(Env& e, int axis, int nseg, int k0, int k1)
 {
     const SweepArgs A = sweep_args(e, axis, COLS, nseg, k0, k1);
     dim3 grid(A.tiles, A.planes);
     int threads = (COLS * nseg + 63) / 64 * 64;
+    const hipError_t st = allow_table_lds((const void*)k_sweep<T, L, COLS>, sweep_lds(A));
+    if (st != hipSuccess) return st;
     k_sweep<T, L, COLS><<<grid, threads, sweep_lds(A), e.stream>>>(A);
     return hipGetLastError();
 }
@@ -985,6 +1006,8 @@ hipError_t launch_stream(Env& e, int axis, int nseg, int k0, int k1)
     const int threads = (COLS * nseg + 63) / 64 * 64;
     const size_t lds = sweep_lds(A);
     const long long total = (long long)A.tiles * A.planes;
+    const hipError_t st = allow_table_lds((const void*)k_sweep_stream<T, L, COLS>, lds);
+    if (st != hipSuccess) return st;
     const int nb = resident_blocks((const void*)k_sweep_stream<T, L, COLS>, threads, lds);
     const int grid = (int)std::min<long long>(total, (long long)nb * compute_units(e));
     k_sweep_stream<T, L, COLS><<<grid, threads, lds, e.stream>>>(A, grid);
@@ -1151,6 +1174,8 @@ hipError_t launch_x_src(Env& e, int nseg, const FusedSources& F)
     const SweepArgs A = sweep_args(e, 0, COLS, nseg);
     dim3 grid(A.tiles, A.planes);
     const int threads = (COLS * nseg + 63) / 64 * 64;
+    const hipError_t st = allow_table_lds((const void*)k_sweep_x_src<T, L, COLS>, sweep_lds(A));
+    if (st != hipSuccess) return st;
     k_sweep_x_src<T, L, COLS><<<grid, threads, sweep_lds(A), e.stream>>>(A, F);
     return hipGetLastError();
 }
@@ -1224,9 +1249,13 @@ hipError_t launch_sweep_x_fused(Env& e, bool bulk, bool skip_agent_voxels, bool 
     return e.precision == BFVM_FP64 ? sweep_x_fused<double>(e, p, F) : sweep_x_fused<float>(e, p, F);
 }
 
+// The shape that runs the axis; lines = 0 when the one-thread-per-line
+// fallback runs it.
 SweepPlan effective_plan(const Env& e, int axis)
 {
-    return plan_for(e, axis);
+    const SweepPlan p = plan_for(e, axis);
+    const bool seg = e.precision == BFVM_FP64 ? segmented<double>(e, axis, p) : segmented<float>(e, axis, p);
+    return seg ? p : SweepPlan{0, 0, 0};
 }
 
 std::vector<SweepPlan> sweep_candidates(const Env& e, int axis)
@@ -1259,13 +1288,15 @@ bool batch_member_ok_t(const Env& e, int axis, int L)
     const size_t gbytes =
         sizeof(double) * (size_t)(e.ax[axis].kg0 + e.ax[axis].kg1 + 3 * kPad + 8) * e.S;
     const int nseg = (gm.n + L - 1) / L;
-    return gm.n >= 2 && nseg <= kMaxThreads / 64 && span < 0xffffffffull && gbytes <= kMaxCorrectionLds;
+    return gm.n >= 2 && nseg <= kMaxThreads / 64 && span < 0xffffffffull && gbytes <= kMaxBatchCorrectionLds;
 }
 
 template <typename T, int L>
 hipError_t launch_batch_t(const SweepArgs* d_args, const int* d_first, int nm, int blocks,
                           size_t lds, hipStream_t st, int spread)
 {
+    const hipError_t a = allow_table_lds((const void*)k_sweep_batch<T, L>, lds, kMaxBatchCorrectionLds);
+    if (a != hipSuccess) return a;
     k_sweep_batch<T, L><<<blocks, kMaxThreads, lds, st>>>(d_args, d_first, nm, spread);
     return hipGetLastError();
 }

@@ -181,6 +181,24 @@ def test_autotune_keeps_state_and_parity(precision):
         assert rel_err(m.download(), expected) <= TOL[precision]
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_many_substrates_keep_the_segmented_kernels(precision):
+    # 24 substrates, one diffusing fast (its end corrections run the whole
+    # line): the x and y tables take 69 and 57 KiB of LDS per block
+    rng = np.random.default_rng(29)
+    nx, ny, nz, S = 128, 96, 24, 24
+    D = np.exp(rng.uniform(np.log(1e2), np.log(1e5), S))
+    D[3] = 1e5
+    lam = np.exp(rng.uniform(np.log(1e-3), np.log(1.0), S))
+    case = W.make_case(nx, ny, nz, S, 10.0, D, lam, 0.01, 2, W.DIFFUSION, rng.uniform(0, 1, nx * ny * nz * S))
+    ref = oracle.oracle_run(case)
+    with setup_case(case, precision) as m:
+        plans = m.sweep_plans()
+        assert plans[0][0] > 0 and plans[1][0] > 0, plans  # not the one-thread-per-line fallback
+        m.run(case["dt"], case["steps"], case["phases"])
+        assert rel_err(m.download(), ref) <= TOL[precision]
+
+
 @pytest.fixture(scope="module")
 def plan_case():
     # large enough that resident (streaming) blocks loop over several tiles on
