@@ -53,6 +53,10 @@ struct SweepArgs {
     int cols;                // lines per block (a power of two)
     int tiles, planes;       // column tiles per plane, planes
     int spread;              // tiles per XCD chunk (see xcd_tile in lod.hip)
+    // 1: every wave's buffer starts at its own segment's first node (cols =
+    // 64, one segment per wave), so a z line may span more than 4 GiB of
+    // addresses: the store's z sweep above 4 GiB
+    int seg_base;
 };
 
 // Arguments of the x sweep on LDS-staged row tiles (k_sweep_rows in lod.hip):

@@ -195,8 +195,14 @@ __device__ __forceinline__ SegPos seg_pos(const SweepArgs& A, int C, int tile, i
     // plane is the same in every lane; say so, so the buffer descriptor
     // built from it stays in scalar registers
     P.plane0 = (size_t)__builtin_amdgcn_readfirstlane(plane) * A.plane_stride;
-    P.vo = ((unsigned)(col / A.S) * A.vstride + (unsigned)P.s + (unsigned)P.p0 * A.line_stride) *
-           (unsigned)sizeof(T);
+    unsigned node0 = (unsigned)P.p0 * A.line_stride;  // values from the plane start to node p0
+    if (A.seg_base) {
+        // C = 64: the wave is one segment, p0 the same in every lane; the
+        // descriptor starts at node p0 and offsets stay below L+1 nodes
+        P.plane0 += (size_t)__builtin_amdgcn_readfirstlane(P.p0) * A.line_stride;
+        node0 = 0;
+    }
+    P.vo = ((unsigned)(col / A.S) * A.vstride + (unsigned)P.s + node0) * (unsigned)sizeof(T);
     return P;
 }
 
@@ -860,7 +866,13 @@ SweepPlan plan_for(const Env& e, int axis)
 {
     const SweepPlan& t = e.plan[axis];
     if (t.cols > 0) return t;
-    return default_plan(axis == 0 ? e.nx : axis == 1 ? e.ny : e.nz);
+    const int n = axis == 0 ? e.nx : axis == 1 ? e.ny : e.nz;
+    if (axis == 2 && e.nvs * e.elem >= 0xffffffffull) {
+        // a store above 4 GiB: z lines one segment per wave (SweepArgs::seg_base)
+        for (int L : {16, 32, 8, 24, 12, 4})
+            if (plan_valid(n, 64, L) && (L <= 8 || L < 2 * n)) return SweepPlan{64, L};
+    }
+    return default_plan(n);
 }
 
 template <typename T>
@@ -924,6 +936,7 @@ SweepArgs sweep_args(const Env& e, int axis, int cols, int nseg, int k0 = 0, int
     A.tiles = (gm.ncols + cols - 1) / cols;
     A.planes = k1 - k0;
     A.spread = e.spread[axis];
+    A.seg_base = axis == 2 && cols == 64 && e.nvs * e.elem >= 0xffffffffull ? 1 : 0;
     return A;
 }
 
@@ -1118,8 +1131,10 @@ bool segmented(const Env& e, int axis, const SweepPlan& p)
     Geometry gm = geometry(e, axis);
     // buffer offsets are 32-bit: the segmented kernel addresses at most 4 GiB
     // from a plane's start (x: from the first of a run of z planes, at least
-    // one)
-    const size_t span = (axis == 2 ? e.nvs : (size_t)e.nx * e.ny * e.S) * sizeof(T);
+    // one; z with 64 lines per block: from a segment's first node)
+    size_t span = (axis == 2 ? e.nvs : (size_t)e.nx * e.ny * e.S) * sizeof(T);
+    if (axis == 2 && span >= 0xffffffffull && p.cols == 64 && p.stream != 2)
+        span = (size_t)e.nx * e.ny * e.S * (size_t)(p.L + 1) * sizeof(T);
     // the end-correction tables live in LDS beside the 16 KiB carry arrays
     const size_t gbytes =
         sizeof(double) * (size_t)(e.ax[axis].kg0 + e.ax[axis].kg1 + 3 * kPad + 8) * e.S;

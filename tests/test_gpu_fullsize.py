@@ -9,6 +9,11 @@
   sums are 1 + c2, so each substrate's mass is multiplied by exactly
   (1 + lambda dt / 3)^-3 per step (no-flux boundaries).  FP64 store within
   1e-12, FP32 store within 1e-6 (its sum is taken in FP64 over FP32 values).
+- A store above 4 GiB (416 x 416 x 400 x 8, FP64, 4.43 GB): the x sweep runs
+  in two runs of z planes and the z sweep with one segment per wave (its
+  buffers start at each segment), both past the 32-bit offsets of one
+  buffer.  A separable cosine mode decays by the closed-form factor of each
+  axis, 1 / (1 + c2 + 4 c1 sin^2(pi m / 2n)), at every value.
 """
 import numpy as np
 import pytest
@@ -58,3 +63,30 @@ def test_400_cubed_mass_decays_by_the_exact_factor(precision, tol):
     c2 = lam * dt / 3.0
     expect = start * (1.0 + c2) ** (-3 * steps)
     assert np.max(np.abs(end - expect) / expect) <= tol
+
+
+def test_store_above_4_gib_cosine_mode_decays_at_its_eigenvalue():
+    nx, ny, nz, S = 416, 416, 400, 8
+    modes = (1, 2, 3)
+    D = np.array([1e5, 3e4, 1e4, 3e3, 1e3, 300.0, 100.0, 30.0])
+    lam = np.array([0.1, 0.01, 0.5, 0.001, 1.0, 0.05, 0.2, 0.003])
+    dt, dx = 0.01, 10.0
+    cx, cy, cz = [np.cos(np.pi * m * (np.arange(n) + 0.5) / n) for m, n in zip(modes, (nx, ny, nz))]
+    amp = np.arange(1.0, S + 1.0)
+    rho0 = (cz[:, None, None, None] * cy[None, :, None, None] * cx[None, None, :, None]
+            * amp[None, None, None, :]).reshape(-1)
+    assert rho0.size * 8 > 2 ** 32
+    case = W.make_case(nx, ny, nz, S, dx, D, lam, dt, 1, W.DIFFUSION, rho0)
+    with setup_case(case, FP64) as m:
+        plans = m.sweep_plans()
+        assert plans[2][0] == 64, plans  # segmented z, one segment per wave
+        m.run(dt, 1, W.DIFFUSION)
+        got = m.download().reshape(-1, S)
+    c1 = D * dt / dx ** 2
+    c2 = lam * dt / 3.0
+    f = np.ones(S)
+    for mo, n in zip(modes, (nx, ny, nz)):
+        f /= 1.0 + c2 + 4.0 * c1 * np.sin(np.pi * mo / (2 * n)) ** 2
+    expect = rho0.reshape(-1, S) * f
+    err = np.max(np.abs(got - expect), axis=0) / amp
+    assert np.all(err < 1e-13), err
