@@ -461,8 +461,10 @@ def main():
         cols, L, streaming = plans["xyz".index(dom[-1])]
         kname = ("k_sweep_x_src" if dom == "sweep_x" and fused else
                  "k_sweep_rows" if streaming == 2 else "k_sweep_stream" if streaming else "k_sweep")
-        traffic = measured_traffic(args.workload,
-                                   f"{kname}<{'float' if main_prec == FP32 else 'double'}, {L}, {cols}>")
+        tname = "float" if main_prec == FP32 else "double"
+        # the row-tile kernel's template has no lines-per-block parameter
+        targs = f"{tname}, {L}" if kname == "k_sweep_rows" else f"{tname}, {L}, {cols}"
+        traffic = measured_traffic(args.workload, f"{kname}<{targs}>")
     step_bytes = algorithmic_bytes(case, w_main)
 
     out_extra = {}
