@@ -280,7 +280,7 @@ def ensemble_bench(args):
         "ms_per_step": 1e3 * wall / steps,
         "events_ms_per_step": ev_ms / steps,
         "eager_ms_per_step": eager_ms,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "store_dtype": "f32",
         "data": "synthetic (seeded numpy default_rng; see biofvm_amd/workloads.py)",
         "config": {"workload": f"ensemble_64: {len(cases)} grids, {nvs} voxel-substrate values, "
                                f"{n_agents} agents, dt={dt}, phases={ph}, one HIP graph per step"},
@@ -387,7 +387,7 @@ def differentiation_bench(args):
         "unit": "voxel-substrate updates/s",
         "n_gpus": 1, "steps": steps, "warmup": 0,
         "ms_per_step": 1e3 * t_bwd / steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "store_dtype": "f32",
         "data": "synthetic (seeded numpy default_rng; see biofvm_amd/workloads.py)",
         "config": {"workload": f"differentiation_128: {case['nx']}^3 voxels x {case['S']} substrates, "
                                f"{n_agents} agents, {steps} steps, checkpoints every 50"},
@@ -509,7 +509,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if main_prec == FP32 else "f64",
+        # the arithmetic type (every store precision solves in FP64 registers
+        # with FP64 coefficients); the store's value type beside it
+        "dtype": "f64",
+        "store_dtype": "f32" if main_prec == FP32 else "f64",
         "data": "synthetic (seeded numpy default_rng; see biofvm_amd/workloads.py)",
         "config": {"workload": f"{args.workload}: {case['nx']}x{case['ny']}x{case['nz']} voxels x "
                                f"{case['S']} substrates, dx={case['dx']}, dt={case['dt']}, "
