@@ -36,6 +36,17 @@ def test_library_exports_every_declared_symbol():
     assert exported == set(declared_functions())
 
 
+def test_one_hip_runtime_in_the_process():
+    # torch is loaded before the library, so the library binds to the HIP
+    # runtime torch brought in (same soname) instead of loading a second one
+    import sys
+    _capi.load()
+    assert "torch" in sys.modules
+    maps = open("/proc/self/maps").read()
+    runtimes = {line.split()[-1] for line in maps.splitlines() if "libamdhip64" in line}
+    assert len(runtimes) == 1, runtimes
+
+
 def test_library_carries_gfx950_code_objects():
     # the offload bundle embedded by hipcc names its target triple
     blob = open(_capi.LIB_PATH, "rb").read()
