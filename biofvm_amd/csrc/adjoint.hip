@@ -436,8 +436,15 @@ hipError_t forward_step_keep(Env& e, Tape& tp, const void* start)
     const bool diff = tp.flags & BFVM_DIFFUSION;
     void* bufs[3] = {tp.s4, tp.s6, tp.s8};
     for (int axis = 0; axis < 3; ++axis) {
+        const bool sweep = diff && axis < e.lod_dims;
+        if (axis > 0 && sweep && e.n_dir == 0) {
+            // the y and z sweeps read the kept state and write the next one
+            if ((st = bfvm::launch_sweep_to(e, axis, bufs[axis - 1], bufs[axis]))) return st;
+            continue;
+        }
+        // the Dirichlet overwrite must not touch the kept state: copy first
         if (axis > 0 && (st = copy_store(e, bufs[axis], bufs[axis - 1]))) return st;
-        if (!diff || axis >= e.lod_dims) continue;
+        if (!sweep) continue;
         st = on_buffer(e, bufs[axis], [&]() -> hipError_t {
             hipError_t q;
             if (e.n_dir > 0 && (q = bfvm::launch_dirichlet(e))) return q;

@@ -41,6 +41,7 @@ struct AxisCoef {
 // lod.hip); also the per-member table of the ensemble launch.
 struct SweepArgs {
     void* rho;               // density store (float or double)
+    void* out;               // where the swept values go: rho (in place) or a store of the same shape
     size_t nvs;              // values in the store
     size_t plane_stride;     // values between planes
     const double* wb;        // per substrate: 1/d, c1/d, K00, K01, K10, K11, 0, 0
@@ -93,6 +94,7 @@ struct Env {
     int cus = 0;  // compute units of the device
 
     void* rho = nullptr;  // density store, nvs values
+    void* out = nullptr;  // set only around an out-of-place sweep (launch_sweep_to)
 
     // Coefficients: D, lambda as set by the caller; the factorisation is
     // rebuilt when they or dt change.
@@ -192,6 +194,9 @@ hipError_t launch_agents(Env& e, bool with_bulk = false);
 // Gradient vectors of the store into out (nvs*3 values, store precision).
 hipError_t launch_gradient(Env& e, void* out);
 hipError_t launch_sweep(Env& e, int axis);
+// The sweep of `axis` reading store `src` and writing store `dst` (same shape
+// and precision as the handle's; src is left unchanged).
+hipError_t launch_sweep_to(Env& e, int axis, const void* src, void* dst);
 // Whether this grid's x sweep can run with sources fused in (a segmented
 // one-block-per-tile shape: the autotuner's fastest, or the default); if so,
 // launch_sweep_x_fused runs it with, in its load prologue, the bulk update

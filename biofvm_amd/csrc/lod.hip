@@ -214,6 +214,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const SweepArgs& A,
 {
     return make_rsrc((T*)A.rho + P.plane0, (A.nvs - P.plane0) * sizeof(T));
 }
+// The same range of the output store (rho itself for an in-place sweep).
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc_out(const SweepArgs& A, const SegPos& P)
+{
+    return make_rsrc((T*)A.out + P.plane0, (A.nvs - P.plane0) * sizeof(T));
+}
 
 // STREAM: one load per node for every lane with a per-lane offset and no
 // branch, so a wait placed before the arithmetic can count every load issued
@@ -339,7 +345,7 @@ __device__ __forceinline__ void seg_solve(const SweepArgs& A, int C, const SegPo
     const double* G1 = gtab + (kg0 + kPad) * S;
     const double* g0r = G0 + (head ? p0 : 0) * S + s;
     const double* g1r = G1 + (tail ? kPad + n - 1 - p0 : 0) * S + s;
-    const __amdgpu_buffer_rsrc_t rs = plane_rsrc<T>(A, P);
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc_out<T>(A, P);
     double QZ = QZ0;
 #pragma unroll
     for (int t = L - 1; t >= 0; --t) {
@@ -640,7 +646,7 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 8 ? 8 : 4) k_sweep_x_src(Swe
         return (double)dval[(size_t)(dbase + __builtin_popcountll(dwin & ((1ull << t) - 1ull))) * S + s];
     };
     auto src = [&](int t, double x) { return dir_on && ((dwin >> t) & 1ull) ? dir_value(t) : x; };
-    const __amdgpu_buffer_rsrc_t rs = plane_rsrc<T>(A, P);
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc_out<T>(A, P);
     const unsigned step = A.line_stride * (unsigned)sizeof(T);
     const bool dir_after = dir_on && F.dir_after;
     seg_core<T, L>(A, COLS, (int)(threadIdx.x & (COLS - 1)), s, p0, P.slot, P.last_writer, raw, r, a, xa, xb,
@@ -711,6 +717,7 @@ __device__ __forceinline__ void rows_tile(const RowTile& X, unsigned tile_index)
     const size_t first = (size_t)tile_index * X.tile_vals;
     const unsigned range = (unsigned)min((size_t)X.tile_vals, X.vals - first) * (unsigned)sizeof(T);
     const __amdgpu_buffer_rsrc_t rs = make_rsrc((T*)A.rho + first, range);
+    const __amdgpu_buffer_rsrc_t rs_out = make_rsrc((T*)A.out + first, range);
     {
         T v[L];
 #pragma unroll
@@ -750,7 +757,7 @@ __device__ __forceinline__ void rows_tile(const RowTile& X, unsigned tile_index)
     for (int k = 0; k < L; ++k) {
         const unsigned e = (unsigned)(tid + k * nthr);
         if (e < (unsigned)X.tile_vals)
-            val_store((T*)nullptr, rs, tile[row_tile_slot<L>(X, e)], e * (unsigned)sizeof(T));
+            val_store((T*)nullptr, rs_out, tile[row_tile_slot<L>(X, e)], e * (unsigned)sizeof(T));
     }
 }
 
@@ -919,6 +926,7 @@ SweepArgs sweep_args(const Env& e, int axis, int cols, int nseg, int k0 = 0, int
     }
     SweepArgs A;
     A.rho = (char*)e.rho + first * e.elem;
+    A.out = e.out ? (char*)e.out + first * e.elem : A.rho;
     A.nvs = e.nvs - first;
     A.plane_stride = gm.plane_stride;
     A.wb = (const double*)a.wb;
@@ -1415,6 +1423,28 @@ hipError_t launch_rows_batch(int precision, int L, int spread, const RowTile* d_
 hipError_t launch_sweep(Env& e, int axis)
 {
     return e.precision == BFVM_FP64 ? sweep<double>(e, axis) : sweep<float>(e, axis);
+}
+
+hipError_t launch_sweep_to(Env& e, int axis, const void* src, void* dst)
+{
+    const SweepPlan p = plan_for(e, axis);
+    const bool seg = e.precision == BFVM_FP64 ? segmented<double>(e, axis, p) : segmented<float>(e, axis, p);
+    void* keep = e.rho;
+    hipError_t st;
+    if (seg) {
+        // the segmented kernels load a line before storing any of it
+        e.rho = const_cast<void*>(src);
+        e.out = dst;
+        st = launch_sweep(e, axis);
+    } else {
+        // the one-thread-per-line fallback sweeps in place
+        st = hipMemcpyAsync(dst, src, e.nvs * e.elem, hipMemcpyDeviceToDevice, e.stream);
+        e.rho = dst;
+        if (st == hipSuccess) st = launch_sweep(e, axis);
+    }
+    e.rho = keep;
+    e.out = nullptr;
+    return st;
 }
 
 bool sweep_chunkable(const Env& e, int axis)
