@@ -13,8 +13,10 @@ inputs; the FP32 result is compared with the FP64 one after the same steps.
 Prints ONE JSON line.  ``value`` is voxel-substrate updates/s of the FP32
 full step (nx*ny*nz*S*K / wall seconds of K steps, inputs resident in HBM).
 ``roofline`` prices the dominant kernel's algorithmic HBM bytes per launch
-against the 8 TB/s HBM3E peak, using its average launch time from HIP events
-on the library's stream.  ``cpu_baseline`` times BioFVM itself (compiled from
+against the 8 TB/s HBM3E peak, using its average launch time from HIP event
+pairs recorded around every launch, on the library's stream, in up to 10
+more steps run right after the timed ones (pairs inside the timed steps
+would slow them: 1.5 % at 400^3 x 8).  ``cpu_baseline`` times BioFVM itself (compiled from
 source, oracle/_ref) on a slab of the same workload on this host's cores.
 
 ``--workload ensemble_64`` times the batching form instead: 64 independent
@@ -117,7 +119,9 @@ def time_precision(case, precision, steps, warmup, fused=True, plan_x=""):
     ev_ms, _ = m.timed_run(dt, steps, ph)
     m.sync()
     wall = time.perf_counter() - t0
-    # a short second pass with an event pair around every launch
+    # right after, the same step again with an event pair around every launch
+    # (on the library's stream): recorded inside the timed steps, the pairs
+    # cost 1.5 % of a 400^3 x 8 step and 9 % of a 200^3 x 4 one
     prof_steps = max(1, min(steps, 10))
     _, per = m.timed_run(dt, prof_steps, ph, per_kernel=True)
     # per kernel: (ms per step, launches per step)

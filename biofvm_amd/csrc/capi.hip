@@ -736,6 +736,16 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
     e.kernel_timing = per_kernel_ms != nullptr;
     e.ev_next = 0;
     e.ev_marks.clear();
+    if (e.kernel_timing) {
+        // the events of a step's launches (up to 8, more with plane chunks),
+        // created before the timed region
+        const size_t want = (size_t)steps * 2 * 16;
+        while (e.ev_pool.size() < want) {
+            hipEvent_t ev;
+            HIP_OR_FAIL(&e, hipEventCreate(&ev));
+            e.ev_pool.push_back(ev);
+        }
+    }
     HIP_OR_FAIL(&e, quiesce(e));
     HIP_OR_FAIL(&e, hipEventRecord(t0, e.stream));
     rc = BFVM_OK;
