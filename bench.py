@@ -466,8 +466,14 @@ def main():
         kname = ("k_sweep_x_src" if dom == "sweep_x" and fused else
                  "k_sweep_rows" if streaming == 2 else "k_sweep_stream" if streaming else "k_sweep")
         tname = "float" if main_prec == FP32 else "double"
-        # the row-tile kernel's template has no lines-per-block parameter
-        targs = f"{tname}, {L}" if kname == "k_sweep_rows" else f"{tname}, {L}, {cols}"
+        # the row-tile kernel's template has no lines-per-block parameter; the
+        # column kernels carry a cache policy (non-temporal, 2, for y and z)
+        if kname == "k_sweep_rows":
+            targs = f"{tname}, {L}"
+        elif kname == "k_sweep_x_src":
+            targs = f"{tname}, {L}, {cols}"
+        else:
+            targs = f"{tname}, {L}, {cols}, {0 if dom == 'sweep_x' else 2}"
         traffic = measured_traffic(args.workload, f"{kname}<{targs}>")
     step_bytes = algorithmic_bytes(case, w_main)
 

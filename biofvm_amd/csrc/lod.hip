@@ -50,9 +50,15 @@ constexpr int kPad = 32;  // zero rows around the correction tables (>= the larg
 // 32-bit-element format used for gfx9-class buffers.
 constexpr int kRsrcWord3 = 0x00020000;
 // Cache policy bits of the loads and stores of the store (gfx950: bit 1 = NT,
-// streaming: each value is read and written once per sweep).
+// streaming).  The y and z sweeps read and write each value once, in whole
+// cache lines, and run faster with non-temporal loads and stores (at 400^3 x
+// 8: FP32 y 4.5 %, z 3.7 %, and the agent kernel after them 15-25 %, the
+// sweeps no longer sweeping the caches clean of its data; the FP64 step 7 %);
+// the x sweep, whose lanes read 4-64-byte pieces of lines that later loads of
+// the same lines reuse, runs 8-65 % slower with them and keeps the default.
 constexpr int kLoadPolicy = 0;
 constexpr int kStorePolicy = 0;
+constexpr int kStreamingPolicy = 2;
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, size_t bytes)
 {
@@ -60,24 +66,28 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, si
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nrec, kRsrcWord3);
 }
 
+template <int POL = kLoadPolicy>
 __device__ __forceinline__ float buf_load(float*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
 {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, kLoadPolicy));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, POL));
 }
+template <int POL = kLoadPolicy>
 __device__ __forceinline__ double buf_load(double*, __amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, kLoadPolicy));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, POL));
 }
+template <int POL = kStorePolicy>
 __device__ __forceinline__ void buf_store(float*, __amdgpu_buffer_rsrc_t r, double x, unsigned vo,
                                           unsigned so)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)x), r, vo, so, kStorePolicy);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, (float)x), r, vo, so, POL);
 }
+template <int POL = kStorePolicy>
 __device__ __forceinline__ void buf_store(double*, __amdgpu_buffer_rsrc_t r, double x, unsigned vo,
                                           unsigned so)
 {
     using u64 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, x), r, vo, so, kStorePolicy);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u64, x), r, vo, so, POL);
 }
 
 // Block barrier that orders LDS only: waits for this wave's LDS accesses,
@@ -226,7 +236,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc_out(const SweepArgs
 // after these (the prefetch of the streaming kernel).  Otherwise the
 // segments below the last one use a scalar offset per node (fewer VGPRs) and
 // only the last segment computes clamped offsets.
-template <typename T, int L, bool STREAM>
+template <typename T, int L, bool STREAM, int POL = kLoadPolicy>
 __device__ __forceinline__ void seg_load(const SweepArgs& A, const SegPos& P, T (&raw)[L])
 {
     const __amdgpu_buffer_rsrc_t rs = plane_rsrc<T>(A, P);
@@ -235,10 +245,10 @@ __device__ __forceinline__ void seg_load(const SweepArgs& A, const SegPos& P, T 
     if (STREAM || P.p0 + L >= A.n) {
 #pragma unroll
         for (int t = 0; t < L; ++t)
-            raw[t] = buf_load((T*)nullptr, rs, P.vo + (unsigned)min(t, last) * step, 0);
+            raw[t] = buf_load<POL>((T*)nullptr, rs, P.vo + (unsigned)min(t, last) * step, 0);
     } else {
 #pragma unroll
-        for (int t = 0; t < L; ++t) raw[t] = buf_load((T*)nullptr, rs, P.vo, t * step);
+        for (int t = 0; t < L; ++t) raw[t] = buf_load<POL>((T*)nullptr, rs, P.vo, t * step);
     }
 }
 
@@ -272,7 +282,7 @@ __device__ __forceinline__ void stage_tables(const SweepArgs& A, double* gtab)
 // (copies store the owner's values; the true value of node n-1 is stored
 // after the others by the same lane), so no store sits behind a lane mask;
 // otherwise only owners store, and only their real nodes.
-template <typename T, int L, bool STREAM>
+template <typename T, int L, bool STREAM, int POL = kStorePolicy>
 __device__ __forceinline__ void seg_solve(const SweepArgs& A, int C, const SegPos& P, const T (&raw)[L],
                                           double r, double a, double* xa, double* xb, double* wfirst,
                                           double* wlast, const double* gtab)
@@ -353,8 +363,8 @@ __device__ __forceinline__ void seg_solve(const SweepArgs& A, int C, const SegPo
         double zf = v[t] + QZ;
         if (head) zf = fma(mu0, g0r[t * S], zf);
         if (tail) zf = fma(mu1, g1r[-t * S], zf);
-        if (STREAM) buf_store((T*)nullptr, rs, zf, P.vo + (unsigned)min(t, cnt - 1) * step, 0);
-        else if (P.act && t < cnt) buf_store((T*)nullptr, rs, zf, P.vo, t * step);
+        if (STREAM) buf_store<POL>((T*)nullptr, rs, zf, P.vo + (unsigned)min(t, cnt - 1) * step, 0);
+        else if (P.act && t < cnt) buf_store<POL>((T*)nullptr, rs, zf, P.vo, t * step);
     }
 }
 
@@ -452,7 +462,7 @@ __device__ __forceinline__ void seg_core(const SweepArgs& A, int C, int lane, in
 }
 
 // Block-level body of the segmented sweep for one plane's column tile.
-template <typename T, int L>
+template <typename T, int L, int POL = 0>
 __device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, int plane,
                                            double* xa, double* xb, double* wfirst, double* wlast,
                                            double* gtab)
@@ -460,13 +470,14 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, 
     const SegPos P = seg_pos<T, L>(A, C, tile, plane);
     const double r = A.wb[P.s * 8 + 0], a = A.wb[P.s * 8 + 1];
     T raw[L];
-    seg_load<T, L, false>(A, P, raw);
+    seg_load<T, L, false, POL>(A, P, raw);
     stage_tables(A, gtab);  // while the loads are in flight
-    seg_solve<T, L, false>(A, C, P, raw, r, a, xa, xb, wfirst, wlast, gtab);
+    seg_solve<T, L, false, POL>(A, C, P, raw, r, a, xa, xb, wfirst, wlast, gtab);
 }
 
-// One grid: block (x, y) = (column tile, plane), COLS lines per block.
-template <typename T, int L, int COLS>
+// One grid: block (x, y) = (column tile, plane), COLS lines per block; POL =
+// the cache policy of the store's loads and stores.
+template <typename T, int L, int COLS, int POL>
 __global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4) k_sweep(SweepArgs A)
 {
     __shared__ double xa[kMaxThreads];
@@ -477,7 +488,8 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4) k_sweep(SweepArg
     const unsigned w =
         xcd_tile(blockIdx.x + blockIdx.y * (unsigned)A.tiles, (unsigned)(A.tiles * A.planes), A.spread);
     const unsigned plane = w / (unsigned)A.tiles;
-    sweep_tile<T, L>(A, COLS, (int)(w - plane * (unsigned)A.tiles), (int)plane, xa, xb, wfirst, wlast, gtab);
+    sweep_tile<T, L, POL>(A, COLS, (int)(w - plane * (unsigned)A.tiles), (int)plane, xa, xb, wfirst, wlast,
+                          gtab);
 }
 
 // One grid, resident blocks: block b solves tiles b, b + gridDim.x, ... (tile
@@ -488,7 +500,7 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4) k_sweep(SweepArg
 // vector memory counter is in order: waiting for a later load would wait for
 // the prefetch too).  nblocks = gridDim.x <= tiles * planes, passed as an
 // argument so it sits in a scalar register.
-template <typename T, int L, int COLS>
+template <typename T, int L, int COLS, int POL>
 __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_stream(SweepArgs A, int nblocks)
 {
     __shared__ double xa[kMaxThreads];
@@ -500,7 +512,7 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_stream(SweepArgs A, in
     int w = blockIdx.x;  // the same in every lane, as is everything derived from it
     T raw[L];
     int wr = (int)xcd_tile((unsigned)w, (unsigned)total, A.spread);  // tile w, renumbered per XCD
-    seg_load<T, L, true>(A, seg_pos<T, L>(A, COLS, wr % A.tiles, wr / A.tiles), raw);
+    seg_load<T, L, true, POL>(A, seg_pos<T, L>(A, COLS, wr % A.tiles, wr / A.tiles), raw);
     stage_tables(A, gtab);
     lds_barrier();
     const double* K = gtab + (A.kg0 + A.kg1 + 3 * kPad) * A.S;
@@ -512,9 +524,10 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_stream(SweepArgs A, in
         // arithmetic below count the prefetch as in flight
         const int wl = (int)xcd_tile((unsigned)(more ? wn : w), (unsigned)total, A.spread);
         T nxt[L];
-        seg_load<T, L, true>(A, seg_pos<T, L>(A, COLS, wl % A.tiles, wl / A.tiles), nxt);
+        seg_load<T, L, true, POL>(A, seg_pos<T, L>(A, COLS, wl % A.tiles, wl / A.tiles), nxt);
         const SegPos P = seg_pos<T, L>(A, COLS, wr % A.tiles, wr / A.tiles);
-        seg_solve<T, L, true>(A, COLS, P, raw, K[P.s * 8 + 0], K[P.s * 8 + 1], xa, xb, wfirst, wlast, gtab);
+        seg_solve<T, L, true, POL>(A, COLS, P, raw, K[P.s * 8 + 0], K[P.s * 8 + 1], xa, xb, wfirst, wlast,
+                                   gtab);
         if (!more) break;
 #pragma unroll
         for (int t = 0; t < L; ++t) raw[t] = nxt[t];
@@ -967,16 +980,34 @@ hipError_t allow_table_lds(const void* kernel, size_t lds, size_t cap = kMaxCorr
     return st;
 }
 
-template <typename T, int L, int COLS>
-hipError_t launch_seg(Env& e, int axis, int nseg, int k0, int k1)
+template <typename T, int L, int COLS, int POL>
+hipError_t launch_seg_pol(Env& e, int axis, int nseg, int k0, int k1)
 {
     const SweepArgs A = sweep_args(e, axis, COLS, nseg, k0, k1);
     dim3 grid(A.tiles, A.planes);
     int threads = (COLS * nseg + 63) / 64 * 64;
-    const hipError_t st = allow_table_lds((const void*)k_sweep<T, L, COLS>, sweep_lds(A));
+    const hipError_t st = allow_table_lds((const void*)k_sweep<T, L, COLS, POL>, sweep_lds(A));
     if (st != hipSuccess) return st;
-    k_sweep<T, L, COLS><<<grid, threads, sweep_lds(A), e.stream>>>(A);
+    k_sweep<T, L, COLS, POL><<<grid, threads, sweep_lds(A), e.stream>>>(A);
     return hipGetLastError();
+}
+
+// The y and z sweeps with non-temporal loads and stores where a row piece of
+// a tile is whole 128-byte lines (with 64-byte pieces they moved 20-25 % more
+// bytes than the sweep's own), x with the default policy (see
+// kStreamingPolicy).
+template <typename T, int COLS>
+constexpr bool streams_whole_lines()
+{
+    return COLS * sizeof(T) >= 128;
+}
+
+template <typename T, int L, int COLS>
+hipError_t launch_seg(Env& e, int axis, int nseg, int k0, int k1)
+{
+    if constexpr (streams_whole_lines<T, COLS>())
+        if (axis != 0) return launch_seg_pol<T, L, COLS, kStreamingPolicy>(e, axis, nseg, k0, k1);
+    return launch_seg_pol<T, L, COLS, kLoadPolicy>(e, axis, nseg, k0, k1);
 }
 
 template <typename T, int COLS>
@@ -1020,19 +1051,27 @@ int compute_units(Env& e)
     return e.cus;
 }
 
-template <typename T, int L, int COLS>
-hipError_t launch_stream(Env& e, int axis, int nseg, int k0, int k1)
+template <typename T, int L, int COLS, int POL>
+hipError_t launch_stream_pol(Env& e, int axis, int nseg, int k0, int k1)
 {
     const SweepArgs A = sweep_args(e, axis, COLS, nseg, k0, k1);
     const int threads = (COLS * nseg + 63) / 64 * 64;
     const size_t lds = sweep_lds(A);
     const long long total = (long long)A.tiles * A.planes;
-    const hipError_t st = allow_table_lds((const void*)k_sweep_stream<T, L, COLS>, lds);
+    const hipError_t st = allow_table_lds((const void*)k_sweep_stream<T, L, COLS, POL>, lds);
     if (st != hipSuccess) return st;
-    const int nb = resident_blocks((const void*)k_sweep_stream<T, L, COLS>, threads, lds);
+    const int nb = resident_blocks((const void*)k_sweep_stream<T, L, COLS, POL>, threads, lds);
     const int grid = (int)std::min<long long>(total, (long long)nb * compute_units(e));
-    k_sweep_stream<T, L, COLS><<<grid, threads, lds, e.stream>>>(A, grid);
+    k_sweep_stream<T, L, COLS, POL><<<grid, threads, lds, e.stream>>>(A, grid);
     return hipGetLastError();
+}
+
+template <typename T, int L, int COLS>
+hipError_t launch_stream(Env& e, int axis, int nseg, int k0, int k1)
+{
+    if constexpr (streams_whole_lines<T, COLS>())
+        if (axis != 0) return launch_stream_pol<T, L, COLS, kStreamingPolicy>(e, axis, nseg, k0, k1);
+    return launch_stream_pol<T, L, COLS, kLoadPolicy>(e, axis, nseg, k0, k1);
 }
 
 template <typename T, int COLS>
