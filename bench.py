@@ -446,6 +446,7 @@ def main():
     m32, wall, ev_ms, per, n_done = time_precision(case, main_prec, steps, warmup, plan_x=args.plan_x)
     m32_chunk = m32.sweep_chunk
     plans = m32.sweep_plans()
+    policies = m32.sweep_cache_policies()
     value = nvs * steps / wall
     step_ms = 1e3 * wall / steps
     ph = case["phases"]
@@ -473,7 +474,7 @@ def main():
         elif kname == "k_sweep_x_src":
             targs = f"{tname}, {L}, {cols}"
         else:
-            targs = f"{tname}, {L}, {cols}, {0 if dom == 'sweep_x' else 2}"
+            targs = f"{tname}, {L}, {cols}, {2 if policies['xyz'.index(dom[-1])] else 0}"
         traffic = measured_traffic(args.workload, f"{kname}<{targs}>")
     step_bytes = algorithmic_bytes(case, w_main)
 
@@ -539,6 +540,7 @@ def main():
         "sweep_xy_chunk_planes": m32_chunk,
         "lod_only": lod_only(per, nvs, w_main),
         "sweep_plans_lines_nodes_streaming": {"x": plans[0], "y": plans[1], "z": plans[2]},
+        "sweep_non_temporal": {"x": policies[0], "y": policies[1], "z": policies[2]},
     }
     line.update(out_extra)
     if not args.no_cpu_baseline:

@@ -44,6 +44,8 @@ SIGNATURES = {
     "bfvm_get_sweep_plan": (_i, [_h, _i, _ip, _ip, _ip]),
     "bfvm_set_sweep_plan": (_i, [_h, _i, _i, _i, _i]),
     "bfvm_get_sweep_spread": (_i, [_h, _i, _ip]),
+    "bfvm_get_sweep_cache_policy": (_i, [_h, _i, _ip]),
+    "bfvm_set_sweep_cache_policy": (_i, [_h, _i, _i]),
     "bfvm_set_fused_sources": (_i, [_h, _i]),
     "bfvm_get_fused_sources": (_i, [_h, _ip]),
     "bfvm_set_sweep_spread": (_i, [_h, _i, _i]),

@@ -77,6 +77,8 @@ struct SweepPlan {
     int L = 0;       // nodes per segment
     int stream = 0;  // kernel: 0 one block per tile, 1 resident blocks that load the
                      // next tile while solving one, 2 x sweep on LDS-staged row tiles
+    int nt = 0;      // 1: non-temporal loads and stores (y and z sweeps whose tiles'
+                     // row pieces are whole 128-byte cache lines)
 };
 
 constexpr int kDefaultSpread = -1;
@@ -266,6 +268,8 @@ void free_writer(Env& e);
 std::vector<SweepPlan> sweep_candidates(const Env& e, int axis);
 // The launch shape a sweep of `axis` uses now (autotuned or default).
 SweepPlan effective_plan(const Env& e, int axis);
+// Whether plan p of `axis` may use non-temporal loads and stores (SweepPlan::nt).
+bool sweep_nt_allowed(const Env& e, int axis, const SweepPlan& p);
 
 }  // namespace bfvm
 

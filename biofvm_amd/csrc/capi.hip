@@ -931,6 +931,30 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
     return fail(h, BFVM_E_ARG, "launch shape not valid for this axis length");
 }
 
+int bfvm_get_sweep_cache_policy(bfvm_handle h, int axis, int* non_temporal)
+{
+    if (!h || !non_temporal) return fail(h, BFVM_E_ARG, "null argument");
+    if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
+    const bfvm::SweepPlan p = bfvm::effective_plan(*h, axis);
+    *non_temporal = p.nt != 0 && bfvm::sweep_nt_allowed(*h, axis, p) ? 1 : 0;
+    return BFVM_OK;
+}
+
+int bfvm_set_sweep_cache_policy(bfvm_handle h, int axis, int non_temporal)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
+    bfvm::SweepPlan p = bfvm::effective_plan(*h, axis);
+    if (non_temporal && (p.cols == 0 || !bfvm::sweep_nt_allowed(*h, axis, p)))
+        return fail(h, BFVM_E_ARG,
+                    "non-temporal loads and stores are for y and z sweeps whose tiles' row pieces are "
+                    "whole 128-byte lines");
+    if (p.cols == 0) return BFVM_OK;  // the one-thread-per-line fallback has no policy
+    p.nt = non_temporal ? 1 : 0;
+    h->plan[axis] = p;
+    return BFVM_OK;
+}
+
 int bfvm_set_fused_sources(bfvm_handle h, int on)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");

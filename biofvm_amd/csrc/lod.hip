@@ -992,34 +992,38 @@ hipError_t launch_seg_pol(Env& e, int axis, int nseg, int k0, int k1)
     return hipGetLastError();
 }
 
-// The y and z sweeps with non-temporal loads and stores where a row piece of
-// a tile is whole 128-byte lines (with 64-byte pieces they moved 20-25 % more
-// bytes than the sweep's own), x with the default policy (see
-// kStreamingPolicy).
+// Non-temporal loads and stores (plan.nt, see kStreamingPolicy) are offered
+// for y and z sweeps whose tiles' row pieces are whole 128-byte lines (with
+// 64-byte pieces they moved 20-25 % more bytes than the sweep's own).
 template <typename T, int COLS>
 constexpr bool streams_whole_lines()
 {
     return COLS * sizeof(T) >= 128;
 }
 
+bool nt_shape(const Env& e, int axis, int cols, int stream)
+{
+    return axis != 0 && stream != 2 && (size_t)cols * e.elem >= 128;
+}
+
 template <typename T, int L, int COLS>
-hipError_t launch_seg(Env& e, int axis, int nseg, int k0, int k1)
+hipError_t launch_seg(Env& e, int axis, int nseg, int k0, int k1, bool nt)
 {
     if constexpr (streams_whole_lines<T, COLS>())
-        if (axis != 0) return launch_seg_pol<T, L, COLS, kStreamingPolicy>(e, axis, nseg, k0, k1);
+        if (nt) return launch_seg_pol<T, L, COLS, kStreamingPolicy>(e, axis, nseg, k0, k1);
     return launch_seg_pol<T, L, COLS, kLoadPolicy>(e, axis, nseg, k0, k1);
 }
 
 template <typename T, int COLS>
-hipError_t launch_cols(Env& e, int axis, int L, int nseg, int k0, int k1)
+hipError_t launch_cols(Env& e, int axis, int L, int nseg, int k0, int k1, bool nt)
 {
     switch (L) {
-    case 4: return launch_seg<T, 4, COLS>(e, axis, nseg, k0, k1);
-    case 8: return launch_seg<T, 8, COLS>(e, axis, nseg, k0, k1);
-    case 12: return launch_seg<T, 12, COLS>(e, axis, nseg, k0, k1);
-    case 16: return launch_seg<T, 16, COLS>(e, axis, nseg, k0, k1);
-    case 24: return launch_seg<T, 24, COLS>(e, axis, nseg, k0, k1);
-    default: return launch_seg<T, 32, COLS>(e, axis, nseg, k0, k1);
+    case 4: return launch_seg<T, 4, COLS>(e, axis, nseg, k0, k1, nt);
+    case 8: return launch_seg<T, 8, COLS>(e, axis, nseg, k0, k1, nt);
+    case 12: return launch_seg<T, 12, COLS>(e, axis, nseg, k0, k1, nt);
+    case 16: return launch_seg<T, 16, COLS>(e, axis, nseg, k0, k1, nt);
+    case 24: return launch_seg<T, 24, COLS>(e, axis, nseg, k0, k1, nt);
+    default: return launch_seg<T, 32, COLS>(e, axis, nseg, k0, k1, nt);
     }
 }
 
@@ -1067,22 +1071,22 @@ hipError_t launch_stream_pol(Env& e, int axis, int nseg, int k0, int k1)
 }
 
 template <typename T, int L, int COLS>
-hipError_t launch_stream(Env& e, int axis, int nseg, int k0, int k1)
+hipError_t launch_stream(Env& e, int axis, int nseg, int k0, int k1, bool nt)
 {
     if constexpr (streams_whole_lines<T, COLS>())
-        if (axis != 0) return launch_stream_pol<T, L, COLS, kStreamingPolicy>(e, axis, nseg, k0, k1);
+        if (nt) return launch_stream_pol<T, L, COLS, kStreamingPolicy>(e, axis, nseg, k0, k1);
     return launch_stream_pol<T, L, COLS, kLoadPolicy>(e, axis, nseg, k0, k1);
 }
 
 template <typename T, int COLS>
-hipError_t launch_cols_stream(Env& e, int axis, int L, int nseg, int k0, int k1)
+hipError_t launch_cols_stream(Env& e, int axis, int L, int nseg, int k0, int k1, bool nt)
 {
     switch (L) {
-    case 8: return launch_stream<T, 8, COLS>(e, axis, nseg, k0, k1);
-    case 12: return launch_stream<T, 12, COLS>(e, axis, nseg, k0, k1);
-    case 16: return launch_stream<T, 16, COLS>(e, axis, nseg, k0, k1);
+    case 8: return launch_stream<T, 8, COLS>(e, axis, nseg, k0, k1, nt);
+    case 12: return launch_stream<T, 12, COLS>(e, axis, nseg, k0, k1, nt);
+    case 16: return launch_stream<T, 16, COLS>(e, axis, nseg, k0, k1, nt);
     default:
-        if constexpr (sizeof(T) == 4) return launch_stream<T, 24, COLS>(e, axis, nseg, k0, k1);
+        if constexpr (sizeof(T) == 4) return launch_stream<T, 24, COLS>(e, axis, nseg, k0, k1, nt);
         return hipErrorInvalidValue;
     }
 }
@@ -1216,17 +1220,18 @@ hipError_t sweep(Env& e, int axis, int k0 = 0, int k1 = -1)
     }
     int nseg = (gm.n + p.L - 1) / p.L;
     const long long tiles = ((long long)gm.ncols + p.cols - 1) / p.cols;
+    const bool nt = p.nt != 0 && nt_shape(e, axis, p.cols, p.stream);
     if (p.stream && stream_nodes_ok(p.L, e.precision) && tiles * planes < (1ll << 31)) {
         switch (p.cols) {
-        case 16: return launch_cols_stream<T, 16>(e, axis, p.L, nseg, k0, k1);
-        case 32: return launch_cols_stream<T, 32>(e, axis, p.L, nseg, k0, k1);
-        default: return launch_cols_stream<T, 64>(e, axis, p.L, nseg, k0, k1);
+        case 16: return launch_cols_stream<T, 16>(e, axis, p.L, nseg, k0, k1, nt);
+        case 32: return launch_cols_stream<T, 32>(e, axis, p.L, nseg, k0, k1, nt);
+        default: return launch_cols_stream<T, 64>(e, axis, p.L, nseg, k0, k1, nt);
         }
     }
     switch (p.cols) {
-    case 16: return launch_cols<T, 16>(e, axis, p.L, nseg, k0, k1);
-    case 32: return launch_cols<T, 32>(e, axis, p.L, nseg, k0, k1);
-    default: return launch_cols<T, 64>(e, axis, p.L, nseg, k0, k1);
+    case 16: return launch_cols<T, 16>(e, axis, p.L, nseg, k0, k1, nt);
+    case 32: return launch_cols<T, 32>(e, axis, p.L, nseg, k0, k1, nt);
+    default: return launch_cols<T, 64>(e, axis, p.L, nseg, k0, k1, nt);
     }
 }
 
@@ -1311,6 +1316,11 @@ hipError_t launch_sweep_x_fused(Env& e, bool bulk, bool skip_agent_voxels, bool 
     return e.precision == BFVM_FP64 ? sweep_x_fused<double>(e, p, F) : sweep_x_fused<float>(e, p, F);
 }
 
+bool sweep_nt_allowed(const Env& e, int axis, const SweepPlan& p)
+{
+    return nt_shape(e, axis, p.cols, p.stream);
+}
+
 // The shape that runs the axis; lines = 0 when the one-thread-per-line
 // fallback runs it.
 SweepPlan effective_plan(const Env& e, int axis)
@@ -1329,6 +1339,10 @@ std::vector<SweepPlan> sweep_candidates(const Env& e, int axis)
             if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n)) {
                 out.push_back(SweepPlan{cols, L, 0});
                 if (stream_nodes_ok(L, e.precision)) out.push_back(SweepPlan{cols, L, 1});
+                if (nt_shape(e, axis, cols, 0)) {
+                    out.push_back(SweepPlan{cols, L, 0, 1});
+                    if (stream_nodes_ok(L, e.precision)) out.push_back(SweepPlan{cols, L, 1, 1});
+                }
             }
     if (axis == 0)
         for (int R = 1; R <= 64; R *= 2)

@@ -198,6 +198,18 @@ class DeviceMicroenvironment:
             out.append((a.value, b.value, c.value))
         return out
 
+    def sweep_cache_policies(self):
+        """[non_temporal] for the x, y and z sweeps (1: streaming loads and stores)."""
+        out = []
+        for axis in range(3):
+            v = ctypes.c_int()
+            self._check(self.lib.bfvm_get_sweep_cache_policy(self.h, axis, ctypes.byref(v)))
+            out.append(v.value)
+        return out
+
+    def set_sweep_cache_policy(self, axis, non_temporal):
+        self._check(self.lib.bfvm_set_sweep_cache_policy(self.h, int(axis), int(bool(non_temporal))))
+
     # ---- differentiation (adjoint) -------------------------------------
     def tape_record(self, dt, steps, flags=ALL, checkpoint_every=50):
         """Runs `steps` steps from the current store, recording checkpoints."""

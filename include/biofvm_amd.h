@@ -178,7 +178,8 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
  * Autotuner: times every valid launch shape of the three segmented sweeps
  * (lines per block x nodes per segment x kernel) for this grid, substrate
  * count and precision on a scratch copy of the store, keeps the fastest per
- * axis, then its XCD spread (bfvm_set_sweep_spread), then (with bulk fields
+ * axis (y and z: with either cache policy, bfvm_set_sweep_cache_policy),
+ * then its XCD spread (bfvm_set_sweep_spread), then (with bulk fields
  * or Dirichlet nodes set and fused sources on) the one-block-per-tile shapes
  * of the x sweep with the sources fused in, then times the x/y plane chunk
  * sizes (bfvm_set_sweep_chunk).
@@ -202,6 +203,18 @@ int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* node
  * (for kernel 2, the shapes bfvm_autotune would try). */
 int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment,
                         int streaming);
+
+/* Cache policy of one axis's sweep: non_temporal = 1 when its loads and
+ * stores of the store carry the non-temporal (streaming) hint, 0 for the
+ * default.  Offered for the y and z sweeps whose tiles' row pieces are whole
+ * 128-byte lines (lines per block x value size >= 128, column kernels):
+ * each value is read and written once per sweep, and the hint keeps the
+ * sweep from evicting the other kernels' data; bfvm_autotune times both.
+ * Setting 1 elsewhere returns BFVM_E_ARG; setting a launch shape by hand
+ * (bfvm_set_sweep_plan) resets the axis to the default policy.  Results do
+ * not depend on it.  No counterpart in the reference. */
+int bfvm_get_sweep_cache_policy(bfvm_handle h, int axis, int* non_temporal);
+int bfvm_set_sweep_cache_policy(bfvm_handle h, int axis, int non_temporal);
 
 /* Whether a step runs its bulk update and the Dirichlet overwrites before the
  * x and y sweeps inside the x sweep (on = 1, the default) or as kernels of

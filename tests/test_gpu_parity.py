@@ -299,6 +299,28 @@ def test_row_tile_x_sweep_equals_column_tiles(precision):
         assert tried > 0, (nx, ny, nz, S)
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_cache_policy_never_changes_results(precision):
+    rng = np.random.default_rng(41)
+    nx, ny, nz, S = 40, 36, 30, 4
+    case = W.make_case(nx, ny, nz, S, 10.0, [1e5, 2e3, 3e4, 500.0], [0.1, 0.3, 0.01, 1.0], 0.01, 3,
+                       W.DIFFUSION, rng.uniform(0, 1, nx * ny * nz * S))
+    outs = []
+    for nt in (0, 1):
+        with setup_case(case, precision) as m:
+            for axis in (1, 2):
+                m.set_sweep_plan(axis, 32, 16)  # row pieces of 128 or 256 bytes
+                m.set_sweep_cache_policy(axis, nt)
+            assert m.sweep_cache_policies() == [0, nt, nt]
+            m.run(case["dt"], case["steps"], case["phases"])
+            outs.append(m.download())
+        with pytest.raises(BfvmError):
+            with setup_case(case, precision) as m:
+                m.set_sweep_cache_policy(0, 1)  # x keeps the default policy
+    np.testing.assert_array_equal(outs[1], outs[0])
+    assert rel_err(outs[0], oracle.oracle_run(case)) <= TOL[precision]
+
+
 def test_unrunnable_launch_shapes_are_refused():
     with DeviceMicroenvironment(40, 40, 300, 2, 10.0, FP64) as m:
         with pytest.raises(BfvmError):
