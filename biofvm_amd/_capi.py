@@ -71,6 +71,8 @@ SIGNATURES = {
     "bfvm_ensemble_set_segment": (_i, [_vp, _i]),
     "bfvm_ensemble_set_row_tiles": (_i, [_vp, _i]),
     "bfvm_ensemble_get_row_tiles": (_i, [_vp, _ip]),
+    "bfvm_ensemble_set_cache_policy": (_i, [_vp, _i]),
+    "bfvm_ensemble_get_cache_policy": (_i, [_vp, _ip]),
     "bfvm_ensemble_last_error": (ctypes.c_char_p, [_vp]),
     "bfvm_tape_record": (_i, [_h, _d, _i, _i, _i]),
     "bfvm_tape_seed": (_i, [_h, _dp]),

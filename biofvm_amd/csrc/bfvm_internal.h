@@ -226,7 +226,7 @@ bool batch_rows_args(const Env& e, int L, RowTile* X, int* threads, size_t* lds,
 hipError_t launch_rows_batch(int precision, int L, int spread, const RowTile* d_tabs, const int* d_map,
                              int blocks, int threads, size_t lds, hipStream_t st);
 hipError_t launch_sweep_batch(int precision, int L, int spread, const SweepArgs* d_args, const int* d_map,
-                              int nm, int blocks, size_t lds, hipStream_t st);
+                              int nm, int blocks, size_t lds, hipStream_t st, bool nt = false);
 
 // Agents of several grids in one launch: one entry per member.
 struct AgentBatch {

@@ -25,6 +25,7 @@ struct bfvm_ensemble_t {
     hipStream_t stream = nullptr;
     int L = 8;  // nodes per segment of the batched sweep
     int spread = 0;  // XCD spread of the batched sweeps' blocks (xcd_tile in lod.hip)
+    int nt = 0;      // y and z batched sweeps with non-temporal loads and stores (autotuned)
     int rows_x = 1;  // x sweeps on LDS-staged row tiles (k_sweep_rows_batch) where a member can
 
     // tables, valid for (dt, L, member coefficient/agent epochs)
@@ -265,7 +266,8 @@ hipError_t enqueue(bfvm_ensemble_t& E, int flags)
                 return st;
             if (E.nbatch[axis] > 0 &&
                 (st = bfvm::launch_sweep_batch(E.precision, E.L, E.spread, E.d_args[axis], E.d_first[axis],
-                                               E.nbatch[axis], E.blocks[axis], E.lds[axis], E.stream)))
+                                               E.nbatch[axis], E.blocks[axis], E.lds[axis], E.stream,
+                                               axis > 0 && E.nt)))
                 return st;
             for (int i : E.solo[axis])
                 if ((st = on_stream(*E.m[i], E.stream, [&] { return bfvm::launch_sweep(*E.m[i], axis); })))
@@ -421,24 +423,29 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
         e->m[i]->rho = scratch[i];
         ++e->m[i]->coef_epoch;  // launch tables point at the scratch stores
     }
-    int best_L = e->L, best_spread = e->spread, best_rows = e->rows_x;
+    int best_L = e->L, best_spread = e->spread, best_rows = e->rows_x, best_nt = e->nt;
     double best = 1e300;
     int rc = BFVM_OK;
     for (int L : {4, 8, 16, 32}) {
         for (int spread : {0, 64, -1}) {
             for (int rows : {0, 1}) {
-                e->L = L;
-                e->spread = spread;
-                e->rows_x = rows;
-                double ms = 0;
-                if ((rc = bfvm_ensemble_timed_run(e, dt, 1, BFVM_DIFFUSION, 0, &ms))) break;
-                if ((rc = bfvm_ensemble_timed_run(e, dt, 3, BFVM_DIFFUSION, 0, &ms))) break;
-                if (ms < best) {
-                    best = ms;
-                    best_L = L;
-                    best_spread = spread;
-                    best_rows = rows;
+                for (int nt : {0, 1}) {
+                    e->L = L;
+                    e->spread = spread;
+                    e->rows_x = rows;
+                    e->nt = nt;
+                    double ms = 0;
+                    if ((rc = bfvm_ensemble_timed_run(e, dt, 1, BFVM_DIFFUSION, 0, &ms))) break;
+                    if ((rc = bfvm_ensemble_timed_run(e, dt, 3, BFVM_DIFFUSION, 0, &ms))) break;
+                    if (ms < best) {
+                        best = ms;
+                        best_L = L;
+                        best_spread = spread;
+                        best_rows = rows;
+                        best_nt = nt;
+                    }
                 }
+                if (rc) break;
             }
             if (rc) break;
         }
@@ -453,6 +460,7 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
     e->L = best_L;
     e->spread = best_spread;
     e->rows_x = best_rows;
+    e->nt = best_nt;
     e->tables_valid = false;
     drop_graph(*e);
     return rc;
@@ -490,6 +498,21 @@ int bfvm_ensemble_get_row_tiles(bfvm_ensemble e, int* on)
 {
     if (!e || !on) return BFVM_E_ARG;
     *on = e->rows_x;
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_set_cache_policy(bfvm_ensemble e, int non_temporal)
+{
+    if (!e) return BFVM_E_ARG;
+    e->nt = non_temporal ? 1 : 0;
+    drop_graph(*e);
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_get_cache_policy(bfvm_ensemble e, int* non_temporal)
+{
+    if (!e || !non_temporal) return BFVM_E_ARG;
+    *non_temporal = e->nt;
     return BFVM_OK;
 }
 

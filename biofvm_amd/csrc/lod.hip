@@ -540,7 +540,7 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_stream(SweepArgs A, in
 // out as planes x column tiles of A[m].cols lines; map[2b], map[2b+1] = the
 // member of block b and b's index within it.  Every block has kMaxThreads
 // threads (A[m].cols * A[m].nseg of them active).
-template <typename T, int L>
+template <typename T, int L, int POL>
 __global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4)
 k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ map, int nm, int spread)
 {
@@ -553,7 +553,7 @@ k_sweep_batch(const SweepArgs* __restrict__ args, const int* __restrict__ map, i
     const int member = __builtin_amdgcn_readfirstlane(map[2 * b]);
     const int local = __builtin_amdgcn_readfirstlane(map[2 * b + 1]);
     const SweepArgs A = args[member];
-    sweep_tile<T, L>(A, A.cols, local % A.tiles, local / A.tiles, xa, xb, wfirst, wlast, gtab);
+    sweep_tile<T, L, POL>(A, A.cols, local % A.tiles, local / A.tiles, xa, xb, wfirst, wlast, gtab);
 }
 
 // The x sweep with sources fused in (launch_sweep_x_fused).  The reference's
@@ -1367,14 +1367,24 @@ bool batch_member_ok_t(const Env& e, int axis, int L)
     return gm.n >= 2 && nseg <= kMaxThreads / 64 && span < 0xffffffffull && gbytes <= kMaxBatchCorrectionLds;
 }
 
+template <typename T, int L, int POL>
+hipError_t launch_batch_pol(const SweepArgs* d_args, const int* d_first, int nm, int blocks, size_t lds,
+                            hipStream_t st, int spread)
+{
+    const hipError_t a = allow_table_lds((const void*)k_sweep_batch<T, L, POL>, lds, kMaxBatchCorrectionLds);
+    if (a != hipSuccess) return a;
+    k_sweep_batch<T, L, POL><<<blocks, kMaxThreads, lds, st>>>(d_args, d_first, nm, spread);
+    return hipGetLastError();
+}
+
+// nt: non-temporal loads and stores (the batch's tiles are 64 lines or
+// more, whole 128-byte row pieces in either precision)
 template <typename T, int L>
 hipError_t launch_batch_t(const SweepArgs* d_args, const int* d_first, int nm, int blocks,
-                          size_t lds, hipStream_t st, int spread)
+                          size_t lds, hipStream_t st, int spread, bool nt)
 {
-    const hipError_t a = allow_table_lds((const void*)k_sweep_batch<T, L>, lds, kMaxBatchCorrectionLds);
-    if (a != hipSuccess) return a;
-    k_sweep_batch<T, L><<<blocks, kMaxThreads, lds, st>>>(d_args, d_first, nm, spread);
-    return hipGetLastError();
+    return nt ? launch_batch_pol<T, L, kStreamingPolicy>(d_args, d_first, nm, blocks, lds, st, spread)
+              : launch_batch_pol<T, L, kLoadPolicy>(d_args, d_first, nm, blocks, lds, st, spread);
 }
 }  // namespace
 
@@ -1396,21 +1406,21 @@ SweepArgs batch_args(const Env& e, int axis, int L)
 size_t batch_lds(const SweepArgs& A) { return sweep_lds(A); }
 
 hipError_t launch_sweep_batch(int precision, int L, int spread, const SweepArgs* d_args, const int* d_first,
-                              int nm, int blocks, size_t lds, hipStream_t st)
+                              int nm, int blocks, size_t lds, hipStream_t st, bool nt)
 {
     if (precision == BFVM_FP64) {
         switch (L) {
-        case 4: return launch_batch_t<double, 4>(d_args, d_first, nm, blocks, lds, st, spread);
-        case 8: return launch_batch_t<double, 8>(d_args, d_first, nm, blocks, lds, st, spread);
-        case 16: return launch_batch_t<double, 16>(d_args, d_first, nm, blocks, lds, st, spread);
-        default: return launch_batch_t<double, 32>(d_args, d_first, nm, blocks, lds, st, spread);
+        case 4: return launch_batch_t<double, 4>(d_args, d_first, nm, blocks, lds, st, spread, nt);
+        case 8: return launch_batch_t<double, 8>(d_args, d_first, nm, blocks, lds, st, spread, nt);
+        case 16: return launch_batch_t<double, 16>(d_args, d_first, nm, blocks, lds, st, spread, nt);
+        default: return launch_batch_t<double, 32>(d_args, d_first, nm, blocks, lds, st, spread, nt);
         }
     }
     switch (L) {
-    case 4: return launch_batch_t<float, 4>(d_args, d_first, nm, blocks, lds, st, spread);
-    case 8: return launch_batch_t<float, 8>(d_args, d_first, nm, blocks, lds, st, spread);
-    case 16: return launch_batch_t<float, 16>(d_args, d_first, nm, blocks, lds, st, spread);
-    default: return launch_batch_t<float, 32>(d_args, d_first, nm, blocks, lds, st, spread);
+    case 4: return launch_batch_t<float, 4>(d_args, d_first, nm, blocks, lds, st, spread, nt);
+    case 8: return launch_batch_t<float, 8>(d_args, d_first, nm, blocks, lds, st, spread, nt);
+    case 16: return launch_batch_t<float, 16>(d_args, d_first, nm, blocks, lds, st, spread, nt);
+    default: return launch_batch_t<float, 32>(d_args, d_first, nm, blocks, lds, st, spread, nt);
     }
 }
 

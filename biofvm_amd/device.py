@@ -361,3 +361,14 @@ class Ensemble:
     @row_tiles.setter
     def row_tiles(self, on):
         self._check(self.lib.bfvm_ensemble_set_row_tiles(self.e, int(bool(on))))
+
+    @property
+    def non_temporal(self):
+        """Batched y and z sweeps with non-temporal loads and stores."""
+        v = ctypes.c_int(0)
+        self._check(self.lib.bfvm_ensemble_get_cache_policy(self.e, ctypes.byref(v)))
+        return bool(v.value)
+
+    @non_temporal.setter
+    def non_temporal(self, on):
+        self._check(self.lib.bfvm_ensemble_set_cache_policy(self.e, int(bool(on))))

@@ -344,6 +344,13 @@ int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment);
 int bfvm_ensemble_set_row_tiles(bfvm_ensemble e, int on);
 int bfvm_ensemble_get_row_tiles(bfvm_ensemble e, int* on);
 
+/* Whether the batched y and z sweeps load and store with the non-temporal
+ * (streaming) hint (1) or the default policy (0; see
+ * bfvm_set_sweep_cache_policy); bfvm_ensemble_autotune times both.  Results
+ * do not depend on it.  No counterpart in the reference. */
+int bfvm_ensemble_set_cache_policy(bfvm_ensemble e, int non_temporal);
+int bfvm_ensemble_get_cache_policy(bfvm_ensemble e, int* non_temporal);
+
 const char* bfvm_ensemble_last_error(bfvm_ensemble e);
 
 /*

@@ -214,3 +214,24 @@ def test_row_tile_x_sweeps_equal_column_tiles(mixed, precision, L):
     for i, (a, b, ref) in enumerate(zip(out[True], out[False], refs)):
         np.testing.assert_array_equal(a, b, err_msg=f"member {i}")
         assert rel_err(a, ref) <= TOL[precision], i
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_cache_policy_never_changes_ensemble_results(mixed, precision):
+    # the batched y and z sweeps with non-temporal loads and stores give the
+    # default policy's bits
+    cases, refs = mixed
+    out = {}
+    for nt in (False, True):
+        members = _members(cases, precision)
+        ens = Ensemble(members)
+        try:
+            ens.non_temporal = nt
+            assert ens.non_temporal == nt
+            ens.run(cases[0]["dt"], cases[0]["steps"], ALL, graph=True)
+            out[nt] = [m.download() for m in members]
+        finally:
+            _close(ens, members)
+    for i, (a, b, ref) in enumerate(zip(out[True], out[False], refs)):
+        np.testing.assert_array_equal(a, b, err_msg=f"member {i}")
+        assert rel_err(a, ref) <= TOL[precision], i
