@@ -328,11 +328,11 @@ int bfvm_ensemble_sync(bfvm_ensemble e);
 int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph,
                             double* ms_total);
 
-/* Times the batched sweeps for each segment length (4, 8, 16, 32 nodes) and
+/* Times the batched sweeps for each segment length (4, 8, 16, 32 nodes),
  * spread of their blocks over the XCDs (as dispatched, chunks of 64 tiles,
- * contiguous eighths) and x tiles (rows or columns) on scratch copies of the
- * stores and keeps the fastest.
- * Densities unchanged. */
+ * contiguous eighths), x tiles (rows or columns) and y/z cache policy
+ * (default or non-temporal) on scratch copies of the stores and keeps the
+ * fastest.  Densities unchanged. */
 int bfvm_ensemble_autotune(bfvm_ensemble e, double dt);
 int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment);
 int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment);
