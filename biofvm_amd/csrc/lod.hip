@@ -51,11 +51,13 @@ constexpr int kPad = 32;  // zero rows around the correction tables (>= the larg
 constexpr int kRsrcWord3 = 0x00020000;
 // Cache policy bits of the loads and stores of the store (gfx950: bit 1 = NT,
 // streaming).  The y and z sweeps read and write each value once, in whole
-// cache lines, and run faster with non-temporal loads and stores (at 400^3 x
-// 8: FP32 y 4.5 %, z 3.7 %, and the agent kernel after them 15-25 %, the
-// sweeps no longer sweeping the caches clean of its data; the FP64 step 7 %);
-// the x sweep, whose lanes read 4-64-byte pieces of lines that later loads of
-// the same lines reuse, runs 8-65 % slower with them and keeps the default.
+// cache lines, and can run faster with non-temporal loads and stores (at
+// 400^3 x 8: FP32 y 4.5 %, z 3.7 %, and the agent kernel after them 15-25 %,
+// the sweeps no longer sweeping the caches clean of its data; the FP64 step
+// 7 %; at 200^3 x 4 FP64 the z sweep was 20 % slower), so the autotuner
+// times both (SweepPlan::nt).  The x sweep, whose lanes read 4-64-byte
+// pieces of lines that later loads of the same lines reuse, ran 8-65 %
+// slower with them and keeps the default.
 constexpr int kLoadPolicy = 0;
 constexpr int kStorePolicy = 0;
 constexpr int kStreamingPolicy = 2;
