@@ -190,9 +190,6 @@ class Microenvironment {
     DeviceState* device(void);
     friend void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, double dt);
     friend void diffusion_decay_solver__constant_coefficients_LOD_2D(Microenvironment& M, double dt);
-/* The 2-D LOD solver (BioFVM_solvers.h:40, BioFVM_solvers.cpp:278-427) on the
- * GPU, for meshes of one z layer. */
-void diffusion_decay_solver__constant_coefficients_LOD_2D(Microenvironment& M, double dt);
 
  private:
     std::vector<std::vector<double> > densities;  // host mirror
