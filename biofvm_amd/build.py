@@ -6,6 +6,7 @@ libbiofvm_facade.so (the BioFVM-compatible C++ host façade over it) in-tree.
 The library is compiled with hipcc for gfx950 only.  Packed FP32 ALU ops are
 disabled for the device code (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
 """
+import concurrent.futures
 import os
 import shlex
 import subprocess
@@ -35,14 +36,20 @@ def build(verbose=False):
     """Compiles every source and links the shared library; raises on failure."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(shlex.quote(c) for c in cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    # one compiler process per source, at most as many as the host has cores
+    workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or os.cpu_count() or 1, 16))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for proc in pool.map(lambda c: subprocess.run(c), cmds):
+            if proc.returncode != 0:
+                raise subprocess.CalledProcessError(proc.returncode, proc.args)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
