@@ -91,7 +91,7 @@ int afail(Env& e, int code, const std::string& msg)
     do {                                                                                         \
         hipError_t st_ = (call);                                                                 \
         if (st_ != hipSuccess) return afail(e, BFVM_E_DEVICE, std::string(#call) + ": " +        \
-                                                                   hipGetErrorString(st_));      \
+                                                                   bfvm::hip_error_text(st_));      \
     } while (0)
 
 // ---------------------------------------------------------------- kernels

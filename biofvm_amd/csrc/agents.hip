@@ -35,7 +35,7 @@ int afail(Env& e, int code, const std::string& msg)
     do {                                                                                          \
         hipError_t st_ = (call);                                                                  \
         if (st_ != hipSuccess) return afail(e, BFVM_E_DEVICE, std::string(#call) + ": " +         \
-                                                                   hipGetErrorString(st_));       \
+                                                                   bfvm::hip_error_text(st_));       \
     } while (0)
 
 // voxel of each agent, key = voxel, value = list index; bad[0] counts agents
@@ -185,7 +185,7 @@ int bfvm_set_agent_rates_device(bfvm_handle h, int n, const double* position, co
     chk(hipMalloc((void**)&e.agent_sat, sizeof(double) * ns));
     if (st != hipSuccess) {
         cleanup();
-        return afail(e, BFVM_E_DEVICE, std::string("agent arrays: ") + hipGetErrorString(st));
+        return afail(e, BFVM_E_DEVICE, std::string("agent arrays: ") + bfvm::hip_error_text(st));
     }
     // key bits: enough for the largest voxel index
     int bits = 1;
@@ -230,7 +230,7 @@ int bfvm_set_agent_rates_device(bfvm_handle h, int n, const double* position, co
     if (out_voxel) chk(hipMemcpyAsync(out_voxel, key, sizeof(int) * n, hipMemcpyDeviceToDevice, e.stream));
     chk(hipStreamSynchronize(e.stream));
     cleanup();
-    if (st != hipSuccess) rc = afail(e, BFVM_E_DEVICE, std::string("agent setup: ") + hipGetErrorString(st));
+    if (st != hipSuccess) rc = afail(e, BFVM_E_DEVICE, std::string("agent setup: ") + bfvm::hip_error_text(st));
     else if (nbad > 0) rc = afail(e, BFVM_E_ARG, std::to_string(nbad) + " agent(s) outside the domain");
     if (rc != BFVM_OK) {
         release(e.agent_seg_voxel);

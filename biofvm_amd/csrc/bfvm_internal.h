@@ -11,6 +11,16 @@
 
 namespace bfvm {
 
+// Message of a failed HIP call.  A failed runtime call (an allocation the
+// device refuses, say) also stays the thread's last error, which the next
+// launch check (hipGetLastError) would report against an unrelated kernel;
+// taking the message consumes it.
+inline std::string hip_error_text(hipError_t st)
+{
+    (void)hipGetLastError();
+    return hipGetErrorString(st);
+}
+
 // Tridiagonal solve of one axis, all substrates, FP64 on the device for
 // either store precision.  Element (i, s) of an (n, S) table is at [i*S + s].
 //
@@ -173,6 +183,10 @@ struct Env {
     // Gradient vectors (nvs*3, store precision), allocated on first use.
     void* grad = nullptr;
 
+    // Workspace of bfvm_mass_balance (reduce.hip), allocated on first use.
+    void* red = nullptr;
+    size_t red_bytes = 0;
+
     // MAT v4 snapshot buffers and writer thread (io.hip).
     struct MatWriter* mat = nullptr;
 
@@ -257,6 +271,20 @@ inline double decay_split(double lambda, double dt, int lod_dims)
     else c2 /= 3.0;
     return c2;
 }
+
+// Mass balance (reduce.hip): the workspace layout for this grid (pointers
+// into `workspace`, which may be null to learn the size) and the two
+// launches that write the S sums and the non-finite count to device memory.
+struct MassLayout {
+    int nblk = 0;
+    double* part = nullptr;
+    unsigned long long* bad = nullptr;
+    double* sums = nullptr;
+    long long* count = nullptr;
+    size_t bytes = 0;
+};
+MassLayout mass_layout(const Env& e, void* workspace);
+hipError_t launch_mass_balance(Env& e, const MassLayout& m, double* sums, long long* nonfinite);
 
 // Frees the adjoint's tape, if any.
 void free_tape(Env& e);

@@ -25,7 +25,7 @@ int fail(Env* e, int code, const std::string& msg)
 
 int hip_fail(Env* e, hipError_t st, const char* what)
 {
-    std::string m = std::string(what) + ": " + hipGetErrorString(st);
+    std::string m = std::string(what) + ": " + bfvm::hip_error_text(st);
     return fail(e, BFVM_E_DEVICE, m);
 }
 
@@ -521,6 +521,7 @@ int bfvm_destroy(bfvm_handle h)
     dfree(e->bulk_A);
     dfree(e->bulk_B);
     dfree(e->grad);
+    dfree(e->red);
     free_agents(*e);
     for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);

@@ -70,7 +70,7 @@ int efail(bfvm_ensemble_t* e, int code, const std::string& msg)
     do {                                                                                       \
         hipError_t st_ = (call);                                                               \
         if (st_ != hipSuccess) return efail(e, BFVM_E_DEVICE, std::string(#call) + ": " +      \
-                                                                  hipGetErrorString(st_));     \
+                                                                  bfvm::hip_error_text(st_));     \
     } while (0)
 
 // Runs `f` with member `m` launching on the ensemble's stream.
@@ -288,7 +288,7 @@ int ensure_graph(bfvm_ensemble_t& E, double dt, int flags)
     if (st != hipSuccess || st2 != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
         return efail(&E, BFVM_E_DEVICE, std::string("graph capture: ") +
-                                            hipGetErrorString(st != hipSuccess ? st : st2));
+                                            bfvm::hip_error_text(st != hipSuccess ? st : st2));
     }
     E.graph = g;
     EHIP(&E, hipGraphInstantiate(&E.exec, g, nullptr, nullptr, 0));
@@ -416,7 +416,7 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
         (void)hipStreamSynchronize(e->stream);
         for (void* p : scratch)
             if (p) (void)hipFree(p);
-        return efail(e, BFVM_E_DEVICE, std::string("autotune scratch: ") + hipGetErrorString(st));
+        return efail(e, BFVM_E_DEVICE, std::string("autotune scratch: ") + bfvm::hip_error_text(st));
     }
     for (size_t i = 0; i < e->m.size(); ++i) {
         keep.push_back(e->m[i]->rho);

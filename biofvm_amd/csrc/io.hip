@@ -76,7 +76,7 @@ int iofail(Env& e, int code, const std::string& msg)
     do {                                                                                          \
         hipError_t st_ = (call);                                                                  \
         if (st_ != hipSuccess) return iofail(e, BFVM_E_DEVICE, std::string(#call) + ": " +        \
-                                                                    hipGetErrorString(st_));      \
+                                                                    bfvm::hip_error_text(st_));      \
     } while (0)
 
 // Writes the header and the packed matrix; returns an error message or "".
@@ -194,7 +194,7 @@ int bfvm_write_matlab_async(bfvm_handle h, const char* filename, double x0, doub
         const hipError_t st = hipStreamSynchronize(w.copy_stream);
         if (st != hipSuccess) {
             w.result = BFVM_E_DEVICE;
-            w.err = std::string("snapshot copy: ") + hipGetErrorString(st);
+            w.err = std::string("snapshot copy: ") + bfvm::hip_error_text(st);
             return;
         }
         const std::string msg = write_file(name, rows, cols, w.host, w.bytes);

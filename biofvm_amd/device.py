@@ -170,6 +170,14 @@ class DeviceMicroenvironment:
     def io_wait(self):
         self._check(self.lib.bfvm_io_wait(self.h))
 
+    def mass_balance(self):
+        """(per-substrate FP64 sums of the store, number of non-finite values),
+        reduced on the device (bfvm_mass_balance)."""
+        sums = np.zeros(self.S)
+        bad = ctypes.c_longlong(0)
+        self._check(self.lib.bfvm_mass_balance(self.h, _dptr(sums), ctypes.byref(bad)))
+        return sums, int(bad.value)
+
     def gradients(self):
         """Gradient vectors of the current densities, shape (nvox, S, 3)."""
         self._check(self.lib.bfvm_compute_gradients(self.h))

@@ -241,6 +241,19 @@ int bfvm_get_sweep_spread(bfvm_handle h, int axis, int* tiles_per_chunk);
 int bfvm_set_sweep_spread(bfvm_handle h, int axis, int tiles_per_chunk);
 
 /*
+ * Mass balance: sums[s] = sum over voxels of rho[v*S + s], in FP64 whatever
+ * the store precision, with a fixed summation order (the same store gives the
+ * same bits on every run), and *nonfinite = the number of values that are NaN
+ * or infinite.  Either output may be NULL (not both).  Every column of each
+ * LOD sweep matrix sums to 1 + c2 (BioFVM_solvers.cpp:121-135), so without
+ * sources, Dirichlet nodes or agents one LOD_3D step multiplies each sum by
+ * (1 + dt*lambda/3)^-3 exactly; the reference has no such check (its solver
+ * and examples never sum the store).  Waits for the handle's work; the
+ * workspace (about 8*(S+1) KiB) is allocated on the first call.
+ */
+int bfvm_mass_balance(bfvm_handle h, double* sums, long long* nonfinite);
+
+/*
  * Gradient vectors of the current densities (Microenvironment::
  * compute_all_gradient_vectors, BioFVM_microenvironment.cpp:699-779): per
  * voxel n and substrate s, for each axis with n strictly inside,

@@ -7,8 +7,9 @@
   steps each, FP64 store, 1e-12 of max|ref|.
 - 400^3 x 8 (the precision form's grid), diffusion only: every sweep's column
   sums are 1 + c2, so each substrate's mass is multiplied by exactly
-  (1 + lambda dt / 3)^-3 per step (no-flux boundaries).  FP64 store within
-  1e-12, FP32 store within 1e-6 (its sum is taken in FP64 over FP32 values).
+  (1 + lambda dt / 3)^-3 per step (no-flux boundaries); the sums are the
+  device's FP64 reduction (bfvm_mass_balance).  FP64 store within 1e-12,
+  FP32 store within 1e-6 (its sum is taken in FP64 over FP32 values).
 - A store above 4 GiB (416 x 416 x 400 x 8, FP64, 4.43 GB): the x sweep runs
   in two runs of z planes and the z sweep with one segment per wave (its
   buffers start at each segment), both past the 32-bit offsets of one
@@ -57,9 +58,10 @@ def test_400_cubed_mass_decays_by_the_exact_factor(precision, tol):
     rho0 = rng.uniform(0.0, 1.0, n ** 3 * S)
     case = W.make_case(n, n, n, S, 10.0, D, lam, dt, steps, W.DIFFUSION, rho0)
     with setup_case(case, precision) as m:
-        start = m.download().reshape(-1, S).sum(axis=0)  # the store's own values (FP32-rounded)
+        start, bad0 = m.mass_balance()  # FP64 sums on the device of the store's own values
         m.run(dt, steps, W.DIFFUSION)
-        end = m.download().reshape(-1, S).sum(axis=0)
+        end, bad = m.mass_balance()
+    assert bad0 == 0 and bad == 0
     c2 = lam * dt / 3.0
     expect = start * (1.0 + c2) ** (-3 * steps)
     assert np.max(np.abs(end - expect) / expect) <= tol
