@@ -50,6 +50,7 @@ SIGNATURES = {
     "bfvm_get_fused_sources": (_i, [_h, _ip]),
     "bfvm_set_sweep_spread": (_i, [_h, _i, _i]),
     "bfvm_mass_balance": (_i, [_h, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "bfvm_mass_balance_noalloc": (_i, [_h, _vp, _vp]),
     "bfvm_compute_gradients": (_i, [_h]),
     "bfvm_download_gradients": (_i, [_h, _dp]),
     "bfvm_gradient_device_ptr": (_i, [_h, ctypes.POINTER(_vp)]),

@@ -9,6 +9,8 @@
 // b owns a fixed voxel range, its threads a fixed stride, partials folded by
 // a fixed tree), so the same store gives the same bits on every run.
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "bfvm_internal.h"
@@ -177,5 +179,73 @@ extern "C" int bfvm_mass_balance(bfvm_handle h, double* sums, long long* nonfini
         return rhip(h, st, "bfvm_mass_balance");
     if (sums) std::copy(hs.begin(), hs.end(), sums);
     if (nonfinite) *nonfinite = hb;
+    return BFVM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// bfvm_mass_balance_noalloc: the same reduction with no allocation.  Its
+// workspace is `scratch`, a file-scope device array (one copy per GPU, held
+// until the process ends), sized for up to kNoAllocMaxS substrates with the
+// largest block count.  Host code never passes `scratch` itself (in host
+// code its name stands for a host-side shadow, not a device address): the
+// device address is looked up once per GPU, with that GPU current, and kept.
+namespace {
+
+constexpr int kNoAllocMaxS = 64;
+constexpr size_t kScratchBytes = sizeof(double) * ((size_t)bfvm::kMaxRedBlocks * kNoAllocMaxS + kNoAllocMaxS) +
+                                 sizeof(unsigned long long) * (bfvm::kMaxRedBlocks + 1);
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (looked up on the first call for
+// that device).
+hipError_t scratch_address(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        hipError_t st = hipGetDevice(&cur);
+        if (st != hipSuccess) return st;
+        if (cur != device && (st = hipSetDevice(device)) != hipSuccess) return st;
+        void* p = nullptr;
+        st = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) {
+            const hipError_t st2 = hipSetDevice(cur);
+            if (st == hipSuccess) st = st2;
+        }
+        if (st != hipSuccess) return st;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" int bfvm_mass_balance_noalloc(bfvm_handle h, double* sums, long long* nonfinite)
+{
+    if (!h) return rfail(nullptr, BFVM_E_ARG, "null handle");
+    Env& e = *h;
+    if (e.S > kNoAllocMaxS)
+        return rfail(h, BFVM_E_ARG,
+                     "bfvm_mass_balance_noalloc: " + std::to_string(e.S) + " substrates exceed its static "
+                     "workspace's limit of " + std::to_string(kNoAllocMaxS) + " (use bfvm_mass_balance)");
+    void* ws = nullptr;
+    hipError_t st = scratch_address(e.device, &ws);
+    if (st != hipSuccess) return rhip(h, st, "bfvm_mass_balance_noalloc: workspace address");
+    const bfvm::MassLayout m = bfvm::mass_layout(e, ws);
+    if (m.bytes > kScratchBytes) return rfail(h, BFVM_E_ARG, "bfvm_mass_balance_noalloc: workspace too small");
+    // a member of an ensemble: its store may still be written by the ensemble's stream
+    if (e.ensemble_stream && (st = hipStreamSynchronize(e.ensemble_stream)) != hipSuccess)
+        return rhip(h, st, "bfvm_mass_balance_noalloc");
+    if ((st = bfvm::launch_mass_balance(e, m, sums ? sums : m.sums, nonfinite ? nonfinite : m.count)) !=
+        hipSuccess)
+        return rhip(h, st, "bfvm_mass_balance_noalloc");
     return BFVM_OK;
 }

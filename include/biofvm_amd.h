@@ -254,6 +254,20 @@ int bfvm_set_sweep_spread(bfvm_handle h, int axis, int tiles_per_chunk);
 int bfvm_mass_balance(bfvm_handle h, double* sums, long long* nonfinite);
 
 /*
+ * bfvm_mass_balance for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The same kernels and the
+ * same bits as bfvm_mass_balance, but nothing is allocated and nothing waits:
+ * the reduction is enqueued on the handle's stream and writes sums[S] and
+ * *nonfinite to DEVICE memory owned by the caller (either may be NULL).  Its
+ * workspace is a static device array of 520 KiB per GPU, held until the
+ * process ends, which limits it to n_substrates <= 64 (BFVM_E_ARG beyond).
+ * Every call on one GPU shares that array: calls that overlap on one GPU (two
+ * streams at once) race; calls in order on one stream do not.
+ */
+int bfvm_mass_balance_noalloc(bfvm_handle h, double* sums, long long* nonfinite);
+
+/*
  * Gradient vectors of the current densities (Microenvironment::
  * compute_all_gradient_vectors, BioFVM_microenvironment.cpp:699-779): per
  * voxel n and substrate s, for each axis with n strictly inside,
