@@ -209,6 +209,61 @@ def cpu_baseline(case, workload):
     return out, one_core
 
 
+def facade_bench(steps=50, warmup=5, every_step_steps=3):
+    """The drop-in configuration (200^3 x 4, Dirichlet faces, bulk callbacks,
+    1e5 agents, FP64) driven through the reference's own API: the C++ façade
+    (biofvm_amd/facade) under tests/cpp/facade_driver, a program written like
+    the reference's examples (bulk -> cells -> diffusion per step).  Dirichlet
+    nodes overwrite the whole value vector {1, 0, 0, 0}, the reference's
+    semantics (the C-ABI drop_in_200 run masks substrates 1-3 instead).  Timed
+    by the driver's steady clock over `steps` steps after `warmup`, until the
+    device has finished; once with the bulk fields kept on the device (the
+    default), once with the callbacks evaluated at every step as the
+    reference does (`every_step_steps` steps).  The result after
+    warmup + steps steps is compared with the C ABI's on the same inputs."""
+    import subprocess
+    import tempfile
+    from biofvm_amd import FP64, setup_case
+    from biofvm_amd import workloads as W
+    driver = os.path.join(ROOT, "tests", "cpp", "facade_driver")
+    case = W.drop_in_200(steps=steps, mask_substrate0=False)
+    nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    out = {"workload": "drop_in_200 through the reference API (C++ facade): 200^3 x 4, FP64, "
+                       "Dirichlet faces, bulk callbacks, 1e5 agents", "warmup": warmup}
+    with tempfile.TemporaryDirectory() as d:
+        runs = {}
+        for mode, n, extra in (("fields_on_device", steps, []),
+                               ("callbacks_every_step", every_step_steps, ["--bulk-every-step"])):
+            c = dict(case, steps=n)
+            cin, cout = os.path.join(d, f"{mode}.bin"), os.path.join(d, f"{mode}.out")
+            W.write_driver_case(cin, c, FP64)
+            r = subprocess.run([driver, cin, cout, "--time", str(warmup), *extra], capture_output=True,
+                               text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(f"facade driver failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
+            line = json.loads(r.stdout.strip().splitlines()[-1])
+            line["value"] = nvs / (line["ms_per_step"] * 1e-3)
+            runs[mode] = (line, cout)
+            print(f"[bench] facade {mode}: {line['ms_per_step']:.4f} ms/step", file=sys.stderr, flush=True)
+        main_line, main_out = runs["fields_on_device"]
+        got = np.fromfile(main_out, dtype="<f8")
+        with setup_case(dict(case, steps=warmup + steps), FP64) as m:
+            m.autotune(case["dt"])
+            m.run(case["dt"], warmup + steps, case["phases"])
+            ref = m.download()
+    out.update({
+        "steps": steps, "ms_per_step": main_line["ms_per_step"], "value": main_line["value"],
+        "unit": "voxel-substrate updates/s", "coefficient_builds": main_line["coefficient_builds"],
+        "bulk_host_seconds_per_step": main_line["bulk_rates_host_seconds"] / steps,
+        "max_rel_diff_vs_c_abi": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))),
+        "callbacks_every_step": {
+            "steps": every_step_steps, "ms_per_step": runs["callbacks_every_step"][0]["ms_per_step"],
+            "bulk_host_seconds_per_step":
+                runs["callbacks_every_step"][0]["bulk_rates_host_seconds"] / every_step_steps},
+    })
+    return out
+
+
 def calibrate_copy(mib):
     """Copies `mib` MiB device to device with hipMemcpy (the runtime's copy
     kernel), three times, so rocprofv3 counters of a kernel with known bytes
@@ -543,6 +598,8 @@ def main():
         "sweep_non_temporal": {"x": policies[0], "y": policies[1], "z": policies[2]},
     }
     line.update(out_extra)
+    if args.workload == "drop_in_200":
+        line["facade"] = facade_bench()
     if not args.no_cpu_baseline:
         line["cpu_baseline"], one_core = cpu_baseline(case, args.workload)
         if one_core:

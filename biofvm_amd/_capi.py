@@ -41,6 +41,9 @@ SIGNATURES = {
     "bfvm_sync": (_i, [_h]),
     "bfvm_timed_run": (_i, [_h, _d, _i, _i, _dp, _dp, _ip]),
     "bfvm_autotune": (_i, [_h, _d]),
+    "bfvm_set_autotune_segments": (_i, [_h, _i]),
+    "bfvm_get_autotune_segments": (_i, [_h, _ip]),
+    "bfvm_get_coefficient_builds": (_i, [_h, ctypes.POINTER(ctypes.c_uint)]),
     "bfvm_get_sweep_plan": (_i, [_h, _i, _ip, _ip, _ip]),
     "bfvm_set_sweep_plan": (_i, [_h, _i, _i, _i, _i]),
     "bfvm_get_sweep_spread": (_i, [_h, _i, _ip]),
@@ -71,6 +74,7 @@ SIGNATURES = {
     "bfvm_ensemble_autotune": (_i, [_vp, _d]),
     "bfvm_ensemble_get_segment": (_i, [_vp, _ip]),
     "bfvm_ensemble_set_segment": (_i, [_vp, _i]),
+    "bfvm_ensemble_set_autotune_segments": (_i, [_vp, _i]),
     "bfvm_ensemble_set_row_tiles": (_i, [_vp, _i]),
     "bfvm_ensemble_get_row_tiles": (_i, [_vp, _ip]),
     "bfvm_ensemble_set_cache_policy": (_i, [_vp, _i]),

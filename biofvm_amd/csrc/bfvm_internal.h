@@ -133,6 +133,14 @@ struct Env {
     // bulk, agents and the Dirichlet overwrites before the x and y sweeps run
     // inside the x sweep (load prologue and store epilogue) when it can
     bool fused_sources = true;
+    // whether bfvm_autotune may choose the segment length (which changes the
+    // solve's rounding); otherwise it keeps default_segment's
+    bool tune_segments = false;
+    // bumped whenever a launch shape, cache policy or XCD spread is set, so
+    // that an ensemble holding this handle rebuilds its tables and graph
+    unsigned plan_epoch = 0;
+    // number of times the Thomas/Woodbury tables were built
+    unsigned coef_builds = 0;
 
     // Dirichlet nodes, one per voxel (the last of repeated entries, as the
     // reference's in-order overwrite leaves it), sorted by voxel index.
@@ -291,9 +299,12 @@ void free_tape(Env& e);
 // Waits for and frees the snapshot writer, if any.
 void free_writer(Env& e);
 
-// Valid segmented shapes for a line of n nodes and a store precision, for
-// the autotuner.
-std::vector<SweepPlan> sweep_candidates(const Env& e, int axis);
+// Segment length of an axis unless the autotuner may choose it (lod.hip).
+int default_segment(const Env& e, int axis);
+// Valid segmented shapes of one axis for this grid and store precision: of
+// every segment length, or only of default_segment's (what the autotuner
+// times unless the handle's tune_segments is set).
+std::vector<SweepPlan> sweep_candidates(const Env& e, int axis, bool any_length);
 // The launch shape a sweep of `axis` uses now (autotuned or default).
 SweepPlan effective_plan(const Env& e, int axis);
 // Whether plan p of `axis` may use non-temporal loads and stores (SweepPlan::nt).

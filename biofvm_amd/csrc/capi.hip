@@ -247,6 +247,7 @@ int ensure_coefficients_impl(Env& e, double dt)
     e.coef_valid = true;
     e.coef_dt = dt;
     ++e.coef_epoch;
+    ++e.coef_builds;
     return BFVM_OK;
 }
 
@@ -533,6 +534,12 @@ int bfvm_set_coefficients(bfvm_handle h, const double* D, const double* lambda)
 {
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (!D || !lambda) return fail(h, BFVM_E_ARG, "D and lambda must be non-NULL");
+    // the same values as before keep the tables (a caller that sets them
+    // before every step rebuilds nothing)
+    const size_t bytes = sizeof(double) * h->S;
+    if (h->D.size() == (size_t)h->S && std::memcmp(h->D.data(), D, bytes) == 0 &&
+        std::memcmp(h->lambda.data(), lambda, bytes) == 0)
+        return BFVM_OK;
     h->D.assign(D, D + h->S);
     h->lambda.assign(lambda, lambda + h->S);
     h->coef_valid = false;
@@ -798,7 +805,7 @@ int bfvm_autotune(bfvm_handle h, double dt)
         bfvm::SweepPlan best{};
         float best_ms = 1e30f;
         float best_block_ms = 1e30f;  // the fastest one-block-per-tile x shape, for the fused x sweep
-        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(e, axis)) {
+        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(e, axis, e.tune_segments)) {
             e.plan[axis] = cand;
             if ((st = bfvm::launch_sweep(e, axis)) != hipSuccess) break;  // warm-up
             (void)hipEventRecord(t0, e.stream);
@@ -843,7 +850,7 @@ int bfvm_autotune(bfvm_handle h, double dt)
         if (e.have_bulk) st = bfvm::ensure_bulk(e, dt);
         bfvm::SweepPlan best_fused = e.fused_plan;
         float best_ms = 1e30f;
-        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(e, 0)) {
+        for (const bfvm::SweepPlan& cand : bfvm::sweep_candidates(e, 0, e.tune_segments)) {
             if (st != hipSuccess) break;
             if (cand.stream != 0) continue;
             e.fused_plan = cand;
@@ -894,11 +901,33 @@ int bfvm_autotune(bfvm_handle h, double dt)
     (void)hipFree(scratch);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    ++e.plan_epoch;
     if (st != hipSuccess) {
         for (auto& p : e.plan) p = bfvm::SweepPlan{};
         e.fused_plan = bfvm::SweepPlan{};
         return hip_fail(&e, st, "bfvm_autotune");
     }
+    return BFVM_OK;
+}
+
+int bfvm_set_autotune_segments(bfvm_handle h, int on)
+{
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    h->tune_segments = on != 0;
+    return BFVM_OK;
+}
+
+int bfvm_get_autotune_segments(bfvm_handle h, int* on)
+{
+    if (!h || !on) return fail(h, BFVM_E_ARG, "null argument");
+    *on = h->tune_segments ? 1 : 0;
+    return BFVM_OK;
+}
+
+int bfvm_get_coefficient_builds(bfvm_handle h, unsigned* builds)
+{
+    if (!h || !builds) return fail(h, BFVM_E_ARG, "null argument");
+    *builds = h->coef_builds;
     return BFVM_OK;
 }
 
@@ -920,11 +949,12 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
     if (axis == 0) h->fused_plan = bfvm::SweepPlan{};  // the fused x sweep follows the x shape set here
+    ++h->plan_epoch;
     if (lines_per_block == 0) {
         h->plan[axis] = bfvm::SweepPlan{};
         return BFVM_OK;
     }
-    for (const bfvm::SweepPlan& c : bfvm::sweep_candidates(*h, axis))
+    for (const bfvm::SweepPlan& c : bfvm::sweep_candidates(*h, axis, true))
         if (c.cols == lines_per_block && c.L == nodes_per_segment && c.stream == streaming) {
             h->plan[axis] = c;
             return BFVM_OK;
@@ -953,6 +983,7 @@ int bfvm_set_sweep_cache_policy(bfvm_handle h, int axis, int non_temporal)
     if (p.cols == 0) return BFVM_OK;  // the one-thread-per-line fallback has no policy
     p.nt = non_temporal ? 1 : 0;
     h->plan[axis] = p;
+    ++h->plan_epoch;
     return BFVM_OK;
 }
 
@@ -986,6 +1017,7 @@ int bfvm_set_sweep_spread(bfvm_handle h, int axis, int tiles_per_chunk)
     if (tiles_per_chunk < -1) return fail(h, BFVM_E_ARG, "tiles_per_chunk must be -1, 0 or positive");
     HIP_OR_FAIL(h, quiesce(*h));
     h->spread[axis] = tiles_per_chunk;
+    ++h->plan_epoch;
     return BFVM_OK;
 }
 

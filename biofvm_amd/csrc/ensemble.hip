@@ -27,12 +27,15 @@ struct bfvm_ensemble_t {
     int spread = 0;  // XCD spread of the batched sweeps' blocks (xcd_tile in lod.hip)
     int nt = 0;      // y and z batched sweeps with non-temporal loads and stores (autotuned)
     int rows_x = 1;  // x sweeps on LDS-staged row tiles (k_sweep_rows_batch) where a member can
+    // whether bfvm_ensemble_autotune may change L (which changes the rounding
+    // of the batched solve); otherwise it tunes the other knobs at this L
+    bool tune_segments = false;
 
     // tables, valid for (dt, L, member coefficient/agent epochs)
     bool tables_valid = false;
     double tab_dt = 0;
     int tab_L = 0;
-    std::vector<unsigned> coef_epoch, agent_epoch, source_epoch;
+    std::vector<unsigned> coef_epoch, agent_epoch, source_epoch, plan_epoch;
     bfvm::SweepArgs* d_args[3] = {nullptr, nullptr, nullptr};
     int* d_first[3] = {nullptr, nullptr, nullptr};  // per block: member, block within the member
     int nbatch[3] = {0, 0, 0};       // members in the batched launch
@@ -128,7 +131,7 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
         }
         if (flags & BFVM_BULK) EHIP(&E, on_stream(m, E.stream, [&] { return bfvm::ensure_bulk(m, dt); }));
         if (E.coef_epoch[i] != m.coef_epoch || E.agent_epoch[i] != m.agent_epoch ||
-            E.source_epoch[i] != m.source_epoch)
+            E.source_epoch[i] != m.source_epoch || E.plan_epoch[i] != m.plan_epoch)
             changed = true;
     }
     if (!changed) return BFVM_OK;
@@ -234,6 +237,7 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
         E.coef_epoch[i] = E.m[i]->coef_epoch;
         E.agent_epoch[i] = E.m[i]->agent_epoch;
         E.source_epoch[i] = E.m[i]->source_epoch;
+        E.plan_epoch[i] = E.m[i]->plan_epoch;
     }
     E.tab_dt = dt;
     E.tab_L = E.L;
@@ -339,6 +343,7 @@ int bfvm_ensemble_create(int count, const bfvm_handle* members, bfvm_ensemble* o
     E->coef_epoch.assign(count, ~0u);
     E->agent_epoch.assign(count, ~0u);
     E->source_epoch.assign(count, ~0u);
+    E->plan_epoch.assign(count, ~0u);
     if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
         delete E;
         return BFVM_E_DEVICE;
@@ -426,7 +431,9 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
     int best_L = e->L, best_spread = e->spread, best_rows = e->rows_x, best_nt = e->nt;
     double best = 1e300;
     int rc = BFVM_OK;
-    for (int L : {4, 8, 16, 32}) {
+    std::vector<int> lengths{e->L};
+    if (e->tune_segments) lengths = {4, 8, 16, 32};
+    for (int L : lengths) {
         for (int spread : {0, 64, -1}) {
             for (int rows : {0, 1}) {
                 for (int nt : {0, 1}) {
@@ -464,6 +471,13 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
     e->tables_valid = false;
     drop_graph(*e);
     return rc;
+}
+
+int bfvm_ensemble_set_autotune_segments(bfvm_ensemble e, int on)
+{
+    if (!e) return BFVM_E_ARG;
+    e->tune_segments = on != 0;
+    return BFVM_OK;
 }
 
 int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment)

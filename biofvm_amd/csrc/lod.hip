@@ -884,16 +884,42 @@ SweepPlan default_plan(int n)
     return SweepPlan{0, 0};
 }
 
+}  // namespace
+
+// The segment length of an axis when the autotuner may not choose it: a
+// fixed function of the line length, axis and store precision, so results
+// do not depend on timing (the segment length changes the rounding of the
+// solve; every other launch parameter leaves the bits unchanged).  Measured
+// on 64^3 .. 400^3 grids with 2-8 substrates, it is the fastest length or
+// within 4 % of it: 32 nodes for y and z lines of 400 or more in an FP32
+// store, 16 for lines of 200 or more, 8 below.
+int default_segment(const Env& e, int axis)
+{
+    const int n = axis == 0 ? e.nx : axis == 1 ? e.ny : e.nz;
+    int L = 8;
+    if (n >= 400 && axis > 0 && e.precision == BFVM_FP32) L = 32;
+    else if (n >= 200) L = 16;
+    for (int cols : {32, 16, 64})
+        if (plan_valid(n, cols, L)) return L;
+    return default_plan(n).L;  // lines too long for this length
+}
+
+namespace {
+
 SweepPlan plan_for(const Env& e, int axis)
 {
     const SweepPlan& t = e.plan[axis];
     if (t.cols > 0) return t;
     const int n = axis == 0 ? e.nx : axis == 1 ? e.ny : e.nz;
+    const int L = default_segment(e, axis);
     if (axis == 2 && e.nvs * e.elem >= 0xffffffffull) {
         // a store above 4 GiB: z lines one segment per wave (SweepArgs::seg_base)
-        for (int L : {16, 32, 8, 24, 12, 4})
-            if (plan_valid(n, 64, L) && (L <= 8 || L < 2 * n)) return SweepPlan{64, L};
+        if (plan_valid(n, 64, L)) return SweepPlan{64, L};
+        for (int Lz : {16, 32, 8, 24, 12, 4})
+            if (plan_valid(n, 64, Lz) && (Lz <= 8 || Lz < 2 * n)) return SweepPlan{64, Lz};
     }
+    for (int cols : {32, 16, 64})
+        if (plan_valid(n, cols, L)) return SweepPlan{cols, L};
     return default_plan(n);
 }
 
@@ -1332,13 +1358,14 @@ SweepPlan effective_plan(const Env& e, int axis)
     return seg ? p : SweepPlan{0, 0, 0};
 }
 
-std::vector<SweepPlan> sweep_candidates(const Env& e, int axis)
+std::vector<SweepPlan> sweep_candidates(const Env& e, int axis, bool any_length)
 {
     const int n = axis == 0 ? e.nx : axis == 1 ? e.ny : e.nz;
+    const int Lfix = any_length ? 0 : default_segment(e, axis);  // 0: any length
     std::vector<SweepPlan> out;
     for (int cols : kColsSet)
         for (int L : kNodes)
-            if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n)) {
+            if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n) && (Lfix == 0 || L == Lfix)) {
                 out.push_back(SweepPlan{cols, L, 0});
                 if (stream_nodes_ok(L, e.precision)) out.push_back(SweepPlan{cols, L, 1});
                 if (nt_shape(e, axis, cols, 0)) {
@@ -1349,7 +1376,7 @@ std::vector<SweepPlan> sweep_candidates(const Env& e, int axis)
     if (axis == 0)
         for (int R = 1; R <= 64; R *= 2)
             for (int L : kNodes)
-                if ((L <= 8 || L < 2 * n) && rows_plan_valid(e, R * e.S, L))
+                if ((L <= 8 || L < 2 * n) && (Lfix == 0 || L == Lfix) && rows_plan_valid(e, R * e.S, L))
                     out.push_back(SweepPlan{R * e.S, L, 2});
     return out;
 }

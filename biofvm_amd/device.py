@@ -132,6 +132,16 @@ class DeviceMicroenvironment:
         """Picks the fastest launch shape of each sweep (on a scratch copy)."""
         self._check(self.lib.bfvm_autotune(self.h, float(dt)))
 
+    def set_autotune_segments(self, on):
+        """Let autotune() choose the nodes per segment too (changes the rounding)."""
+        self._check(self.lib.bfvm_set_autotune_segments(self.h, 1 if on else 0))
+
+    def coefficient_builds(self):
+        """Times the Thomas/Woodbury tables were built for this handle."""
+        n = ctypes.c_uint(0)
+        self._check(self.lib.bfvm_get_coefficient_builds(self.h, ctypes.byref(n)))
+        return int(n.value)
+
     def set_sweep_plan(self, axis, lines_per_block, nodes_per_segment, streaming=0):
         """streaming (the kernel): 0 one block per column tile, 1 resident
         streaming blocks, 2 row tiles through LDS (x only; lines = rows x S)."""

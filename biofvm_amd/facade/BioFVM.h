@@ -42,6 +42,8 @@
 #include <string>
 #include <vector>
 
+struct bfvm_env;  // the handle type of the biofvm_amd C ABI (biofvm_amd.h)
+
 namespace BioFVM {
 
 class Microenvironment;
@@ -176,6 +178,16 @@ class Microenvironment {
     // MAT v4 snapshot (BioFVM_microenvironment.cpp:557-580), packed on the GPU.
     void write_to_matlab(std::string filename);
 
+    // Bulk source/sink fields (BioFVM_microenvironment.h:121-124, filled by
+    // update_rates, BioFVM_microenvironment.cpp:635-656): the three callbacks
+    // evaluated per voxel, supply_target_densities_times_supply_rates = T*S.
+    // Here update_rates also hands S, T and U to the device, where the bulk
+    // step reads them (see bulk_rates_every_step).
+    std::vector<std::vector<double> > supply_target_densities_times_supply_rates;
+    std::vector<std::vector<double> > supply_rates;
+    std::vector<std::vector<double> > uptake_rates;
+    void update_rates(void);
+
     /* Extensions (no counterpart in the reference). */
     // Restrict Dirichlet overwrites to substrates with mask[s] != 0
     // (empty = every substrate, the reference's behaviour).
@@ -185,6 +197,25 @@ class Microenvironment {
     int device_precision;
     // Copy the device state into the host mirror now.
     void synchronize_host(void);
+    // Bulk callbacks: false (the default) evaluates them once, at the first
+    // simulate_bulk_sources_and_sinks, and keeps the fields on the device,
+    // evaluating them again only after update_rates(), after any of the three
+    // callback pointers changes, or after a resize; true evaluates them at
+    // every call, as the reference does (BioFVM_microenvironment.cpp:595-600),
+    // for callbacks that depend on time or on the densities.  With all three
+    // callbacks zero_function the bulk step is the identity and is skipped.
+    bool bulk_rates_every_step;
+    // Host seconds spent evaluating the bulk callbacks and handing their
+    // fields to the device, summed over the calls.
+    double bulk_rates_host_seconds;
+    // Waits until every step requested so far has run on the device.  The
+    // bulk and cell phases are enqueued with the next diffusion step (one
+    // device step, sources fused into its x sweep) or, before any density is
+    // read, on their own.
+    void wait_for_device(void);
+    // The C-ABI handle of the device copy (null before the first device use),
+    // for calls such as bfvm_mass_balance on the same store.
+    bfvm_env* device_handle(void);
 
     // internal: the device handle (created on first use)
     DeviceState* device(void);
@@ -201,6 +232,8 @@ class Microenvironment {
     void reset_space(void);
     void host_view(void);           // mirror current + mark it the newer copy
     int push_to_device(void);       // upload whatever the device lacks
+    int flush_pending(void);        // run the bulk/cell phases not yet enqueued
+    int load_bulk_fields(void);     // evaluate the callbacks, hand S, T, U to the device
 };
 
 class Basic_Agent {

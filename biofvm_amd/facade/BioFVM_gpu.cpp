@@ -20,6 +20,7 @@
 #include "BioFVM.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -224,16 +225,30 @@ struct DeviceState {
     bool host_newer = true;     // the mirror holds changes the device lacks
     bool device_newer = false;  // the device holds changes the mirror lacks
     bool dirichlet_dirty = true;
-    unsigned long long agents_seen = 0;  // sum of agent revisions last uploaded
+    // the agents last uploaded: the global agent revision then, and the list
+    // (its address, storage and length); any change to an agent's voxel or
+    // constants, or any agent created or deleted, moves the revision
+    unsigned long long agents_revision = 0;
+    const void* agents_list = nullptr;
+    const void* agents_data = nullptr;
     size_t agents_count = (size_t)-1;
     bool tuned = false;  // launch shapes picked (bfvm_autotune) for this handle
+    // bulk (BFVM_BULK) and cell (BFVM_CELLS) phases requested but not yet
+    // enqueued: they go with the next diffusion step, or alone before any
+    // density is read (pending_dt: the bulk phase's dt)
+    int pending = 0;
+    double pending_dt = 0;
+    // the device holds the bulk fields of these three callbacks
+    bool bulk_loaded = false;
+    void* bulk_callbacks[3] = {nullptr, nullptr, nullptr};
     std::vector<double> flat;  // staging for host<->device copies
 };
 
 // ---------------------------------------------------------------- Microenvironment
 
 Microenvironment::Microenvironment()
-    : spatial_units("none"), time_units("none"), name("unnamed"), device_precision(0), dev(nullptr)
+    : spatial_units("none"), time_units("none"), name("unnamed"), device_precision(0),
+      bulk_rates_every_step(false), bulk_rates_host_seconds(0.0), dev(nullptr)
 {
     diffusion_decay_solver = diffusion_decay_solver__constant_coefficients_LOD_3D;
     bulk_supply_rate_function = zero_function;
@@ -378,6 +393,7 @@ Voxel& Microenvironment::nearest_voxel(std::vector<double>& position)
 // (the caller may write through the returned reference).
 void Microenvironment::host_view(void)
 {
+    if (dev && dev->pending) flush_pending();
     if (dev && dev->device_newer && dev->h) {
         const size_t S = densities[0].size();
         dev->flat.resize(densities.size() * S);
@@ -391,6 +407,28 @@ void Microenvironment::host_view(void)
 }
 
 void Microenvironment::synchronize_host(void) { host_view(); }
+
+// The bulk and cell phases requested since the last device step, as one step.
+int Microenvironment::flush_pending(void)
+{
+    if (!dev || !dev->h || !dev->pending) return BFVM_OK;
+    const int flags = dev->pending;
+    dev->pending = 0;
+    int rc = bfvm_step(dev->h, dev->pending_dt, flags);
+    if (rc) report("simulate_bulk/cell_sources_and_sinks", rc, dev->h);
+    dev->device_newer = true;
+    return rc;
+}
+
+void Microenvironment::wait_for_device(void)
+{
+    if (!dev || !dev->h) return;
+    int rc = flush_pending();
+    if (!rc) rc = bfvm_sync(dev->h);
+    if (rc) report("wait_for_device", rc, dev->h);
+}
+
+bfvm_env* Microenvironment::device_handle(void) { return dev ? dev->h : nullptr; }
 
 std::vector<double>& Microenvironment::nearest_density_vector(std::vector<double>& position)
 {
@@ -506,6 +544,7 @@ void Microenvironment::compute_all_gradient_vectors(void)
     if (gradient_vectors.size() != nvox || gradient_vectors[0].size() != S) reset_all_gradient_vectors();
     DeviceState* d = device();
     int rc = d->h ? push_to_device() : BFVM_E_DEVICE;
+    if (!rc) rc = flush_pending();
     if (!rc) rc = bfvm_compute_gradients(d->h);
     std::vector<double> g(nvox * S * 3);
     if (!rc) rc = bfvm_download_gradients(d->h, g.data());
@@ -569,6 +608,7 @@ void Microenvironment::write_to_matlab(std::string filename)
 {
     DeviceState* d = device();
     int rc = d->h ? push_to_device() : BFVM_E_DEVICE;
+    if (!rc) rc = flush_pending();
     if (!rc)
         rc = bfvm_write_matlab(d->h, filename.c_str(), mesh.bounding_box[0], mesh.bounding_box[1],
                                mesh.bounding_box[2]);
@@ -638,12 +678,14 @@ void Microenvironment::simulate_diffusion_decay(double dt)
     }
 }
 
-// Per-step compatible bulk step: the three callbacks fill persistent per-voxel
-// vectors exactly as the reference's temps (a callback that writes only some
-// entries leaves the others as they were), then the device applies
-// rho = (rho + dt*(T*S)) / ((U+S)*dt + 1).
-void Microenvironment::simulate_bulk_sources_and_sinks(double dt)
+// The three bulk callbacks evaluated per voxel into persistent vectors exactly
+// as the reference's temps (a callback that writes only some entries leaves
+// the others as they were; BioFVM_microenvironment.cpp:595-600), then S, T
+// and U handed to the device, which builds dt*(T*S) and (U+S)*dt + 1 in the
+// reference's operation order.
+int Microenvironment::load_bulk_fields(void)
 {
+    const auto t0 = std::chrono::steady_clock::now();
     const size_t nvox = densities.size(), S = densities[0].size();
     if (bulk_temp_S.size() != nvox) {
         bulk_temp_S.assign(nvox, zero);
@@ -661,26 +703,79 @@ void Microenvironment::simulate_bulk_sources_and_sinks(double dt)
         std::memcpy(&ft[i * S], bulk_temp_T[i].data(), S * sizeof(double));
         std::memcpy(&fu[i * S], bulk_temp_U[i].data(), S * sizeof(double));
     }
+    int rc = bfvm_set_bulk(dev->h, fs.data(), ft.data(), fu.data());
+    if (!rc) {
+        dev->bulk_loaded = true;
+        dev->bulk_callbacks[0] = (void*)bulk_supply_rate_function;
+        dev->bulk_callbacks[1] = (void*)bulk_supply_target_densities_function;
+        dev->bulk_callbacks[2] = (void*)bulk_uptake_rate_function;
+    }
+    bulk_rates_host_seconds +=
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+// update_rates (BioFVM_microenvironment.cpp:635-656): the three fields per
+// voxel, supply_target_densities_times_supply_rates = T*S; the device's
+// bulk fields are evaluated again from the same callbacks.
+void Microenvironment::update_rates(void)
+{
+    const size_t nvox = densities.size();
+    if (supply_target_densities_times_supply_rates.size() != nvox)
+        supply_target_densities_times_supply_rates.assign(nvox, zero);
+    if (supply_rates.size() != nvox) supply_rates.assign(nvox, zero);
+    if (uptake_rates.size() != nvox) uptake_rates.assign(nvox, zero);
+    for (size_t i = 0; i < nvox; ++i) {
+        bulk_uptake_rate_function(this, (int)i, &uptake_rates[i]);
+        bulk_supply_rate_function(this, (int)i, &supply_rates[i]);
+        bulk_supply_target_densities_function(this, (int)i, &supply_target_densities_times_supply_rates[i]);
+        std::vector<double>& ts = supply_target_densities_times_supply_rates[i];
+        for (size_t s = 0; s < ts.size(); ++s) ts[s] *= supply_rates[i][s];
+    }
+    if (dev) dev->bulk_loaded = false;  // the next bulk step hands them to the device
+}
+
+// rho = (rho + dt*(T*S)) / ((U+S)*dt + 1) (BioFVM_microenvironment.cpp:584-613)
+// on the device.  The callbacks are evaluated at every call with
+// bulk_rates_every_step, otherwise only when the device lacks the fields of
+// the callbacks set now (see BioFVM.h).  The phase itself is enqueued with
+// the next diffusion step (or before any density is read).
+void Microenvironment::simulate_bulk_sources_and_sinks(double dt)
+{
     int rc = push_to_device();
-    if (!rc) rc = bfvm_set_bulk(dev->h, fs.data(), ft.data(), fu.data());
-    if (!rc) rc = bfvm_step(dev->h, dt, BFVM_BULK);
+    // phases run in the order they were requested: a bulk or cell phase
+    // already waiting goes first
+    if (!rc && (dev->pending & (BFVM_BULK | BFVM_CELLS))) rc = flush_pending();
     if (rc) {
         report("simulate_bulk_sources_and_sinks", rc, dev ? dev->h : nullptr);
         return;
     }
-    dev->device_newer = true;
+    // with every callback zero_function the update is rho = (rho + 0)/1
+    if (bulk_supply_rate_function == zero_function && bulk_supply_target_densities_function == zero_function &&
+        bulk_uptake_rate_function == zero_function)
+        return;
+    const bool same = dev->bulk_loaded && dev->bulk_callbacks[0] == (void*)bulk_supply_rate_function &&
+                      dev->bulk_callbacks[1] == (void*)bulk_supply_target_densities_function &&
+                      dev->bulk_callbacks[2] == (void*)bulk_uptake_rate_function;
+    if (bulk_rates_every_step || !same) rc = load_bulk_fields();
+    if (rc) {
+        report("simulate_bulk_sources_and_sinks", rc, dev->h);
+        return;
+    }
+    dev->pending |= BFVM_BULK;
+    dev->pending_dt = dt;
 }
 
-void Microenvironment::simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>& list, double)
+void Microenvironment::simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>& list, double dt)
 {
     int rc = push_to_device();
+    if (!rc && (dev->pending & BFVM_CELLS)) rc = flush_pending();
     if (rc) {
         report("simulate_cell_sources_and_sinks", rc, dev ? dev->h : nullptr);
         return;
     }
-    unsigned long long sum = 0;
-    for (Basic_Agent* a : list) sum += a->revision * 1469598103934665603ull + (unsigned long long)(size_t)a;
-    if (sum != dev->agents_seen || list.size() != dev->agents_count) {
+    if (g_agent_revision != dev->agents_revision || (const void*)&list != dev->agents_list ||
+        (const void*)list.data() != dev->agents_data || list.size() != dev->agents_count) {
         const size_t S = densities[0].size();
         std::vector<int> vox(list.size());
         std::vector<double> t1(list.size() * S), t2(list.size() * S);
@@ -693,14 +788,13 @@ void Microenvironment::simulate_cell_sources_and_sinks(std::vector<Basic_Agent*>
             report("bfvm_set_agents", rc, dev->h);
             return;
         }
-        dev->agents_seen = sum;
+        dev->agents_revision = g_agent_revision;
+        dev->agents_list = &list;
+        dev->agents_data = list.data();
         dev->agents_count = list.size();
     }
-    if ((rc = bfvm_step(dev->h, 0.0, BFVM_CELLS))) {
-        report("simulate_cell_sources_and_sinks", rc, dev->h);
-        return;
-    }
-    dev->device_newer = true;
+    if (!(dev->pending & BFVM_BULK)) dev->pending_dt = dt;
+    dev->pending |= BFVM_CELLS;
 }
 
 // The reference's global overload walks every agent of every microenvironment
@@ -749,6 +843,25 @@ static int tune_once(DeviceState* d, double dt)
     return bfvm_autotune(d->h, dt);
 }
 
+// One device step: the bulk and cell phases waiting (in the reference's order,
+// bulk -> cells -> diffusion) and the diffusion step, so that the sources run
+// fused into the x sweep; a bulk phase requested with another dt goes alone.
+static int step_with_pending(DeviceState* d, double dt)
+{
+    int flags = BFVM_DIFFUSION;
+    if (d->pending) {
+        if ((d->pending & BFVM_BULK) && d->pending_dt != dt) {
+            const int rc = bfvm_step(d->h, d->pending_dt, d->pending);
+            d->pending = 0;
+            if (rc) return rc;
+        } else {
+            flags |= d->pending;
+            d->pending = 0;
+        }
+    }
+    return bfvm_step(d->h, dt, flags);
+}
+
 // The GPU LOD_3D solver (replaces BioFVM_solvers.cpp:77-276).
 void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, double dt)
 {
@@ -765,7 +878,7 @@ void diffusion_decay_solver__constant_coefficients_LOD_3D(Microenvironment& M, d
         if (!rc) M.dev->lod_dims = 3;
     }
     if (!rc) rc = tune_once(M.dev, dt);
-    if (!rc) rc = bfvm_step(M.dev->h, dt, BFVM_DIFFUSION);
+    if (!rc) rc = step_with_pending(M.dev, dt);
     if (rc) {
         report("diffusion_decay_solver__constant_coefficients_LOD_3D", rc, M.dev ? M.dev->h : nullptr);
         return;
@@ -790,7 +903,7 @@ void diffusion_decay_solver__constant_coefficients_LOD_2D(Microenvironment& M, d
         if (!rc) M.dev->lod_dims = 2;
     }
     if (!rc) rc = tune_once(M.dev, dt);
-    if (!rc) rc = bfvm_step(M.dev->h, dt, BFVM_DIFFUSION);
+    if (!rc) rc = step_with_pending(M.dev, dt);
     if (rc) {
         report("diffusion_decay_solver__constant_coefficients_LOD_2D", rc, M.dev ? M.dev->h : nullptr);
         return;
@@ -896,6 +1009,7 @@ Basic_Agent* create_basic_agent(void)
     Basic_Agent* p = new Basic_Agent;
     all_basic_agents.push_back(p);
     p->index = (int)all_basic_agents.size() - 1;
+    ++g_agent_revision;
     return p;
 }
 
@@ -908,6 +1022,7 @@ void delete_basic_agent(int index)
     all_basic_agents[index] = last;
     all_basic_agents.pop_back();
     delete victim;
+    ++g_agent_revision;
 }
 
 void delete_basic_agent(Basic_Agent* p) { delete_basic_agent(p->index); }

@@ -184,3 +184,32 @@ def adjoint_small(seed=7, steps=6):
     case.update(dir_idx=idx, dir_val=rng.uniform(0.0, 1.0, idx.size * S),
                 dir_mask=np.array([1, 0], dtype=np.uint8))
     return case
+
+
+def write_driver_case(path, case, precision):
+    """Writes `case` in the input format of the façade driver
+    (tests/cpp/facade_driver.cpp, which reads it back through the reference's
+    API) for a store of `precision`; returns the number of values of the store."""
+    S = case["S"]
+    nvox = case["nx"] * case["ny"] * case["nz"]
+    n_dir = 0 if case.get("dir_idx") is None else len(case["dir_idx"])
+    n_ag = 0 if case.get("agent_volume") is None else len(case["agent_volume"])
+    has_bulk = case.get("bulk_supply") is not None
+    with open(path, "wb") as f:
+        phases = case["phases"] | (8 if case.get("lod_dims", 3) == 2 else 0)  # 8: the LOD_2D solver
+        np.array([case["nx"], case["ny"], case["nz"], S, case["steps"], phases, n_dir, n_ag,
+                  int(has_bulk), precision], dtype="<i4").tofile(f)
+        np.array([case["dx"], case["dt"]], dtype="<f8").tofile(f)
+        for k in ("D", "lam", "rho0"):
+            np.asarray(case[k], dtype="<f8").tofile(f)
+        if n_dir:
+            np.asarray(case["dir_idx"], dtype="<i4").tofile(f)
+            np.asarray(case["dir_val"], dtype="<f8").tofile(f)
+        if has_bulk:
+            for k in ("bulk_supply", "bulk_target", "bulk_uptake"):
+                np.asarray(case[k], dtype="<f8").tofile(f)
+        if n_ag:
+            for k in ("agent_pos", "agent_volume", "agent_secretion", "agent_saturation",
+                      "agent_uptake"):
+                np.asarray(case[k], dtype="<f8").tofile(f)
+    return nvox * S

@@ -176,17 +176,33 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
 
 /*
  * Autotuner: times every valid launch shape of the three segmented sweeps
- * (lines per block x nodes per segment x kernel) for this grid, substrate
- * count and precision on a scratch copy of the store, keeps the fastest per
- * axis (y and z: with either cache policy, bfvm_set_sweep_cache_policy),
- * then its XCD spread (bfvm_set_sweep_spread), then (with bulk fields
- * or Dirichlet nodes set and fused sources on) the one-block-per-tile shapes
- * of the x sweep with the sources fused in, then times the x/y plane chunk
- * sizes (bfvm_set_sweep_chunk).
- * The densities are left unchanged.  No counterpart in the reference (its
- * OpenMP loops have no launch shape).
+ * (lines per block x kernel) for this grid, substrate count and precision on
+ * a scratch copy of the store, keeps the fastest per axis (y and z: with
+ * either cache policy, bfvm_set_sweep_cache_policy), then its XCD spread
+ * (bfvm_set_sweep_spread), then (with bulk fields or Dirichlet nodes set and
+ * fused sources on) the one-block-per-tile shapes of the x sweep with the
+ * sources fused in, then times the x/y plane chunk sizes
+ * (bfvm_set_sweep_chunk).  The densities are left unchanged.  No counterpart
+ * in the reference (its OpenMP loops have no launch shape).
+ *
+ * Reproducibility: none of the knobs above changes a result bit, so results
+ * do not depend on which shape won the timing.  The one parameter that does
+ * change the rounding of the solve, the number of nodes per segment, stays at
+ * a fixed function of the line length, axis and store precision (32 for y and
+ * z lines of >= 400 nodes in an FP32 store, 16 for lines of >= 200 nodes, 8
+ * below) unless bfvm_set_autotune_segments(h, 1) lets the autotuner time the
+ * segment lengths too; then results may differ in their last bits between
+ * runs whose timings pick different lengths.
  */
 int bfvm_autotune(bfvm_handle h, double dt);
+int bfvm_set_autotune_segments(bfvm_handle h, int on);
+int bfvm_get_autotune_segments(bfvm_handle h, int* on);
+
+/* Number of times the handle built its Thomas/Woodbury tables (at the first
+ * diffusion step and after D, lambda, dt or the LOD dimension changed;
+ * bfvm_set_coefficients with the values already set changes nothing).  The
+ * reference builds them once per Microenvironment (BioFVM_solvers.cpp:87,172). */
+int bfvm_get_coefficient_builds(bfvm_handle h, unsigned* builds);
 
 /* The launch shape in use for one axis (0 = x, 1 = y, 2 = z):
  * lines per block, nodes per segment, and the kernel ("streaming"): 0 one
@@ -355,14 +371,20 @@ int bfvm_ensemble_sync(bfvm_ensemble e);
 int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph,
                             double* ms_total);
 
-/* Times the batched sweeps for each segment length (4, 8, 16, 32 nodes),
- * spread of their blocks over the XCDs (as dispatched, chunks of 64 tiles,
+/* Times the batched sweeps for each segment length (4, 8, 16, 32 nodes; only
+ * the ensemble's own unless bfvm_ensemble_set_autotune_segments), spread of
+ * their blocks over the XCDs (as dispatched, chunks of 64 tiles,
  * contiguous eighths), x tiles (rows or columns) and y/z cache policy
  * (default or non-temporal) on scratch copies of the stores and keeps the
  * fastest.  Densities unchanged. */
 int bfvm_ensemble_autotune(bfvm_ensemble e, double dt);
 int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment);
 int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment);
+/* Whether bfvm_ensemble_autotune also times the segment lengths (1), which
+ * changes the rounding of the batched solve, or keeps the ensemble's segment
+ * length (0, the default: 8 nodes, or what bfvm_ensemble_set_segment set) and
+ * tunes only the knobs that leave the bits unchanged. */
+int bfvm_ensemble_set_autotune_segments(bfvm_ensemble e, int on);
 /* Whether the members' x sweeps run on LDS-staged row tiles (on = 1, the
  * default; members that cannot take part run with column tiles) or all on
  * column tiles (0).  Row tiles load whole x rows contiguously, which matters

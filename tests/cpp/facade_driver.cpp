@@ -4,6 +4,7 @@
 // bulk callbacks and agents, then loop bulk -> cells -> diffusion.
 //
 //   facade_driver <case.bin> <out.bin> [<grad.bin> <snapshot.mat>]
+//   facade_driver <case.bin> <out.bin> --time W [--bulk-every-step]
 //
 // case.bin (little endian): int32 nx ny nz S steps phases n_dir n_agents
 // has_bulk precision; float64 dx dt; D[S] lambda[S]; rho0[nvox*S];
@@ -15,12 +16,21 @@
 // Halfway through, every density is read through density_vector() (which
 // synchronises the host mirror) and written back unchanged, so the result
 // also covers the mirror's download/upload path.
+//
+// With --time W the driver runs W steps untimed (the first LOD call picks its
+// launch shapes), then the case's steps timed with a steady clock until the
+// device has finished them (no density is read in between), and prints one
+// JSON line: seconds, ms per step, host seconds spent on bulk callbacks, and
+// how often the handle built its Thomas tables.  --bulk-every-step evaluates
+// the bulk callbacks at every step, as the reference does, instead of once.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
 #include "BioFVM.h"
+#include "biofvm_amd.h"
 
 using namespace BioFVM;
 
@@ -43,6 +53,19 @@ void rd(FILE* f, T* p, size_t n)
 
 int main(int argc, char** argv)
 {
+    int warmup = -1;  // --time W: timing mode
+    bool every_step = false;
+    {
+        std::vector<char*> pos;
+        for (int i = 0; i < argc; ++i) {
+            const std::string a = argv[i];
+            if (a == "--time" && i + 1 < argc) warmup = std::atoi(argv[++i]);
+            else if (a == "--bulk-every-step") every_step = true;
+            else pos.push_back(argv[i]);
+        }
+        argc = (int)pos.size();
+        for (int i = 0; i < argc; ++i) argv[i] = pos[i];
+    }
     if (argc != 3 && argc != 5 && argc != 7) {
         fprintf(stderr, "usage: facade_driver case.bin out.bin [grad.bin snapshot.mat [mcds_base flags]]\n"
                         "  flags: m = mesh as MAT, x = data as CSV lines, - = neither\n");
@@ -52,8 +75,9 @@ int main(int argc, char** argv)
     if (!f) return 2;
     int hdr[10];
     rd(f, hdr, 10);
-    const int nx = hdr[0], ny = hdr[1], nz = hdr[2], S = hdr[3], steps = hdr[4], phases = hdr[5],
+    const int nx = hdr[0], ny = hdr[1], nz = hdr[2], S = hdr[3], phases = hdr[5],
               n_dir = hdr[6], n_agents = hdr[7], has_bulk = hdr[8], precision = hdr[9];
+    int steps = hdr[4];
     double dd[2];
     rd(f, dd, 2);
     const double dx = dd[0], dt = dd[1];
@@ -83,6 +107,7 @@ int main(int argc, char** argv)
 
     Microenvironment M;
     M.device_precision = precision;
+    M.bulk_rates_every_step = every_step;
     M.name = "microenvironment";
     M.set_density(0, "s0", "dimensionless");
     for (int s = 1; s < S; ++s) M.add_density("s" + std::to_string(s), "mmHg");
@@ -122,6 +147,28 @@ int main(int argc, char** argv)
         }
         p->set_internal_uptake_constants(dt);
     }
+    auto step = [&] {
+        if ((phases & 1) && has_bulk) M.simulate_bulk_sources_and_sinks(dt);
+        if (phases & 2) M.simulate_cell_sources_and_sinks(dt);
+        if (phases & 4) M.simulate_diffusion_decay(dt);
+    };
+    if (warmup >= 0) {
+        for (int n = 0; n < warmup; ++n) step();
+        M.wait_for_device();
+        const double host0 = M.bulk_rates_host_seconds;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int n = 0; n < steps; ++n) step();
+        M.wait_for_device();
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        unsigned builds = 0;
+        bfvm_get_coefficient_builds(M.device_handle(), &builds);
+        printf("{\"steps\": %d, \"warmup\": %d, \"seconds\": %.6f, \"ms_per_step\": %.6f, "
+               "\"bulk_rates_host_seconds\": %.6f, \"bulk_every_step\": %s, \"coefficient_builds\": %u}\n",
+               steps, warmup, secs, 1e3 * secs / steps, M.bulk_rates_host_seconds - host0,
+               every_step ? "true" : "false", builds);
+        fflush(stdout);
+        steps = 0;  // the timed steps are done; write the result below
+    }
     for (int n = 0; n < steps; ++n) {
         if (n == steps / 2) {
             for (size_t v = 0; v < nvox; ++v) {
@@ -129,9 +176,7 @@ int main(int argc, char** argv)
                 for (int s = 0; s < S; ++s) d[s] = d[s] + 0.0;
             }
         }
-        if ((phases & 1) && has_bulk) M.simulate_bulk_sources_and_sinks(dt);
-        if (phases & 2) M.simulate_cell_sources_and_sinks(dt);
-        if (phases & 4) M.simulate_diffusion_decay(dt);
+        step();
     }
     FILE* o = fopen(argv[2], "wb");
     if (!o) return 2;

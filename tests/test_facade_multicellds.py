@@ -17,7 +17,7 @@ import oracle
 from biofvm_amd import FP64
 from biofvm_amd import workloads as W
 
-from test_gpu_facade import DRIVER, write_case
+from test_gpu_facade import DRIVER
 
 pytestmark = pytest.mark.skipif(not (oracle.have_ref() and os.path.exists(DRIVER)),
                                 reason="compiled reference or façade driver not built")
@@ -44,7 +44,7 @@ def _files(base, flags):
 def _snapshots(tmp_path, case, flags):
     base = str(tmp_path / "snap")
     cin, cout = tmp_path / "case.bin", tmp_path / "out.bin"
-    write_case(cin, case, FP64)
+    W.write_driver_case(cin, case, FP64)
     r = subprocess.run([DRIVER, str(cin), str(cout), "-", "-", base, flags], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr

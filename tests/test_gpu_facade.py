@@ -5,7 +5,10 @@ API (set_density/add_density, resize_space_uniform, density_vector,
 add_dirichlet_node, bulk callbacks, create_basic_agent/assign_position/
 set_internal_uptake_constants) and loops bulk -> cells -> diffusion; its
 output must match the compiled reference's fixtures (FP64 <= 1e-12) and the
-oracle (FP32 store <= 1e-5).
+oracle (FP32 store <= 1e-5).  In its timing mode (--time) the same loop runs
+without reading densities in between: the Thomas tables are built once for
+the whole loop, and the bulk fields kept on the device (the default) give the
+same bits as the callbacks evaluated at every step (--bulk-every-step).
 """
 import os
 import subprocess
@@ -16,6 +19,7 @@ import pytest
 import gen_golden
 import oracle
 from biofvm_amd import FP32, FP64
+from biofvm_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -23,35 +27,9 @@ DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "facade
 TOL = {FP64: 1e-12, FP32: 1e-5}
 
 
-def write_case(path, case, precision):
-    S = case["S"]
-    nvox = case["nx"] * case["ny"] * case["nz"]
-    n_dir = 0 if case.get("dir_idx") is None else len(case["dir_idx"])
-    n_ag = 0 if case.get("agent_volume") is None else len(case["agent_volume"])
-    has_bulk = case.get("bulk_supply") is not None
-    with open(path, "wb") as f:
-        phases = case["phases"] | (8 if case.get("lod_dims", 3) == 2 else 0)  # 8: the LOD_2D solver
-        np.array([case["nx"], case["ny"], case["nz"], S, case["steps"], phases, n_dir, n_ag,
-                  int(has_bulk), precision], dtype="<i4").tofile(f)
-        np.array([case["dx"], case["dt"]], dtype="<f8").tofile(f)
-        for k in ("D", "lam", "rho0"):
-            np.asarray(case[k], dtype="<f8").tofile(f)
-        if n_dir:
-            np.asarray(case["dir_idx"], dtype="<i4").tofile(f)
-            np.asarray(case["dir_val"], dtype="<f8").tofile(f)
-        if has_bulk:
-            for k in ("bulk_supply", "bulk_target", "bulk_uptake"):
-                np.asarray(case[k], dtype="<f8").tofile(f)
-        if n_ag:
-            for k in ("agent_pos", "agent_volume", "agent_secretion", "agent_saturation",
-                      "agent_uptake"):
-                np.asarray(case[k], dtype="<f8").tofile(f)
-    return nvox * S
-
-
 def run_driver(tmp_path, case, precision):
     cin, cout = tmp_path / "case.bin", tmp_path / "out.bin"
-    n = write_case(cin, case, precision)
+    n = W.write_driver_case(cin, case, precision)
     r = subprocess.run([DRIVER, str(cin), str(cout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     out = np.fromfile(cout, dtype="<f8")
@@ -70,7 +48,6 @@ def test_facade_matches_reference_fixture(tmp_path, name, precision):
 
 
 def test_facade_many_agents_per_voxel(tmp_path):
-    from biofvm_amd import workloads as W
     rng = np.random.default_rng(21)
     n = 12
     case = W.make_case(n, n, n, 2, 10.0, [1e4, 1e2], [0.1, 0.01], 0.01, 6, W.ALL,
@@ -89,7 +66,7 @@ def test_facade_gradients_and_matlab_snapshot(tmp_path, precision):
     from biofvm_amd.matlab import read_matlab
     case, _, _ = gen_golden.load("g8_shell")
     cin, cout, cgrad, cmat = (tmp_path / n for n in ("case.bin", "out.bin", "grad.bin", "snap.mat"))
-    write_case(cin, case, precision)
+    W.write_driver_case(cin, case, precision)
     r = subprocess.run([DRIVER, str(cin), str(cout), str(cgrad), str(cmat)], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -114,7 +91,6 @@ def test_facade_gradients_and_matlab_snapshot(tmp_path, precision):
 @pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
 def test_facade_lod2d_solver(tmp_path, precision):
     # diffusion_decay_solver__constant_coefficients_LOD_2D assigned as the examples do
-    from biofvm_amd import workloads as W
     rng = np.random.default_rng(23)
     nx, ny, S = 17, 11, 2
     case = W.make_case(nx, ny, 1, S, 20.0, [3e4, 8e2], [0.2, 0.02], 0.01, 5, W.ALL, rng.uniform(0, 1, nx * ny * S))
@@ -127,3 +103,35 @@ def test_facade_lod2d_solver(tmp_path, precision):
     ref = oracle.oracle_run(case)
     got = run_driver(tmp_path, case, precision)
     assert float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))) <= TOL[precision]
+
+
+def shell_case(steps):
+    case, _, _ = gen_golden.load("g8_shell")  # every phase: Dirichlet, bulk, agents
+    case = dict(case)
+    case["steps"] = steps
+    return case
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_facade_steady_loop_keeps_its_tables_and_bulk_fields(tmp_path, precision):
+    import json
+    warm, steps = 3, 12
+    case = shell_case(steps)
+    cin = tmp_path / "case.bin"
+    W.write_driver_case(cin, case, precision)
+    outs, lines = [], []
+    for mode in ([], ["--bulk-every-step"]):
+        cout = tmp_path / f"out{len(outs)}.bin"
+        r = subprocess.run([DRIVER, str(cin), str(cout), "--time", str(warm), *mode], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        lines.append(json.loads(r.stdout.strip().splitlines()[-1]))
+        outs.append(np.fromfile(cout, dtype="<f8"))
+    assert lines[0]["coefficient_builds"] == 1 and lines[1]["coefficient_builds"] == 1, lines
+    assert lines[0]["bulk_rates_host_seconds"] == 0.0  # fields evaluated once, during the warm-up
+    assert lines[1]["bulk_rates_host_seconds"] > 0.0
+    np.testing.assert_array_equal(outs[0], outs[1])
+    ref_case = dict(case)
+    ref_case["steps"] = warm + steps
+    ref = oracle.oracle_run(ref_case)
+    assert float(np.max(np.abs(outs[0] - ref)) / np.max(np.abs(ref))) <= TOL[precision]

@@ -321,6 +321,60 @@ def test_cache_policy_never_changes_results(precision):
     assert rel_err(outs[0], oracle.oracle_run(case)) <= TOL[precision]
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_cache_policy_of_every_shape_never_changes_results(precision):
+    # every y and z shape whose tiles' row pieces are whole 128-byte lines,
+    # one block per tile and streaming: non-temporal = default, bit for bit
+    rng = np.random.default_rng(43)
+    nx, ny, nz, S = 24, 44, 37, 4
+    case = W.make_case(nx, ny, nz, S, 10.0, [1e5, 2e3, 3e4, 500.0], [0.1, 0.3, 0.01, 1.0], 0.01, 2,
+                       W.DIFFUSION, rng.uniform(0, 1, nx * ny * nz * S))
+    elem = 8 if precision == FP64 else 4
+    tried = 0
+    with setup_case(case, precision) as m:
+        for axis, n in ((1, ny), (2, nz)):
+            for cols, L, streaming in _plans(n, precision):
+                if cols * elem < 128:
+                    continue
+                outs = []
+                for nt in (0, 1):
+                    m.upload(case["rho0"])
+                    m.set_sweep_plan(axis, cols, L, streaming)
+                    m.set_sweep_cache_policy(axis, nt)
+                    assert m.sweep_cache_policies()[axis] == nt
+                    m.run(case["dt"], case["steps"], case["phases"])
+                    outs.append(m.download())
+                np.testing.assert_array_equal(outs[1], outs[0], err_msg=str((axis, cols, L, streaming)))
+                tried += 1
+            m.set_sweep_plan(axis, 0, 0)
+    assert tried >= 20
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_autotuned_results_do_not_depend_on_timing(precision):
+    # the autotuner keeps the segment length fixed, so two handles tuned
+    # separately (whatever shapes their timings pick) give the same bits as
+    # the untuned default shapes
+    rng = np.random.default_rng(45)
+    nx, ny, nz, S = 64, 48, 40, 3
+    case = W.make_case(nx, ny, nz, S, 10.0, [1e5, 2e3, 3e4], [0.1, 0.3, 0.01], 0.01, 3, W.ALL,
+                       rng.uniform(0, 1, nx * ny * nz * S))
+    case.update(W.random_bulk(rng, nx * ny * nz, S))
+    case.update(W.random_agents(rng, 400, S, nx, ny, nz, 10.0, share_voxel=6))
+    idx = W.outer_face_voxels(nx, ny, nz)
+    case.update(dir_idx=idx, dir_val=np.tile([1.0, 0.0, 0.5], idx.size))
+    outs = []
+    for tune in (False, True, True):
+        with setup_case(case, precision) as m:
+            if tune:
+                m.autotune(case["dt"])
+            m.run(case["dt"], case["steps"], case["phases"])
+            outs.append(m.download())
+    np.testing.assert_array_equal(outs[1], outs[0])
+    np.testing.assert_array_equal(outs[2], outs[0])
+    assert rel_err(outs[0], oracle.oracle_run(case)) <= TOL[precision]
+
+
 def test_unrunnable_launch_shapes_are_refused():
     with DeviceMicroenvironment(40, 40, 300, 2, 10.0, FP64) as m:
         with pytest.raises(BfvmError):
