@@ -70,6 +70,8 @@ def build_facade(verbose=False):
     cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall",
            f"-I{INCLUDE}", f"-I{FACADE_DIR}", os.path.join(FACADE_DIR, "BioFVM_gpu.cpp"),
            os.path.join(FACADE_DIR, "BioFVM_MultiCellDS_gpu.cpp"),
+           os.path.join(FACADE_DIR, "BioFVM_vector_utilities.cpp"),
+           os.path.join(FACADE_DIR, "BioFVM_matlab_io.cpp"),
            f"-L{HERE}", "-lbiofvm_amd", "-Wl,-rpath,$ORIGIN", "-o", tmp]
     if verbose:
         print(" ".join(shlex.quote(c) for c in cmd), flush=True)

@@ -549,6 +549,7 @@ extern "C" {
 
 int bfvm_tape_record(bfvm_handle h, double dt, int steps, int flags, int checkpoint_every)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     Env& e = *h;
     int rc;
@@ -581,6 +582,7 @@ int bfvm_tape_record(bfvm_handle h, double dt, int steps, int flags, int checkpo
 
 int bfvm_tape_seed(bfvm_handle h, const double* seed)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     Env& e = *h;
     Tape* tp = e.tape;
@@ -595,6 +597,7 @@ int bfvm_tape_seed(bfvm_handle h, const double* seed)
 
 int bfvm_tape_seed_l2(bfvm_handle h, const double* target, double* loss)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     Env& e = *h;
     Tape* tp = e.tape;
@@ -634,6 +637,7 @@ int bfvm_tape_seed_l2(bfvm_handle h, const double* target, double* loss)
 int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double* grad_lambda,
                        double* grad_secretion, double* grad_uptake)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     Env& e = *h;
     Tape* tp = e.tape;
@@ -725,6 +729,7 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
 
 int bfvm_tape_free(bfvm_handle h)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     bfvm::free_tape(*h);
     return BFVM_OK;

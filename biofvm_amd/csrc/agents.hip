@@ -135,6 +135,7 @@ int bfvm_set_agent_rates_device(bfvm_handle h, int n, const double* position, co
                                 const double* secretion, const double* saturation, const double* uptake,
                                 double dt, int* out_voxel)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     Env& e = *h;
     if (n < 0) return afail(e, BFVM_E_ARG, "n < 0");

@@ -4,12 +4,29 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <string>
 #include <vector>
 
 #include "biofvm_amd.h"
 
 namespace bfvm {
+
+// Runs the enclosing scope with a private state of the C library's random()
+// generator (which rand() shares), restoring the caller's state on exit.  The
+// HIP runtime seeds that generator from the clock when it first loads a code
+// object (at the first launch of a kernel of each source file), so without
+// it a program's rand() sequence would depend on when this library first
+// ran a kernel.  Every entry point of the C ABI holds one.  (Other threads
+// that call rand() while an entry point runs draw from the private state.)
+struct RandGuard {
+    char state[8];
+    char* saved;
+    RandGuard() : saved(initstate(1u, state, sizeof state)) {}
+    ~RandGuard() { setstate(saved); }
+    RandGuard(const RandGuard&) = delete;
+    RandGuard& operator=(const RandGuard&) = delete;
+};
 
 // Message of a failed HIP call.  A failed runtime call (an allocation the
 // device refuses, say) also stays the thread's last error, which the next

@@ -440,6 +440,7 @@ extern "C" {
 int bfvm_create_mesh(int nx, int ny, int nz, int n_substrates, double dx, double dy, double dz,
                      int precision, bfvm_handle* out)
 {
+    bfvm::RandGuard rand_guard;
     if (!out) return fail(nullptr, BFVM_E_ARG, "out is NULL");
     *out = nullptr;
     if (nx < 1 || ny < 1 || nz < 1 || n_substrates < 1)
@@ -489,11 +490,13 @@ int bfvm_create_mesh(int nx, int ny, int nz, int n_substrates, double dx, double
 
 int bfvm_create(int nx, int ny, int nz, int n_substrates, double dx, int precision, bfvm_handle* out)
 {
+    bfvm::RandGuard rand_guard;
     return bfvm_create_mesh(nx, ny, nz, n_substrates, dx, dx, dx, precision, out);
 }
 
 int bfvm_destroy(bfvm_handle h)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_OK;
     if (h->ensemble_stream)
         return fail(h, BFVM_E_STATE, "the handle belongs to an ensemble: destroy the ensemble first");
@@ -532,6 +535,7 @@ int bfvm_destroy(bfvm_handle h)
 
 int bfvm_set_coefficients(bfvm_handle h, const double* D, const double* lambda)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (!D || !lambda) return fail(h, BFVM_E_ARG, "D and lambda must be non-NULL");
     // the same values as before keep the tables (a caller that sets them
@@ -548,6 +552,7 @@ int bfvm_set_coefficients(bfvm_handle h, const double* D, const double* lambda)
 
 int bfvm_upload_density(bfvm_handle h, const double* host)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (!host) return fail(h, BFVM_E_ARG, "host is NULL");
     HIP_OR_FAIL(h, quiesce(*h));
@@ -557,6 +562,7 @@ int bfvm_upload_density(bfvm_handle h, const double* host)
 
 int bfvm_download_density(bfvm_handle h, double* host)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (!host) return fail(h, BFVM_E_ARG, "host is NULL");
     HIP_OR_FAIL(h, quiesce(*h));
@@ -573,6 +579,7 @@ int bfvm_download_density(bfvm_handle h, double* host)
 int bfvm_set_dirichlet(bfvm_handle h, int n, const int* voxel_index, const double* values,
                        const unsigned char* mask)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (n < 0) return fail(h, BFVM_E_ARG, "n < 0");
     HIP_OR_FAIL(h, quiesce(*h));
@@ -625,6 +632,7 @@ int bfvm_set_dirichlet(bfvm_handle h, int n, const int* voxel_index, const doubl
 int bfvm_set_bulk(bfvm_handle h, const double* supply_rate, const double* supply_target,
                   const double* uptake_rate)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     HIP_OR_FAIL(h, quiesce(*h));
     bool none = !supply_rate && !supply_target && !uptake_rate;
@@ -659,6 +667,7 @@ int bfvm_set_bulk(bfvm_handle h, const double* supply_rate, const double* supply
 int bfvm_set_agents(bfvm_handle h, int n, const int* voxel_index, const double* temp1,
                     const double* temp2)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (n < 0) return fail(h, BFVM_E_ARG, "n < 0");
     if (n > 0 && (!voxel_index || !temp1 || !temp2))
@@ -671,6 +680,7 @@ int bfvm_set_agent_rates(bfvm_handle h, int n, const double* position, const dou
                          const double* secretion, const double* saturation, const double* uptake,
                          double dt, int* out_voxel)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (n < 0) return fail(h, BFVM_E_ARG, "n < 0");
     if (n > 0 && (!position || !volume || !secretion || !saturation || !uptake))
@@ -703,6 +713,7 @@ int bfvm_set_agent_rates(bfvm_handle h, int n, const double* position, const dou
 
 int bfvm_step(bfvm_handle h, double dt, int flags)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (h->ensemble_stream) HIP_OR_FAIL(h, hipStreamSynchronize(h->ensemble_stream));
     return step_impl(*h, dt, flags);
@@ -710,6 +721,7 @@ int bfvm_step(bfvm_handle h, double dt, int flags)
 
 int bfvm_run(bfvm_handle h, double dt, int steps, int flags)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (steps < 0) return fail(h, BFVM_E_ARG, "steps < 0");
     if (h->ensemble_stream) HIP_OR_FAIL(h, hipStreamSynchronize(h->ensemble_stream));
@@ -722,6 +734,7 @@ int bfvm_run(bfvm_handle h, double dt, int steps, int flags)
 
 int bfvm_sync(bfvm_handle h)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     HIP_OR_FAIL(h, quiesce(*h));
     return BFVM_OK;
@@ -730,6 +743,7 @@ int bfvm_sync(bfvm_handle h)
 int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_total,
                    double* per_kernel_ms, int* launches)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (steps < 0) return fail(h, BFVM_E_ARG, "steps < 0");
     Env& e = *h;
@@ -785,6 +799,7 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
 
 int bfvm_autotune(bfvm_handle h, double dt)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     Env& e = *h;
     if (e.D.empty()) return fail(&e, BFVM_E_STATE, "coefficients not set (bfvm_set_coefficients)");
@@ -912,6 +927,7 @@ int bfvm_autotune(bfvm_handle h, double dt)
 
 int bfvm_set_autotune_segments(bfvm_handle h, int on)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     h->tune_segments = on != 0;
     return BFVM_OK;
@@ -919,6 +935,7 @@ int bfvm_set_autotune_segments(bfvm_handle h, int on)
 
 int bfvm_get_autotune_segments(bfvm_handle h, int* on)
 {
+    bfvm::RandGuard rand_guard;
     if (!h || !on) return fail(h, BFVM_E_ARG, "null argument");
     *on = h->tune_segments ? 1 : 0;
     return BFVM_OK;
@@ -926,6 +943,7 @@ int bfvm_get_autotune_segments(bfvm_handle h, int* on)
 
 int bfvm_get_coefficient_builds(bfvm_handle h, unsigned* builds)
 {
+    bfvm::RandGuard rand_guard;
     if (!h || !builds) return fail(h, BFVM_E_ARG, "null argument");
     *builds = h->coef_builds;
     return BFVM_OK;
@@ -934,6 +952,7 @@ int bfvm_get_coefficient_builds(bfvm_handle h, unsigned* builds)
 int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment,
                         int* streaming)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
     bfvm::SweepPlan p = bfvm::effective_plan(*h, axis);
@@ -946,6 +965,7 @@ int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* node
 int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment,
                         int streaming)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
     if (axis == 0) h->fused_plan = bfvm::SweepPlan{};  // the fused x sweep follows the x shape set here
@@ -964,6 +984,7 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
 
 int bfvm_get_sweep_cache_policy(bfvm_handle h, int axis, int* non_temporal)
 {
+    bfvm::RandGuard rand_guard;
     if (!h || !non_temporal) return fail(h, BFVM_E_ARG, "null argument");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
     const bfvm::SweepPlan p = bfvm::effective_plan(*h, axis);
@@ -973,6 +994,7 @@ int bfvm_get_sweep_cache_policy(bfvm_handle h, int axis, int* non_temporal)
 
 int bfvm_set_sweep_cache_policy(bfvm_handle h, int axis, int non_temporal)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
     bfvm::SweepPlan p = bfvm::effective_plan(*h, axis);
@@ -989,6 +1011,7 @@ int bfvm_set_sweep_cache_policy(bfvm_handle h, int axis, int non_temporal)
 
 int bfvm_set_fused_sources(bfvm_handle h, int on)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     HIP_OR_FAIL(h, quiesce(*h));
     h->fused_sources = on != 0;
@@ -997,6 +1020,7 @@ int bfvm_set_fused_sources(bfvm_handle h, int on)
 
 int bfvm_get_fused_sources(bfvm_handle h, int* on)
 {
+    bfvm::RandGuard rand_guard;
     if (!h || !on) return fail(h, BFVM_E_ARG, "null argument");
     *on = h->fused_sources ? 1 : 0;
     return BFVM_OK;
@@ -1004,6 +1028,7 @@ int bfvm_get_fused_sources(bfvm_handle h, int* on)
 
 int bfvm_get_sweep_spread(bfvm_handle h, int axis, int* tiles_per_chunk)
 {
+    bfvm::RandGuard rand_guard;
     if (!h || !tiles_per_chunk) return fail(h, BFVM_E_ARG, "null argument");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
     *tiles_per_chunk = h->spread[axis];
@@ -1012,6 +1037,7 @@ int bfvm_get_sweep_spread(bfvm_handle h, int axis, int* tiles_per_chunk)
 
 int bfvm_set_sweep_spread(bfvm_handle h, int axis, int tiles_per_chunk)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (axis < 0 || axis > 2) return fail(h, BFVM_E_ARG, "axis must be 0, 1 or 2");
     if (tiles_per_chunk < -1) return fail(h, BFVM_E_ARG, "tiles_per_chunk must be -1, 0 or positive");
@@ -1023,6 +1049,7 @@ int bfvm_set_sweep_spread(bfvm_handle h, int axis, int tiles_per_chunk)
 
 int bfvm_compute_gradients(bfvm_handle h)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (h->ensemble_stream) HIP_OR_FAIL(h, hipStreamSynchronize(h->ensemble_stream));
     if (!h->grad) HIP_OR_FAIL(h, hipMalloc(&h->grad, h->nvs * 3 * h->elem));
@@ -1032,6 +1059,7 @@ int bfvm_compute_gradients(bfvm_handle h)
 
 int bfvm_download_gradients(bfvm_handle h, double* host)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (!host) return fail(h, BFVM_E_ARG, "host is NULL");
     if (!h->grad) return fail(h, BFVM_E_STATE, "no gradients computed (bfvm_compute_gradients)");
@@ -1049,6 +1077,7 @@ int bfvm_download_gradients(bfvm_handle h, double* host)
 
 int bfvm_gradient_device_ptr(bfvm_handle h, void** ptr)
 {
+    bfvm::RandGuard rand_guard;
     if (!h || !ptr) return fail(h, BFVM_E_ARG, "null argument");
     *ptr = h->grad;
     return BFVM_OK;
@@ -1056,6 +1085,7 @@ int bfvm_gradient_device_ptr(bfvm_handle h, void** ptr)
 
 int bfvm_set_lod_dimensions(bfvm_handle h, int dims)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (dims != 2 && dims != 3) return fail(h, BFVM_E_ARG, "dims must be 2 or 3");
     if (dims == 2 && h->nz != 1) return fail(h, BFVM_E_MESH, "LOD_2D needs a mesh with nz == 1");
@@ -1069,6 +1099,7 @@ int bfvm_set_lod_dimensions(bfvm_handle h, int dims)
 
 int bfvm_get_sweep_chunk(bfvm_handle h, int* planes)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (planes) *planes = h->xy_chunk;
     return BFVM_OK;
@@ -1076,6 +1107,7 @@ int bfvm_get_sweep_chunk(bfvm_handle h, int* planes)
 
 int bfvm_set_sweep_chunk(bfvm_handle h, int planes)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (planes < 0) return fail(h, BFVM_E_ARG, "planes < 0");
     h->xy_chunk = planes;
@@ -1085,6 +1117,7 @@ int bfvm_set_sweep_chunk(bfvm_handle h, int planes)
 int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, int* precision,
                   size_t* density_bytes)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (nx) *nx = h->nx;
     if (ny) *ny = h->ny;
@@ -1097,6 +1130,7 @@ int bfvm_get_info(bfvm_handle h, int* nx, int* ny, int* nz, int* n_substrates, i
 
 int bfvm_density_device_ptr(bfvm_handle h, void** ptr)
 {
+    bfvm::RandGuard rand_guard;
     if (!h || !ptr) return fail(h, BFVM_E_ARG, "null argument");
     *ptr = h->rho;
     return BFVM_OK;
@@ -1104,6 +1138,7 @@ int bfvm_density_device_ptr(bfvm_handle h, void** ptr)
 
 const char* bfvm_last_error(bfvm_handle h)
 {
+    bfvm::RandGuard rand_guard;
     return h ? h->err.c_str() : g_thread_err.c_str();
 }
 

@@ -326,6 +326,7 @@ extern "C" {
 
 int bfvm_ensemble_create(int count, const bfvm_handle* members, bfvm_ensemble* out)
 {
+    bfvm::RandGuard rand_guard;
     if (!out) return BFVM_E_ARG;
     *out = nullptr;
     if (count < 1 || !members) return BFVM_E_ARG;
@@ -355,6 +356,7 @@ int bfvm_ensemble_create(int count, const bfvm_handle* members, bfvm_ensemble* o
 
 int bfvm_ensemble_destroy(bfvm_ensemble e)
 {
+    bfvm::RandGuard rand_guard;
     if (!e) return BFVM_OK;
     (void)hipStreamSynchronize(e->stream);
     for (Env* pm : e->m) pm->ensemble_stream = nullptr;
@@ -367,6 +369,7 @@ int bfvm_ensemble_destroy(bfvm_ensemble e)
 
 int bfvm_ensemble_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph)
 {
+    bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     if (steps < 0) return efail(e, BFVM_E_ARG, "steps < 0");
     int rc = ready(*e, dt, flags, use_graph);
@@ -375,6 +378,7 @@ int bfvm_ensemble_run(bfvm_ensemble e, double dt, int steps, int flags, int use_
 
 int bfvm_ensemble_sync(bfvm_ensemble e)
 {
+    bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     EHIP(e, hipStreamSynchronize(e->stream));
     return BFVM_OK;
@@ -383,6 +387,7 @@ int bfvm_ensemble_sync(bfvm_ensemble e)
 int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph,
                             double* ms)
 {
+    bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     if (steps < 0) return efail(e, BFVM_E_ARG, "steps < 0");
     int rc = ready(*e, dt, flags, use_graph);
@@ -405,6 +410,7 @@ int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, in
 
 int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
 {
+    bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     // time the three batched sweeps for each segment length on scratch copies
     for (Env* pm : e->m) EHIP(e, hipStreamSynchronize(pm->stream));
@@ -475,6 +481,7 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
 
 int bfvm_ensemble_set_autotune_segments(bfvm_ensemble e, int on)
 {
+    bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     e->tune_segments = on != 0;
     return BFVM_OK;
@@ -482,6 +489,7 @@ int bfvm_ensemble_set_autotune_segments(bfvm_ensemble e, int on)
 
 int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment)
 {
+    bfvm::RandGuard rand_guard;
     if (!e || !nodes_per_segment) return BFVM_E_ARG;
     *nodes_per_segment = e->L;
     return BFVM_OK;
@@ -489,6 +497,7 @@ int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment)
 
 int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment)
 {
+    bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     if (nodes_per_segment != 4 && nodes_per_segment != 8 && nodes_per_segment != 16 &&
         nodes_per_segment != 32)
@@ -501,6 +510,7 @@ int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment)
 
 int bfvm_ensemble_set_row_tiles(bfvm_ensemble e, int on)
 {
+    bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     e->rows_x = on ? 1 : 0;
     e->tables_valid = false;
@@ -510,6 +520,7 @@ int bfvm_ensemble_set_row_tiles(bfvm_ensemble e, int on)
 
 int bfvm_ensemble_get_row_tiles(bfvm_ensemble e, int* on)
 {
+    bfvm::RandGuard rand_guard;
     if (!e || !on) return BFVM_E_ARG;
     *on = e->rows_x;
     return BFVM_OK;
@@ -517,6 +528,7 @@ int bfvm_ensemble_get_row_tiles(bfvm_ensemble e, int* on)
 
 int bfvm_ensemble_set_cache_policy(bfvm_ensemble e, int non_temporal)
 {
+    bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     e->nt = non_temporal ? 1 : 0;
     drop_graph(*e);
@@ -525,6 +537,7 @@ int bfvm_ensemble_set_cache_policy(bfvm_ensemble e, int non_temporal)
 
 int bfvm_ensemble_get_cache_policy(bfvm_ensemble e, int* non_temporal)
 {
+    bfvm::RandGuard rand_guard;
     if (!e || !non_temporal) return BFVM_E_ARG;
     *non_temporal = e->nt;
     return BFVM_OK;
@@ -532,6 +545,7 @@ int bfvm_ensemble_get_cache_policy(bfvm_ensemble e, int* non_temporal)
 
 const char* bfvm_ensemble_last_error(bfvm_ensemble e)
 {
+    bfvm::RandGuard rand_guard;
     return e ? e->err.c_str() : "null ensemble";
 }
 

@@ -163,6 +163,7 @@ extern "C" {
 
 int bfvm_write_matlab(bfvm_handle h, const char* filename, double x0, double y0, double z0)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     Env& e = *h;
     if (!filename) return iofail(e, BFVM_E_ARG, "filename is NULL");
@@ -178,6 +179,7 @@ int bfvm_write_matlab(bfvm_handle h, const char* filename, double x0, double y0,
 
 int bfvm_write_matlab_async(bfvm_handle h, const char* filename, double x0, double y0, double z0)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     Env& e = *h;
     if (!filename) return iofail(e, BFVM_E_ARG, "filename is NULL");
@@ -208,6 +210,7 @@ int bfvm_write_matlab_async(bfvm_handle h, const char* filename, double x0, doub
 
 int bfvm_io_wait(bfvm_handle h)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     return join_writer(*h);
 }

@@ -155,6 +155,7 @@ int rhip(Env* e, hipError_t st, const char* what)
 
 extern "C" int bfvm_mass_balance(bfvm_handle h, double* sums, long long* nonfinite)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return rfail(nullptr, BFVM_E_ARG, "null handle");
     if (!sums && !nonfinite) return rfail(h, BFVM_E_ARG, "sums and nonfinite are both NULL");
     Env& e = *h;
@@ -230,6 +231,7 @@ hipError_t scratch_address(int device, void** out)
 
 extern "C" int bfvm_mass_balance_noalloc(bfvm_handle h, double* sums, long long* nonfinite)
 {
+    bfvm::RandGuard rand_guard;
     if (!h) return rfail(nullptr, BFVM_E_ARG, "null handle");
     Env& e = *h;
     if (e.S > kNoAllocMaxS)

@@ -38,6 +38,15 @@
 #ifndef BIOFVM_AMD_FACADE_H
 #define BIOFVM_AMD_FACADE_H
 
+// the standard headers the reference's BioFVM.h brings in (through its
+// BioFVM_utilities.h, BioFVM_matlab.h and BioFVM_mesh.h), which programs
+// written against it may rely on
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -101,8 +110,11 @@ class Microenvironment {
     Microenvironment();
     explicit Microenvironment(std::string name);
     ~Microenvironment();
-    Microenvironment(const Microenvironment&) = delete;
-    Microenvironment& operator=(const Microenvironment&) = delete;
+    // A copy holds the densities and settings at the time of the copy (the
+    // source's device state is read back first) and gets a device copy of
+    // its own at its first device use; agents stay with the source.
+    Microenvironment(const Microenvironment& other);
+    Microenvironment& operator=(const Microenvironment& other);
 
     Cartesian_Mesh mesh;
     std::string spatial_units, time_units, name;
@@ -202,8 +214,13 @@ class Microenvironment {
     // evaluating them again only after update_rates(), after any of the three
     // callback pointers changes, or after a resize; true evaluates them at
     // every call, as the reference does (BioFVM_microenvironment.cpp:595-600),
-    // for callbacks that depend on time or on the densities.  With all three
-    // callbacks zero_function the bulk step is the identity and is skipped.
+    // for callbacks that depend on time or on the densities.  Callbacks that
+    // leave some entries unwritten (found at the first evaluation), and
+    // callbacks set after a first evaluation, are evaluated at every call
+    // too: the reference then carries its in-place updated values of those
+    // entries from step to step (:603-609), and so does this façade.  With
+    // all three callbacks zero_function the bulk step is the identity and is
+    // skipped.
     bool bulk_rates_every_step;
     // Host seconds spent evaluating the bulk callbacks and handing their
     // fields to the device, summed over the calls.
@@ -230,10 +247,12 @@ class Microenvironment {
     std::vector<double> zero, one;
     DeviceState* dev;
     void reset_space(void);
+    void refresh_mirror(void);      // mirror current (the device copy stays valid)
     void host_view(void);           // mirror current + mark it the newer copy
+    void copy_from(const Microenvironment& other);
     int push_to_device(void);       // upload whatever the device lacks
     int flush_pending(void);        // run the bulk/cell phases not yet enqueued
-    int load_bulk_fields(void);     // evaluate the callbacks, hand S, T, U to the device
+    int load_bulk_fields(double dt);  // evaluate the callbacks, hand S, T, U to the device
 };
 
 class Basic_Agent {
@@ -322,6 +341,60 @@ void zero_function(Microenvironment* pMicroenvironment, int voxel_index,
                    std::vector<double>* write_destination);
 void one_function(Microenvironment* pMicroenvironment, int voxel_index,
                   std::vector<double>* write_destination);
+
+/* Host helpers of the reference's public API used by its example programs:
+ * element-wise vector arithmetic and norms (BioFVM_vector.h:57-107) and the
+ * stopwatch (BioFVM_utilities.h:56-73). */
+std::vector<double> operator-(const std::vector<double>& v1, const std::vector<double>& v2);
+std::vector<double> operator+(const std::vector<double>& v1, const std::vector<double>& v2);
+std::vector<double> operator*(const std::vector<double>& v1, const std::vector<double>& v2);
+std::vector<double> operator/(const std::vector<double>& v1, const std::vector<double>& v2);
+std::vector<double> operator*(double d, const std::vector<double>& v1);
+std::vector<double> operator+(double d, const std::vector<double>& v1);
+std::vector<double> operator+(const std::vector<double>& v1, double d);
+std::vector<double> operator-(double d, const std::vector<double>& v1);
+std::vector<double> operator-(const std::vector<double>& v1, double d);
+void operator+=(std::vector<double>& v1, const std::vector<double>& v2);
+void operator-=(std::vector<double>& v1, const std::vector<double>& v2);
+void operator/=(std::vector<double>& v1, const std::vector<double>& v2);
+void operator*=(std::vector<double>& v1, const double& a);
+void operator*=(std::vector<double>& v1, const std::vector<double>& v2);
+void operator/=(std::vector<double>& v1, const double& a);
+std::ostream& operator<<(std::ostream& os, const std::vector<double>& v);
+std::vector<double> normalize(std::vector<double>& v);
+void normalize(std::vector<double>* v);
+double norm_squared(const std::vector<double>& v);
+double norm(const std::vector<double>& v);
+double maxabs(const std::vector<double>& v);
+double max_abs_difference(const std::vector<double>& v1, const std::vector<double>& v2);
+std::vector<double> exponentiate(const std::vector<double>& exponent);
+void randomize(std::vector<double>* v);
+void axpy(std::vector<double>* y, double& a, std::vector<double>& x);
+void axpy(std::vector<double>* y, std::vector<double>& a, std::vector<double>& x);
+void naxpy(std::vector<double>* y, double& a, std::vector<double>& x);
+void naxpy(std::vector<double>* y, std::vector<double>& a, std::vector<double>& x);
+void csv_to_vector(const char* buffer, std::vector<double>& vect);
+char* vector_to_csv(const std::vector<double>& vect);
+void vector_to_csv_safe(const std::vector<double>& vect, char*& buffer);
+void vector_to_csv(const std::vector<double>& vect, char*& buffer);
+
+/* MAT v4 files on the host (BioFVM_matlab.h:57-72): read_matlab returns
+ * rows x columns (output[row][column]); write_matlab stores input[i] as
+ * column i. */
+std::vector<std::vector<double> > read_matlab(std::string filename);
+FILE* read_matlab_header(int* rows, int* cols, std::string filename);
+FILE* write_matlab_header(int rows, int cols, std::string filename, std::string variable_name);
+bool write_matlab(std::vector<std::vector<double> >& input, std::string filename);
+
+void TIC(void);
+void TOC(void);
+void RUNTIME_TIC(void);
+void RUNTIME_TOC(void);
+double stopwatch_value(void);
+double runtime_stopwatch_value(void);
+double total_stopwatch_time(void);
+void display_stopwatch_value(std::ostream& os, double dIn);
+std::string format_stopwatch_value(double dIn);
 
 }  // namespace BioFVM
 

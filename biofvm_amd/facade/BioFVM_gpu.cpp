@@ -241,6 +241,10 @@ struct DeviceState {
     // the device holds the bulk fields of these three callbacks
     bool bulk_loaded = false;
     void* bulk_callbacks[3] = {nullptr, nullptr, nullptr};
+    // the callbacks leave some entries unwritten, which the reference's
+    // per-step update carries over from its previous step (see
+    // load_bulk_fields): evaluated at every step
+    bool bulk_carry = false;
     std::vector<double> flat;  // staging for host<->device copies
 };
 
@@ -393,6 +397,12 @@ Voxel& Microenvironment::nearest_voxel(std::vector<double>& position)
 // (the caller may write through the returned reference).
 void Microenvironment::host_view(void)
 {
+    refresh_mirror();
+    if (dev) dev->host_newer = true;
+}
+
+void Microenvironment::refresh_mirror(void)
+{
     if (dev && dev->pending) flush_pending();
     if (dev && dev->device_newer && dev->h) {
         const size_t S = densities[0].size();
@@ -403,10 +413,56 @@ void Microenvironment::host_view(void)
             std::memcpy(densities[v].data(), &dev->flat[v * S], S * sizeof(double));
         dev->device_newer = false;
     }
-    if (dev) dev->host_newer = true;
 }
 
 void Microenvironment::synchronize_host(void) { host_view(); }
+
+// Every member but the device copy; the source's mirror is brought up to date
+// without marking it newer, so the source keeps its device copy.
+void Microenvironment::copy_from(const Microenvironment& o)
+{
+    const_cast<Microenvironment&>(o).refresh_mirror();
+    mesh = o.mesh;
+    spatial_units = o.spatial_units;
+    time_units = o.time_units;
+    name = o.name;
+    density_names = o.density_names;
+    density_units = o.density_units;
+    diffusion_coefficients = o.diffusion_coefficients;
+    decay_rates = o.decay_rates;
+    diffusion_decay_solver = o.diffusion_decay_solver;
+    bulk_supply_rate_function = o.bulk_supply_rate_function;
+    bulk_supply_target_densities_function = o.bulk_supply_target_densities_function;
+    bulk_uptake_rate_function = o.bulk_uptake_rate_function;
+    gradient_vectors = o.gradient_vectors;
+    gradient_vector_computed = o.gradient_vector_computed;
+    supply_target_densities_times_supply_rates = o.supply_target_densities_times_supply_rates;
+    supply_rates = o.supply_rates;
+    uptake_rates = o.uptake_rates;
+    dirichlet_substrate_mask = o.dirichlet_substrate_mask;
+    device_precision = o.device_precision;
+    bulk_rates_every_step = o.bulk_rates_every_step;
+    bulk_rates_host_seconds = o.bulk_rates_host_seconds;
+    densities = o.densities;
+    dirichlet_indices = o.dirichlet_indices;
+    dirichlet_value_vectors = o.dirichlet_value_vectors;
+    bulk_temp_S = o.bulk_temp_S;
+    bulk_temp_T = o.bulk_temp_T;
+    bulk_temp_U = o.bulk_temp_U;
+    zero = o.zero;
+    one = o.one;
+}
+
+Microenvironment::Microenvironment(const Microenvironment& other) : dev(nullptr) { copy_from(other); }
+
+Microenvironment& Microenvironment::operator=(const Microenvironment& other)
+{
+    if (this != &other) {
+        reset_space();
+        copy_from(other);
+    }
+    return *this;
+}
 
 // The bulk and cell phases requested since the last device step, as one step.
 int Microenvironment::flush_pending(void)
@@ -678,24 +734,57 @@ void Microenvironment::simulate_diffusion_decay(double dt)
     }
 }
 
-// The three bulk callbacks evaluated per voxel into persistent vectors exactly
-// as the reference's temps (a callback that writes only some entries leaves
-// the others as they were; BioFVM_microenvironment.cpp:595-600), then S, T
-// and U handed to the device, which builds dt*(T*S) and (U+S)*dt + 1 in the
-// reference's operation order.
-int Microenvironment::load_bulk_fields(void)
+// The three bulk callbacks evaluated per voxel into the persistent vectors S,
+// T, U (the reference's bulk_source_sink_solver_temp1..3), S, T and U handed
+// to the device, which builds dt*(T*S) and (U+S)*dt + 1 in the reference's
+// operation order.  The reference updates T and U in place each step
+// (temp2 *= temp1; temp3 += temp1; temp3 *= dt; temp3 += one:
+// BioFVM_microenvironment.cpp:603-609), so an entry a callback does not write
+// at the next step carries that updated value; evaluated at every step
+// (bulk_rates_every_step, or callbacks found to leave entries unwritten),
+// the host vectors are updated the same way after each hand-over.  Whether
+// the callbacks write every entry is found at the first evaluation (vectors
+// filled with NaN first); if one does not, every later step evaluates them,
+// from zero vectors as the reference starts.
+int Microenvironment::load_bulk_fields(double dt)
 {
     const auto t0 = std::chrono::steady_clock::now();
     const size_t nvox = densities.size(), S = densities[0].size();
-    if (bulk_temp_S.size() != nvox) {
+    const bool fresh = bulk_temp_S.size() != nvox;
+    if (fresh) {
         bulk_temp_S.assign(nvox, zero);
         bulk_temp_T.assign(nvox, zero);
         bulk_temp_U.assign(nvox, zero);
     }
-    for (size_t i = 0; i < nvox; ++i) {
-        bulk_supply_rate_function(this, (int)i, &bulk_temp_S[i]);
-        bulk_supply_target_densities_function(this, (int)i, &bulk_temp_T[i]);
-        bulk_uptake_rate_function(this, (int)i, &bulk_temp_U[i]);
+    auto evaluate = [&] {
+        for (size_t i = 0; i < nvox; ++i) {
+            bulk_supply_rate_function(this, (int)i, &bulk_temp_S[i]);
+            bulk_supply_target_densities_function(this, (int)i, &bulk_temp_T[i]);
+            bulk_uptake_rate_function(this, (int)i, &bulk_temp_U[i]);
+        }
+    };
+    bool carry = bulk_rates_every_step || dev->bulk_carry || !fresh;
+    if (!carry) {
+        const std::vector<double> nan(S, std::nan(""));
+        for (size_t i = 0; i < nvox; ++i) bulk_temp_S[i] = bulk_temp_T[i] = bulk_temp_U[i] = nan;
+        evaluate();
+        bool all = true;
+        for (size_t i = 0; i < nvox && all; ++i)
+            for (size_t q = 0; q < S; ++q)
+                if (std::isnan(bulk_temp_S[i][q]) || std::isnan(bulk_temp_T[i][q]) || std::isnan(bulk_temp_U[i][q])) {
+                    all = false;
+                    break;
+                }
+        if (!all) {
+            carry = true;
+            bulk_temp_S.assign(nvox, zero);
+            bulk_temp_T.assign(nvox, zero);
+            bulk_temp_U.assign(nvox, zero);
+        }
+    }
+    if (carry) {
+        dev->bulk_carry = true;
+        evaluate();
     }
     std::vector<double> fs(nvox * S), ft(nvox * S), fu(nvox * S);
     for (size_t i = 0; i < nvox; ++i) {
@@ -704,6 +793,17 @@ int Microenvironment::load_bulk_fields(void)
         std::memcpy(&fu[i * S], bulk_temp_U[i].data(), S * sizeof(double));
     }
     int rc = bfvm_set_bulk(dev->h, fs.data(), ft.data(), fu.data());
+    if (!rc && carry) {
+        for (size_t i = 0; i < nvox; ++i)
+            for (size_t q = 0; q < S; ++q) {
+                bulk_temp_T[i][q] *= bulk_temp_S[i][q];
+                double u = bulk_temp_U[i][q];
+                u += bulk_temp_S[i][q];
+                u *= dt;
+                u += 1.0;
+                bulk_temp_U[i][q] = u;
+            }
+    }
     if (!rc) {
         dev->bulk_loaded = true;
         dev->bulk_callbacks[0] = (void*)bulk_supply_rate_function;
@@ -737,9 +837,10 @@ void Microenvironment::update_rates(void)
 
 // rho = (rho + dt*(T*S)) / ((U+S)*dt + 1) (BioFVM_microenvironment.cpp:584-613)
 // on the device.  The callbacks are evaluated at every call with
-// bulk_rates_every_step, otherwise only when the device lacks the fields of
-// the callbacks set now (see BioFVM.h).  The phase itself is enqueued with
-// the next diffusion step (or before any density is read).
+// bulk_rates_every_step or when they leave entries unwritten, otherwise only
+// when the device lacks the fields of the callbacks set now (see BioFVM.h
+// and load_bulk_fields).  The phase itself is enqueued with the next
+// diffusion step (or before any density is read).
 void Microenvironment::simulate_bulk_sources_and_sinks(double dt)
 {
     int rc = push_to_device();
@@ -757,7 +858,7 @@ void Microenvironment::simulate_bulk_sources_and_sinks(double dt)
     const bool same = dev->bulk_loaded && dev->bulk_callbacks[0] == (void*)bulk_supply_rate_function &&
                       dev->bulk_callbacks[1] == (void*)bulk_supply_target_densities_function &&
                       dev->bulk_callbacks[2] == (void*)bulk_uptake_rate_function;
-    if (bulk_rates_every_step || !same) rc = load_bulk_fields();
+    if (bulk_rates_every_step || dev->bulk_carry || !same) rc = load_bulk_fields(dt);
     if (rc) {
         report("simulate_bulk_sources_and_sinks", rc, dev->h);
         return;
