@@ -177,6 +177,18 @@ def agent_constants(S, dt, voxel_volume, volume, secretion, saturation, uptake):
     return t1, t2
 
 
+def agent_step(S, rho, voxel, temp1, temp2):
+    """One agent phase on `rho` (flat, S values per voxel) in place: every
+    agent in list order applies rho[v] = (rho[v] + temp1) / temp2
+    (oracle_agent_step; BioFVM_basic_agent.cpp:229-238)."""
+    lib = _load_oracle()
+    assert rho.dtype == np.float64 and rho.flags.c_contiguous
+    voxel, t1, t2 = _i32(voxel), _f64(temp1), _f64(temp2)
+    lib.oracle_agent_step(ctypes.c_int(S), _ptr(rho), ctypes.c_int(int(voxel.size)), _ptr(voxel, _ip),
+                          _ptr(t1), _ptr(t2))
+    return rho
+
+
 def nearest_voxel(nx, ny, nz, dx, x, y, z):
     """Voxel index of a point, or -1 outside the closed box (BioFVM_mesh.cpp:930-948)."""
     return _load_oracle().oracle_nearest_voxel(nx, ny, nz, dx, 0.0, 0.0, 0.0, x, y, z)
