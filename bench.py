@@ -159,54 +159,133 @@ def error_curve(case, total, every):
     return curve
 
 
-def cpu_baseline(case, workload):
-    """BioFVM itself (oracle/_ref) on a z-slab of the workload, all host threads."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_case(workload):
+    """The workload the CPU baseline runs: the whole configuration (the
+    drop-in one with whole-vector Dirichlet values, the reference's only
+    kind)."""
+    from biofvm_amd import workloads as W
+    if workload == "drop_in_200":
+        return W.drop_in_200(mask_substrate0=False)
+    return getattr(W, workload)()
+
+
+def cpu_child(workload):
+    """Child process of the CPU baseline: sets the compiled reference
+    (oracle/_ref) up on the whole workload, says so on stdout, waits for a
+    line on stdin (sent once the GPU timings are done, so the two never share
+    the host), then times, on the OpenMP threads its environment names: one
+    untimed warm-up step (the reference builds its Thomas coefficients in its
+    first diffusion call), K full steps, K/2 LOD-only steps, then full steps
+    on one thread.  Prints one JSON line and exits without tearing the
+    reference's 64e6 voxel objects down."""
+    import oracle
+    t0 = time.perf_counter()
+    case = cpu_case(workload)
+    print(f"[bench cpu] {workload} generated ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sess = oracle.RefSession(case)
+    setup = time.perf_counter() - t0
+    print(f"[bench cpu] reference set up ({setup:.1f} s)", file=sys.stderr, flush=True)
+    print(json.dumps({"ready": True}), flush=True)
+    sys.stdin.readline()
+    n = int(os.environ["OMP_NUM_THREADS"])
+    nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    ph = case["phases"]
+    warm = sess.steps(1, ph, n)
+    k = int(min(200, max(5, round(5.0 / max(warm, 1e-6)))))
+    full = sess.steps(k, ph, n)
+    print(f"[bench cpu] {k} steps on {n} threads: {full:.2f} s", file=sys.stderr, flush=True)
+    k_lod = max(3, k // 2)
+    lod = sess.steps(k_lod, 4, n)
+    k1 = int(max(1, min(200, round(5.0 / max(full / k * n, 1e-6)))))
+    one = sess.steps(k1, ph, 1)
+    print(f"[bench cpu] {k1} steps on 1 thread: {one:.2f} s", file=sys.stderr, flush=True)
+    print(json.dumps({"nvs": nvs, "threads": n, "setup_s": setup, "warmup_s": warm,
+                      "steps": k, "seconds": full, "lod_steps": k_lod, "lod_seconds": lod,
+                      "steps_1core": k1, "seconds_1core": one}), flush=True)
+    os._exit(0)
+
+
+class CpuBaseline:
+    """The compiled reference on the whole workload in a child process
+    (cpu_child), started before the GPU is touched so that its set-up
+    overlaps the GPU timings; its steps run after them."""
+
+    def __init__(self, workload):
+        import subprocess
+        import oracle
+        self.workload = workload
+        self.proc = None
+        if not oracle.have_ref():
+            return
+        self.threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        self.threads = min(self.threads, os.cpu_count() or self.threads)
+        env = dict(os.environ, OMP_NUM_THREADS=str(self.threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        self.proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-child", workload],
+                                     stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True, env=env)
+
+    def _json_line(self):
+        for line in self.proc.stdout:  # the reference prints lines of its own
+            if line.startswith("{"):
+                return json.loads(line)
+        raise RuntimeError(f"CPU baseline child ended (code {self.proc.wait()})")
+
+    def result(self, case):
+        if self.proc is None:
+            return port_baseline(case, self.workload), None
+        self._json_line()  # ready
+        self.proc.stdin.write("go\n")
+        self.proc.stdin.flush()
+        r = self._json_line()
+        self.proc.wait()
+        nvs = r["nvs"]
+        build = "g++ -O3 -march=x86-64-v3 -fopenmp (oracle/Makefile)"
+        common = {"unit": "voxel-substrate updates/s", "kind": "reference", "cpu_model": cpu_model(),
+                  "timer": "std::chrono::steady_clock around the step loop"}
+        out = dict(common, value=nvs * r["steps"] / r["seconds"], cores=r["threads"],
+                   ms_per_step=1e3 * r["seconds"] / r["steps"],
+                   sample=f"the whole {self.workload} workload, {r['steps']} full steps after one warm-up "
+                          f"step, {r['threads']} threads, OMP_PROC_BIND=close OMP_PLACES=cores; {build}; "
+                          f"set-up {r['setup_s']:.0f} s, outside the timing",
+                   lod_only={"value": nvs * r["lod_steps"] / r["lod_seconds"], "steps": r["lod_steps"],
+                             "ms_per_step": 1e3 * r["lod_seconds"] / r["lod_steps"]})
+        one = dict(common, value=nvs * r["steps_1core"] / r["seconds_1core"], cores=1,
+                   ms_per_step=1e3 * r["seconds_1core"] / r["steps_1core"],
+                   sample=f"the same session, {r['steps_1core']} full steps on one thread")
+        return out, one
+
+    def cancel(self):
+        if self.proc is not None and self.proc.poll() is None:
+            self.proc.kill()
+            self.proc.wait()
+
+
+def port_baseline(case, workload):
+    """Without the compiled reference: the C restatement (one thread) on a
+    16-plane z-slab of the workload."""
     import oracle
     from biofvm_amd import workloads as W
-    nz_s = 16 if case["nx"] >= 200 else case["nz"]
+    nz_s = min(16, case["nz"])
     nx, ny, S, dx = case["nx"], case["ny"], case["S"], case["dx"]
-    slab = W.make_case(nx, ny, nz_s, S, dx, case["D"], case["lam"], case["dt"], 1, case["phases"],
+    slab = W.make_case(nx, ny, nz_s, S, dx, case["D"], case["lam"], case["dt"], 2, 4,
                        case["rho0"][: nx * ny * nz_s * S])
-    desc = f"{nx}x{ny}x{nz_s} slab of {workload}"
-    if case.get("agent_pos") is not None:
-        pos = case["agent_pos"].reshape(-1, 3)
-        zc = case["nz"] * dx / 2
-        keep = np.flatnonzero(np.abs(pos[:, 2] - zc) < nz_s * dx / 2)
-        p = pos[keep].copy()
-        p[:, 2] -= zc - nz_s * dx / 2
-        S_ = S
-        slab.update(agent_pos=p.reshape(-1), agent_volume=case["agent_volume"][keep],
-                    agent_secretion=case["agent_secretion"].reshape(-1, S_)[keep].reshape(-1),
-                    agent_saturation=case["agent_saturation"].reshape(-1, S_)[keep].reshape(-1),
-                    agent_uptake=case["agent_uptake"].reshape(-1, S_)[keep].reshape(-1))
-        desc += f", {keep.size} agents"
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    nthreads = min(nthreads, os.cpu_count() or nthreads)
-    nvs = nx * ny * nz_s * S
-    one_core = None
-    if oracle.have_ref():
-        slab["steps"] = 2
-        _, secs = oracle.ref_run(slab, nthreads=nthreads, return_seconds=True)
-        steps = int(max(2, min(200, 10.0 / max(secs / 2, 1e-6))))
-        slab["steps"] = steps
-        _, secs = oracle.ref_run(slab, nthreads=nthreads, return_seconds=True)
-        kind, cores = "reference", nthreads
-        if nthreads > 1:  # the same slab on one core, about as long
-            steps1 = max(2, steps // nthreads)
-            s1 = dict(slab, steps=steps1)
-            _, secs1 = oracle.ref_run(s1, nthreads=1, return_seconds=True)
-            one_core = {"value": nvs * steps1 / secs1, "unit": "voxel-substrate updates/s", "cores": 1,
-                        "kind": "reference", "sample": f"{desc}, {steps1} steps, {secs1:.2f} s wall"}
-    else:
-        slab["steps"] = 2
-        t0 = time.perf_counter()
-        oracle.oracle_run(slab)
-        secs = time.perf_counter() - t0
-        steps = 2
-        kind, cores = "port", 1
-    out = {"value": nvs * steps / secs, "unit": "voxel-substrate updates/s", "cores": cores,
-           "kind": kind, "sample": f"{desc}, {steps} steps, {secs:.2f} s wall"}
-    return out, one_core
+    t0 = time.perf_counter()
+    oracle.oracle_run(slab)
+    secs = time.perf_counter() - t0
+    return {"value": nx * ny * nz_s * S * 2 / secs, "unit": "voxel-substrate updates/s", "cores": 1,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"{nx}x{ny}x{nz_s} slab of {workload}, LOD only, 2 steps, {secs:.2f} s wall"}
 
 
 def facade_bench(steps=50, warmup=5, every_step_steps=3):
@@ -281,6 +360,70 @@ def calibrate_copy(mib):
     hip.hipFree(a)
     hip.hipFree(b)
     print(json.dumps({"copy_mib": mib, "bytes_read": n, "bytes_written": n}), flush=True)
+
+
+def ceiling_bench(args):
+    """`--ceiling`: the bandwidth ceilings the sweeps are judged against,
+    from libbfvm_ceiling.so (biofvm_amd/csrc/ceiling.hip): streaming passes
+    over a buffer the size of the FP32 400^3 x 8 store (in place, copy, read,
+    write; the in-place pass also over a 128 MB region that stays in the
+    Infinity Cache), then per store precision the three sweeps of the
+    library (autotuned, diffusion-only steps, HIP events per launch) beside
+    the same sweeps' access pattern alone (same tiles, block shapes, strides
+    and XCD numbering; loads, one barrier, stores; no solve)."""
+    import ctypes
+    from biofvm_amd import DIFFUSION, FP32, FP64, setup_case
+    from biofvm_amd import build as B
+    from biofvm_amd import workloads as W
+    lib = ctypes.CDLL(B.CEILING_LIB)
+    dpp = ctypes.POINTER(ctypes.c_double)
+    lib.bfc_stream_pass.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, dpp, dpp]
+    lib.bfc_sweep_pattern.argtypes = [ctypes.c_int] * 8 + [dpp, dpp]
+    n, S = 400, 8
+    store = n ** 3 * S * 4
+    out = {"mode": "ceiling", "unit": "GB/s", "peak": HBM_PEAK_GBS, "reps": 10}
+
+    def stream(kind, buf, region):
+        g, ms = ctypes.c_double(), ctypes.c_double()
+        rc = lib.bfc_stream_pass(kind, buf, region, 10, ctypes.byref(g), ctypes.byref(ms))
+        if rc:
+            raise RuntimeError(f"bfc_stream_pass({kind}) failed: {rc}")
+        return {"gbps": g.value, "ms": ms.value, "region_bytes": region}
+
+    out["streams"] = {
+        "in_place_read_write": stream(0, store, store),
+        "in_place_read_write_128MB_region": stream(0, store, 128 << 20),
+        "copy": stream(1, store // 2, store // 2),
+        "read": stream(2, store, store),
+        "write": stream(3, store, store),
+    }
+    print(f"[bench] streams: {json.dumps(out['streams'])}", file=sys.stderr, flush=True)
+    case = W.precision_400(steps=1)
+    case = dict(case, phases=DIFFUSION, agent_pos=None, agent_volume=None)
+    for prec, name, elem in ((FP32, "fp32", 4), (FP64, "fp64", 8)):
+        m = setup_case(case, prec)
+        m.autotune(case["dt"])
+        plans = m.sweep_plans()
+        spreads = [m.sweep_spread(a) for a in range(3)]
+        m.run(case["dt"], 2, DIFFUSION)
+        _, per = m.timed_run(case["dt"], 10, DIFFUSION, per_kernel=True)
+        m.close()
+        res = {}
+        for a, k in enumerate(("x", "y", "z")):
+            cols, L, streaming = plans[a]
+            ms = per[f"sweep_{k}"][0] / 10
+            sweep = 2 * n ** 3 * S * elem / (ms * 1e-3) / 1e9
+            g, pms = ctypes.c_double(), ctypes.c_double()
+            rc = lib.bfc_sweep_pattern(elem, n, S, a, cols, L, spreads[a], 10, ctypes.byref(g), ctypes.byref(pms))
+            if rc:
+                raise RuntimeError(f"bfc_sweep_pattern({k}) failed: {rc}")
+            res[k] = {"plan_lines_nodes_streaming": plans[a], "spread": spreads[a], "sweep_ms": ms,
+                      "sweep_gbps": sweep, "pattern_ms": pms.value, "pattern_gbps": g.value,
+                      "sweep_over_pattern": sweep / g.value,
+                      "sweep_over_in_place": sweep / out["streams"]["in_place_read_write"]["gbps"]}
+        out[name] = res
+        print(f"[bench] {name}: {json.dumps(res)}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
 
 
 def ensemble_bench(args):
@@ -472,7 +615,14 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device hipMemcpy of MIB MiB (a plain copy "
                          "kernel of known size, to scale HBM counters) and exit")
+    ap.add_argument("--ceiling", action="store_true",
+                    help="only measure the bandwidth ceilings (streaming passes, the sweeps' access "
+                         "pattern) beside the sweeps, and exit")
+    ap.add_argument("--cpu-child", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(args.cpu_child)
+        return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
@@ -481,9 +631,24 @@ def main():
               "(the z-slab decomposition is a design, not code). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    # the CPU baseline's child starts before this process touches the GPU
+    cpu = None
+    if not args.no_cpu_baseline and not args.ceiling and args.workload in ("precision_400", "drop_in_200", "lod_50"):
+        cpu = CpuBaseline(args.workload)
+    try:
+        gpu_main(args, cpu)
+    finally:
+        if cpu is not None:
+            cpu.cancel()
+
+
+def gpu_main(args, cpu):
     from biofvm_amd import FP32, FP64, _capi
     from biofvm_amd import workloads as W
     _capi.load()
+    if args.ceiling:
+        ceiling_bench(args)
+        return
     if args.workload == "ensemble_64":
         ensemble_bench(args)
         return
@@ -600,8 +765,8 @@ def main():
     line.update(out_extra)
     if args.workload == "drop_in_200":
         line["facade"] = facade_bench()
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"], one_core = cpu_baseline(case, args.workload)
+    if cpu is not None:
+        line["cpu_baseline"], one_core = cpu.result(case)
         if one_core:
             line["cpu_baseline_1core"] = one_core
     print(json.dumps(line), flush=True)

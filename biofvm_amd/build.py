@@ -1,5 +1,6 @@
-"""Builds libbiofvm_amd.so (hand-written gfx950 HIP kernels + the C ABI) and
-libbiofvm_facade.so (the BioFVM-compatible C++ host façade over it) in-tree.
+"""Builds libbiofvm_amd.so (hand-written gfx950 HIP kernels + the C ABI),
+libbiofvm_facade.so (the BioFVM-compatible C++ host façade over it) and
+libbfvm_ceiling.so (the bandwidth probes of `bench.py --ceiling`) in-tree.
 
     python -m biofvm_amd.build          # or biofvm_amd.build.build()
 
@@ -57,7 +58,23 @@ def build(verbose=False):
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     build_facade(verbose)
+    build_ceiling(verbose)
     return LIB
+
+
+CEILING_LIB = os.path.join(HERE, "libbfvm_ceiling.so")
+
+
+def build_ceiling(verbose=False):
+    """The bandwidth-ceiling kernels `bench.py --ceiling` times (a library of
+    their own: nothing in the product library calls them)."""
+    tmp = CEILING_LIB + ".tmp"
+    cmd = [HIPCC, *CFLAGS, "-shared", os.path.join(CSRC, "ceiling.hip"), "-o", tmp]
+    if verbose:
+        print(" ".join(shlex.quote(c) for c in cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, CEILING_LIB)
+    return CEILING_LIB
 
 
 FACADE_DIR = os.path.join(HERE, "facade")

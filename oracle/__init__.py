@@ -6,9 +6,9 @@ Two CPU implementations of BioFVM 1.0.4's step, loaded through ctypes:
   (built to ``oracle/libbfvm_oracle.so``), which follows the reference's
   arithmetic order (BioFVM_solvers.cpp:77-276, BioFVM_microenvironment.cpp:
   192-197, 584-633, BioFVM_basic_agent.cpp:114-148, 229-238).
-* ``ref_run`` — the compiled reference itself (``oracle/_ref/libbfvm_ref.so``,
-  built from the unmodified reference sources by ``oracle/Makefile``), driven
-  through ``oracle/ref_driver.cpp``.
+* ``ref_run`` / ``RefSession`` — the compiled reference itself
+  (``oracle/_ref/libbfvm_ref.so``, built from the unmodified reference sources
+  by ``oracle/Makefile``), driven through ``oracle/ref_driver.cpp``.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
 leg import this package, and only as the checker.  The product library under
@@ -65,6 +65,14 @@ def _load_ref():
             raise FileNotFoundError(f"{REF_SO} missing: run `make -C oracle ref` where the reference tree exists")
         lib = ctypes.CDLL(REF_SO)
         lib.ref_run.restype = ctypes.c_int
+        lib.ref_create.restype = ctypes.c_void_p
+        lib.ref_steps.restype = ctypes.c_int
+        lib.ref_steps.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  _dp]
+        lib.ref_read.restype = None
+        lib.ref_read.argtypes = [ctypes.c_void_p, _dp]
+        lib.ref_destroy.restype = None
+        lib.ref_destroy.argtypes = [ctypes.c_void_p]
         _ref = lib
     return _ref
 
@@ -149,6 +157,57 @@ def ref_run(case, nthreads=1, return_voxels=False, return_seconds=False):
     if return_seconds:
         out.append(secs.value)
     return out[0] if len(out) == 1 else tuple(out)
+
+
+class RefSession:
+    """The compiled reference set up once on `case` and stepped on demand
+    (ref_create / ref_steps / ref_read / ref_destroy in ref_driver.cpp), so
+    that its set-up (tens of seconds at 400^3) stays out of a timing.  One
+    session at a time: the reference keeps its agents in globals."""
+
+    def __init__(self, case):
+        lib = self.lib = _load_ref()
+        if case.get("dir_mask") is not None and not np.all(np.asarray(case["dir_mask"]) != 0):
+            raise ValueError("the reference has no per-substrate Dirichlet mask")
+        S, D, lam, rho, n_dir, dir_idx, dir_val, bulk, n_agents, ag = _common(case)
+        self.case, self.nvs = case, rho.size
+        self._keep = (bulk, ag, dir_idx, dir_val)  # the bulk callbacks read these at every step
+        self.voxels = np.zeros(max(n_agents, 1), dtype=np.int32)
+        rc = ctypes.c_int(0)
+        self.h = lib.ref_create(
+            ctypes.c_int(case["nx"]), ctypes.c_int(case["ny"]), ctypes.c_int(case["nz"]),
+            ctypes.c_int(S), ctypes.c_double(case["dx"]), _ptr(D), _ptr(lam), ctypes.c_double(case["dt"]),
+            ctypes.c_int(_phases(case)), _ptr(rho), ctypes.c_int(n_dir), _ptr(dir_idx, _ip), _ptr(dir_val),
+            _ptr(bulk[0]), _ptr(bulk[1]), _ptr(bulk[2]),
+            ctypes.c_int(n_agents), _ptr(ag[0]), _ptr(ag[1]), _ptr(ag[2]), _ptr(ag[3]), _ptr(ag[4]),
+            _ptr(self.voxels, _ip), ctypes.byref(rc))
+        if not self.h:
+            raise ValueError(f"ref_create rejected the case (code {rc.value})")
+        self.voxels = self.voxels[:n_agents]
+
+    def steps(self, steps, phases=None, nthreads=0):
+        """Runs `steps` steps; returns the wall-clock seconds of the step loop."""
+        secs = ctypes.c_double(0.0)
+        ph = _phases(self.case) if phases is None else int(phases)
+        self.lib.ref_steps(self.h, ctypes.c_double(self.case["dt"]), int(steps), ph, int(nthreads),
+                           ctypes.byref(secs))
+        return secs.value
+
+    def read(self):
+        rho = np.empty(self.nvs)
+        self.lib.ref_read(self.h, _ptr(rho))
+        return rho
+
+    def close(self):
+        if self.h:
+            self.lib.ref_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def thomas_coefficients(n, S, dx, dt, D, lam):

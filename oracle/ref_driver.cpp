@@ -55,15 +55,148 @@ extern "C" {
 /* Flags for ref_run's `phases`: which of the three step methods run each step. */
 enum { REF_BULK = 1, REF_CELLS = 2, REF_DIFFUSION = 4, REF_LOD_2D = 8 };
 
+void ref_destroy(void* h);
+
+/* One reference Microenvironment with its agents, set up once and stepped
+ * any number of times (ref_create / ref_steps / ref_read / ref_destroy). */
+struct RefSession {
+    Microenvironment* M = nullptr;
+    std::vector<Basic_Agent*> agents;
+    int nvox = 0, S = 0;
+};
+
 /*
- * Runs `steps` steps of the reference on an nx*ny*nz grid of S substrates.
- *   rho        in/out, nx*ny*nz*S doubles, voxel-major / substrate-innermost
+ * Sets up the reference on an nx*ny*nz grid of S substrates.
+ *   rho        nx*ny*nz*S doubles, voxel-major / substrate-innermost
  *   dir_*      n_dir Dirichlet nodes (voxel index, S values each)
- *   bulk_*     per-voxel supply rate S, target T, uptake U (nvox*S each), or NULL
+ *   bulk_*     per-voxel supply rate S, target T, uptake U (nvox*S each), or
+ *              NULL; read by the callbacks at every step, so they must stay
+ *              valid until ref_destroy
  *   agent_*    n_agents agents: position (3 each), volume, and S rates each
  *   out_voxel  optional, n_agents ints: the voxel each agent was assigned
- *   seconds    optional: wall-clock seconds of the step loop only
- * Returns 0 on success, a negative code if the reference rejected the input.
+ * Returns the session, or NULL with *rc < 0 if the reference rejected the
+ * input.  One session at a time (the reference keeps its agents and the
+ * default microenvironment in globals).
+ */
+void* ref_create(int nx, int ny, int nz, int S, double dx, const double* D, const double* lambda,
+                 double dt, int phases, const double* rho, int n_dir, const int* dir_idx,
+                 const double* dir_val, const double* bulk_supply, const double* bulk_target,
+                 const double* bulk_uptake, int n_agents, const double* agent_pos,
+                 const double* agent_volume, const double* agent_secretion,
+                 const double* agent_saturation, const double* agent_uptake, int* out_voxel, int* rc)
+{
+    *rc = 0;
+    RefSession* R = new RefSession;
+    Microenvironment* M = R->M = new Microenvironment;
+    set_default_microenvironment(M);
+    M->set_density(0, "s0", "dimensionless");
+    for (int s = 1; s < S; ++s) M->add_density("s", "dimensionless");
+    M->resize_space_uniform(0.0, nx * dx, 0.0, ny * dx, 0.0, nz * dx, dx);
+    if ((int)M->mesh.x_coordinates.size() != nx || (int)M->mesh.y_coordinates.size() != ny ||
+        (int)M->mesh.z_coordinates.size() != nz || M->number_of_densities() != S) {
+        *rc = -1;
+    }
+    M->diffusion_decay_solver = (phases & REF_LOD_2D) ? diffusion_decay_solver__constant_coefficients_LOD_2D
+                                                      : diffusion_decay_solver__constant_coefficients_LOD_3D;
+    R->nvox = nx * ny * nz;
+    R->S = S;
+    if (*rc == 0) {
+        for (int s = 0; s < S; ++s) {
+            M->diffusion_coefficients[s] = D[s];
+            M->decay_rates[s] = lambda[s];
+        }
+        for (int v = 0; v < R->nvox; ++v) {
+            std::vector<double>& d = M->density_vector(v);
+            for (int s = 0; s < S; ++s) d[s] = rho[(size_t)v * S + s];
+        }
+        for (int d = 0; d < n_dir; ++d) {
+            std::vector<double> value(dir_val + (size_t)d * S, dir_val + (size_t)(d + 1) * S);
+            M->add_dirichlet_node(dir_idx[d], value);
+        }
+        g_S = S;
+        g_bulk_supply = bulk_supply;
+        g_bulk_target = bulk_target;
+        g_bulk_uptake = bulk_uptake;
+        if (bulk_supply && bulk_target && bulk_uptake) {
+            M->bulk_supply_rate_function = supply_cb;
+            M->bulk_supply_target_densities_function = target_cb;
+            M->bulk_uptake_rate_function = uptake_cb;
+        }
+        R->agents.reserve(n_agents);
+        for (int a = 0; a < n_agents; ++a) {
+            Basic_Agent* p = create_basic_agent();
+            R->agents.push_back(p);
+            p->register_microenvironment(M);
+            if (!p->assign_position(agent_pos[3 * a], agent_pos[3 * a + 1], agent_pos[3 * a + 2])) {
+                *rc = -2;  // the reference would leave this agent's voxel uninitialised
+                break;
+            }
+            p->volume = agent_volume[a];
+            for (int s = 0; s < S; ++s) {
+                (*p->secretion_rates)[s] = agent_secretion[(size_t)a * S + s];
+                (*p->saturation_densities)[s] = agent_saturation[(size_t)a * S + s];
+                (*p->uptake_rates)[s] = agent_uptake[(size_t)a * S + s];
+            }
+            p->set_internal_uptake_constants(dt);
+            if (out_voxel) out_voxel[a] = p->nearest_voxel_index();
+        }
+    }
+    if (*rc != 0) {
+        ref_destroy(R);
+        return nullptr;
+    }
+    return R;
+}
+
+/* `steps` steps in the reference's order (bulk -> cells -> diffusion, the
+ * phases flagged) on `nthreads` OpenMP threads (0: the runtime's default);
+ * *seconds = wall-clock seconds of the step loop only. */
+int ref_steps(void* h, double dt, int steps, int phases, int nthreads, double* seconds)
+{
+    RefSession* R = (RefSession*)h;
+    if (!R) return -1;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    Microenvironment* M = R->M;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int n = 0; n < steps; ++n) {
+        if (phases & REF_BULK) M->simulate_bulk_sources_and_sinks(dt);
+        if (phases & REF_CELLS) M->simulate_cell_sources_and_sinks(R->agents, dt);
+        if (phases & REF_DIFFUSION) M->simulate_diffusion_decay(dt);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return 0;
+}
+
+/* The densities, voxel-major / substrate-innermost. */
+void ref_read(void* h, double* rho)
+{
+    const RefSession* R = (const RefSession*)h;
+    for (int v = 0; v < R->nvox; ++v) {
+        const std::vector<double>& d = R->M->density_vector(v);
+        for (int s = 0; s < R->S; ++s) rho[(size_t)v * R->S + s] = d[s];
+    }
+}
+
+/* Tears the session down: its agents must not leak into the next one (the
+ * reference keeps them in the global all_basic_agents list). */
+void ref_destroy(void* h)
+{
+    RefSession* R = (RefSession*)h;
+    if (!R) return;
+    for (Basic_Agent* p : R->agents) delete p;
+    all_basic_agents.clear();
+    set_default_microenvironment(nullptr);
+    delete R->M;
+    delete R;
+    g_bulk_supply = g_bulk_target = g_bulk_uptake = nullptr;
+}
+
+/*
+ * Runs `steps` steps of the reference on one fresh session (ref_create's
+ * arguments; rho in/out) and tears it down.  seconds: optional, wall-clock
+ * seconds of the step loop only.  Returns 0 on success, a negative code if
+ * the reference rejected the input.
  */
 int ref_run(int nx, int ny, int nz, int S, double dx,
             const double* D, const double* lambda, double dt, int steps, int phases,
@@ -75,86 +208,15 @@ int ref_run(int nx, int ny, int nz, int S, double dx,
             int* out_voxel, int nthreads, double* seconds)
 {
     if (nthreads > 0) omp_set_num_threads(nthreads);
-
-    Microenvironment* M = new Microenvironment;
-    set_default_microenvironment(M);
-    M->set_density(0, "s0", "dimensionless");
-    for (int s = 1; s < S; ++s) M->add_density("s", "dimensionless");
-    M->resize_space_uniform(0.0, nx * dx, 0.0, ny * dx, 0.0, nz * dx, dx);
-    if ((int)M->mesh.x_coordinates.size() != nx || (int)M->mesh.y_coordinates.size() != ny ||
-        (int)M->mesh.z_coordinates.size() != nz || M->number_of_densities() != S) {
-        delete M;
-        return -1;
-    }
-    M->diffusion_decay_solver = (phases & REF_LOD_2D) ? diffusion_decay_solver__constant_coefficients_LOD_2D
-                                                      : diffusion_decay_solver__constant_coefficients_LOD_3D;
-    for (int s = 0; s < S; ++s) {
-        M->diffusion_coefficients[s] = D[s];
-        M->decay_rates[s] = lambda[s];
-    }
-    const int nvox = nx * ny * nz;
-    for (int v = 0; v < nvox; ++v) {
-        std::vector<double>& d = M->density_vector(v);
-        for (int s = 0; s < S; ++s) d[s] = rho[(size_t)v * S + s];
-    }
-    for (int d = 0; d < n_dir; ++d) {
-        std::vector<double> value(dir_val + (size_t)d * S, dir_val + (size_t)(d + 1) * S);
-        M->add_dirichlet_node(dir_idx[d], value);
-    }
-    g_S = S;
-    g_bulk_supply = bulk_supply;
-    g_bulk_target = bulk_target;
-    g_bulk_uptake = bulk_uptake;
-    if (bulk_supply && bulk_target && bulk_uptake) {
-        M->bulk_supply_rate_function = supply_cb;
-        M->bulk_supply_target_densities_function = target_cb;
-        M->bulk_uptake_rate_function = uptake_cb;
-    }
-
     int rc = 0;
-    std::vector<Basic_Agent*> agents;
-    agents.reserve(n_agents);
-    for (int a = 0; a < n_agents; ++a) {
-        Basic_Agent* p = create_basic_agent();
-        p->register_microenvironment(M);
-        if (!p->assign_position(agent_pos[3 * a], agent_pos[3 * a + 1], agent_pos[3 * a + 2])) {
-            rc = -2;  // the reference would leave this agent's voxel uninitialised
-            break;
-        }
-        p->volume = agent_volume[a];
-        for (int s = 0; s < S; ++s) {
-            (*p->secretion_rates)[s] = agent_secretion[(size_t)a * S + s];
-            (*p->saturation_densities)[s] = agent_saturation[(size_t)a * S + s];
-            (*p->uptake_rates)[s] = agent_uptake[(size_t)a * S + s];
-        }
-        p->set_internal_uptake_constants(dt);
-        if (out_voxel) out_voxel[a] = p->nearest_voxel_index();
-        agents.push_back(p);
-    }
-
-    if (rc == 0) {
-        auto t0 = std::chrono::steady_clock::now();
-        for (int n = 0; n < steps; ++n) {
-            if (phases & REF_BULK) M->simulate_bulk_sources_and_sinks(dt);
-            if (phases & REF_CELLS) M->simulate_cell_sources_and_sinks(agents, dt);
-            if (phases & REF_DIFFUSION) M->simulate_diffusion_decay(dt);
-        }
-        auto t1 = std::chrono::steady_clock::now();
-        if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
-        for (int v = 0; v < nvox; ++v) {
-            std::vector<double>& d = M->density_vector(v);
-            for (int s = 0; s < S; ++s) rho[(size_t)v * S + s] = d[s];
-        }
-    }
-
-    // Tear down: the agents of this run must not leak into the next one
-    // (the reference keeps them in the global all_basic_agents list).
-    for (Basic_Agent* p : agents) delete p;
-    all_basic_agents.clear();
-    set_default_microenvironment(nullptr);
-    delete M;
-    g_bulk_supply = g_bulk_target = g_bulk_uptake = nullptr;
-    return rc;
+    void* R = ref_create(nx, ny, nz, S, dx, D, lambda, dt, phases, rho, n_dir, dir_idx, dir_val, bulk_supply,
+                         bulk_target, bulk_uptake, n_agents, agent_pos, agent_volume, agent_secretion,
+                         agent_saturation, agent_uptake, out_voxel, &rc);
+    if (!R) return rc;
+    ref_steps(R, dt, steps, phases, nthreads, seconds);
+    ref_read(R, rho);
+    ref_destroy(R);
+    return 0;
 }
 
 /*
