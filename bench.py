@@ -667,6 +667,7 @@ def gpu_main(args, cpu):
     m32_chunk = m32.sweep_chunk
     plans = m32.sweep_plans()
     policies = m32.sweep_cache_policies()
+    spreads = [m32.sweep_spread(ax) for ax in range(3)]
     value = nvs * steps / wall
     step_ms = 1e3 * wall / steps
     ph = case["phases"]
@@ -704,11 +705,17 @@ def gpu_main(args, cpu):
         assert n64 == n_done
         a = m32.download()
         b = m64.download()
+        plans64, policies64 = m64.sweep_plans(), m64.sweep_cache_policies()
+        spreads64 = [m64.sweep_spread(ax) for ax in range(3)]
         m64.close()
         diff = np.abs(a - b)
         scale = np.abs(b)
         big = scale > 1e-3 * scale.max()
         out_extra["fp64"] = {"value": nvs * steps / wall64, "ms_per_step": 1e3 * wall64 / steps,
+                             "kernels_ms_per_step": {k: v[0] for k, v in per64.items()},
+                             "sweep_plans_lines_nodes_streaming": dict(zip("xyz", plans64)),
+                             "sweep_non_temporal": dict(zip("xyz", policies64)),
+                             "sweep_spread": dict(zip("xyz", spreads64)),
                              "lod_only": lod_only(per64, nvs, 8),
                              "hbm_frac_step": algorithmic_bytes(case, 8) / (wall64 / steps) / 1e9 / HBM_PEAK_GBS}
         out_extra["fp32_speedup_vs_fp64"] = wall64 / wall
@@ -761,6 +768,7 @@ def gpu_main(args, cpu):
         "lod_only": lod_only(per, nvs, w_main),
         "sweep_plans_lines_nodes_streaming": {"x": plans[0], "y": plans[1], "z": plans[2]},
         "sweep_non_temporal": {"x": policies[0], "y": policies[1], "z": policies[2]},
+        "sweep_spread": dict(zip("xyz", spreads)),
     }
     line.update(out_extra)
     if args.workload == "drop_in_200":

@@ -480,7 +480,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs& A, int C, int tile, 
 // One grid: block (x, y) = (column tile, plane), COLS lines per block; POL =
 // the cache policy of the store's loads and stores.
 template <typename T, int L, int COLS, int POL>
-__global__ void __launch_bounds__(kMaxThreads, L <= 16 ? 8 : 4) k_sweep(SweepArgs A)
+__global__ void __launch_bounds__(kMaxThreads, L <= 20 ? 8 : 4) k_sweep(SweepArgs A)
 {
     __shared__ double xa[kMaxThreads];
     __shared__ double xb[kMaxThreads];
@@ -847,13 +847,15 @@ Geometry geometry(const Env& e, int axis)
 }
 
 // Launch shape of one sweep: COLS lines per block and L nodes per segment,
-// L one of kNodes, nseg = ceil(n/L) segments with COLS*nseg <= 1024, either
-// one block per tile (k_sweep) or resident blocks streaming through the
-// tiles (k_sweep_stream, L one of kStreamNodes; it holds two tiles' raw
-// values, so an FP64 store is limited to 16 nodes to keep within 128 VGPRs).
+// nseg = ceil(n/L) segments with COLS*nseg <= 1024, either one block per
+// tile (k_sweep, L one of kColNodes) or resident blocks streaming through
+// the tiles (k_sweep_stream, L one of kStreamNodes; it holds two tiles'
+// raw values, so an FP64 store is limited to 16 nodes to keep within 128
+// VGPRs), or for x row tiles through LDS (k_sweep_rows, L one of kNodes).
 // cols == 0 means the line is too long for the segmented kernel.  A plan set
 // by the autotuner (bfvm_autotune) wins over the default.
 constexpr int kNodes[] = {4, 8, 12, 16, 24, 32};
+constexpr int kColNodes[] = {4, 8, 12, 16, 20, 24, 32};
 constexpr int kStreamNodes[] = {8, 12, 16, 24};
 constexpr int kColsSet[] = {16, 32, 64};
 
@@ -891,13 +893,15 @@ SweepPlan default_plan(int n)
 // do not depend on timing (the segment length changes the rounding of the
 // solve; every other launch parameter leaves the bits unchanged).  Measured
 // on 64^3 .. 400^3 grids with 2-8 substrates, it is the fastest length or
-// within 4 % of it: 32 nodes for y and z lines of 400 or more in an FP32
-// store, 16 for lines of 200 or more, 8 below.
+// within 4 % of it: for y and z lines of 400 or more in an FP32 store 20
+// nodes while 32 lines of them fit a block (n <= 640: the block's 64 VGPRs
+// let three blocks share a compute unit, against two of the 32-node
+// segments' 94), 32 beyond; 16 for lines of 200 or more, 8 below.
 int default_segment(const Env& e, int axis)
 {
     const int n = axis == 0 ? e.nx : axis == 1 ? e.ny : e.nz;
     int L = 8;
-    if (n >= 400 && axis > 0 && e.precision == BFVM_FP32) L = 32;
+    if (n >= 400 && axis > 0 && e.precision == BFVM_FP32) L = plan_valid(n, 32, 20) ? 20 : 32;
     else if (n >= 200) L = 16;
     for (int cols : {32, 16, 64})
         if (plan_valid(n, cols, L)) return L;
@@ -1050,6 +1054,7 @@ hipError_t launch_cols(Env& e, int axis, int L, int nseg, int k0, int k1, bool n
     case 8: return launch_seg<T, 8, COLS>(e, axis, nseg, k0, k1, nt);
     case 12: return launch_seg<T, 12, COLS>(e, axis, nseg, k0, k1, nt);
     case 16: return launch_seg<T, 16, COLS>(e, axis, nseg, k0, k1, nt);
+    case 20: return launch_seg<T, 20, COLS>(e, axis, nseg, k0, k1, nt);
     case 24: return launch_seg<T, 24, COLS>(e, axis, nseg, k0, k1, nt);
     default: return launch_seg<T, 32, COLS>(e, axis, nseg, k0, k1, nt);
     }
@@ -1283,6 +1288,7 @@ hipError_t launch_x_src_L(Env& e, int L, int nseg, const FusedSources& F)
     case 8: return launch_x_src<T, 8, COLS>(e, nseg, F);
     case 12: return launch_x_src<T, 12, COLS>(e, nseg, F);
     case 16: return launch_x_src<T, 16, COLS>(e, nseg, F);
+    case 20: return launch_x_src<T, 20, COLS>(e, nseg, F);
     case 24: return launch_x_src<T, 24, COLS>(e, nseg, F);
     default: return launch_x_src<T, 32, COLS>(e, nseg, F);
     }
@@ -1364,7 +1370,7 @@ std::vector<SweepPlan> sweep_candidates(const Env& e, int axis, bool any_length)
     const int Lfix = any_length ? 0 : default_segment(e, axis);  // 0: any length
     std::vector<SweepPlan> out;
     for (int cols : kColsSet)
-        for (int L : kNodes)
+        for (int L : kColNodes)
             if (plan_valid(n, cols, L) && (L <= 8 || L < 2 * n) && (Lfix == 0 || L == Lfix)) {
                 out.push_back(SweepPlan{cols, L, 0});
                 if (stream_nodes_ok(L, e.precision)) out.push_back(SweepPlan{cols, L, 1});

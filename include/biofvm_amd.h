@@ -188,9 +188,9 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
  * Reproducibility: none of the knobs above changes a result bit, so results
  * do not depend on which shape won the timing.  The one parameter that does
  * change the rounding of the solve, the number of nodes per segment, stays at
- * a fixed function of the line length, axis and store precision (32 for y and
- * z lines of >= 400 nodes in an FP32 store, 16 for lines of >= 200 nodes, 8
- * below) unless bfvm_set_autotune_segments(h, 1) lets the autotuner time the
+ * a fixed function of the line length, axis and store precision (for y and
+ * z lines of 400-640 nodes in an FP32 store 20, longer ones 32; 16 for lines
+ * of >= 200 nodes, 8 below) unless bfvm_set_autotune_segments(h, 1) lets the autotuner time the
  * segment lengths too; then results may differ in their last bits between
  * runs whose timings pick different lengths.
  */

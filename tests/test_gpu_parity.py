@@ -212,7 +212,7 @@ def plan_case():
 
 def _plans(n, precision):
     for cols in (16, 32, 64):
-        for L in (4, 8, 12, 16, 24, 32):
+        for L in (4, 8, 12, 16, 20, 24, 32):
             if cols * -(-n // L) > 1024 or (L > 8 and L >= 2 * n):
                 continue
             yield cols, L, False
