@@ -52,6 +52,8 @@ SIGNATURES = {
     "bfvm_set_fused_sources": (_i, [_h, _i]),
     "bfvm_get_fused_sources": (_i, [_h, _ip]),
     "bfvm_set_sweep_spread": (_i, [_h, _i, _i]),
+    "bfvm_set_graph_steps": (_i, [_h, _i]),
+    "bfvm_get_graph_steps": (_i, [_h, _ip]),
     "bfvm_mass_balance": (_i, [_h, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "bfvm_mass_balance_noalloc": (_i, [_h, _vp, _vp]),
     "bfvm_compute_gradients": (_i, [_h]),

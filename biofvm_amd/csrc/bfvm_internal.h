@@ -215,6 +215,13 @@ struct Env {
     // MAT v4 snapshot buffers and writer thread (io.hip).
     struct MatWriter* mat = nullptr;
 
+    // Steps replayed from one HIP graph by bfvm_run and bfvm_timed_run
+    // (without per-launch events); 0 = every kernel launched on its own.
+    // The graph (capi.hip, StepGraph) is captured from the same launches an
+    // eager step makes and rebuilt when anything baked into them changes.
+    int graph_steps = 8;
+    struct StepGraph* graph = nullptr;
+
     // Per-launch timing (bfvm_timed_run with per-kernel events).
     bool kernel_timing = false;
     std::vector<hipEvent_t> ev_pool;

@@ -345,6 +345,97 @@ int step_impl(Env& e, double dt, int flags)
     return BFVM_OK;
 }
 
+}  // namespace
+
+// A HIP graph of `steps` consecutive steps, and what its launches have baked
+// in: the store, the tables and sources (through their epochs), the launch
+// shapes, dt and the phases.
+struct bfvm::StepGraph {
+    hipGraphExec_t exec = nullptr;
+    int steps = 0, flags = 0, lod_dims = 0, xy_chunk = 0;
+    double dt = 0;
+    bool fused = false;
+    unsigned coef_epoch = 0, plan_epoch = 0, source_epoch = 0, agent_epoch = 0;
+    const void *rho = nullptr, *out = nullptr, *bulk_A = nullptr, *bulk_B = nullptr;
+
+    bool matches(const Env& e, double dt_, int flags_, int steps_) const
+    {
+        return exec && steps == steps_ && flags == flags_ && dt == dt_ && lod_dims == e.lod_dims &&
+               xy_chunk == e.xy_chunk && fused == e.fused_sources && coef_epoch == e.coef_epoch &&
+               plan_epoch == e.plan_epoch && source_epoch == e.source_epoch && agent_epoch == e.agent_epoch &&
+               rho == e.rho && out == e.out && bulk_A == e.bulk_A && bulk_B == e.bulk_B;
+    }
+};
+
+namespace {
+
+using bfvm::StepGraph;
+
+void free_graph(Env& e)
+{
+    if (!e.graph) return;
+    if (e.graph->exec) (void)hipGraphExecDestroy(e.graph->exec);
+    delete e.graph;
+    e.graph = nullptr;
+}
+
+// Captures `g` steps into the handle's graph.  Every step before it ran
+// eagerly with the same arguments, so the tables and bulk terms are built and
+// the kernels' attributes set: the capture holds kernel launches only.
+int capture_steps(Env& e, double dt, int flags, int g)
+{
+    free_graph(e);
+    HIP_OR_FAIL(&e, hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+    int rc = BFVM_OK;
+    for (int n = 0; n < g && rc == BFVM_OK; ++n) rc = step_impl(e, dt, flags);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(e.stream, &graph);
+    if (rc != BFVM_OK || end != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != BFVM_OK ? rc : hip_fail(&e, end, "hipStreamEndCapture");
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t st = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (st != hipSuccess) return hip_fail(&e, st, "hipGraphInstantiate");
+    StepGraph* G = new StepGraph;
+    G->exec = exec;
+    G->steps = g;
+    G->flags = flags;
+    G->dt = dt;
+    G->lod_dims = e.lod_dims;
+    G->xy_chunk = e.xy_chunk;
+    G->fused = e.fused_sources;
+    G->coef_epoch = e.coef_epoch;
+    G->plan_epoch = e.plan_epoch;
+    G->source_epoch = e.source_epoch;
+    G->agent_epoch = e.agent_epoch;
+    G->rho = e.rho;
+    G->out = e.out;
+    G->bulk_A = e.bulk_A;
+    G->bulk_B = e.bulk_B;
+    e.graph = G;
+    return BFVM_OK;
+}
+
+// `steps` steps on the handle's stream: the first eagerly, then whole graphs
+// of e.graph_steps steps, then the rest eagerly.  The graph replays exactly
+// the launches of the eager steps (same kernels, arguments and order), so
+// the results are the same bits either way.
+int run_steps(Env& e, double dt, int steps, int flags)
+{
+    int done = 0, rc = BFVM_OK;
+    const int g = e.graph_steps;
+    if (steps >= 2 && g >= 2 && !e.kernel_timing && steps - 1 >= g) {
+        if ((rc = step_impl(e, dt, flags))) return rc;
+        done = 1;
+        if (!(e.graph && e.graph->matches(e, dt, flags, g)) && (rc = capture_steps(e, dt, flags, g))) return rc;
+        for (; steps - done >= g; done += g) HIP_OR_FAIL(&e, hipGraphLaunch(e.graph->exec, e.stream));
+    }
+    for (; done < steps && rc == BFVM_OK; ++done) rc = step_impl(e, dt, flags);
+    return rc;
+}
+
 void free_agents(Env& e)
 {
     dfree_t(e.agent_seg_voxel);
@@ -503,6 +594,7 @@ int bfvm_destroy(bfvm_handle h)
     bfvm_env* e = h;
     bfvm::free_tape(*e);
     bfvm::free_writer(*e);
+    free_graph(*e);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     dfree(e->rho);
     dfree(e->c1);
@@ -725,11 +817,7 @@ int bfvm_run(bfvm_handle h, double dt, int steps, int flags)
     if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
     if (steps < 0) return fail(h, BFVM_E_ARG, "steps < 0");
     if (h->ensemble_stream) HIP_OR_FAIL(h, hipStreamSynchronize(h->ensemble_stream));
-    for (int n = 0; n < steps; ++n) {
-        int rc = step_impl(*h, dt, flags);
-        if (rc) return rc;
-    }
-    return BFVM_OK;
+    return run_steps(*h, dt, steps, flags);
 }
 
 int bfvm_sync(bfvm_handle h)
@@ -770,8 +858,7 @@ int bfvm_timed_run(bfvm_handle h, double dt, int steps, int flags, double* ms_to
     }
     HIP_OR_FAIL(&e, quiesce(e));
     HIP_OR_FAIL(&e, hipEventRecord(t0, e.stream));
-    rc = BFVM_OK;
-    for (int n = 0; n < steps && rc == BFVM_OK; ++n) rc = step_impl(e, dt, flags);
+    rc = run_steps(e, dt, steps, flags);
     HIP_OR_FAIL(&e, hipEventRecord(t1, e.stream));
     HIP_OR_FAIL(&e, hipEventSynchronize(t1));
     e.kernel_timing = false;
@@ -1094,6 +1181,26 @@ int bfvm_set_lod_dimensions(bfvm_handle h, int dims)
         h->lod_dims = dims;
         h->coef_valid = false;  // the decay split changes
     }
+    return BFVM_OK;
+}
+
+int bfvm_set_graph_steps(bfvm_handle h, int steps_per_graph)
+{
+    bfvm::RandGuard rand_guard;
+    if (!h) return fail(nullptr, BFVM_E_ARG, "null handle");
+    if (steps_per_graph < 0 || steps_per_graph > 4096)
+        return fail(h, BFVM_E_ARG, "steps_per_graph must be 0 to 4096");
+    HIP_OR_FAIL(h, quiesce(*h));
+    h->graph_steps = steps_per_graph;
+    free_graph(*h);
+    return BFVM_OK;
+}
+
+int bfvm_get_graph_steps(bfvm_handle h, int* steps_per_graph)
+{
+    bfvm::RandGuard rand_guard;
+    if (!h || !steps_per_graph) return fail(h, BFVM_E_ARG, "null argument");
+    *steps_per_graph = h->graph_steps;
     return BFVM_OK;
 }
 

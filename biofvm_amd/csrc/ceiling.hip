@@ -106,6 +106,7 @@ hipError_t pattern_dispatch(T* rho, int n, int S, int axis, int C, int L, int sp
     switch (L) {
     case 8: return launch_pattern<T, 8>(rho, n, S, axis, C, spread, st);
     case 16: return launch_pattern<T, 16>(rho, n, S, axis, C, spread, st);
+    case 20: return launch_pattern<T, 20>(rho, n, S, axis, C, spread, st);
     case 32: return launch_pattern<T, 32>(rho, n, S, axis, C, spread, st);
     default: return hipErrorInvalidValue;
     }
@@ -167,7 +168,7 @@ int bfc_stream_pass(int kind, size_t buffer_bytes, size_t region_bytes, int reps
 }
 
 // The access pattern of one sweep on an n^3 grid of S substrates (elem = 4
-// for an FP32 store, 8 for FP64): C lines per block, L in {8, 16, 32} nodes
+// for an FP32 store, 8 for FP64): C lines per block, L in {8, 16, 20, 32} nodes
 // per thread, XCD spread as the sweeps take it.  *gbps = 2 x the store's
 // bytes / average launch time.
 int bfc_sweep_pattern(int elem, int n, int S, int axis, int C, int L, int spread, int reps, double* gbps,

@@ -168,6 +168,17 @@ class DeviceMicroenvironment:
     def set_sweep_spread(self, axis, tiles_per_chunk):
         self._check(self.lib.bfvm_set_sweep_spread(self.h, int(axis), int(tiles_per_chunk)))
 
+    @property
+    def graph_steps(self):
+        """Steps per HIP graph replay in run/timed_run (0 = every kernel launched on its own)."""
+        v = ctypes.c_int()
+        self._check(self.lib.bfvm_get_graph_steps(self.h, ctypes.byref(v)))
+        return v.value
+
+    @graph_steps.setter
+    def graph_steps(self, steps):
+        self._check(self.lib.bfvm_set_graph_steps(self.h, int(steps)))
+
     def set_lod_dimensions(self, dims):
         """3 = LOD_3D (default), 2 = LOD_2D (nz must be 1)."""
         self._check(self.lib.bfvm_set_lod_dimensions(self.h, int(dims)))

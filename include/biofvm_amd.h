@@ -148,8 +148,22 @@ int bfvm_set_agent_rates_device(bfvm_handle h, int n, const double* position, co
  */
 int bfvm_step(bfvm_handle h, double dt, int flags);
 
-/* `steps` consecutive calls of bfvm_step, enqueued back to back. */
+/* `steps` consecutive calls of bfvm_step, enqueued back to back (whole
+ * groups of steps replayed from a HIP graph, bfvm_set_graph_steps). */
 int bfvm_run(bfvm_handle h, double dt, int steps, int flags);
+
+/* Steps per HIP graph for bfvm_run and bfvm_timed_run (without per-launch
+ * events).  A run of K steps makes its first step launch by launch, captures
+ * the launches of the next steps_per_graph steps into a graph (once: it is
+ * kept for later runs), replays it while whole graphs fit and launches the
+ * rest one by one.  The graph holds the kernels of the eager steps with the
+ * same arguments in the same order, so the results are the same bits as with
+ * steps_per_graph = 0 (every kernel launched on its own).  It is captured
+ * again after the tables, sources, agents, launch shapes, dt or phases
+ * change.  Default 8.  No counterpart in the reference (its steps are host
+ * loops). */
+int bfvm_set_graph_steps(bfvm_handle h, int steps_per_graph);
+int bfvm_get_graph_steps(bfvm_handle h, int* steps_per_graph);
 
 /* Waits for all work enqueued on the handle. */
 int bfvm_sync(bfvm_handle h);
