@@ -19,7 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libbiofvm_amd.so")
 
-SOURCES = ["capi.hip", "sources.hip", "lod.hip", "ensemble.hip", "adjoint.hip", "io.hip", "agents.hip", "reduce.hip"]
+SOURCES = ["capi.hip", "sources.hip", "lod.hip", "ensemble.hip", "adjoint.hip", "io.hip", "agents.hip", "reduce.hip",
+           "sort.hip"]
 HEADERS = ["bfvm_internal.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -4,11 +4,10 @@
 // the implicit constants (Basic_Agent::set_internal_uptake_constants,
 // BioFVM_basic_agent.cpp:114-148) and the grouping by voxel that k_agents
 // uses, with a stable radix sort of (voxel, list index) so that list order is
-// kept inside a voxel.  The result equals bfvm_set_agent_rates on the same
-// values; agents that move are re-binned by calling it again, without a host
-// round trip of their arrays.
-#include <hipcub/hipcub.hpp>
-
+// kept inside a voxel (sort.hip).  The result equals bfvm_set_agent_rates on
+// the same values; agents that move are re-binned by calling it again,
+// without a host round trip of their arrays.
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -191,10 +190,7 @@ int bfvm_set_agent_rates_device(bfvm_handle h, int n, const double* position, co
     // key bits: enough for the largest voxel index
     int bits = 1;
     while (bits < 32 && ((size_t)1 << bits) < e.nvox) ++bits;
-    size_t sort_bytes = 0, scan_bytes = 0;
-    chk(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (unsigned*)key, (unsigned*)key_s, (int*)idx,
-                                           (int*)idx_s, n, 0, bits, e.stream));
-    chk(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (int*)head, (int*)gid, n, e.stream));
+    const size_t sort_bytes = bfvm::radix_sort_tmp_bytes(n), scan_bytes = bfvm::exclusive_scan_tmp_bytes(n);
     chk(hipMalloc(&tmp, std::max(sort_bytes, scan_bytes)));
     chk(hipMemsetAsync(bad, 0, sizeof(int), e.stream));
     if (st == hipSuccess) {
@@ -202,13 +198,14 @@ int bfvm_set_agent_rates_device(bfvm_handle h, int n, const double* position, co
                                                             (unsigned*)key, (int*)idx, (int*)bad);
         chk(hipGetLastError());
     }
-    chk(hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, (unsigned*)key, (unsigned*)key_s, (int*)idx,
-                                           (int*)idx_s, n, 0, bits, e.stream));
+    if (st == hipSuccess)
+        chk(bfvm::radix_sort_pairs((const unsigned*)key, (const int*)idx, (unsigned*)key_s, (int*)idx_s, n, bits,
+                                   tmp, sort_bytes, e.stream));
     if (st == hipSuccess) {
         k_group_heads<<<grid_for(n), kBlock, 0, e.stream>>>((const unsigned*)key_s, n, (int*)head);
         chk(hipGetLastError());
     }
-    chk(hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, (int*)head, (int*)gid, n, e.stream));
+    if (st == hipSuccess) chk(bfvm::exclusive_scan((const int*)head, (int*)gid, n, tmp, scan_bytes, e.stream));
     if (st == hipSuccess) {
         k_group_tables<<<grid_for(n), kBlock, 0, e.stream>>>((const unsigned*)key_s, (const int*)head,
                                                             (const int*)gid, n, e.agent_seg_voxel,

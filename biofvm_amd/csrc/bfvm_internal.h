@@ -94,8 +94,9 @@ struct RowTile {
     size_t vals;             // values swept: rows * nx * S
     int R, C;                // rows per tile, lines per tile (R*S)
     int nxS;                 // values per row
-    unsigned mag_nxS, mag_S;  // ceil(2^32/d), d >= 2: e/d = umulhi(e, mag) for e*d < 2^32
+    unsigned mag_nxS;        // ceil(2^32/nxS), nxS >= 2: e/nxS = umulhi(e, mag) for e*nxS < 2^32
     int tile_vals;           // R*nx*S
+    int P;                   // LDS row pitch in values: nx*S, the last segment's nodes past n-1, = S mod 64
 };
 
 // Launch shape of the segmented sweep of one axis (see lod.hip).
@@ -319,6 +320,15 @@ MassLayout mass_layout(const Env& e, void* workspace);
 hipError_t launch_mass_balance(Env& e, const MassLayout& m, double* sums, long long* nonfinite);
 
 // Frees the adjoint's tape, if any.
+// Stable ascending sort of n (key, value) pairs by the low `bits` bits of
+// the keys (sort.hip), and an exclusive prefix sum of n ints; tmp holds at
+// least the bytes the *_tmp_bytes functions give.  Enqueued on `st`.
+size_t radix_sort_tmp_bytes(int n);
+hipError_t radix_sort_pairs(const unsigned* key_in, const int* val_in, unsigned* key_out, int* val_out, int n,
+                            int bits, void* tmp, size_t tmp_bytes, hipStream_t st);
+size_t exclusive_scan_tmp_bytes(int n);
+hipError_t exclusive_scan(const int* in, int* out, int n, void* tmp, size_t tmp_bytes, hipStream_t st);
+
 void free_tape(Env& e);
 // Waits for and frees the snapshot writer, if any.
 void free_writer(Env& e);

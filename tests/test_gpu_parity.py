@@ -545,6 +545,45 @@ def test_device_agent_setup_equals_host_setup(precision):
                                      dev["agent_saturation"], dev["agent_uptake"], case["dt"])
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_device_agent_setup_at_scale_equals_host_setup(precision):
+    # 400^3 voxels (26-bit voxel keys: four radix passes over many blocks) and
+    # 3e5 agents, 1e5 of them stacked 2000 deep on 50 voxels: the device
+    # grouping gives the host grouping's step, bit for bit, and the voxels of
+    # BioFVM_mesh.cpp:930-948
+    import torch
+    rng = np.random.default_rng(92)
+    n, S, dx, dt = 400, 1, 10.0, 0.01
+    ag = W.random_agents(rng, 200_000, S, n, n, n, dx, cluster_sigma=300.0)
+    stack_vox = rng.integers(0, n, (50, 3))
+    stacked = np.repeat((stack_vox + 0.5) * dx, 2000, axis=0)
+    pos = np.concatenate([ag["agent_pos"].reshape(-1, 3), stacked]).reshape(-1)
+    na = pos.size // 3
+    vol = np.full(na, 2494.0)
+    sec, upt = rng.uniform(0, 10, na * S), rng.uniform(0, 10, na * S)
+    sat = np.ones(na * S)
+    rho0 = rng.uniform(0, 1, n ** 3 * S)
+    p3 = pos.reshape(-1, 3)
+    ijk = np.clip(np.floor(p3 / dx).astype(np.int64), 0, n - 1)
+    want_vox = (ijk[:, 2] * n + ijk[:, 1]) * n + ijk[:, 0]
+    with DeviceMicroenvironment(n, n, n, S, dx, precision) as host, \
+            DeviceMicroenvironment(n, n, n, S, dx, precision) as dev:
+        for m in (host, dev):
+            m.set_coefficients([1e3], [0.1])
+            m.upload(rho0)
+        got_host = host.set_agent_rates(pos, vol, sec, sat, upt, dt)
+        t = {k: torch.tensor(v, dtype=torch.float64, device="cuda") for k, v in
+             (("pos", pos), ("vol", vol), ("sec", sec), ("sat", sat), ("upt", upt))}
+        vox = torch.empty(na, dtype=torch.int32, device="cuda")
+        dev.set_agent_rates_device(t["pos"], t["vol"], t["sec"], t["sat"], t["upt"], dt, out_voxel=vox)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(vox.cpu().numpy(), want_vox)
+        np.testing.assert_array_equal(np.asarray(got_host), want_vox)
+        for m in (host, dev):
+            m.run(dt, 2, W.CELLS)
+        np.testing.assert_array_equal(dev.download(), host.download())
+
+
 def test_sources_only_steps_are_bit_exact():
     # bulk and agent updates (FP64 store) have no reassociation: bit for bit the oracle's
     rng = np.random.default_rng(101)
