@@ -17,13 +17,17 @@ against the 8 TB/s HBM3E peak, using its average launch time from HIP event
 pairs recorded around every launch, on the library's stream, in up to 10
 more steps run right after the timed ones (pairs inside the timed steps
 would slow them: 1.5 % at 400^3 x 8).  ``cpu_baseline`` times BioFVM itself (compiled from
-source, oracle/_ref) on a slab of the same workload on this host's cores.
+source, oracle/_ref) on the whole workload on this host's cores (a child process set up while
+the GPU timings run, its steps timed after them; see cpu_child), ``cpu_baseline_1core`` the
+same on one thread.  The line also carries compact ``ensemble_64`` and
+``differentiation_128`` objects (ensemble_result, differentiation_result).
 
 ``--workload ensemble_64`` times the batching form instead: 64 independent
 uneven grids (FP32 stores) advanced together, one step = one HIP graph replay
 of batched agent and sweep launches; its roofline prices the three batched
 sweeps of a diffusion-only step.  ``--workload differentiation_128`` times the
-adjoint (see differentiation_bench).
+adjoint (see differentiation_result).  ``--ceiling`` measures the bandwidth
+ceilings beside the sweeps (ceiling_bench).
 """
 import argparse
 import json
@@ -51,6 +55,22 @@ def algorithmic_bytes(case, w):
         b += nvs * 2 * 8  # read A and B (FP64); the store pass is shared with the x-sweep when fused
     if ph & 2 and case.get("agent_volume") is not None:
         b += len(case["agent_volume"]) * (4 + 2 * S * 8)  # voxel key + FP64 temp1/temp2
+    if ph & 4 and case.get("dir_idx") is not None:
+        b += 3 * len(case["dir_idx"]) * (4 + S * w)
+    return b
+
+
+def survey_bytes(case, w):
+    """The same per-step figure with every term at the store's width w (agent
+    constants counted as w-byte values, bulk A/B as w-byte values): the
+    formula the HBM targets for this path are stated in."""
+    nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    S, ph = case["S"], case["phases"]
+    b = nvs * 6 * w
+    if ph & 1 and case.get("bulk_supply") is not None:
+        b += nvs * 2 * w
+    if ph & 2 and case.get("agent_volume") is not None:
+        b += len(case["agent_volume"]) * (4 + 2 * S * w)
     if ph & 4 and case.get("dir_idx") is not None:
         b += 3 * len(case["dir_idx"]) * (4 + S * w)
     return b
@@ -426,14 +446,16 @@ def ceiling_bench(args):
     print(json.dumps(out), flush=True)
 
 
-def ensemble_bench(args):
+def ensemble_result(steps, warmup, with_fp64):
+    """ensemble_64: 64 uneven grids advanced together, one HIP graph replay per
+    step (FP32 stores), then the FP64 stores on the same inputs.  Returns the
+    line's fields (value, timings, the batched sweeps' roofline)."""
     from biofvm_amd import FP32, FP64, Ensemble, setup_case
     from biofvm_amd import workloads as W
     cases = W.ensemble_64()
     dt, ph = cases[0]["dt"], cases[0]["phases"]
     nvs = sum(c["nx"] * c["ny"] * c["nz"] * c["S"] for c in cases)
     n_agents = sum(len(c["agent_volume"]) for c in cases)
-    steps, warmup = args.steps, args.warmup
 
     def run(precision):
         members = [setup_case(c, precision) for c in cases]
@@ -451,12 +473,27 @@ def ensemble_bench(args):
     out = [m.download() for m in members]
     # diffusion-only steps (after the result was taken) price the three batched sweeps
     sweep_steps = max(3, min(steps, 20))
+    ens.run(dt, 2, 4, graph=True)  # the diffusion-only graph captured before the timing
     sweep_ms = ens.timed_run(dt, sweep_steps, 4, graph=True) / sweep_steps
+    knobs = {"segment_nodes": L, "x_row_tiles": ens.row_tiles, "yz_non_temporal": ens.non_temporal}
     ens.close()
     for m in members:
         m.close()
-    extra = {}
-    if not args.no_fp64:
+    sweep_bytes = 3 * 2 * nvs * 4
+    achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9  # three launches' bytes over their summed time
+    res = {
+        "value": nvs * steps / wall, "unit": "voxel-substrate updates/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * wall / steps, "events_ms_per_step": ev_ms / steps, "eager_ms_per_step": eager_ms,
+        "store_dtype": "f32",
+        "config": {"workload": f"ensemble_64: {len(cases)} grids, {nvs} voxel-substrate values, "
+                               f"{n_agents} agents, dt={dt}, phases={ph}, one HIP graph per step"},
+        "roofline": {"bound": "hbm", "kernel": "k_sweep_batch / k_sweep_rows_batch (3 per step, average)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": sweep_bytes / 3, "avg_launch_ms": sweep_ms / 3},
+    }
+    res.update(knobs)
+    if with_fp64:
         ens64, mem64, L64, wall64, ev64, eager64 = run(FP64)
         err_n, err_l2n, err_l2d = 0.0, 0.0, 0.0
         for m, a in zip(mem64, out):
@@ -467,46 +504,47 @@ def ensemble_bench(args):
         ens64.close()
         for m in mem64:
             m.close()
-        extra = {"fp64": {"value": nvs * steps / wall64, "ms_per_step": 1e3 * wall64 / steps,
-                          "segment_nodes": L64},
-                 "error_vs_fp64": {"steps": warmup + steps + max(1, steps // 4),
-                                   "max_rel_normwise_per_member": err_n,
-                                   "l2_rel": (err_l2n / err_l2d) ** 0.5}}
-    sweep_bytes = 3 * 2 * nvs * 4
-    achieved = sweep_bytes / (sweep_ms * 1e-3) / 1e9  # three launches' bytes over their summed time
+        res["fp64"] = {"value": nvs * steps / wall64, "ms_per_step": 1e3 * wall64 / steps, "segment_nodes": L64}
+        res["error_vs_fp64"] = {"steps": warmup + steps + max(1, steps // 4),
+                                "max_rel_normwise_per_member": err_n, "l2_rel": (err_l2n / err_l2d) ** 0.5}
+    return res
+
+
+def ensemble_bench(args):
+    res = ensemble_result(args.steps, args.warmup, not args.no_fp64)
     line = {
         "metric": "voxel-substrate updates/s (ensemble step: agents + LOD diffusion-decay, all members)",
-        "value": nvs * steps / wall,
-        "unit": "voxel-substrate updates/s",
-        "n_gpus": 1, "steps": steps, "warmup": warmup,
-        "ms_per_step": 1e3 * wall / steps,
-        "events_ms_per_step": ev_ms / steps,
-        "eager_ms_per_step": eager_ms,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "store_dtype": "f32",
+        "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded numpy default_rng; see biofvm_amd/workloads.py)",
-        "config": {"workload": f"ensemble_64: {len(cases)} grids, {nvs} voxel-substrate values, "
-                               f"{n_agents} agents, dt={dt}, phases={ph}, one HIP graph per step"},
-        "segment_nodes": L,
-        "roofline": {"bound": "hbm", "kernel": "k_sweep_batch (3 per step, average)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": sweep_bytes / 3, "avg_launch_ms": sweep_ms / 3},
     }
-    line.update(extra)
+    line.update(res)
     print(json.dumps(line), flush=True)
 
 
-def differentiation_bench(args):
+def adjoint_bytes(case, w):
+    """Algorithmic HBM bytes of one backward step of the adjoint: the step
+    recomputed twice from the segment's checkpoint (3 sweeps of the store,
+    w bytes per value, read and written; the agents' FP64 constants), the
+    transposed sweeps of the FP64 adjoint state (3, read and written), the
+    D/lambda sums of each axis (adjoint state and forward state read once),
+    the agents' adjoint (their constants once more)."""
+    nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    agents = len(case["agent_volume"]) * (4 + 2 * case["S"] * 8)
+    return 2 * (6 * nvs * w + agents) + 6 * nvs * 8 + 3 * nvs * (8 + w) + agents
+
+
+def differentiation_result(full):
     """Adjoint of 500 steps at 128^3 x 2 with 2e4 agents (FP32 store,
     checkpoints every 50 steps).  The target fields come from the hidden
     parameters (FP64 run); the gradient is taken at the start of a fit (D,
     lambda and secretion rates off by a factor exp(U[-0.3, 0.3])).  Times the
-    recorded forward and the backward pass, and checks the gradient against
-    FP64 central differences (h = 1e-6 relative) on 32 random directions in
-    the joint parameter space (initial field, D, lambda, secretion, uptake).
-    Per-group checks of D and lambda at h = 1e-6 and 1e-4 are reported too:
-    their directional derivatives are small next to J, so h = 1e-6 differences
-    carry J's rounding noise (~1e-13 J / h)."""
+    recorded forward and the backward pass and checks the gradient against
+    FP64 differences of J (summed with math.fsum) on random directions in the
+    joint parameter space (initial field, D, lambda, secretion, uptake):
+    4th-order central differences (Richardson, h = 1e-4 relative,
+    truncation O(h^4)).  full: 8 directions and the FP64 adjoint beside the
+    FP32 one; otherwise 2 directions, FP32 adjoint only."""
+    import math
     from biofvm_amd import FP32, FP64, setup_case
     from biofvm_amd import workloads as W
     hidden = W.differentiation_128()
@@ -518,10 +556,9 @@ def differentiation_bench(args):
         m.run(dt, steps, ph)
         target = m.download()
     case = dict(hidden)
-    case["D"] = hidden["D"] * np.exp(rng.uniform(-0.3, 0.3, hidden["S"]))
-    case["lam"] = hidden["lam"] * np.exp(rng.uniform(-0.3, 0.3, hidden["S"]))
-    case["agent_secretion"] = hidden["agent_secretion"] * np.exp(
-        rng.uniform(-0.3, 0.3, hidden["agent_secretion"].shape))
+    for k, shape in (("D", hidden["S"]), ("lam", hidden["S"]), ("agent_secretion", hidden["agent_secretion"].shape)):
+        case[k] = np.asarray(hidden[k]) * np.exp(rng.uniform(-0.3, 0.3, shape))
+
     def adjoint(precision):
         m = setup_case(case, precision)
         m.autotune(dt)
@@ -538,63 +575,63 @@ def differentiation_bench(args):
         return g, t_fwd, t_bwd, loss
 
     g, t_fwd, t_bwd, loss = adjoint(FP32)
-    out = {"forward_s": t_fwd, "backward_s": t_bwd, "loss": loss}
-    if not args.no_fp64:
-        keys = ["rho0", "D", "lam", "agent_secretion", "agent_uptake"]
-        gk = {"rho0": "grad_rho0", "D": "grad_D", "lam": "grad_lambda", "agent_secretion": "grad_secretion",
-              "agent_uptake": "grad_uptake"}
-        # the FP64 adjoint: the same transposes on the FP64 store's trajectory
+    step_bytes = adjoint_bytes(case, 4)
+    achieved = step_bytes / (t_bwd / steps) / 1e9
+    out = {"value": nvs * steps / t_bwd, "unit": "voxel-substrate updates/s", "steps": steps,
+           "ms_per_step": 1e3 * t_bwd / steps, "forward_s": t_fwd, "backward_s": t_bwd, "loss": loss,
+           "store_dtype": "f32",
+           "config": {"workload": f"differentiation_128: {case['nx']}^3 voxels x {case['S']} substrates, "
+                                  f"{n_agents} agents, {steps} steps, checkpoints every 50"},
+           "roofline": {"bound": "hbm", "kernel": "backward step (recomputed forward, transposed sweeps, "
+                                                  "D/lambda sums, agents' adjoint)",
+                        "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_step": step_bytes}}
+    keys = ["rho0", "D", "lam", "agent_secretion", "agent_uptake"]
+    gk = {"rho0": "grad_rho0", "D": "grad_D", "lam": "grad_lambda", "agent_secretion": "grad_secretion",
+          "agent_uptake": "grad_uptake"}
+    grads = {"fp32": g}
+    if full:
         g64, _, t_bwd64, _ = adjoint(FP64)
+        grads["fp64"] = g64
         out["fp64_backward_s"] = t_bwd64
         out["fp32_vs_fp64_adjoint"] = {
             k: {"max_abs_over_max": float(np.abs(g[v] - g64[v]).max() / np.abs(g64[v]).max()),
                 "l2_rel": float(np.linalg.norm(g[v] - g64[v]) / np.linalg.norm(g64[v]))}
             for k, v in gk.items()}
 
-        def J(ds, h):
-            c = dict(case)
-            for k, d in ds.items():
-                c[k] = np.asarray(case[k]) + h * d
-            with setup_case(c, FP64) as mm:
-                mm.run(dt, steps, ph)
-                r = mm.download() - target
-                return 0.5 * float(r @ r)
+    def J(ds, h):
+        c = dict(case)
+        for k, d in ds.items():
+            c[k] = np.asarray(case[k]) + h * d
+        with setup_case(c, FP64) as mm:
+            mm.run(dt, steps, ph)
+            r = mm.download() - target
+        return 0.5 * math.fsum(r * r)
 
-        def check(ks, h, grads=(g,)):
-            """Per gradient in `grads`: |ad - fd| / |fd| and |ad - fd| / sum |terms|."""
-            ds = {k: rng.normal(size=np.shape(case[k])) * np.asarray(case[k]) for k in ks}
-            fd = (J(ds, h) - J(ds, -h)) / (2 * h)
-            res = []
-            for gr in grads:
-                terms = np.concatenate([gr[gk[k]].reshape(-1) * ds[k].reshape(-1) for k in ks])
-                ad = float(terms.sum())
-                res.append((abs(ad - fd) / abs(fd), abs(ad - fd) / float(np.abs(terms).sum())))
-            return res
+    h, n_dir = 1e-4, (8 if full else 2)
+    worst = {k: 0.0 for k in grads}
+    for _ in range(n_dir):
+        ds = {k: rng.normal(size=np.shape(case[k])) * np.asarray(case[k]) for k in keys}
+        fd = (8 * (J(ds, h) - J(ds, -h)) - (J(ds, 2 * h) - J(ds, -2 * h))) / (12 * h)
+        for name, gr in grads.items():
+            ad = math.fsum(np.concatenate([np.asarray(gr[gk[k]]).reshape(-1) * ds[k].reshape(-1) for k in keys]))
+            worst[name] = max(worst[name], abs(ad - fd) / abs(fd))
+    out["fd_check"] = {"directions": n_dir, "h": h, "difference": "4th-order central (Richardson), J by fsum",
+                       "space": "joint (rho0, D, lambda, secretion, uptake)",
+                       "max_rel_err": worst["fp32"]}
+    if "fp64" in worst:
+        out["fd_check"]["fp64_adjoint_max_rel_err"] = worst["fp64"]
+    return out
 
-        joint = [check(keys, 1e-6, (g, g64)) for _ in range(32)]
-        out["fd_check"] = {"directions": 32, "h": 1e-6, "space": "joint (rho0, D, lambda, secretion, uptake)",
-                           "max_rel_err": max(r[0][0] for r in joint),
-                           "max_rel_err_vs_term_scale": max(r[0][1] for r in joint),
-                           "directions_within_1e-4": sum(r[0][0] <= 1e-4 for r in joint),
-                           # the same directions with the FP64 adjoint: the differences' own error
-                           "fp64_adjoint_max_rel_err": max(r[1][0] for r in joint),
-                           "fp64_adjoint_directions_within_1e-4": sum(r[1][0] <= 1e-4 for r in joint)}
-        dl = {h: [check(["D", "lam"], h)[0] for _ in range(4)] for h in (1e-6, 1e-4)}
-        out["fd_check_D_lambda"] = {str(h): {"max_rel_err": max(a for a, _ in v),
-                                             "max_rel_err_vs_term_scale": max(b for _, b in v)}
-                                    for h, v in dl.items()}
+
+def differentiation_bench(args):
+    res = differentiation_result(not args.no_fp64)
     line = {
         "metric": "voxel-substrate adjoint step updates/s (backward pass incl. checkpoint recomputation)",
-        "value": nvs * steps / t_bwd,
-        "unit": "voxel-substrate updates/s",
-        "n_gpus": 1, "steps": steps, "warmup": 0,
-        "ms_per_step": 1e3 * t_bwd / steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "store_dtype": "f32",
-        "data": "synthetic (seeded numpy default_rng; see biofvm_amd/workloads.py)",
-        "config": {"workload": f"differentiation_128: {case['nx']}^3 voxels x {case['S']} substrates, "
-                               f"{n_agents} agents, {steps} steps, checkpoints every 50"},
+        "n_gpus": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded numpy default_rng; see biofvm_amd/workloads.py)",
     }
-    line.update(out)
+    line.update(res)
     print(json.dumps(line), flush=True)
 
 
@@ -608,6 +645,8 @@ def main():
                              "differentiation_128"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="leave the compact ensemble_64 and differentiation_128 objects out of the default line")
     ap.add_argument("--plan-x", default="", metavar="LINES,NODES,KERNEL",
                     help="after autotuning, run the FP32 x sweep with this shape (profiling aid)")
     ap.add_argument("--error-steps", type=int, default=1000,
@@ -757,6 +796,8 @@ def gpu_main(args, cpu):
                                f"{0 if case.get('agent_volume') is None else len(case['agent_volume'])} agents, "
                                f"phases={case['phases']}"},
         "hbm_frac_step": step_bytes / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+        # the same with every term at the store's width (agent constants too)
+        "hbm_frac_step_store_width": survey_bytes(case, w_main) / (wall / steps) / 1e9 / HBM_PEAK_GBS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_step": dom_bytes, "ms_per_step": dom_ms, "launches_per_step": dom_n,
@@ -773,6 +814,12 @@ def gpu_main(args, cpu):
     line.update(out_extra)
     if args.workload == "drop_in_200":
         line["facade"] = facade_bench()
+    if args.workload == "precision_400" and not args.no_forms:
+        # the batching and adjoint forms, compact (their own --workload gives the full lines)
+        print("[bench] ensemble_64", file=sys.stderr, flush=True)
+        line["ensemble_64"] = ensemble_result(20, 5, False)
+        print("[bench] differentiation_128", file=sys.stderr, flush=True)
+        line["differentiation_128"] = differentiation_result(False)
     if cpu is not None:
         line["cpu_baseline"], one_core = cpu.result(case)
         if one_core:

@@ -676,32 +676,15 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 8 ? 8 : 4) k_sweep_x_src(Swe
 // x sweep on row tiles staged in LDS.  A row is the nx*S values of one (j, k)
 // pair, contiguous in the store; a tile is R consecutive rows (one contiguous
 // run of R*nx*S values), loaded and stored with one value per lane per
-// instruction, consecutive lanes on consecutive values, and transposed
-// through LDS so that each thread finds its (row, substrate) line segment
-// there as L consecutive values.  The solve is seg_core's (bit-identical to
-// k_sweep with the same L).
+// instruction, consecutive lanes on consecutive values.  In LDS the tile
+// keeps the store's order, row r at r*P (P = X.P values, at least nx*S plus
+// the nodes the last segment holds past n-1, which are zeros, and P = S mod
+// 64 so that the lanes of a wave, lines r*S + s, fall in different banks):
+// the thread of line (r, s), segment g finds node t at r*P + (g*L + t)*S + s.
+// The solve is seg_core's (bit-identical to k_sweep with the same L).
 //
 // LDS: xa, xb (one double per thread), wfirst, wlast (C each), the correction
-// tables (stage_tables), then the tile:
-// node t of segment g of line l = r*S + s at (g*C + l)*(L+1) + t.  L+1 is
-// odd, so the 64 lanes of a wave, reading node t of 64 consecutive (g, l)
-// slots, meet 64 different banks.  The nodes past n-1 of the last segment
-// are zeros (they reach no real node: the backward step into node n-1
-// multiplies them by 0).
-
-
-// LDS slot of tile value e (row-major, substrate innermost in the store).
-template <int L>
-__device__ __forceinline__ unsigned row_tile_slot(const RowTile& X, unsigned e)
-{
-    const unsigned S = (unsigned)X.A.S;
-    const unsigned r = __umulhi(e, X.mag_nxS);
-    const unsigned w = e - r * (unsigned)X.nxS;
-    const unsigned i = S == 1 ? w : __umulhi(w, X.mag_S);
-    const unsigned s = w - i * S;
-    const unsigned g = i / L, t = i - g * L;
-    return (g * (unsigned)X.C + r * S + s) * (L + 1) + t;
-}
+// tables (stage_tables), then the tile.
 
 __device__ __forceinline__ void val_store(float*, __amdgpu_buffer_rsrc_t r, float x, unsigned vo)
 {
@@ -726,6 +709,11 @@ __device__ __forceinline__ void rows_tile(const RowTile& X, unsigned tile_index)
     double* wlast = wfirst + X.C;
     double* gtab = wlast + X.C;
     T* tile = (T*)(gtab + (A.kg0 + A.kg1 + 3 * kPad + 8) * A.S);
+    const unsigned nxS = (unsigned)X.nxS, P = (unsigned)X.P;
+    // value e = tid + k*nthr of the tile is (row r, offset w): walked from
+    // e = tid in steps of nthr = dr rows and dw values
+    const unsigned r0 = __umulhi((unsigned)tid, X.mag_nxS), w0 = (unsigned)tid - r0 * nxS;
+    const unsigned dr = (unsigned)nthr / nxS, dw = (unsigned)nthr - dr * nxS;
 
     // the buffer ends with the tile's last real value: reads past it give 0
     // (the rows past the last are zeros) and stores past it are dropped
@@ -740,45 +728,61 @@ __device__ __forceinline__ void rows_tile(const RowTile& X, unsigned tile_index)
             v[k] = buf_load((T*)nullptr, rs, (unsigned)(tid + k * nthr) * (unsigned)sizeof(T), 0);
         stage_tables(A, gtab);  // while the loads are in flight
         // zeros past node n-1 of each line's last segment
-        const int tail = A.nseg * L - A.n;
-        T* last = tile + (size_t)(A.nseg - 1) * X.C * (L + 1) + (L - tail);
-        for (int q = tid; q < X.C * tail; q += nthr) {
-            const int l = q / tail;
-            last[l * (L + 1) + (q - l * tail)] = (T)0;
+        const int pad = (int)(P - nxS);
+        for (int q = tid; q < X.R * pad; q += nthr) {
+            const int r = q / pad;
+            tile[(unsigned)r * P + nxS + (unsigned)(q - r * pad)] = (T)0;
         }
+        unsigned r = r0, w = w0;
 #pragma unroll
         for (int k = 0; k < L; ++k) {
-            const unsigned e = (unsigned)(tid + k * nthr);
-            if (e < (unsigned)X.tile_vals) tile[row_tile_slot<L>(X, e)] = v[k];
+            if (tid + k * nthr < X.tile_vals) tile[r * P + w] = v[k];
+            w += dw;
+            r += dr;
+            if (w >= nxS) {
+                w -= nxS;
+                ++r;
+            }
         }
     }
     lds_barrier();
 
     const int lane = tid % X.C, graw = tid / X.C;
     const int g = min(graw, A.nseg - 1);  // threads past the last segment are copies of it
-    const int s = lane % A.S;
-    T* seg = tile + (g * X.C + lane) * (L + 1);
+    const int S = A.S;
+    const int s = lane % S;
+    unsigned base = (unsigned)(lane / S) * P + (unsigned)(g * L * S + s);
     T raw[L];
 #pragma unroll
-    for (int t = 0; t < L; ++t) raw[t] = seg[t];
+    for (int t = 0; t < L; ++t) raw[t] = tile[base + (unsigned)(t * S)];
     const double* K = gtab + (A.kg0 + A.kg1 + 3 * kPad) * A.S;
     const bool act = graw < A.nseg;
+    // the addresses are recomputed after the solve rather than kept in
+    // registers across it (the compiler would otherwise hold all L of them)
+    asm volatile("" : "+v"(base));
     seg_core<T, L>(A, X.C, lane, s, g * L, g * X.C + lane, graw == A.nseg - 1, raw, K[s * 8 + 0], K[s * 8 + 1],
                    xa, xb, wfirst, wlast, gtab, NoSource{}, [&](int t, double zf) {
-                       if (act) seg[t] = (T)zf;
+                       if (act) tile[base + (unsigned)(t * S)] = (T)zf;
                    });
     lds_barrier();
+    unsigned r = r0, w = w0;
+    asm volatile("" : "+v"(r), "+v"(w));
 #pragma unroll
     for (int k = 0; k < L; ++k) {
         const unsigned e = (unsigned)(tid + k * nthr);
-        if (e < (unsigned)X.tile_vals)
-            val_store((T*)nullptr, rs_out, tile[row_tile_slot<L>(X, e)], e * (unsigned)sizeof(T));
+        if (e < (unsigned)X.tile_vals) val_store((T*)nullptr, rs_out, tile[r * P + w], e * (unsigned)sizeof(T));
+        w += dw;
+        r += dr;
+        if (w >= nxS) {
+            w -= nxS;
+            ++r;
+        }
     }
 }
 
 // Grid: one block per row tile.
 template <typename T, int L>
-__global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_rows(RowTile X)
+__global__ void __launch_bounds__(kMaxThreads, L <= 8 ? 8 : 4) k_sweep_rows(RowTile X)
 {
     rows_tile<T, L>(X, xcd_tile(blockIdx.x, gridDim.x, X.A.spread));
 }
@@ -786,7 +790,7 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_sweep_rows(RowTile X)
 // Row tiles of several grids' x sweeps: map[2b], map[2b+1] = the grid of block
 // b and b's tile within it; blockDim and the LDS fit every grid's tiles.
 template <typename T, int L>
-__global__ void __launch_bounds__(kMaxThreads, 4)
+__global__ void __launch_bounds__(kMaxThreads, L <= 8 ? 8 : 4)
 k_sweep_rows_batch(const RowTile* __restrict__ tabs, const int* __restrict__ map, int spread)
 {
     const unsigned b = xcd_tile(blockIdx.x, gridDim.x, spread);
@@ -1130,6 +1134,7 @@ constexpr size_t kMaxRowTileLds = 150 * 1024;
 
 struct RowShape {
     int R = 0, C = 0, nseg = 0, threads = 0;
+    int P = 0;  // LDS row pitch in values (rows_tile)
     size_t lds = 0;
 };
 
@@ -1141,8 +1146,13 @@ RowShape row_shape(const Env& e, int C, int L, int elem)
     r.R = C / e.S;
     r.nseg = (e.nx + L - 1) / L;
     r.threads = (C * r.nseg + 63) / 64 * 64;
+    // a row's values, then the last segment's nodes past n-1, then up to the
+    // next pitch = S (mod 64)
+    const int nxS = e.nx * e.S;
+    r.P = nxS + (r.nseg * L - e.nx) * e.S;
+    r.P += ((e.S - r.P) % 64 + 64) % 64;
     const size_t tables = (size_t)(e.ax[0].kg0 + e.ax[0].kg1 + 3 * kPad + 8) * e.S;
-    r.lds = sizeof(double) * (2 * r.threads + 2 * C + tables) + (size_t)elem * C * r.nseg * (L + 1);
+    r.lds = sizeof(double) * (2 * r.threads + 2 * C + tables) + (size_t)elem * r.R * r.P;
     return r;
 }
 
@@ -1169,9 +1179,9 @@ RowTile row_tile_args(const Env& e, const RowShape& sh, int k0, int k1)
     X.C = sh.C;
     X.nxS = e.nx * e.S;
     X.mag_nxS = div_magic((unsigned)X.nxS);
-    X.mag_S = e.S > 1 ? div_magic((unsigned)e.S) : 0;  // S = 1 divides by nothing
     X.tile_vals = X.R * X.nxS;
     X.vals = (size_t)rows * X.nxS;
+    X.P = sh.P;
     return X;
 }
 
