@@ -385,8 +385,9 @@ hipError_t sweep_adjoint(Env& e, double* lam, int axis)
     e.precision = BFVM_FP64;
     e.elem = sizeof(double);  // offsets and 4 GiB limits in FP64 values
     // row tiles tuned for an FP32 store take twice the LDS on the FP64
-    // adjoint (22 us against 14 us for the default shape at 128^3 x 2)
-    if (keep != BFVM_FP64 && keep_plan.stream == 2) e.plan[axis] = bfvm::SweepPlan{};
+    // adjoint (22 us against 14 us for the default shape at 128^3 x 2); the
+    // whole-cache-line x kernel tuned for an FP32 store is not timed on FP64
+    if (keep != BFVM_FP64 && keep_plan.stream >= 2) e.plan[axis] = bfvm::SweepPlan{};
     hipError_t st = on_buffer(e, lam, [&] { return bfvm::launch_sweep(e, axis); });
     e.plan[axis] = keep_plan;
     e.elem = keep_elem;

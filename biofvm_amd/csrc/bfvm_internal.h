@@ -104,7 +104,8 @@ struct SweepPlan {
     int cols = 0;    // lines per block (16, 32 or 64; row tiles: a multiple of S); 0 = default
     int L = 0;       // nodes per segment
     int stream = 0;  // kernel: 0 one block per tile, 1 resident blocks that load the
-                     // next tile while solving one, 2 x sweep on LDS-staged row tiles
+                     // next tile while solving one, 2 x sweep on LDS-staged row tiles,
+                     // 3 x sweep with whole-cache-line loads and stores (k_sweep_xt)
     int nt = 0;      // 1: non-temporal loads and stores (y and z sweeps whose tiles'
                      // row pieces are whole 128-byte cache lines)
 };

@@ -382,14 +382,16 @@ struct NoSource {
     __device__ __forceinline__ double operator()(int, double x) const { return x; }
 };
 
+// xslot = the LDS slot this thread writes its partials to (its own slot, or
+// the slot of the segment it copies: the same value); walker = this thread
+// walks the carry chain of column `lane` (one thread per column).
 template <typename T, int L, class Src, class Sink>
-__device__ __forceinline__ void seg_core(const SweepArgs& A, int C, int lane, int s, int p0, int slot,
-                                         bool last_writer, const T (&raw)[L], double r, double a, double* xa,
-                                         double* xb, double* wfirst, double* wlast, const double* gtab,
-                                         Src&& src, Sink&& sink)
+__device__ __forceinline__ void seg_core_at(const SweepArgs& A, int C, int lane, int s, int p0, int slot,
+                                            bool last_writer, int xslot, bool walker, const T (&raw)[L],
+                                            double r, double a, double* xa, double* xb, double* wfirst,
+                                            double* wlast, const double* gtab, Src&& src, Sink&& sink)
 {
     const int n = A.n, nseg = A.nseg, S = A.S, kg0 = A.kg0, kg1 = A.kg1;
-    const int tid = threadIdx.x;
     const bool has_last = p0 + L >= n;  // holds node n-1 (and nodes past it)
     const double* K = gtab + (kg0 + kg1 + 3 * kPad) * S;  // read after the first barrier
 
@@ -401,9 +403,9 @@ __device__ __forceinline__ void seg_core(const SweepArgs& A, int C, int lane, in
     double aL = a;
 #pragma unroll
     for (int t = 1; t < L; ++t) aL *= a;
-    xa[tid] = y;
+    xa[xslot] = y;
     lds_barrier();
-    if (tid < C) chain(xa, aL, lane, C, nseg, true);
+    if (walker) chain(xa, aL, lane, C, nseg, true);
     lds_barrier();
     {
         double PY = xa[slot];
@@ -430,9 +432,9 @@ __device__ __forceinline__ void seg_core(const SweepArgs& A, int C, int lane, in
 #pragma unroll
         for (int t = L - 1; t >= 0; --t) v[t] = z = fma(p0 + t < n - 1 ? a : 0.0, z, v[t]);
     }
-    xb[tid] = z;
+    xb[xslot] = z;
     lds_barrier();
-    if (tid < C) wfirst[lane] = chain(xb, aL, lane, C, nseg, false);
+    if (walker) wfirst[lane] = chain(xb, aL, lane, C, nseg, false);
     lds_barrier();
 
     // Last pass, from the top of the segment down, each node finished and
@@ -461,6 +463,19 @@ __device__ __forceinline__ void seg_core(const SweepArgs& A, int C, int lane, in
         if (tail) zf = fma(mu1, g1r[-t * S], zf);
         sink(t, zf);
     }
+}
+
+// Thread tid = segment * C + column (k_sweep's layout): partials in slot tid,
+// threads 0..C-1 walk the chains.
+template <typename T, int L, class Src, class Sink>
+__device__ __forceinline__ void seg_core(const SweepArgs& A, int C, int lane, int s, int p0, int slot,
+                                         bool last_writer, const T (&raw)[L], double r, double a, double* xa,
+                                         double* xb, double* wfirst, double* wlast, const double* gtab,
+                                         Src&& src, Sink&& sink)
+{
+    const int tid = threadIdx.x;
+    seg_core_at<T, L>(A, C, lane, s, p0, slot, last_writer, tid, tid < C, raw, r, a, xa, xb, wfirst, wlast, gtab,
+                      src, sink);
 }
 
 // Block-level body of the segmented sweep for one plane's column tile.
@@ -798,6 +813,130 @@ k_sweep_rows_batch(const RowTile* __restrict__ tabs, const int* __restrict__ map
     const int local = __builtin_amdgcn_readfirstlane(map[2 * b + 1]);
     const RowTile X = tabs[member];
     rows_tile<T, L>(X, (unsigned)local);
+}
+
+// x sweep with whole-cache-line loads and stores (k_sweep_xt).  In the
+// column kernels a lane owns one x line, so a load instruction reads an
+// S-value piece of each of several rows and the next nodes of those rows
+// come from the same cache lines again (through L1).  Here a load
+// instruction reads whole G-node chunks (G*S contiguous values, G = 16 bytes
+// / value size): lane j of a segment loads the 16 bytes j of the segment's
+// next chunk, and the chunk goes through a per-wave LDS buffer to the lanes
+// that own its lines (lane s takes node u of line s); the solved values go
+// back the same way and are stored 16 bytes per lane.  Needs S | 64, G | S
+// and G | L (FP32: S in {4, 8, 16}; FP64: S in {2, 4, 8, 16}).
+//
+// Block: R rows (C = R*S lines, column rl*S + s) x nseg segments; its
+// Q = R*nseg segments q = rl*nseg + g are dealt to the waves, 64/S per
+// wave (wave w holds q = w*64/S + qq on lanes qq*S .. qq*S + S-1), so a
+// wave's load instruction covers whole chunks of 64/S segments.  Threads
+// past segment Q-1 copy it (same loads, same values, no stores); rows past
+// the last are copies of it (no stores).  The solve is seg_core's:
+// bit-identical to k_sweep with the same L.
+//
+// LDS: the correction tables (stage_tables), the carry slots xa, xb (Q*S
+// each), wfirst, wlast (C each), then one chunk buffer per wave: segment qq's
+// chunk at qq*(G+1)*S values (one node of padding), node u of line s at
+// u*S + s in it, so that the 64 lanes reading one node fall in different
+// banks and every lane's LDS offsets are one register plus constants.
+
+// Orders the LDS accesses of one wave: a wave's LDS operations complete in
+// order, so this only keeps the compiler from moving them across it.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T, int S>
+constexpr bool xt_shape_ok(int L)
+{
+    return 16 % sizeof(T) == 0 && S % (16 / (int)sizeof(T)) == 0 && 64 % S == 0 && S <= 16 &&
+           L % (16 / (int)sizeof(T)) == 0;
+}
+
+// Bytes of one wave's chunk buffer.
+template <typename T>
+constexpr int xt_wave_lds()
+{
+    return 64 * (16 / (int)sizeof(T) + 1) * (int)sizeof(T);
+}
+
+template <typename T, int L, int S>
+__global__ void __launch_bounds__(kMaxThreads, L <= 20 ? 8 : 4) k_sweep_xt(SweepArgs A)
+{
+    constexpr int G = 16 / (int)sizeof(T);  // nodes per chunk = values per lane piece
+    constexpr int kSegW = 64 / S;            // segments per wave
+    constexpr int kPitch = (G + 1) * S;      // values per segment in the chunk buffer
+    static_assert(xt_shape_ok<T, S>(L), "k_sweep_xt shape");
+    extern __shared__ __attribute__((aligned(16))) double smem_xt[];
+    const int C = A.cols, nseg = A.nseg, n = A.n;
+    const int R = C / S, Q = R * nseg;
+    double* gtab = smem_xt;
+    double* xa = gtab + (A.kg0 + A.kg1 + 3 * kPad + 8) * S;
+    double* xb = xa + Q * S;
+    double* wfirst = xb + Q * S;
+    double* wlast = wfirst + C;
+    const int tid = threadIdx.x, qq = (tid & 63) / S, s = tid % S;
+    // this lane's segment's part of the wave's chunk buffer
+    T* const mine = (T*)(wlast + C) + (tid >> 6) * (kSegW * kPitch) + qq * kPitch;
+    const int q = (tid >> 6) * kSegW + qq;
+    const int qc = min(q, Q - 1);
+    const int rl = qc / nseg, g = qc - rl * nseg;
+    const int col = rl * S + s, p0 = g * L;
+    const int rows = A.cols_per_plane / S;
+    const int row = (int)xcd_tile(blockIdx.x, gridDim.x, A.spread) * R + rl;
+    const bool act = q < Q && row < rows;
+    const unsigned row0 = (unsigned)min(row, rows - 1) * A.vstride;  // values before the row
+    const int uj = G * s / S, oj = G * s % S;  // this lane's piece: node uj, values oj .. oj+G-1
+    const int wj = uj * S + oj;                // the piece's place in the chunk buffer
+    // byte offset of this lane's piece of chunk 0; chunk k is k*G*S values on
+    constexpr unsigned kChunk = (unsigned)(G * S * sizeof(T));
+    const unsigned vo = (row0 + (unsigned)(p0 + uj) * S + oj) * (unsigned)sizeof(T);
+
+    const double r = A.wb[s * 8 + 0], a = A.wb[s * 8 + 1];
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(A.rho, A.nvs * sizeof(T));
+    u32x4 piece[L / G];
+    if (p0 + L <= n) {
+#pragma unroll
+        for (int k = 0; k < L / G; ++k)
+            piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, k * kChunk, kLoadPolicy);
+    } else {  // the segment holding node n-1: nodes past it read node n-1
+#pragma unroll
+        for (int k = 0; k < L / G; ++k) {
+            const unsigned node = (unsigned)min(p0 + G * k + uj, n - 1);
+            piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, (row0 + node * S + oj) * (unsigned)sizeof(T), 0,
+                                                             kLoadPolicy);
+        }
+    }
+    stage_tables(A, gtab);  // while the loads are in flight
+    T raw[L];
+#pragma unroll
+    for (int k = 0; k < L / G; ++k) {
+        *(u32x4*)(mine + wj) = piece[k];
+        wave_lds_sync();
+#pragma unroll
+        for (int u = 0; u < G; ++u) raw[G * k + u] = mine[u * S + s];
+        wave_lds_sync();
+    }
+
+    const __amdgpu_buffer_rsrc_t ro = make_rsrc(A.out, A.nvs * sizeof(T));
+    const int slot = g * C + col;
+    seg_core_at<T, L>(A, C, col, s, p0, slot, g == nseg - 1, slot, q < Q && g == 0, raw, r, a, xa, xb, wfirst,
+                      wlast, gtab, NoSource{}, [&](int t, double zf) {
+                          mine[(t % G) * S + s] = (T)zf;
+                          if (t % G == 0) {  // chunk t / G is complete
+                              wave_lds_sync();
+                              const u32x4 out = *(const u32x4*)(mine + wj);
+                              if (act && p0 + t + uj < n)
+                                  __builtin_amdgcn_raw_buffer_store_b128(out, ro, vo, (t / G) * kChunk,
+                                                                         kStorePolicy);
+                              wave_lds_sync();
+                          }
+                      });
 }
 
 // One thread owns one whole (line, substrate) pair and runs the reference's
@@ -1217,11 +1356,94 @@ hipError_t launch_rows_L(Env& e, const SweepPlan& p, int k0, int k1)
     }
 }
 
+// Whole-cache-line x sweeps (k_sweep_xt): plan.cols = C = R*S lines, R rows
+// per block, L one of kXtNodes.
+constexpr int kXtNodes[] = {8, 16, 20};
+
+template <typename T>
+bool xt_supported(int S, int L)
+{
+    bool nodes = false;
+    for (int v : kXtNodes) nodes = nodes || v == L;
+    if (!nodes) return false;
+    switch (S) {
+    case 4: return xt_shape_ok<T, 4>(L);
+    case 8: return xt_shape_ok<T, 8>(L);
+    case 16: return xt_shape_ok<T, 16>(L);
+    default: return false;
+    }
+}
+
+struct XtShape {
+    int Q = 0, threads = 0;
+    size_t lds = 0;
+};
+
+template <typename T>
+XtShape xt_shape(const Env& e, int C, int L)
+{
+    XtShape x;
+    const int S = e.S;
+    if (C <= 0 || L <= 0 || C % S != 0) return x;
+    x.Q = (C / S) * ((e.nx + L - 1) / L);
+    const int waves = (x.Q + 64 / S - 1) / (64 / S);
+    x.threads = waves * 64;
+    const size_t tables = (size_t)(e.ax[0].kg0 + e.ax[0].kg1 + 3 * kPad + 8) * S;
+    x.lds = sizeof(double) * (tables + 2 * (size_t)x.Q * S + 2 * (size_t)C) + (size_t)waves * xt_wave_lds<T>();
+    return x;
+}
+
+template <typename T>
+bool xt_plan_valid(const Env& e, int C, int L)
+{
+    if (!xt_supported<T>(e.S, L) || e.nx < 2) return false;
+    const XtShape x = xt_shape<T>(e, C, L);
+    return x.Q > 0 && x.threads <= kMaxThreads && x.lds <= kMaxCorrectionLds;
+}
+
+template <typename T, int L, int S>
+hipError_t launch_xt(Env& e, const SweepPlan& p, int k0, int k1)
+{
+    if constexpr (!xt_shape_ok<T, S>(L)) {
+        return hipErrorInvalidValue;
+    } else {
+        const XtShape x = xt_shape<T>(e, p.cols, L);
+        const SweepArgs A = sweep_args(e, 0, p.cols, (e.nx + L - 1) / L, k0, k1);
+        const hipError_t st = allow_table_lds((const void*)k_sweep_xt<T, L, S>, x.lds);
+        if (st != hipSuccess) return st;
+        k_sweep_xt<T, L, S><<<(unsigned)A.tiles, x.threads, x.lds, e.stream>>>(A);
+        return hipGetLastError();
+    }
+}
+
+template <typename T, int L>
+hipError_t launch_xt_S(Env& e, const SweepPlan& p, int k0, int k1)
+{
+    switch (e.S) {
+    case 4: return launch_xt<T, L, 4>(e, p, k0, k1);
+    case 8: return launch_xt<T, L, 8>(e, p, k0, k1);
+    case 16: return launch_xt<T, L, 16>(e, p, k0, k1);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_xt_L(Env& e, const SweepPlan& p, int k0, int k1)
+{
+    switch (p.L) {
+    case 8: return launch_xt_S<T, 8>(e, p, k0, k1);
+    case 16: return launch_xt_S<T, 16>(e, p, k0, k1);
+    case 20: return launch_xt_S<T, 20>(e, p, k0, k1);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 // Whether the segmented kernels run this axis (else the per-line fallback).
 template <typename T>
 bool segmented(const Env& e, int axis, const SweepPlan& p)
 {
     if (p.stream == 2) return axis == 0 && rows_plan_valid(e, p.cols, p.L);
+    if (p.stream == 3 && (axis != 0 || !xt_plan_valid<T>(e, p.cols, p.L))) return false;
     Geometry gm = geometry(e, axis);
     // buffer offsets are 32-bit: the segmented kernel addresses at most 4 GiB
     // from a plane's start (x: from the first of a run of z planes, at least
@@ -1260,6 +1482,7 @@ hipError_t sweep(Env& e, int axis, int k0 = 0, int k1 = -1)
         }
         gm.ncols = (k1 - k0) * e.ny * e.S;
         planes = 1;
+        if (p.stream == 3) return launch_xt_L<T>(e, p, k0, k1);
     }
     int nseg = (gm.n + p.L - 1) / p.L;
     const long long tiles = ((long long)gm.ncols + p.cols - 1) / p.cols;
@@ -1394,6 +1617,13 @@ std::vector<SweepPlan> sweep_candidates(const Env& e, int axis, bool any_length)
             for (int L : kNodes)
                 if ((L <= 8 || L < 2 * n) && (Lfix == 0 || L == Lfix) && rows_plan_valid(e, R * e.S, L))
                     out.push_back(SweepPlan{R * e.S, L, 2});
+    if (axis == 0)
+        for (int R = 1; R <= 16; R *= 2)
+            for (int L : kXtNodes) {
+                const bool ok = e.precision == BFVM_FP64 ? xt_plan_valid<double>(e, R * e.S, L)
+                                                         : xt_plan_valid<float>(e, R * e.S, L);
+                if ((L <= 8 || L < 2 * n) && (Lfix == 0 || L == Lfix) && ok) out.push_back(SweepPlan{R * e.S, L, 3});
+            }
     return out;
 }
 

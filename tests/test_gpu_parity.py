@@ -299,6 +299,68 @@ def test_row_tile_x_sweep_equals_column_tiles(precision):
         assert tried > 0, (nx, ny, nz, S)
 
 
+# substrate counts whose nodes split into whole 16-byte pieces (FP32: S in
+# {4, 8, 16}; FP64 also 2); line lengths below, at and past the segment
+# lengths, with partial last segments and partial last tiles
+WHOLE_LINE_SHAPES = [(400, 6, 3, 8), (37, 5, 4, 8), (61, 7, 3, 4), (20, 9, 5, 16), (3, 11, 2, 8),
+                     (130, 4, 3, 2), (48, 13, 2, 4)]
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_whole_line_x_sweep_equals_column_tiles(precision):
+    # the x sweep with whole-cache-line loads (kernel 3) runs the column
+    # kernel's arithmetic with the same segment length: bit-identical stores
+    # for every shape the library accepts, and within tolerance of the oracle
+    rng = np.random.default_rng(93)
+    tried = 0
+    for nx, ny, nz, S in WHOLE_LINE_SHAPES:
+        case = W.make_case(nx, ny, nz, S, 10.0, 10.0 ** rng.uniform(2, 5, S), 10.0 ** rng.uniform(-3, 0, S),
+                           0.01, 3, W.DIFFUSION, rng.uniform(0, 1, nx * ny * nz * S))
+        ref = oracle.oracle_run(case)
+        with setup_case(case, precision) as m:
+            for L in (8, 16, 20):
+                base = None
+                for cols in (16, 32, 64):
+                    try:
+                        m.set_sweep_plan(0, cols, L, 0)
+                    except BfvmError:
+                        continue
+                    m.upload(case["rho0"])
+                    m.run(case["dt"], case["steps"], case["phases"])
+                    base = m.download()
+                    break
+                for R in (1, 2, 4, 8, 16):
+                    try:
+                        m.set_sweep_plan(0, R * S, L, 3)
+                    except BfvmError:
+                        continue
+                    assert m.sweep_plans()[0] == (R * S, L, 3)
+                    for spread in (0, -1):
+                        m.set_sweep_spread(0, spread)
+                        m.upload(case["rho0"])
+                        m.run(case["dt"], case["steps"], case["phases"])
+                        got = m.download()
+                        tried += 1
+                        assert rel_err(got, ref) <= TOL[precision], (nx, ny, nz, S, R, L)
+                        if base is not None:
+                            np.testing.assert_array_equal(got, base, err_msg=str((nx, ny, nz, S, R, L, spread)))
+    assert tried > 0
+
+
+def test_whole_line_x_sweep_refuses_other_substrate_counts():
+    # S = 3 does not split into 16-byte pieces; y and z take no such kernel
+    with DeviceMicroenvironment(40, 8, 6, 3, 10.0, FP32) as m:
+        m.set_coefficients([1e3, 1e4, 1e5], [0.1, 0.2, 0.3])
+        with pytest.raises(BfvmError):
+            m.set_sweep_plan(0, 12, 16, 3)
+    with DeviceMicroenvironment(40, 40, 40, 8, 10.0, FP32) as m:
+        m.set_coefficients([1e3] * 8, [0.1] * 8)
+        m.set_sweep_plan(0, 32, 16, 3)
+        for axis in (1, 2):
+            with pytest.raises(BfvmError):
+                m.set_sweep_plan(axis, 32, 16, 3)
+
+
 @pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
 def test_cache_policy_never_changes_results(precision):
     rng = np.random.default_rng(41)
