@@ -399,6 +399,7 @@ def ceiling_bench(args):
     dpp = ctypes.POINTER(ctypes.c_double)
     lib.bfc_stream_pass.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, dpp, dpp]
     lib.bfc_sweep_pattern.argtypes = [ctypes.c_int] * 8 + [dpp, dpp]
+    lib.bfc_sweep_pattern_xt.argtypes = [ctypes.c_int] * 6 + [dpp, dpp]
     n, S = 400, 8
     store = n ** 3 * S * 4
     out = {"mode": "ceiling", "unit": "GB/s", "peak": HBM_PEAK_GBS, "reps": 10}
@@ -434,7 +435,11 @@ def ceiling_bench(args):
             ms = per[f"sweep_{k}"][0] / 10
             sweep = 2 * n ** 3 * S * elem / (ms * 1e-3) / 1e9
             g, pms = ctypes.c_double(), ctypes.c_double()
-            rc = lib.bfc_sweep_pattern(elem, n, S, a, cols, L, spreads[a], 10, ctypes.byref(g), ctypes.byref(pms))
+            if streaming == 3:  # the whole-cache-line x kernel has its own pattern
+                rc = lib.bfc_sweep_pattern_xt(elem, n, cols, L, spreads[a], 10, ctypes.byref(g), ctypes.byref(pms))
+            else:
+                rc = lib.bfc_sweep_pattern(elem, n, S, a, cols, L, spreads[a], 10, ctypes.byref(g),
+                                           ctypes.byref(pms))
             if rc:
                 raise RuntimeError(f"bfc_sweep_pattern({k}) failed: {rc}")
             res[k] = {"plan_lines_nodes_streaming": plans[a], "spread": spreads[a], "sweep_ms": ms,
@@ -725,12 +730,16 @@ def gpu_main(args, cpu):
     if dom.startswith("sweep"):
         cols, L, streaming = plans["xyz".index(dom[-1])]
         kname = ("k_sweep_x_src" if dom == "sweep_x" and fused else
-                 "k_sweep_rows" if streaming == 2 else "k_sweep_stream" if streaming else "k_sweep")
+                 "k_sweep_rows" if streaming == 2 else "k_sweep_xt" if streaming == 3 else
+                 "k_sweep_stream" if streaming else "k_sweep")
         tname = "float" if main_prec == FP32 else "double"
-        # the row-tile kernel's template has no lines-per-block parameter; the
+        # the row-tile kernel's template has no lines-per-block parameter, the
+        # whole-cache-line x kernel's has the substrate count instead; the
         # column kernels carry a cache policy (non-temporal, 2, for y and z)
         if kname == "k_sweep_rows":
             targs = f"{tname}, {L}"
+        elif kname == "k_sweep_xt":
+            targs = f"{tname}, {L}, {case['S']}"
         elif kname == "k_sweep_x_src":
             targs = f"{tname}, {L}, {cols}"
         else:

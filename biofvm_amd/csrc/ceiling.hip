@@ -112,6 +112,81 @@ hipError_t pattern_dispatch(T* rho, int n, int S, int axis, int C, int L, int sp
     }
 }
 
+// The access pattern of the whole-cache-line x sweep (k_sweep_xt): blocks of
+// R rows x nseg segments, 64/S segments per wave, each lane loading 16 bytes
+// of its segment's next G-node chunk (G = 16 / value size), the chunk handed
+// through a per-wave LDS buffer (pitch (G+1)*S values per segment) to the
+// lanes of its lines, one barrier, and the same way back to 16-byte stores.
+template <typename T, int L, int S>
+__global__ void __launch_bounds__(1024) k_pattern_xt(T* rho, int n, int R, int nseg, int rows, int spread, T scale)
+{
+    constexpr int G = 16 / (int)sizeof(T), kSegW = 64 / S, kPitch = (G + 1) * S;
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_pattern[];
+    const int tid = threadIdx.x, qq = (tid & 63) / S, s = tid % S;
+    T* const mine = (T*)smem_pattern + (tid >> 6) * (kSegW * kPitch) + qq * kPitch;
+    const int Q = R * nseg, q = (tid >> 6) * kSegW + qq, qc = min(q, Q - 1);
+    const int rl = qc / nseg, g = qc - rl * nseg, p0 = g * L;
+    const int row = (int)xcd_map(blockIdx.x, gridDim.x, spread) * R + rl;
+    const bool act = q < Q && row < rows;
+    const size_t row0 = (size_t)min(row, rows - 1) * n * S;
+    const int uj = G * s / S, oj = G * s % S, wj = uj * S + oj;
+    u32x4 piece[L / G];
+#pragma unroll
+    for (int k = 0; k < L / G; ++k)
+        piece[k] = *(const u32x4*)(rho + row0 + (size_t)min(p0 + G * k + uj, n - 1) * S + oj);
+    T v[L];
+#pragma unroll
+    for (int k = 0; k < L / G; ++k) {
+        *(u32x4*)(mine + wj) = piece[k];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+#pragma unroll
+        for (int u = 0; u < G; ++u) v[G * k + u] = mine[u * S + s];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < L / G; ++k) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) mine[u * S + s] = v[G * k + u] * scale;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        const u32x4 out = *(const u32x4*)(mine + wj);
+        if (act && p0 + G * k + uj < n) *(u32x4*)(rho + row0 + (size_t)(p0 + G * k + uj) * S + oj) = out;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    }
+}
+
+template <typename T, int L>
+hipError_t launch_pattern_xt(T* rho, int n, int C, int spread, hipStream_t st)
+{
+    constexpr int S = 8, G = 16 / (int)sizeof(T);
+    const int R = C / S, nseg = (n + L - 1) / L, rows = n * n;
+    const int waves = (R * nseg + 64 / S - 1) / (64 / S);
+    const size_t lds = (size_t)waves * 64 * (G + 1) * sizeof(T);
+    const unsigned blocks = (unsigned)((rows + R - 1) / R);
+    k_pattern_xt<T, L, S><<<blocks, waves * 64, lds, st>>>(rho, n, R, nseg, rows, spread, T(1));
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t pattern_xt_dispatch(T* rho, int n, int C, int L, int spread, hipStream_t st)
+{
+    switch (L) {
+    case 8: return launch_pattern_xt<T, 8>(rho, n, C, spread, st);
+    case 16: return launch_pattern_xt<T, 16>(rho, n, C, spread, st);
+    case 20: return launch_pattern_xt<T, 20>(rho, n, C, spread, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 // Runs `launch` once untimed, then `reps` times between two events; average ms.
 template <typename F>
 int time_reps(F&& launch, int reps, double* ms)
@@ -186,6 +261,30 @@ int bfc_sweep_pattern(int elem, int n, int S, int axis, int C, int L, int spread
         [&] {
             return elem == 4 ? pattern_dispatch<float>((float*)rho, n, S, axis, C, L, spread, nullptr)
                              : pattern_dispatch<double>((double*)rho, n, S, axis, C, L, spread, nullptr);
+        },
+        reps, ms);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(rho);
+    *gbps = rc == 0 ? 2.0 * (double)bytes / (*ms * 1e-3) / 1e9 : 0.0;
+    return rc;
+}
+
+// The access pattern of the whole-cache-line x sweep on an n^3 grid of 8
+// substrates: C = 8R lines per block, L in {8, 16, 20} nodes per segment.
+int bfc_sweep_pattern_xt(int elem, int n, int C, int L, int spread, int reps, double* gbps, double* ms)
+{
+    constexpr int S = 8;
+    if ((elem != 4 && elem != 8) || C <= 0 || C % S || reps < 1 || n < 2) return 1;
+    const int nseg = (n + L - 1) / L;
+    if ((C / S) * nseg * S > 1024) return 1;
+    const size_t bytes = (size_t)n * n * n * S * elem;
+    void* rho = nullptr;
+    if (hipMalloc(&rho, bytes) != hipSuccess) return 3;
+    (void)hipMemset(rho, 0, bytes);
+    const int rc = time_reps(
+        [&] {
+            return elem == 4 ? pattern_xt_dispatch<float>((float*)rho, n, C, L, spread, nullptr)
+                             : pattern_xt_dispatch<double>((double*)rho, n, C, L, spread, nullptr);
         },
         reps, ms);
     (void)hipDeviceSynchronize();
