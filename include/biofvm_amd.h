@@ -222,15 +222,20 @@ int bfvm_get_coefficient_builds(bfvm_handle h, unsigned* builds);
  * lines per block, nodes per segment, and the kernel ("streaming"): 0 one
  * block per column tile, 1 resident blocks that load the next tile while
  * solving one, 2 (x only) tiles of whole rows loaded contiguously and
- * transposed through LDS, lines_per_block = rows per tile x substrates;
- * *lines_per_block = 0 means the one-thread-per-line fallback.  Any output
- * pointer may be NULL. */
+ * transposed through LDS, lines_per_block = rows per tile x substrates,
+ * 3 (x only, a node's values whole 16-byte pieces: FP32 with 4, 8 or 16
+ * substrates, FP64 with 2, 4, 8 or 16) 16-byte loads and stores of whole
+ * 4-node (FP32) or 2-node (FP64) chunks handed to the lines' lanes through
+ * LDS, lines_per_block = rows per block x substrates, nodes per segment 8,
+ * 16 or 20; *lines_per_block = 0 means the one-thread-per-line fallback.
+ * Any output pointer may be NULL. */
 int bfvm_get_sweep_plan(bfvm_handle h, int axis, int* lines_per_block, int* nodes_per_segment,
                         int* streaming);
 
 /* Sets the launch shape of one axis by hand (lines_per_block = 0 restores
  * the default).  Returns BFVM_E_ARG for a shape the kernel cannot run
- * (for kernel 2, the shapes bfvm_autotune would try). */
+ * (for kernels 2 and 3, the shapes bfvm_autotune would try).  Every shape
+ * with the same nodes per segment gives the same result bits. */
 int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_per_segment,
                         int streaming);
 
