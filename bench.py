@@ -465,7 +465,8 @@ def ensemble_result(steps, warmup, with_fp64):
     def run(precision):
         members = [setup_case(c, precision) for c in cases]
         ens = Ensemble(members)
-        L = ens.autotune(dt)
+        ens.autotune(dt)
+        L = ens.axis_segments()  # nodes per segment of the batched x, y, z sweeps
         ens.run(dt, warmup, ph, graph=True)
         ens.sync()
         t0 = time.perf_counter()
@@ -480,7 +481,7 @@ def ensemble_result(steps, warmup, with_fp64):
     sweep_steps = max(3, min(steps, 20))
     ens.run(dt, 2, 4, graph=True)  # the diffusion-only graph captured before the timing
     sweep_ms = ens.timed_run(dt, sweep_steps, 4, graph=True) / sweep_steps
-    knobs = {"segment_nodes": L, "x_row_tiles": ens.row_tiles, "yz_non_temporal": ens.non_temporal}
+    knobs = {"segment_nodes_xyz": L, "x_row_tiles": ens.row_tiles, "yz_non_temporal": ens.non_temporal}
     ens.close()
     for m in members:
         m.close()
@@ -509,7 +510,7 @@ def ensemble_result(steps, warmup, with_fp64):
         ens64.close()
         for m in mem64:
             m.close()
-        res["fp64"] = {"value": nvs * steps / wall64, "ms_per_step": 1e3 * wall64 / steps, "segment_nodes": L64}
+        res["fp64"] = {"value": nvs * steps / wall64, "ms_per_step": 1e3 * wall64 / steps, "segment_nodes_xyz": L64}
         res["error_vs_fp64"] = {"steps": warmup + steps + max(1, steps // 4),
                                 "max_rel_normwise_per_member": err_n, "l2_rel": (err_l2n / err_l2d) ** 0.5}
     return res

@@ -76,6 +76,7 @@ SIGNATURES = {
     "bfvm_ensemble_autotune": (_i, [_vp, _d]),
     "bfvm_ensemble_get_segment": (_i, [_vp, _ip]),
     "bfvm_ensemble_set_segment": (_i, [_vp, _i]),
+    "bfvm_ensemble_get_axis_segment": (_i, [_vp, _i, _ip]),
     "bfvm_ensemble_set_autotune_segments": (_i, [_vp, _i]),
     "bfvm_ensemble_set_row_tiles": (_i, [_vp, _i]),
     "bfvm_ensemble_get_row_tiles": (_i, [_vp, _ip]),

@@ -9,8 +9,11 @@
 //   - diffusion, per axis: Dirichlet launches for members that have nodes,
 //     then one k_sweep_batch launch over every member (members whose lines
 //     do not fit the batched kernel take their own sweep launch).
-// The launch shape of the batched sweep (nodes per segment L) is picked by
-// bfvm_ensemble_autotune.
+// The batched sweep of an axis runs at the segment length the members take
+// on their own (default_segment in lod.hip: the most common one among the
+// members; members with another length sweep on their own), so a member's
+// result equals its solo run bit for bit; bfvm_ensemble_set_segment fixes
+// one length for every axis and member instead.
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -23,7 +26,7 @@ struct bfvm_ensemble_t {
     std::vector<Env*> m;
     int precision = BFVM_FP64;
     hipStream_t stream = nullptr;
-    int L = 8;  // nodes per segment of the batched sweep
+    int L = 0;  // nodes per segment of every batched sweep; 0 = the members' own, per axis
     int spread = 0;  // XCD spread of the batched sweeps' blocks (xcd_tile in lod.hip)
     int nt = 0;      // y and z batched sweeps with non-temporal loads and stores (autotuned)
     int rows_x = 1;  // x sweeps on LDS-staged row tiles (k_sweep_rows_batch) where a member can
@@ -34,7 +37,7 @@ struct bfvm_ensemble_t {
     // tables, valid for (dt, L, member coefficient/agent epochs)
     bool tables_valid = false;
     double tab_dt = 0;
-    int tab_L = 0;
+    int tab_L[3] = {0, 0, 0};
     std::vector<unsigned> coef_epoch, agent_epoch, source_epoch, plan_epoch;
     bfvm::SweepArgs* d_args[3] = {nullptr, nullptr, nullptr};
     int* d_first[3] = {nullptr, nullptr, nullptr};  // per block: member, block within the member
@@ -118,9 +121,39 @@ void drop_graph(bfvm_ensemble_t& E)
 // Host-side preparation of one step: coefficients and bulk terms of every
 // member for dt, then the batched launch tables.  No kernel is enqueued
 // here except bulk preparation, so a step captured after this is launches only.
+// The segment length of the batched sweep of `axis`: the fixed one, or the
+// one the members' own sweeps of that axis take most (by values swept; the
+// shorter on a tie).
+int axis_segment(const bfvm_ensemble_t& E, int axis)
+{
+    if (E.L > 0) return E.L;
+    size_t best_n = 0;
+    int best = 8;
+    for (int L : {4, 8, 12, 16, 20, 24, 32}) {
+        size_t n = 0;
+        for (const Env* pm : E.m)
+            if (axis < pm->lod_dims && bfvm::default_segment(*pm, axis) == L) n += pm->nvs;
+        if (n > best_n) {
+            best_n = n;
+            best = L;
+        }
+    }
+    return best;
+}
+
+// Whether member m takes part in the batched sweep of `axis` at length L:
+// it fits the batched kernel and, with the members' own lengths, L is its own.
+bool joins_batch(const bfvm_ensemble_t& E, const Env& m, int axis, int L)
+{
+    return m.coef_valid && bfvm::batch_member_ok(m, axis, L) && (E.L > 0 || bfvm::default_segment(m, axis) == L);
+}
+
 int prepare(bfvm_ensemble_t& E, double dt, int flags)
 {
-    bool changed = !E.tables_valid || E.tab_dt != dt || E.tab_L != E.L || E.tab_rows_x != E.rows_x;
+    int Lax[3];
+    for (int a = 0; a < 3; ++a) Lax[a] = axis_segment(E, a);
+    bool changed = !E.tables_valid || E.tab_dt != dt || E.tab_rows_x != E.rows_x;
+    for (int a = 0; a < 3; ++a) changed = changed || E.tab_L[a] != Lax[a];
     for (size_t i = 0; i < E.m.size(); ++i) {
         Env& m = *E.m[i];
         if (flags & BFVM_DIFFUSION) {
@@ -149,7 +182,8 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
             bfvm::RowTile X;
             int threads = 0, tiles = 0;
             size_t lds = 0;
-            if (!m.coef_valid || !bfvm::batch_rows_args(m, E.L, &X, &threads, &lds, &tiles)) continue;
+            if (!joins_batch(E, m, 0, Lax[0]) || !bfvm::batch_rows_args(m, Lax[0], &X, &threads, &lds, &tiles))
+                continue;
             on_rows[i] = true;
             for (int t = 0; t < tiles; ++t) {
                 map.push_back((int)tabs.size());
@@ -182,11 +216,11 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
             Env& m = *E.m[i];
             if (axis >= m.lod_dims) continue;  // LOD_2D members have no z sweep
             if (axis == 0 && on_rows[i]) continue;
-            if (!m.coef_valid || !bfvm::batch_member_ok(m, axis, E.L)) {
+            if (!joins_batch(E, m, axis, Lax[axis])) {
                 E.solo[axis].push_back((int)i);
                 continue;
             }
-            bfvm::SweepArgs A = bfvm::batch_args(m, axis, E.L);
+            bfvm::SweepArgs A = bfvm::batch_args(m, axis, Lax[axis]);
             args.push_back(A);
             first.push_back(first.back() + A.tiles * A.planes);
             lds = std::max(lds, bfvm::batch_lds(A));
@@ -240,7 +274,7 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
         E.plan_epoch[i] = E.m[i]->plan_epoch;
     }
     E.tab_dt = dt;
-    E.tab_L = E.L;
+    for (int a = 0; a < 3; ++a) E.tab_L[a] = Lax[a];
     E.tab_rows_x = E.rows_x;
     E.tables_valid = true;
     return BFVM_OK;
@@ -265,11 +299,11 @@ hipError_t enqueue(bfvm_ensemble_t& E, int flags)
                     (st = on_stream(*pm, E.stream, [&] { return bfvm::launch_dirichlet(*pm); })))
                     return st;
             if (axis == 0 && E.rblocks > 0 &&
-                (st = bfvm::launch_rows_batch(E.precision, E.L, E.spread, E.d_rows, E.d_rmap, E.rblocks,
+                (st = bfvm::launch_rows_batch(E.precision, E.tab_L[0], E.spread, E.d_rows, E.d_rmap, E.rblocks,
                                               E.rthreads, E.rlds, E.stream)))
                 return st;
             if (E.nbatch[axis] > 0 &&
-                (st = bfvm::launch_sweep_batch(E.precision, E.L, E.spread, E.d_args[axis], E.d_first[axis],
+                (st = bfvm::launch_sweep_batch(E.precision, E.tab_L[axis], E.spread, E.d_args[axis], E.d_first[axis],
                                                E.nbatch[axis], E.blocks[axis], E.lds[axis], E.stream,
                                                axis > 0 && E.nt)))
                 return st;
@@ -438,7 +472,7 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
     double best = 1e300;
     int rc = BFVM_OK;
     std::vector<int> lengths{e->L};
-    if (e->tune_segments) lengths = {4, 8, 16, 32};
+    if (e->tune_segments) lengths = {0, 4, 8, 16, 32};  // 0: the members' own lengths
     for (int L : lengths) {
         for (int spread : {0, 64, -1}) {
             for (int rows : {0, 1}) {
@@ -495,13 +529,21 @@ int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment)
     return BFVM_OK;
 }
 
+int bfvm_ensemble_get_axis_segment(bfvm_ensemble e, int axis, int* nodes_per_segment)
+{
+    bfvm::RandGuard rand_guard;
+    if (!e || !nodes_per_segment || axis < 0 || axis > 2) return BFVM_E_ARG;
+    *nodes_per_segment = axis_segment(*e, axis);
+    return BFVM_OK;
+}
+
 int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment)
 {
     bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
-    if (nodes_per_segment != 4 && nodes_per_segment != 8 && nodes_per_segment != 16 &&
+    if (nodes_per_segment != 0 && nodes_per_segment != 4 && nodes_per_segment != 8 && nodes_per_segment != 16 &&
         nodes_per_segment != 32)
-        return efail(e, BFVM_E_ARG, "nodes per segment must be 4, 8, 16 or 32");
+        return efail(e, BFVM_E_ARG, "nodes per segment must be 0 (the members' own), 4, 8, 16 or 32");
     e->L = nodes_per_segment;
     e->tables_valid = false;
     drop_graph(*e);

@@ -380,6 +380,15 @@ class Ensemble:
     def segment(self, nodes):
         self._check(self.lib.bfvm_ensemble_set_segment(self.e, int(nodes)))
 
+    def axis_segments(self):
+        """Nodes per segment the batched x, y and z sweeps run at."""
+        out = []
+        for axis in range(3):
+            v = ctypes.c_int(0)
+            self._check(self.lib.bfvm_ensemble_get_axis_segment(self.e, axis, ctypes.byref(v)))
+            out.append(v.value)
+        return out
+
     @property
     def row_tiles(self):
         """x sweeps on LDS-staged row tiles (True) or column tiles (False)."""

@@ -397,12 +397,19 @@ int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, in
  * (default or non-temporal) on scratch copies of the stores and keeps the
  * fastest.  Densities unchanged. */
 int bfvm_ensemble_autotune(bfvm_ensemble e, double dt);
+/* Nodes per segment of the batched sweeps: 0 (the default) = each axis at
+ * the length the members' own sweeps take (the most common one among them,
+ * by values; a member whose own length differs sweeps that axis on its own),
+ * so every member's result equals its solo run bit for bit; 4, 8, 16 or 32 =
+ * that length for every axis and member. */
 int bfvm_ensemble_get_segment(bfvm_ensemble e, int* nodes_per_segment);
 int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment);
-/* Whether bfvm_ensemble_autotune also times the segment lengths (1), which
- * changes the rounding of the batched solve, or keeps the ensemble's segment
- * length (0, the default: 8 nodes, or what bfvm_ensemble_set_segment set) and
- * tunes only the knobs that leave the bits unchanged. */
+/* The length the batched sweep of `axis` runs at (0 = x, 1 = y, 2 = z). */
+int bfvm_ensemble_get_axis_segment(bfvm_ensemble e, int axis, int* nodes_per_segment);
+/* Whether bfvm_ensemble_autotune also times the segment lengths (1: the
+ * members' own, 4, 8, 16 and 32), which changes the rounding of the batched
+ * solve, or keeps the ensemble's segment length (0, the default) and tunes
+ * only the knobs that leave the bits unchanged. */
 int bfvm_ensemble_set_autotune_segments(bfvm_ensemble e, int on);
 /* Whether the members' x sweeps run on LDS-staged row tiles (on = 1, the
  * default; members that cannot take part run with column tiles) or all on

@@ -118,6 +118,29 @@ def test_ensemble_equals_single_runs_after_changes(mixed):
         assert rel_err(got[i], ref) <= TOL[FP64], i
 
 
+def test_members_at_their_own_segment_lengths_equal_solo_runs():
+    # by default each axis is batched at the length most members take on
+    # their own (FP32 y and z lines of 32 nodes or more: 16; shorter ones and
+    # x lines below 200: 8) and members with another length sweep on their
+    # own: every member's bits equal its solo run's
+    cases = W.ensemble_mixed(seed=5, steps=3)
+    members = _members(cases, FP32)
+    ens = Ensemble(members)
+    try:
+        assert ens.segment == 0
+        assert ens.axis_segments() == [8, 8, 16]  # one 40-node z axis holds most of the values
+        ens.run(cases[0]["dt"], 3, ALL, graph=True)
+        got = [m.download() for m in members]
+        ens.segment = 8
+        assert ens.axis_segments() == [8, 8, 8]
+    finally:
+        _close(ens, members)
+    for i, c in enumerate(cases):
+        with setup_case(c, FP32) as m:
+            m.run(c["dt"], 3, ALL)
+            np.testing.assert_array_equal(got[i], m.download(), err_msg=f"member {i}")
+
+
 def test_autotune_keeps_densities(mixed):
     cases, refs = mixed
     members = _members(cases, FP32)
@@ -125,7 +148,7 @@ def test_autotune_keeps_densities(mixed):
     try:
         before = [m.download() for m in members]
         L = ens.autotune(cases[0]["dt"])
-        assert L in (4, 8, 16, 32)
+        assert L in (0, 4, 8, 16, 32)  # 0: the members' own lengths
         for m, b in zip(members, before):
             np.testing.assert_array_equal(m.download(), b)
         ens.run(cases[0]["dt"], cases[0]["steps"], ALL, graph=True)
