@@ -481,7 +481,8 @@ def ensemble_result(steps, warmup, with_fp64):
     sweep_steps = max(3, min(steps, 20))
     ens.run(dt, 2, 4, graph=True)  # the diffusion-only graph captured before the timing
     sweep_ms = ens.timed_run(dt, sweep_steps, 4, graph=True) / sweep_steps
-    knobs = {"segment_nodes_xyz": L, "x_row_tiles": ens.row_tiles, "yz_non_temporal": ens.non_temporal}
+    knobs = {"segment_nodes_xyz": L, "x_chunks": ens.x_chunks, "x_row_tiles": ens.row_tiles,
+             "yz_non_temporal": ens.non_temporal}
     ens.close()
     for m in members:
         m.close()

@@ -77,6 +77,10 @@ SIGNATURES = {
     "bfvm_ensemble_get_segment": (_i, [_vp, _ip]),
     "bfvm_ensemble_set_segment": (_i, [_vp, _i]),
     "bfvm_ensemble_get_axis_segment": (_i, [_vp, _i, _ip]),
+    "bfvm_ensemble_set_graph_steps": (_i, [_vp, _i]),
+    "bfvm_ensemble_set_x_chunks": (_i, [_vp, _i]),
+    "bfvm_ensemble_get_x_chunks": (_i, [_vp, _ip]),
+    "bfvm_ensemble_get_graph_steps": (_i, [_vp, _ip]),
     "bfvm_ensemble_set_autotune_segments": (_i, [_vp, _i]),
     "bfvm_ensemble_set_row_tiles": (_i, [_vp, _i]),
     "bfvm_ensemble_get_row_tiles": (_i, [_vp, _ip]),

@@ -85,6 +85,9 @@ struct SweepArgs {
     // 64, one segment per wave), so a z line may span more than 4 GiB of
     // addresses: the store's z sweep above 4 GiB
     int seg_base;
+    // values per segment in the chunk buffers of the whole-cache-line x
+    // sweep's runtime-S form (k_sweep_xt in lod.hip); 0 for other kernels
+    int xt_pitch;
 };
 
 // Arguments of the x sweep on LDS-staged row tiles (k_sweep_rows in lod.hip):
@@ -275,6 +278,12 @@ hipError_t launch_rows_batch(int precision, int L, int spread, const RowTile* d_
                              int blocks, int threads, size_t lds, hipStream_t st);
 hipError_t launch_sweep_batch(int precision, int L, int spread, const SweepArgs* d_args, const int* d_map,
                               int nm, int blocks, size_t lds, hipStream_t st, bool nt = false);
+// Whole-cache-line x sweep arguments of a grid for an ensemble's batched
+// launch of `threads`-thread blocks (false: the grid cannot take part: FP32,
+// rows of a multiple of 4 nodes, L of 8, 16 or 20), and that launch.
+bool batch_xt_args(const Env& e, int L, int threads, SweepArgs* A, size_t* lds, int* tiles);
+hipError_t launch_xt_batch(int L, int spread, const SweepArgs* d_args, const int* d_map, int blocks, int threads,
+                           size_t lds, hipStream_t st);
 
 // Agents of several grids in one launch: one entry per member.
 struct AgentBatch {

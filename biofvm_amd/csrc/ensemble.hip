@@ -29,6 +29,7 @@ struct bfvm_ensemble_t {
     int L = 0;  // nodes per segment of every batched sweep; 0 = the members' own, per axis
     int spread = 0;  // XCD spread of the batched sweeps' blocks (xcd_tile in lod.hip)
     int nt = 0;      // y and z batched sweeps with non-temporal loads and stores (autotuned)
+    int xt_x = 1;    // x sweeps with whole-cache-line chunk loads (k_sweep_xt_batch) where a member can
     int rows_x = 1;  // x sweeps on LDS-staged row tiles (k_sweep_rows_batch) where a member can
     // whether bfvm_ensemble_autotune may change L (which changes the rounding
     // of the batched solve); otherwise it tunes the other knobs at this L
@@ -51,13 +52,23 @@ struct bfvm_ensemble_t {
     int rblocks = 0, rthreads = 0;
     size_t rlds = 0;
     int tab_rows_x = -1;
+    // x sweeps with whole-cache-line chunk loads: per-member arguments, per-block (member, tile)
+    bfvm::SweepArgs* d_xt = nullptr;
+    int* d_xtmap = nullptr;
+    int xtblocks = 0;
+    size_t xtlds = 0;
+    int tab_xt_x = -1;
     bfvm::AgentBatch* d_agents = nullptr;
     int n_agent_members = 0;
     long long agent_threads = 0;
 
-    // graph of one step
+    // graphs of one step and of graph_steps steps (replays of the latter
+    // carry runs of that many steps: one launch gap instead of one per step)
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
+    hipGraphExec_t exec_multi = nullptr;
+    hipGraph_t graph_multi = nullptr;
+    int graph_steps = 8;
     double graph_dt = 0;
     int graph_flags = -1;
 
@@ -65,6 +76,10 @@ struct bfvm_ensemble_t {
 };
 
 namespace {
+
+// Threads per block of the batched whole-cache-line x sweep: each member's
+// blocks hold as many rows as fill 8 waves with segments.
+constexpr int kXtBatchThreads = 512;
 
 int efail(bfvm_ensemble_t* e, int code, const std::string& msg)
 {
@@ -106,6 +121,12 @@ void free_tables(bfvm_ensemble_t& E)
     E.d_rows = nullptr;
     E.d_rmap = nullptr;
     E.rblocks = 0;
+    if (E.d_xt) (void)hipFree(E.d_xt);
+    if (E.d_xtmap) (void)hipFree(E.d_xtmap);
+    E.d_xt = nullptr;
+    E.d_xtmap = nullptr;
+    E.xtblocks = 0;
+    E.xtlds = 0;
     E.tables_valid = false;
 }
 
@@ -113,8 +134,12 @@ void drop_graph(bfvm_ensemble_t& E)
 {
     if (E.exec) (void)hipGraphExecDestroy(E.exec);
     if (E.graph) (void)hipGraphDestroy(E.graph);
+    if (E.exec_multi) (void)hipGraphExecDestroy(E.exec_multi);
+    if (E.graph_multi) (void)hipGraphDestroy(E.graph_multi);
     E.exec = nullptr;
     E.graph = nullptr;
+    E.exec_multi = nullptr;
+    E.graph_multi = nullptr;
     E.graph_flags = -1;
 }
 
@@ -152,7 +177,7 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
 {
     int Lax[3];
     for (int a = 0; a < 3; ++a) Lax[a] = axis_segment(E, a);
-    bool changed = !E.tables_valid || E.tab_dt != dt || E.tab_rows_x != E.rows_x;
+    bool changed = !E.tables_valid || E.tab_dt != dt || E.tab_rows_x != E.rows_x || E.tab_xt_x != E.xt_x;
     for (int a = 0; a < 3; ++a) changed = changed || E.tab_L[a] != Lax[a];
     for (size_t i = 0; i < E.m.size(); ++i) {
         Env& m = *E.m[i];
@@ -170,7 +195,34 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
     if (!changed) return BFVM_OK;
     drop_graph(E);
     free_tables(E);
-    std::vector<bool> on_rows(E.m.size(), false);
+    std::vector<bool> on_rows(E.m.size(), false), on_xt(E.m.size(), false);
+    if (E.xt_x) {
+        // x sweeps with whole-cache-line chunk loads, one launch for every member that can
+        std::vector<bfvm::SweepArgs> args;
+        std::vector<int> map;
+        for (size_t i = 0; i < E.m.size(); ++i) {
+            Env& m = *E.m[i];
+            bfvm::SweepArgs A;
+            size_t lds = 0;
+            int tiles = 0;
+            if (!joins_batch(E, m, 0, Lax[0]) || !bfvm::batch_xt_args(m, Lax[0], kXtBatchThreads, &A, &lds, &tiles))
+                continue;
+            on_xt[i] = true;
+            for (int t = 0; t < tiles; ++t) {
+                map.push_back((int)args.size());
+                map.push_back(t);
+            }
+            args.push_back(A);
+            E.xtlds = std::max(E.xtlds, lds);
+        }
+        if (!args.empty()) {
+            E.xtblocks = (int)(map.size() / 2);
+            EHIP(&E, hipMalloc((void**)&E.d_xt, sizeof(bfvm::SweepArgs) * args.size()));
+            EHIP(&E, hipMalloc((void**)&E.d_xtmap, sizeof(int) * map.size()));
+            EHIP(&E, hipMemcpy(E.d_xt, args.data(), sizeof(bfvm::SweepArgs) * args.size(), hipMemcpyHostToDevice));
+            EHIP(&E, hipMemcpy(E.d_xtmap, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
+        }
+    }
     if (E.rows_x) {
         // x sweeps on row tiles, one launch for every member that can
         std::vector<bfvm::RowTile> tabs;
@@ -182,7 +234,8 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
             bfvm::RowTile X;
             int threads = 0, tiles = 0;
             size_t lds = 0;
-            if (!joins_batch(E, m, 0, Lax[0]) || !bfvm::batch_rows_args(m, Lax[0], &X, &threads, &lds, &tiles))
+            if (on_xt[i] || !joins_batch(E, m, 0, Lax[0]) ||
+                !bfvm::batch_rows_args(m, Lax[0], &X, &threads, &lds, &tiles))
                 continue;
             on_rows[i] = true;
             for (int t = 0; t < tiles; ++t) {
@@ -215,7 +268,7 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
         for (size_t i = 0; i < E.m.size(); ++i) {
             Env& m = *E.m[i];
             if (axis >= m.lod_dims) continue;  // LOD_2D members have no z sweep
-            if (axis == 0 && on_rows[i]) continue;
+            if (axis == 0 && (on_rows[i] || on_xt[i])) continue;
             if (!joins_batch(E, m, axis, Lax[axis])) {
                 E.solo[axis].push_back((int)i);
                 continue;
@@ -276,6 +329,7 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
     E.tab_dt = dt;
     for (int a = 0; a < 3; ++a) E.tab_L[a] = Lax[a];
     E.tab_rows_x = E.rows_x;
+    E.tab_xt_x = E.xt_x;
     E.tables_valid = true;
     return BFVM_OK;
 }
@@ -298,6 +352,10 @@ hipError_t enqueue(bfvm_ensemble_t& E, int flags)
                 if (pm->n_dir > 0 && axis < pm->lod_dims &&
                     (st = on_stream(*pm, E.stream, [&] { return bfvm::launch_dirichlet(*pm); })))
                     return st;
+            if (axis == 0 && E.xtblocks > 0 &&
+                (st = bfvm::launch_xt_batch(E.tab_L[0], E.spread, E.d_xt, E.d_xtmap, E.xtblocks, kXtBatchThreads,
+                                            E.xtlds, E.stream)))
+                return st;
             if (axis == 0 && E.rblocks > 0 &&
                 (st = bfvm::launch_rows_batch(E.precision, E.tab_L[0], E.spread, E.d_rows, E.d_rmap, E.rblocks,
                                               E.rthreads, E.rlds, E.stream)))
@@ -315,12 +373,12 @@ hipError_t enqueue(bfvm_ensemble_t& E, int flags)
     return hipSuccess;
 }
 
-int ensure_graph(bfvm_ensemble_t& E, double dt, int flags)
+// Captures `steps` steps' launches into a graph and instantiates it.
+int capture_steps(bfvm_ensemble_t& E, int flags, int steps, hipGraph_t* graph, hipGraphExec_t* exec)
 {
-    if (E.exec && E.graph_dt == dt && E.graph_flags == flags) return BFVM_OK;
-    drop_graph(E);
     EHIP(&E, hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
-    hipError_t st = enqueue(E, flags);
+    hipError_t st = hipSuccess;
+    for (int n = 0; n < steps && st == hipSuccess; ++n) st = enqueue(E, flags);
     hipGraph_t g = nullptr;
     hipError_t st2 = hipStreamEndCapture(E.stream, &g);
     if (st != hipSuccess || st2 != hipSuccess) {
@@ -328,26 +386,44 @@ int ensure_graph(bfvm_ensemble_t& E, double dt, int flags)
         return efail(&E, BFVM_E_DEVICE, std::string("graph capture: ") +
                                             bfvm::hip_error_text(st != hipSuccess ? st : st2));
     }
-    E.graph = g;
-    EHIP(&E, hipGraphInstantiate(&E.exec, g, nullptr, nullptr, 0));
-    E.graph_dt = dt;
-    E.graph_flags = flags;
+    *graph = g;
+    EHIP(&E, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return BFVM_OK;
+}
+
+// The one-step graph, and the graph_steps-step one when a run holds that many.
+int ensure_graph(bfvm_ensemble_t& E, double dt, int flags, int steps)
+{
+    if (!(E.exec && E.graph_dt == dt && E.graph_flags == flags)) {
+        drop_graph(E);
+        int rc = capture_steps(E, flags, 1, &E.graph, &E.exec);
+        if (rc) return rc;
+        E.graph_dt = dt;
+        E.graph_flags = flags;
+    }
+    if (E.graph_steps > 1 && steps >= E.graph_steps && !E.exec_multi)
+        return capture_steps(E, flags, E.graph_steps, &E.graph_multi, &E.exec_multi);
     return BFVM_OK;
 }
 
 // Everything before the launches of `steps` steps: members' own work
 // drained, coefficients and tables for dt, the graph if asked for.
-int ready(bfvm_ensemble_t& E, double dt, int flags, int use_graph)
+int ready(bfvm_ensemble_t& E, double dt, int flags, int use_graph, int steps)
 {
     for (Env* pm : E.m) EHIP(&E, hipStreamSynchronize(pm->stream));
     int rc = prepare(E, dt, flags);
-    if (rc == BFVM_OK && use_graph) rc = ensure_graph(E, dt, flags);
+    if (rc == BFVM_OK && use_graph) rc = ensure_graph(E, dt, flags, steps);
     return rc;
 }
 
+// Graph runs: whole graph_steps-step graphs while they fit, then one-step ones
+// (the same launches in the same order as the eager steps).
 int launch_steps(bfvm_ensemble_t& E, int steps, int flags, int use_graph)
 {
-    for (int n = 0; n < steps; ++n) {
+    int n = 0;
+    if (use_graph && E.exec_multi)
+        for (; n + E.graph_steps <= steps; n += E.graph_steps) EHIP(&E, hipGraphLaunch(E.exec_multi, E.stream));
+    for (; n < steps; ++n) {
         if (use_graph) EHIP(&E, hipGraphLaunch(E.exec, E.stream));
         else EHIP(&E, enqueue(E, flags));
     }
@@ -406,7 +482,7 @@ int bfvm_ensemble_run(bfvm_ensemble e, double dt, int steps, int flags, int use_
     bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     if (steps < 0) return efail(e, BFVM_E_ARG, "steps < 0");
-    int rc = ready(*e, dt, flags, use_graph);
+    int rc = ready(*e, dt, flags, use_graph, steps);
     return rc ? rc : launch_steps(*e, steps, flags, use_graph);
 }
 
@@ -424,7 +500,7 @@ int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, in
     bfvm::RandGuard rand_guard;
     if (!e) return BFVM_E_ARG;
     if (steps < 0) return efail(e, BFVM_E_ARG, "steps < 0");
-    int rc = ready(*e, dt, flags, use_graph);
+    int rc = ready(*e, dt, flags, use_graph, steps);
     if (rc) return rc;
     hipEvent_t t0, t1;
     EHIP(e, hipEventCreate(&t0));
@@ -468,18 +544,19 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
         e->m[i]->rho = scratch[i];
         ++e->m[i]->coef_epoch;  // launch tables point at the scratch stores
     }
-    int best_L = e->L, best_spread = e->spread, best_rows = e->rows_x, best_nt = e->nt;
+    int best_L = e->L, best_spread = e->spread, best_rows = e->rows_x, best_nt = e->nt, best_xt = e->xt_x;
     double best = 1e300;
     int rc = BFVM_OK;
     std::vector<int> lengths{e->L};
     if (e->tune_segments) lengths = {0, 4, 8, 16, 32};  // 0: the members' own lengths
     for (int L : lengths) {
         for (int spread : {0, 64, -1}) {
-            for (int rows : {0, 1}) {
+            for (int xrow : {0, 1, 2}) {  // x: column tiles, row tiles, whole-cache-line chunks (+ row tiles)
                 for (int nt : {0, 1}) {
                     e->L = L;
                     e->spread = spread;
-                    e->rows_x = rows;
+                    e->rows_x = xrow > 0;
+                    e->xt_x = xrow == 2;
                     e->nt = nt;
                     double ms = 0;
                     if ((rc = bfvm_ensemble_timed_run(e, dt, 1, BFVM_DIFFUSION, 0, &ms))) break;
@@ -488,7 +565,8 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
                         best = ms;
                         best_L = L;
                         best_spread = spread;
-                        best_rows = rows;
+                        best_rows = e->rows_x;
+                        best_xt = e->xt_x;
                         best_nt = nt;
                     }
                 }
@@ -507,10 +585,29 @@ int bfvm_ensemble_autotune(bfvm_ensemble e, double dt)
     e->L = best_L;
     e->spread = best_spread;
     e->rows_x = best_rows;
+    e->xt_x = best_xt;
     e->nt = best_nt;
     e->tables_valid = false;
     drop_graph(*e);
     return rc;
+}
+
+int bfvm_ensemble_set_graph_steps(bfvm_ensemble e, int steps_per_graph)
+{
+    bfvm::RandGuard rand_guard;
+    if (!e) return BFVM_E_ARG;
+    if (steps_per_graph < 1) return efail(e, BFVM_E_ARG, "steps per graph must be at least 1");
+    e->graph_steps = steps_per_graph;
+    drop_graph(*e);
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_get_graph_steps(bfvm_ensemble e, int* steps_per_graph)
+{
+    bfvm::RandGuard rand_guard;
+    if (!e || !steps_per_graph) return BFVM_E_ARG;
+    *steps_per_graph = e->graph_steps;
+    return BFVM_OK;
 }
 
 int bfvm_ensemble_set_autotune_segments(bfvm_ensemble e, int on)
@@ -547,6 +644,24 @@ int bfvm_ensemble_set_segment(bfvm_ensemble e, int nodes_per_segment)
     e->L = nodes_per_segment;
     e->tables_valid = false;
     drop_graph(*e);
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_set_x_chunks(bfvm_ensemble e, int on)
+{
+    bfvm::RandGuard rand_guard;
+    if (!e) return BFVM_E_ARG;
+    e->xt_x = on != 0;
+    e->tables_valid = false;
+    drop_graph(*e);
+    return BFVM_OK;
+}
+
+int bfvm_ensemble_get_x_chunks(bfvm_ensemble e, int* on)
+{
+    bfvm::RandGuard rand_guard;
+    if (!e || !on) return BFVM_E_ARG;
+    *on = e->xt_x;
     return BFVM_OK;
 }
 

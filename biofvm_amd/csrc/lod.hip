@@ -823,22 +823,29 @@ k_sweep_rows_batch(const RowTile* __restrict__ tabs, const int* __restrict__ map
 // / value size): lane j of a segment loads the 16 bytes j of the segment's
 // next chunk, and the chunk goes through a per-wave LDS buffer to the lanes
 // that own its lines (lane s takes node u of line s); the solved values go
-// back the same way and are stored 16 bytes per lane.  Needs S | 64, G | S
-// and G | L (FP32: S in {4, 8, 16}; FP64: S in {2, 4, 8, 16}).
+// back the same way and are stored 16 bytes per lane.  Two forms:
+//   SC = S, a compile-time 4, 8 or 16 (FP32) or 2, 4, 8 or 16 (FP64): a
+//     lane's 16 bytes lie in one node, any row length;
+//   SC = 0, any S up to 64 (FP32): a lane's 16 bytes may span two nodes;
+//     rows of a multiple of 4 nodes, so every chunk starts 16-byte aligned
+//     and lies wholly inside or wholly past the row.
 //
 // Block: R rows (C = R*S lines, column rl*S + s) x nseg segments; its
-// Q = R*nseg segments q = rl*nseg + g are dealt to the waves, 64/S per
-// wave (wave w holds q = w*64/S + qq on lanes qq*S .. qq*S + S-1), so a
-// wave's load instruction covers whole chunks of 64/S segments.  Threads
+// Q = R*nseg segments q = rl*nseg + g are dealt to the waves, segw = 64/S
+// (rounded down) per wave (wave w holds q = w*segw + qq on lanes
+// qq*S .. qq*S + S-1; lanes past segw*S copy the wave's last segment), so a
+// wave's load instruction covers whole chunks of segw segments.  Threads
 // past segment Q-1 copy it (same loads, same values, no stores); rows past
 // the last are copies of it (no stores).  The solve is seg_core's:
 // bit-identical to k_sweep with the same L.
 //
 // LDS: the correction tables (stage_tables), the carry slots xa, xb (Q*S
-// each), wfirst, wlast (C each), then one chunk buffer per wave: segment qq's
-// chunk at qq*(G+1)*S values (one node of padding), node u of line s at
-// u*S + s in it, so that the 64 lanes reading one node fall in different
-// banks and every lane's LDS offsets are one register plus constants.
+// each), wfirst, wlast (C each), then one chunk buffer per wave (at an even
+// double offset): segment qq's chunk at qq*P values, node u of line s at
+// u*S + s in it; P = (G+1)*S for SC (one node of padding: the 64 lanes
+// reading one node fall in different banks and every lane's LDS offsets are
+// one register plus constants), else SweepArgs::xt_pitch, a multiple of 4
+// picked on the host for few bank conflicts.
 
 // Orders the LDS accesses of one wave: a wave's LDS operations complete in
 // order, so this only keeps the compiler from moving them across it.
@@ -854,48 +861,58 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 template <typename T, int S>
 constexpr bool xt_shape_ok(int L)
 {
+    if (S == 0) return sizeof(T) == 4 && L % 4 == 0;  // runtime S: FP32
     return 16 % sizeof(T) == 0 && S % (16 / (int)sizeof(T)) == 0 && 64 % S == 0 && S <= 16 &&
            L % (16 / (int)sizeof(T)) == 0;
 }
 
-// Bytes of one wave's chunk buffer.
-template <typename T>
-constexpr int xt_wave_lds()
+// Doubles before the chunk buffers (tables, carries, w at the line ends),
+// rounded up to an even count so the buffers are 16-byte aligned.
+__host__ __device__ __forceinline__ int xt_buffer_offset(int kg0, int kg1, int S, int Q, int C)
 {
-    return 64 * (16 / (int)sizeof(T) + 1) * (int)sizeof(T);
+    const int d = (kg0 + kg1 + 3 * kPad + 8) * S + 2 * Q * S + 2 * C;
+    return (d + 1) & ~1;
 }
 
-template <typename T, int L, int S>
-__global__ void __launch_bounds__(kMaxThreads, L <= 20 ? 8 : 4) k_sweep_xt(SweepArgs A)
+// One block's tile of the whole-cache-line x sweep (see above).
+template <typename T, int L, int SC>
+__device__ __forceinline__ void xt_tile(const SweepArgs& A, unsigned tile, double* smem)
 {
     constexpr int G = 16 / (int)sizeof(T);  // nodes per chunk = values per lane piece
-    constexpr int kSegW = 64 / S;            // segments per wave
-    constexpr int kPitch = (G + 1) * S;      // values per segment in the chunk buffer
-    static_assert(xt_shape_ok<T, S>(L), "k_sweep_xt shape");
-    extern __shared__ __attribute__((aligned(16))) double smem_xt[];
+    static_assert(xt_shape_ok<T, SC>(L), "k_sweep_xt shape");
+    const int S = SC ? SC : A.S;
+    const int segw = SC ? 64 / SC : 64 / S;  // segments per wave
+    const int P = SC ? (G + 1) * SC : A.xt_pitch;
     const int C = A.cols, nseg = A.nseg, n = A.n;
     const int R = C / S, Q = R * nseg;
-    double* gtab = smem_xt;
+    double* gtab = smem;
     double* xa = gtab + (A.kg0 + A.kg1 + 3 * kPad + 8) * S;
     double* xb = xa + Q * S;
     double* wfirst = xb + Q * S;
     double* wlast = wfirst + C;
-    const int tid = threadIdx.x, qq = (tid & 63) / S, s = tid % S;
+    const int tid = threadIdx.x, l = tid & 63;
+    int qq = l / S, s = l - qq * S;
+    const bool dup = !SC && qq >= segw;
+    if (dup) {  // lanes past the wave's whole segments copy its last one (and walk no chain)
+        qq = segw - 1;
+        s = l - segw * S;
+    }
     // this lane's segment's part of the wave's chunk buffer
-    T* const mine = (T*)(wlast + C) + (tid >> 6) * (kSegW * kPitch) + qq * kPitch;
-    const int q = (tid >> 6) * kSegW + qq;
+    T* const mine =
+        (T*)(smem + xt_buffer_offset(A.kg0, A.kg1, S, Q, C)) + (tid >> 6) * (segw * P) + qq * P;
+    const int q = (tid >> 6) * segw + qq;
     const int qc = min(q, Q - 1);
     const int rl = qc / nseg, g = qc - rl * nseg;
     const int col = rl * S + s, p0 = g * L;
     const int rows = A.cols_per_plane / S;
-    const int row = (int)xcd_tile(blockIdx.x, gridDim.x, A.spread) * R + rl;
+    const int row = (int)tile * R + rl;
     const bool act = q < Q && row < rows;
     const unsigned row0 = (unsigned)min(row, rows - 1) * A.vstride;  // values before the row
-    const int uj = G * s / S, oj = G * s % S;  // this lane's piece: node uj, values oj .. oj+G-1
-    const int wj = uj * S + oj;                // the piece's place in the chunk buffer
+    const int uj = G * s / S;  // node of the chunk where this lane's piece starts
+    const int wj = G * s;      // the piece's place in the chunk buffer
     // byte offset of this lane's piece of chunk 0; chunk k is k*G*S values on
-    constexpr unsigned kChunk = (unsigned)(G * S * sizeof(T));
-    const unsigned vo = (row0 + (unsigned)(p0 + uj) * S + oj) * (unsigned)sizeof(T);
+    const unsigned chunk = (unsigned)(G * S * sizeof(T));
+    const unsigned vo = (row0 + (unsigned)p0 * S + wj) * (unsigned)sizeof(T);
 
     const double r = A.wb[s * 8 + 0], a = A.wb[s * 8 + 1];
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(A.rho, A.nvs * sizeof(T));
@@ -903,13 +920,20 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 20 ? 8 : 4) k_sweep_xt(Sweep
     if (p0 + L <= n) {
 #pragma unroll
         for (int k = 0; k < L / G; ++k)
-            piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, k * kChunk, kLoadPolicy);
-    } else {  // the segment holding node n-1: nodes past it read node n-1
+            piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, k * chunk, kLoadPolicy);
+    } else if (SC) {  // the segment holding node n-1: nodes past it read node n-1
+        const int oj = G * s - uj * S;
 #pragma unroll
         for (int k = 0; k < L / G; ++k) {
             const unsigned node = (unsigned)min(p0 + G * k + uj, n - 1);
             piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, (row0 + node * S + oj) * (unsigned)sizeof(T), 0,
                                                              kLoadPolicy);
+        }
+    } else {  // rows of whole chunks: chunks past node n-1 read the row's last chunk
+#pragma unroll
+        for (int k = 0; k < L / G; ++k) {
+            const int kk = min(k, (n - p0) / G - 1);
+            piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, kk * chunk, kLoadPolicy);
         }
     }
     stage_tables(A, gtab);  // while the loads are in flight
@@ -925,18 +949,43 @@ __global__ void __launch_bounds__(kMaxThreads, L <= 20 ? 8 : 4) k_sweep_xt(Sweep
 
     const __amdgpu_buffer_rsrc_t ro = make_rsrc(A.out, A.nvs * sizeof(T));
     const int slot = g * C + col;
-    seg_core_at<T, L>(A, C, col, s, p0, slot, g == nseg - 1, slot, q < Q && g == 0, raw, r, a, xa, xb, wfirst,
-                      wlast, gtab, NoSource{}, [&](int t, double zf) {
+    // a chunk is stored when its lane's piece lies in the row (SC: the
+    // piece's node; SC = 0: the chunk's first node, chunks being whole)
+    const int store_node = SC ? p0 + uj : p0;
+    seg_core_at<T, L>(A, C, col, s, p0, slot, g == nseg - 1, slot, q < Q && g == 0 && !dup, raw, r, a, xa, xb,
+                      wfirst, wlast, gtab, NoSource{}, [&](int t, double zf) {
                           mine[(t % G) * S + s] = (T)zf;
                           if (t % G == 0) {  // chunk t / G is complete
                               wave_lds_sync();
                               const u32x4 out = *(const u32x4*)(mine + wj);
-                              if (act && p0 + t + uj < n)
-                                  __builtin_amdgcn_raw_buffer_store_b128(out, ro, vo, (t / G) * kChunk,
+                              if (act && !dup && store_node + t < n)
+                                  __builtin_amdgcn_raw_buffer_store_b128(out, ro, vo, (t / G) * chunk,
                                                                          kStorePolicy);
                               wave_lds_sync();
                           }
                       });
+}
+
+template <typename T, int L, int SC>
+__global__ void __launch_bounds__(kMaxThreads, L <= 20 ? 8 : 4) k_sweep_xt(SweepArgs A)
+{
+    extern __shared__ __attribute__((aligned(16))) double smem_xt[];
+    xt_tile<T, L, SC>(A, xcd_tile(blockIdx.x, gridDim.x, A.spread), smem_xt);
+}
+
+// Whole-cache-line x sweeps of several grids (runtime S): map[2b], map[2b+1] =
+// the grid of block b and b's tile within it; blockDim and the LDS fit every
+// grid's tiles (a grid needing fewer waves has its extra ones copy).
+template <typename T, int L>
+__global__ void __launch_bounds__(kMaxThreads, 8)
+k_sweep_xt_batch(const SweepArgs* __restrict__ args, const int* __restrict__ map, int spread)
+{
+    extern __shared__ __attribute__((aligned(16))) double smem_xtb[];
+    const unsigned b = xcd_tile(blockIdx.x, gridDim.x, spread);
+    const int member = __builtin_amdgcn_readfirstlane(map[2 * b]);
+    const int local = __builtin_amdgcn_readfirstlane(map[2 * b + 1]);
+    const SweepArgs A = args[member];
+    xt_tile<T, L, 0>(A, (unsigned)local, smem_xtb);
 }
 
 // One thread owns one whole (line, substrate) pair and runs the reference's
@@ -1039,17 +1088,20 @@ SweepPlan default_plan(int n)
 // within 4 % of it: for y and z lines of 400 or more in an FP32 store 20
 // nodes while 32 lines of them fit a block (n <= 640: the block's 64 VGPRs
 // let three blocks share a compute unit, against two of the 32-node
-// segments' 94), 32 beyond; 16 for lines of 200 or more and for FP32 y and
-// z lines of 32 or more (twice the bytes in flight per thread of 8 nodes:
-// the batched y and z sweeps of ensembles of 32-128-node grids ran 11 %
-// faster), 8 below.  Ensembles batch each axis at its members' own length.
+// segments' 94), 32 beyond; 16 for lines of 200 or more, for FP32 y and z
+// lines of 32 or more, and for FP32 x lines of 32 or more whose rows are
+// whole 4-node chunks (the whole-cache-line x kernel takes any S up to 64
+// there): twice the bytes in flight per thread of 8 nodes (ensembles of
+// 32-128-node grids: diffusion step 0.62 ms at 8-node x and 16-node y/z
+// segments, 0.58 ms at 16 on every axis); 8 below.  Ensembles batch each
+// axis at its members' own length.
 int default_segment(const Env& e, int axis)
 {
     const int n = axis == 0 ? e.nx : axis == 1 ? e.ny : e.nz;
     int L = 8;
     if (n >= 400 && axis > 0 && e.precision == BFVM_FP32) L = plan_valid(n, 32, 20) ? 20 : 32;
     else if (n >= 200) L = 16;
-    else if (n >= 32 && axis > 0 && e.precision == BFVM_FP32) L = 16;
+    else if (n >= 32 && e.precision == BFVM_FP32 && (axis > 0 || (n % 4 == 0 && e.S <= 64))) L = 16;
     for (int cols : {32, 16, 64})
         if (plan_valid(n, cols, L)) return L;
     return default_plan(n).L;  // lines too long for this length
@@ -1137,6 +1189,7 @@ SweepArgs sweep_args(const Env& e, int axis, int cols, int nseg, int k0 = 0, int
     A.planes = k1 - k0;
     A.spread = e.spread[axis];
     A.seg_base = axis == 2 && cols == 64 && e.nvs * e.elem >= 0xffffffffull ? 1 : 0;
+    A.xt_pitch = 0;
     return A;
 }
 
@@ -1361,25 +1414,52 @@ hipError_t launch_rows_L(Env& e, const SweepPlan& p, int k0, int k1)
 }
 
 // Whole-cache-line x sweeps (k_sweep_xt): plan.cols = C = R*S lines, R rows
-// per block, L one of kXtNodes.
+// per block, L one of kXtNodes.  S of 4, 8 or 16 (FP32 or FP64) take the
+// compile-time form; any other S up to 64 the runtime form (FP32, rows of a
+// multiple of 4 nodes).
 constexpr int kXtNodes[] = {8, 16, 20};
+constexpr int kMaxXtRuntimeS = 64;
+
+bool xt_compile_time_s(int S) { return S == 4 || S == 8 || S == 16; }
 
 template <typename T>
-bool xt_supported(int S, int L)
+bool xt_supported(const Env& e, int L)
 {
     bool nodes = false;
     for (int v : kXtNodes) nodes = nodes || v == L;
-    if (!nodes) return false;
-    switch (S) {
+    if (!nodes || e.nx < 2) return false;
+    switch (e.S) {
     case 4: return xt_shape_ok<T, 4>(L);
     case 8: return xt_shape_ok<T, 8>(L);
     case 16: return xt_shape_ok<T, 16>(L);
-    default: return false;
+    default: return e.S <= kMaxXtRuntimeS && e.nx % 4 == 0 && xt_shape_ok<T, 0>(L);
     }
 }
 
+// Chunk-buffer pitch (values per segment) of the runtime-S form: the
+// multiple of 4 values (16-byte pieces) from 4*S up that spreads the 64
+// lanes reading one node of their segments over the most LDS banks.
+int xt_pitch(int S)
+{
+    const int segw = 64 / S;
+    int best = 4 * S, best_worst = 1 << 30;
+    for (int P = 4 * S; P <= 4 * S + 64; P += 4) {
+        int worst = 0;
+        for (int u = 0; u < 4; ++u) {
+            int cnt[64] = {0};
+            for (int qq = 0; qq < segw; ++qq)
+                for (int s = 0; s < S; ++s) worst = std::max(worst, ++cnt[(qq * P + u * S + s) % 64]);
+        }
+        if (worst < best_worst) {
+            best_worst = worst;
+            best = P;
+        }
+    }
+    return best;
+}
+
 struct XtShape {
-    int Q = 0, threads = 0;
+    int Q = 0, threads = 0, pitch = 0;
     size_t lds = 0;
 };
 
@@ -1388,34 +1468,38 @@ XtShape xt_shape(const Env& e, int C, int L)
 {
     XtShape x;
     const int S = e.S;
-    if (C <= 0 || L <= 0 || C % S != 0) return x;
+    if (C <= 0 || L <= 0 || C % S != 0 || S > kMaxXtRuntimeS) return x;
+    constexpr int G = 16 / (int)sizeof(T);
+    const int segw = 64 / S;
     x.Q = (C / S) * ((e.nx + L - 1) / L);
-    const int waves = (x.Q + 64 / S - 1) / (64 / S);
+    const int waves = (x.Q + segw - 1) / segw;
     x.threads = waves * 64;
-    const size_t tables = (size_t)(e.ax[0].kg0 + e.ax[0].kg1 + 3 * kPad + 8) * S;
-    x.lds = sizeof(double) * (tables + 2 * (size_t)x.Q * S + 2 * (size_t)C) + (size_t)waves * xt_wave_lds<T>();
+    x.pitch = xt_compile_time_s(S) ? (G + 1) * S : xt_pitch(S);
+    x.lds = sizeof(double) * (size_t)xt_buffer_offset(e.ax[0].kg0, e.ax[0].kg1, S, x.Q, C) +
+            (size_t)waves * segw * x.pitch * sizeof(T);
     return x;
 }
 
 template <typename T>
 bool xt_plan_valid(const Env& e, int C, int L)
 {
-    if (!xt_supported<T>(e.S, L) || e.nx < 2) return false;
+    if (!xt_supported<T>(e, L)) return false;
     const XtShape x = xt_shape<T>(e, C, L);
     return x.Q > 0 && x.threads <= kMaxThreads && x.lds <= kMaxCorrectionLds;
 }
 
-template <typename T, int L, int S>
+template <typename T, int L, int SC>
 hipError_t launch_xt(Env& e, const SweepPlan& p, int k0, int k1)
 {
-    if constexpr (!xt_shape_ok<T, S>(L)) {
+    if constexpr (!xt_shape_ok<T, SC>(L)) {
         return hipErrorInvalidValue;
     } else {
         const XtShape x = xt_shape<T>(e, p.cols, L);
-        const SweepArgs A = sweep_args(e, 0, p.cols, (e.nx + L - 1) / L, k0, k1);
-        const hipError_t st = allow_table_lds((const void*)k_sweep_xt<T, L, S>, x.lds);
+        SweepArgs A = sweep_args(e, 0, p.cols, (e.nx + L - 1) / L, k0, k1);
+        A.xt_pitch = x.pitch;
+        const hipError_t st = allow_table_lds((const void*)k_sweep_xt<T, L, SC>, x.lds);
         if (st != hipSuccess) return st;
-        k_sweep_xt<T, L, S><<<(unsigned)A.tiles, x.threads, x.lds, e.stream>>>(A);
+        k_sweep_xt<T, L, SC><<<(unsigned)A.tiles, x.threads, x.lds, e.stream>>>(A);
         return hipGetLastError();
     }
 }
@@ -1427,7 +1511,7 @@ hipError_t launch_xt_S(Env& e, const SweepPlan& p, int k0, int k1)
     case 4: return launch_xt<T, L, 4>(e, p, k0, k1);
     case 8: return launch_xt<T, L, 8>(e, p, k0, k1);
     case 16: return launch_xt<T, L, 16>(e, p, k0, k1);
-    default: return hipErrorInvalidValue;
+    default: return launch_xt<T, L, 0>(e, p, k0, k1);
     }
 }
 
@@ -1622,7 +1706,7 @@ std::vector<SweepPlan> sweep_candidates(const Env& e, int axis, bool any_length)
                 if ((L <= 8 || L < 2 * n) && (Lfix == 0 || L == Lfix) && rows_plan_valid(e, R * e.S, L))
                     out.push_back(SweepPlan{R * e.S, L, 2});
     if (axis == 0)
-        for (int R = 1; R <= 16; R *= 2)
+        for (int R = 1; R <= 128; R *= 2)
             for (int L : kXtNodes) {
                 const bool ok = e.precision == BFVM_FP64 ? xt_plan_valid<double>(e, R * e.S, L)
                                                          : xt_plan_valid<float>(e, R * e.S, L);
@@ -1760,6 +1844,49 @@ hipError_t launch_rows_batch(int precision, int L, int spread, const RowTile* d_
     case 16: return launch_rows_batch_t<float, 16>(d_tabs, d_map, blocks, threads, lds, spread, st);
     default: return launch_rows_batch_t<float, 32>(d_tabs, d_map, blocks, threads, lds, spread, st);
     }
+}
+
+// Whole-cache-line x sweep of an ensemble member (k_sweep_xt_batch, the
+// runtime-S form: FP32, rows of a multiple of 4 nodes): the member's
+// arguments for blocks of `threads` threads, R = the most rows whose
+// segments fill them; false when the member cannot take part.
+bool batch_xt_args(const Env& e, int L, int threads, SweepArgs* A, size_t* lds, int* tiles)
+{
+    if (e.precision != BFVM_FP32 || e.S > kMaxXtRuntimeS || e.nx % 4 != 0 || !xt_shape_ok<float, 0>(L) ||
+        !xt_supported<float>(e, L) || !batch_member_ok(e, 0, L))
+        return false;
+    const int S = e.S, segw = 64 / S, nseg = (e.nx + L - 1) / L;
+    const int rows = e.ny * e.nz;
+    const int R = std::min(rows, (threads / 64) * segw / nseg);
+    if (R < 1) return false;
+    XtShape x = xt_shape<float>(e, R * S, L);
+    x.pitch = xt_pitch(S);  // the batch kernel is the runtime form for every S
+    x.lds = sizeof(double) * (size_t)xt_buffer_offset(e.ax[0].kg0, e.ax[0].kg1, S, x.Q, R * S) +
+            (size_t)(threads / 64) * segw * x.pitch * sizeof(float);
+    if (x.threads > threads || x.lds > kMaxBatchCorrectionLds) return false;
+    *A = sweep_args(e, 0, R * S, nseg);
+    A->xt_pitch = x.pitch;
+    *lds = x.lds;
+    *tiles = A->tiles * A->planes;
+    return true;
+}
+
+hipError_t launch_xt_batch(int L, int spread, const SweepArgs* d_args, const int* d_map, int blocks, int threads,
+                           size_t lds, hipStream_t st)
+{
+    if (blocks == 0) return hipSuccess;
+    const void* fn = L == 8 ? (const void*)k_sweep_xt_batch<float, 8>
+                   : L == 16 ? (const void*)k_sweep_xt_batch<float, 16>
+                             : (const void*)k_sweep_xt_batch<float, 20>;
+    if (L != 8 && L != 16 && L != 20) return hipErrorInvalidValue;
+    const hipError_t a = allow_table_lds(fn, lds, kMaxBatchCorrectionLds);
+    if (a != hipSuccess) return a;
+    switch (L) {
+    case 8: k_sweep_xt_batch<float, 8><<<blocks, threads, lds, st>>>(d_args, d_map, spread); break;
+    case 16: k_sweep_xt_batch<float, 16><<<blocks, threads, lds, st>>>(d_args, d_map, spread); break;
+    default: k_sweep_xt_batch<float, 20><<<blocks, threads, lds, st>>>(d_args, d_map, spread); break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_sweep(Env& e, int axis)

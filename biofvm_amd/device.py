@@ -380,6 +380,17 @@ class Ensemble:
     def segment(self, nodes):
         self._check(self.lib.bfvm_ensemble_set_segment(self.e, int(nodes)))
 
+    @property
+    def graph_steps(self):
+        """Steps per graph replay of run(..., graph=True)."""
+        v = ctypes.c_int(0)
+        self._check(self.lib.bfvm_ensemble_get_graph_steps(self.e, ctypes.byref(v)))
+        return v.value
+
+    @graph_steps.setter
+    def graph_steps(self, steps):
+        self._check(self.lib.bfvm_ensemble_set_graph_steps(self.e, int(steps)))
+
     def axis_segments(self):
         """Nodes per segment the batched x, y and z sweeps run at."""
         out = []
@@ -388,6 +399,17 @@ class Ensemble:
             self._check(self.lib.bfvm_ensemble_get_axis_segment(self.e, axis, ctypes.byref(v)))
             out.append(v.value)
         return out
+
+    @property
+    def x_chunks(self):
+        """x sweeps with whole-cache-line chunk loads where a member can (True)."""
+        v = ctypes.c_int(0)
+        self._check(self.lib.bfvm_ensemble_get_x_chunks(self.e, ctypes.byref(v)))
+        return bool(v.value)
+
+    @x_chunks.setter
+    def x_chunks(self, on):
+        self._check(self.lib.bfvm_ensemble_set_x_chunks(self.e, 1 if on else 0))
 
     @property
     def row_tiles(self):

@@ -382,6 +382,13 @@ int bfvm_ensemble_destroy(bfvm_ensemble e);
  * flags) and replayed `steps` times.  Asynchronous. */
 int bfvm_ensemble_run(bfvm_ensemble e, double dt, int steps, int flags, int use_graph);
 
+/* Steps per graph replay of bfvm_ensemble_run (default 8): runs of that many
+ * steps replay one graph holding them (one launch gap instead of one per
+ * step), the rest one-step graphs; the same launches in the same order, the
+ * same bits.  No counterpart in the reference. */
+int bfvm_ensemble_set_graph_steps(bfvm_ensemble e, int steps_per_graph);
+int bfvm_ensemble_get_graph_steps(bfvm_ensemble e, int* steps_per_graph);
+
 /* Waits for all work enqueued on the ensemble. */
 int bfvm_ensemble_sync(bfvm_ensemble e);
 
@@ -393,9 +400,9 @@ int bfvm_ensemble_timed_run(bfvm_ensemble e, double dt, int steps, int flags, in
 /* Times the batched sweeps for each segment length (4, 8, 16, 32 nodes; only
  * the ensemble's own unless bfvm_ensemble_set_autotune_segments), spread of
  * their blocks over the XCDs (as dispatched, chunks of 64 tiles,
- * contiguous eighths), x tiles (rows or columns) and y/z cache policy
- * (default or non-temporal) on scratch copies of the stores and keeps the
- * fastest.  Densities unchanged. */
+ * contiguous eighths), x kernel (column tiles, row tiles, whole-cache-line
+ * chunks) and y/z cache policy (default or non-temporal) on scratch copies
+ * of the stores and keeps the fastest.  Densities unchanged. */
 int bfvm_ensemble_autotune(bfvm_ensemble e, double dt);
 /* Nodes per segment of the batched sweeps: 0 (the default) = each axis at
  * the length the members' own sweeps take (the most common one among them,
@@ -418,6 +425,14 @@ int bfvm_ensemble_set_autotune_segments(bfvm_ensemble e, int on);
  * bfvm_ensemble_autotune. */
 int bfvm_ensemble_set_row_tiles(bfvm_ensemble e, int on);
 int bfvm_ensemble_get_row_tiles(bfvm_ensemble e, int* on);
+/* Whether the members' x sweeps load and store whole cache lines (on = 1,
+ * the default: 16-byte pieces of 4-node chunks handed to the lines' lanes
+ * through LDS, for FP32 members whose x rows are a multiple of 4 nodes and
+ * at segment lengths of 8, 16 or 20; the others run on row or column tiles).
+ * The result is the same bit for bit.  Set by bfvm_ensemble_autotune.  No
+ * counterpart in the reference. */
+int bfvm_ensemble_set_x_chunks(bfvm_ensemble e, int on);
+int bfvm_ensemble_get_x_chunks(bfvm_ensemble e, int* on);
 
 /* Whether the batched y and z sweeps load and store with the non-temporal
  * (streaming) hint (1) or the default policy (0; see

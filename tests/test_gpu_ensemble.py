@@ -118,17 +118,45 @@ def test_ensemble_equals_single_runs_after_changes(mixed):
         assert rel_err(got[i], ref) <= TOL[FP64], i
 
 
+def test_multi_step_graphs_equal_eager_steps():
+    # runs of graph_steps steps replay one graph holding them, the rest
+    # one-step graphs: the bits of launching every step eagerly
+    cases = W.ensemble_mixed(seed=8, steps=1)
+    out = {}
+    for mode in ("eager", "graph3", "graph1"):
+        members = _members(cases, FP32)
+        ens = Ensemble(members)
+        try:
+            if mode == "graph3":
+                ens.graph_steps = 3
+                assert ens.graph_steps == 3
+            if mode == "graph1":
+                ens.graph_steps = 1
+            with pytest.raises(BfvmError):
+                ens.graph_steps = 0
+            for steps in (7, 2, 3):  # 2 x 3 + 1, one-step graphs, one 3-step graph
+                ens.run(cases[0]["dt"], steps, ALL, graph=mode != "eager")
+            out[mode] = [m.download() for m in members]
+        finally:
+            _close(ens, members)
+    for mode in ("graph3", "graph1"):
+        for i, (a, b) in enumerate(zip(out[mode], out["eager"])):
+            np.testing.assert_array_equal(a, b, err_msg=f"{mode} member {i}")
+
+
 def test_members_at_their_own_segment_lengths_equal_solo_runs():
     # by default each axis is batched at the length most members take on
-    # their own (FP32 y and z lines of 32 nodes or more: 16; shorter ones and
-    # x lines below 200: 8) and members with another length sweep on their
-    # own: every member's bits equal its solo run's
+    # their own (FP32 lines of 32 nodes or more: 16, for x if the rows are
+    # whole 4-node chunks; shorter ones: 8) and members with another length
+    # sweep on their own: every member's bits equal its solo run's
     cases = W.ensemble_mixed(seed=5, steps=3)
     members = _members(cases, FP32)
     ens = Ensemble(members)
     try:
         assert ens.segment == 0
-        assert ens.axis_segments() == [8, 8, 16]  # one 40-node z axis holds most of the values
+        # x: the 64-node member (12288 values) against the rest; z: one
+        # 40-node axis holds most of the values
+        assert ens.axis_segments() == [8, 8, 16]
         ens.run(cases[0]["dt"], 3, ALL, graph=True)
         got = [m.download() for m in members]
         ens.segment = 8
@@ -219,23 +247,27 @@ def test_ensemble_with_lod2d_members():
 @pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("L", [8, 16])
 def test_row_tile_x_sweeps_equal_column_tiles(mixed, precision, L):
-    # members' x sweeps on row tiles (one batched launch) give the bits of the
-    # column-tile batch with the same segment length
+    # members' x sweeps on row tiles or whole-cache-line chunks (one batched
+    # launch each) give the bits of the column-tile batch with the same
+    # segment length
     cases, refs = mixed
     out = {}
-    for rows in (False, True):
+    for chunks, rows in ((False, False), (False, True), (True, False), (True, True)):
         members = _members(cases, precision)
         ens = Ensemble(members)
         try:
             ens.segment = L
             ens.row_tiles = rows
-            assert ens.row_tiles == rows
+            ens.x_chunks = chunks
+            assert ens.row_tiles == rows and ens.x_chunks == chunks
             ens.run(cases[0]["dt"], cases[0]["steps"], ALL, graph=rows)
-            out[rows] = [m.download() for m in members]
+            out[chunks, rows] = [m.download() for m in members]
         finally:
             _close(ens, members)
-    for i, (a, b, ref) in enumerate(zip(out[True], out[False], refs)):
-        np.testing.assert_array_equal(a, b, err_msg=f"member {i}")
+    for key in ((False, True), (True, False), (True, True)):
+        for i, (a, b, ref) in enumerate(zip(out[key], out[False, False], refs)):
+            np.testing.assert_array_equal(a, b, err_msg=f"member {i} (chunks, rows) = {key}")
+            assert rel_err(a, ref) <= TOL[precision], (i, key)
         assert rel_err(a, ref) <= TOL[precision], i
 
 

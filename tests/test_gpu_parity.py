@@ -299,11 +299,13 @@ def test_row_tile_x_sweep_equals_column_tiles(precision):
         assert tried > 0, (nx, ny, nz, S)
 
 
-# substrate counts whose nodes split into whole 16-byte pieces (FP32: S in
-# {4, 8, 16}; FP64 also 2); line lengths below, at and past the segment
-# lengths, with partial last segments and partial last tiles
+# substrate counts whose nodes split into whole 16-byte pieces (S in {4, 8,
+# 16}; line lengths below, at and past the segment lengths, partial last
+# segments and tiles), and FP32 rows of a multiple of 4 nodes with any S (a
+# lane's 16 bytes spanning two nodes, lanes past a wave's whole segments)
 WHOLE_LINE_SHAPES = [(400, 6, 3, 8), (37, 5, 4, 8), (61, 7, 3, 4), (20, 9, 5, 16), (3, 11, 2, 8),
-                     (130, 4, 3, 2), (48, 13, 2, 4)]
+                     (130, 4, 3, 2), (48, 13, 2, 4), (64, 5, 3, 3), (32, 7, 2, 1), (48, 6, 4, 5),
+                     (128, 3, 2, 6), (36, 4, 3, 2), (40, 5, 3, 12), (4, 9, 2, 7)]
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
@@ -329,7 +331,7 @@ def test_whole_line_x_sweep_equals_column_tiles(precision):
                     m.run(case["dt"], case["steps"], case["phases"])
                     base = m.download()
                     break
-                for R in (1, 2, 4, 8, 16):
+                for R in (1, 2, 4, 8, 16, 32, 64, 128):
                     try:
                         m.set_sweep_plan(0, R * S, L, 3)
                     except BfvmError:
@@ -348,11 +350,20 @@ def test_whole_line_x_sweep_equals_column_tiles(precision):
 
 
 def test_whole_line_x_sweep_refuses_other_substrate_counts():
-    # S = 3 does not split into 16-byte pieces; y and z take no such kernel
-    with DeviceMicroenvironment(40, 8, 6, 3, 10.0, FP32) as m:
+    # S = 3 does not split into 16-byte pieces: FP32 rows of 42 nodes are not
+    # whole chunks, and the FP64 form needs S in {4, 8, 16}; y and z take no
+    # such kernel
+    with DeviceMicroenvironment(42, 8, 6, 3, 10.0, FP32) as m:
         m.set_coefficients([1e3, 1e4, 1e5], [0.1, 0.2, 0.3])
         with pytest.raises(BfvmError):
             m.set_sweep_plan(0, 12, 16, 3)
+    with DeviceMicroenvironment(40, 8, 6, 3, 10.0, FP64) as m:
+        m.set_coefficients([1e3, 1e4, 1e5], [0.1, 0.2, 0.3])
+        with pytest.raises(BfvmError):
+            m.set_sweep_plan(0, 12, 16, 3)
+    with DeviceMicroenvironment(40, 8, 6, 3, 10.0, FP32) as m:
+        m.set_coefficients([1e3, 1e4, 1e5], [0.1, 0.2, 0.3])
+        m.set_sweep_plan(0, 12, 16, 3)
     with DeviceMicroenvironment(40, 40, 40, 8, 10.0, FP32) as m:
         m.set_coefficients([1e3] * 8, [0.1] * 8)
         m.set_sweep_plan(0, 32, 16, 3)
