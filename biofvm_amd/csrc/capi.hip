@@ -398,6 +398,12 @@ int capture_steps(Env& e, double dt, int flags, int g)
     const hipError_t st = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (st != hipSuccess) return hip_fail(&e, st, "hipGraphInstantiate");
+    // stage the graph's launch resources now rather than at its first replay
+    const hipError_t up = hipGraphUpload(exec, e.stream);
+    if (up != hipSuccess) {
+        (void)hipGraphExecDestroy(exec);
+        return hip_fail(&e, up, "hipGraphUpload");
+    }
     StepGraph* G = new StepGraph;
     G->exec = exec;
     G->steps = g;

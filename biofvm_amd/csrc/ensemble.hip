@@ -388,6 +388,8 @@ int capture_steps(bfvm_ensemble_t& E, int flags, int steps, hipGraph_t* graph, h
     }
     *graph = g;
     EHIP(&E, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    // stage the graph's launch resources now rather than at its first replay
+    EHIP(&E, hipGraphUpload(*exec, E.stream));
     return BFVM_OK;
 }
 
