@@ -482,9 +482,11 @@ def ensemble_result(steps, warmup, with_fp64):
 
     ens, members, L, wall, ev_ms, eager_ms, wu = run(FP32)
     out = [m.download() for m in members]
-    # diffusion-only steps (after the result was taken) price the three batched sweeps
-    sweep_steps = max(3, min(steps, 20))
-    ens.run(dt, ens.graph_steps, 4, graph=True)  # the diffusion-only graphs captured and replayed before the timing
+    # diffusion-only steps (after the result was taken) price the three batched
+    # sweeps: whole multi-step graphs (each replay boundary costs a few us)
+    sweep_steps = 3 * ens.graph_steps
+    # the diffusion-only graphs captured and replayed for ~20 ms before the timing
+    ens.run(dt, 4 * ens.graph_steps, 4, graph=True)
     sweep_ms = ens.timed_run(dt, sweep_steps, 4, graph=True) / sweep_steps
     knobs = {"segment_nodes_xyz": L, "x_chunks": ens.x_chunks, "x_row_tiles": ens.row_tiles,
              "yz_non_temporal": ens.non_temporal}
