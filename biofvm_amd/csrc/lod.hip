@@ -1706,7 +1706,7 @@ std::vector<SweepPlan> sweep_candidates(const Env& e, int axis, bool any_length)
                 if ((L <= 8 || L < 2 * n) && (Lfix == 0 || L == Lfix) && rows_plan_valid(e, R * e.S, L))
                     out.push_back(SweepPlan{R * e.S, L, 2});
     if (axis == 0)
-        for (int R = 1; R <= 128; R *= 2)
+        for (int R = 1; R <= 128; R = R < 8 ? R + 1 : 2 * R)  // rows per block
             for (int L : kXtNodes) {
                 const bool ok = e.precision == BFVM_FP64 ? xt_plan_valid<double>(e, R * e.S, L)
                                                          : xt_plan_valid<float>(e, R * e.S, L);

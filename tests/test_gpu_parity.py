@@ -331,7 +331,7 @@ def test_whole_line_x_sweep_equals_column_tiles(precision):
                     m.run(case["dt"], case["steps"], case["phases"])
                     base = m.download()
                     break
-                for R in (1, 2, 4, 8, 16, 32, 64, 128):
+                for R in (1, 2, 3, 4, 5, 6, 7, 8, 16, 32, 64, 128):
                     try:
                         m.set_sweep_plan(0, R * S, L, 3)
                     except BfvmError:
