@@ -31,6 +31,7 @@
 // recomputed forward states stay in the store's precision.
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "bfvm_internal.h"
@@ -174,42 +175,73 @@ __device__ void block_sums(double (&acc)[M], double* __restrict__ out, bool accu
     }
 }
 
-// Per block: for substrates s0 .. s0+sc-1 (sc <= kChunk), the sums over the
-// block's voxels of (l_v - l_w)(y_v - y_w) over edges v -> w = v + step along
-// `axis`, and of l_v y_v, added to partial[block][0..kChunk) and
-// [kChunk..2 kChunk).
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_axis_grad(const double* __restrict__ lam, const T* __restrict__ y,
-                                                      int nx, int ny, int nz, int S, int axis, int s0,
-                                                      int sc, double* __restrict__ partial)
+// Per block: for substrates s0 .. s0+SC-1 (SC <= kChunk), the sums over the
+// block's voxels of (l_v - l_w)(y_v - y_w) over edges v -> w = v + 1 along x,
+// and of l_v y_v, added to partial[block][0..kChunk) and [kChunk..2 kChunk).
+// Consecutive lanes hold consecutive voxels, so a lane takes its neighbour's
+// values from the next lane (the wave's last lane loads them).  kAhead voxels
+// of the grid-stride walk at a time: their loads are all issued before the
+// first is used, and they are added in walk order (the sums of a
+// one-at-a-time walk, bit for bit).
+template <typename T, int SC>
+__global__ void __launch_bounds__(kBlock) k_axis_grad_x(const double* __restrict__ lam, const T* __restrict__ y,
+                                                        int nx, int ny, int nz, int S, int s0,
+                                                        double* __restrict__ partial)
 {
-    double acc[2 * kChunk];
+    // fewer voxels ahead for more substrates: at most about 128 VGPRs (the
+    // grid's 512 blocks put 8 waves on a compute unit)
+    constexpr int kAhead = SC <= 2 ? 4 : SC <= 4 ? 2 : 1;
+    double acc[2 * SC];
 #pragma unroll
-    for (int m = 0; m < 2 * kChunk; ++m) acc[m] = 0.0;
-    // voxel indices in 32 bits (the handle has fewer than 2^31 voxels): the
-    // coordinate along `axis` costs a 32-bit division, not a 64-bit one
-    const unsigned nxu = (unsigned)nx, nxy = (unsigned)nx * (unsigned)ny;
-    const unsigned nvox = nxy * (unsigned)nz;
-    const unsigned stride = axis == 0 ? 1u : axis == 1 ? nxu : nxy;
-    const unsigned n = (unsigned)(axis == 0 ? nx : axis == 1 ? ny : nz);
-    for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += gridDim.x * blockDim.x) {
-        const unsigned coord = axis == 0 ? v % nxu : axis == 1 ? (v / nxu) % (unsigned)ny : v / nxy;
-        const bool edge = coord + 1 < n;
-        const size_t w = (size_t)v + stride;
+    for (int m = 0; m < 2 * SC; ++m) acc[m] = 0.0;
+    const unsigned nxu = (unsigned)nx;
+    const unsigned nvox = nxu * (unsigned)ny * (unsigned)nz;
+    const unsigned step = gridDim.x * blockDim.x;
+    const bool last_lane = (threadIdx.x & 63) == 63;
+    for (unsigned v0 = blockIdx.x * blockDim.x + threadIdx.x; v0 - threadIdx.x % 64 < nvox; v0 += kAhead * step) {
+        double l[kAhead][SC], yv[kAhead][SC], ln[kAhead][SC], yn[kAhead][SC];
 #pragma unroll
-        for (int m = 0; m < kChunk; ++m) {
-            if (m < sc) {
-                const size_t q = (size_t)v * S + s0 + m;
-                const double l = (double)lam[q], yv = (double)y[q];
-                acc[kChunk + m] += l * yv;
-                if (edge) {
-                    const size_t p = w * S + s0 + m;
-                    acc[m] += (l - (double)lam[p]) * (yv - (double)y[p]);
+        for (int k = 0; k < kAhead; ++k) {
+            const unsigned v = v0 + (unsigned)k * step;
+            const unsigned vc = v < nvox ? v : nvox - 1;  // lanes past the end load a real value, add nothing
+#pragma unroll
+            for (int m = 0; m < SC; ++m) {
+                const size_t q = (size_t)vc * S + s0 + m;
+                l[k][m] = lam[q];
+                yv[k][m] = (double)y[q];
+                // the wave's last lane loads its neighbour (v + 1 < nvox when it has an edge)
+                const size_t p = last_lane && vc + 1 < nvox ? q + S : q;
+                ln[k][m] = last_lane ? lam[p] : 0.0;
+                yn[k][m] = last_lane ? (double)y[p] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kAhead; ++k) {
+            const unsigned v = v0 + (unsigned)k * step;
+#pragma unroll
+            for (int m = 0; m < SC; ++m) {
+                const double lnext = __shfl_down(l[k][m], 1, 64), ynext = __shfl_down(yv[k][m], 1, 64);
+                if (!last_lane) {
+                    ln[k][m] = lnext;
+                    yn[k][m] = ynext;
                 }
+            }
+            if (v >= nvox) continue;  // (whole waves past the end left the loop)
+            const bool edge = v % nxu + 1 < nxu;
+#pragma unroll
+            for (int m = 0; m < SC; ++m) {
+                acc[SC + m] += l[k][m] * yv[k][m];
+                if (edge) acc[m] += (l[k][m] - ln[k][m]) * (yv[k][m] - yn[k][m]);
             }
         }
     }
-    block_sums<2 * kChunk>(acc, partial + (size_t)blockIdx.x * 2 * kChunk, true);
+    double out[2 * kChunk];
+#pragma unroll
+    for (int m = 0; m < kChunk; ++m) {
+        out[m] = m < SC ? acc[m] : 0.0;
+        out[kChunk + m] = m < SC ? acc[SC + m] : 0.0;
+    }
+    block_sums<2 * kChunk>(out, partial + (size_t)blockIdx.x * 2 * kChunk, true);
 }
 
 // The same sums for the y or z axis, whose neighbour is nx or nx*ny voxels
@@ -219,51 +251,89 @@ __global__ void __launch_bounds__(kBlock) k_axis_grad(const double* __restrict__
 // (run, column) with the column fastest, so a wave's loads are contiguous.
 constexpr int kRun = 8;
 
-template <typename T>
+template <typename T, int SC>
 __global__ void __launch_bounds__(kBlock) k_axis_grad_runs(const double* __restrict__ lam,
                                                            const T* __restrict__ y, int nx, int ny, int nz,
-                                                           int S, int axis, int s0, int sc,
+                                                           int S, int axis, int s0,
                                                            double* __restrict__ partial)
 {
-    double acc[2 * kChunk];
+    // work items of the grid-stride walk loaded together (all their loads
+    // issued before the first is used, then added in walk order: the sums of
+    // a one-at-a-time walk, bit for bit); fewer for more substrates (at most
+    // about 128 VGPRs: the grid's 512 blocks put 8 waves on a compute unit)
+    constexpr int kAheadRuns = SC <= 2 ? 2 : 1;
+    double acc[2 * SC];
 #pragma unroll
-    for (int m = 0; m < 2 * kChunk; ++m) acc[m] = 0.0;
+    for (int m = 0; m < 2 * SC; ++m) acc[m] = 0.0;
     const unsigned nxu = (unsigned)nx, nxy = (unsigned)nx * (unsigned)ny;
     const unsigned n = (unsigned)(axis == 1 ? ny : nz);
     const unsigned stride = axis == 1 ? nxu : nxy;            // voxels between nodes of a line
     const unsigned ncol = axis == 1 ? nxu * (unsigned)nz : nxy;  // lines
     const unsigned nrun = (n + kRun - 1) / kRun;
     const unsigned items = ncol * nrun;
-    for (unsigned it = blockIdx.x * blockDim.x + threadIdx.x; it < items; it += gridDim.x * blockDim.x) {
-        const unsigned run = it / ncol, c = it - run * ncol;
-        // line c's node 0: z lines start in plane 0; y line (i, k) at (k, 0, i)
-        const unsigned base = axis == 1 ? (c / nxu) * nxy + c % nxu : c;
-        const unsigned p0 = run * kRun;
-        const int cnt = (int)min((unsigned)kRun, n - p0);         // nodes of this run
-        const bool more = p0 + kRun < n;                          // the next run's node 0 exists
+    const unsigned step = gridDim.x * blockDim.x;
+    for (unsigned it0 = blockIdx.x * blockDim.x + threadIdx.x; it0 < items; it0 += kAheadRuns * step) {
+        // with up to 2 substrates, the values of kAheadRuns items at once; with
+        // more, one substrate's values at a time (as many registers)
+        constexpr int kM = SC <= 2 ? SC : 1;
+        double l[kAheadRuns][kM][kRun + 1], yv[kAheadRuns][kM][kRun + 1];
+        unsigned base[kAheadRuns], p0[kAheadRuns];
+        int cnt[kAheadRuns];
+        bool more[kAheadRuns];
 #pragma unroll
-        for (int m = 0; m < kChunk; ++m) {
-            if (m < sc) {
-                double l[kRun + 1], yv[kRun + 1];
+        for (int k = 0; k < kAheadRuns; ++k) {
+            const unsigned it = it0 + (unsigned)k * step;
+            const bool live = it < items;
+            const unsigned run = live ? it / ncol : 0, c = live ? it - run * ncol : 0;
+            // line c's node 0: z lines start in plane 0; y line (i, k) at (k, 0, i)
+            base[k] = axis == 1 ? (c / nxu) * nxy + c % nxu : c;
+            p0[k] = run * kRun;
+            cnt[k] = live ? (int)min((unsigned)kRun, n - p0[k]) : 0;  // nodes of this run
+            more[k] = live && p0[k] + kRun < n;                        // the next run's node 0 exists
+        }
+        auto load = [&](int k, int mi, int m) {
 #pragma unroll
-                for (int u = 0; u <= kRun; ++u) {
-                    const bool in = u < cnt || (u == kRun && more);
-                    const size_t q = (size_t)(base + (p0 + (unsigned)u) * stride) * S + s0 + m;
-                    l[u] = in ? lam[q] : 0.0;
-                    yv[u] = in ? (double)y[q] : 0.0;
+            for (int u = 0; u <= kRun; ++u) {
+                const bool in = u < cnt[k] || (u == kRun && more[k]);
+                const size_t q = (size_t)(base[k] + (p0[k] + (unsigned)u) * stride) * S + s0 + m;
+                l[k][mi][u] = in ? lam[q] : 0.0;
+                yv[k][mi][u] = in ? (double)y[q] : 0.0;
+            }
+        };
+        auto add = [&](int k, int mi, int m) {
+#pragma unroll
+            for (int u = 0; u < kRun; ++u) {
+                if (u < cnt[k]) {
+                    acc[SC + m] += l[k][mi][u] * yv[k][mi][u];
+                    if (u + 1 < cnt[k] || (u + 1 == kRun && more[k]))
+                        acc[m] += (l[k][mi][u] - l[k][mi][u + 1]) * (yv[k][mi][u] - yv[k][mi][u + 1]);
                 }
+            }
+        };
+        if constexpr (SC <= 2) {
 #pragma unroll
-                for (int u = 0; u < kRun; ++u) {
-                    if (u < cnt) {
-                        acc[kChunk + m] += l[u] * yv[u];
-                        if (u + 1 < cnt || (u + 1 == kRun && more))
-                            acc[m] += (l[u] - l[u + 1]) * (yv[u] - yv[u + 1]);
-                    }
-                }
+            for (int k = 0; k < kAheadRuns; ++k)
+#pragma unroll
+                for (int m = 0; m < SC; ++m) load(k, m, m);
+#pragma unroll
+            for (int k = 0; k < kAheadRuns; ++k)
+#pragma unroll
+                for (int m = 0; m < SC; ++m) add(k, m, m);
+        } else {
+#pragma unroll
+            for (int m = 0; m < SC; ++m) {
+                load(0, 0, m);
+                add(0, 0, m);
             }
         }
     }
-    block_sums<2 * kChunk>(acc, partial + (size_t)blockIdx.x * 2 * kChunk, true);
+    double out[2 * kChunk];
+#pragma unroll
+    for (int m = 0; m < kChunk; ++m) {
+        out[m] = m < SC ? acc[m] : 0.0;
+        out[kChunk + m] = m < SC ? acc[SC + m] : 0.0;
+    }
+    block_sums<2 * kChunk>(out, partial + (size_t)blockIdx.x * 2 * kChunk, true);
 }
 
 // gc[s] -= sum of the blocks' edge sums, gc[S + s] -= sum of the l.y sums.
@@ -340,12 +410,27 @@ hipError_t axis_grad(Env& e, Tape& tp, const void* y, int axis)
     for (int s0 = 0; s0 < e.S; s0 += kChunk) {
         const int sc = std::min(kChunk, e.S - s0);
         double* part = tp.gpart + (size_t)(s0 / kChunk) * kRedBlocks * 2 * kChunk;
-        if (axis == 0)  // the neighbour is the next voxel: the same or the next cache line
-            k_axis_grad<T><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny, e.nz,
-                                                               e.S, axis, s0, sc, part);
-        else
-            k_axis_grad_runs<T><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny,
-                                                                    e.nz, e.S, axis, s0, sc, part);
+        // the neighbour is the next voxel for x (the next lane's), nx or nx*ny
+        // voxels away for y and z (runs of nodes along the line)
+        auto launch = [&](auto sc_const) {
+            constexpr int SC = decltype(sc_const)::value;
+            if (axis == 0)
+                k_axis_grad_x<T, SC><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny, e.nz,
+                                                                          e.S, s0, part);
+            else
+                k_axis_grad_runs<T, SC><<<kRedBlocks, kBlock, 0, e.stream>>>(tp.lam, (const T*)y, e.nx, e.ny,
+                                                                             e.nz, e.S, axis, s0, part);
+        };
+        switch (sc) {
+        case 1: launch(std::integral_constant<int, 1>{}); break;
+        case 2: launch(std::integral_constant<int, 2>{}); break;
+        case 3: launch(std::integral_constant<int, 3>{}); break;
+        case 4: launch(std::integral_constant<int, 4>{}); break;
+        case 5: launch(std::integral_constant<int, 5>{}); break;
+        case 6: launch(std::integral_constant<int, 6>{}); break;
+        case 7: launch(std::integral_constant<int, 7>{}); break;
+        default: launch(std::integral_constant<int, 8>{}); break;
+        }
         hipError_t st = hipGetLastError();
         if (st != hipSuccess) return st;
     }
