@@ -63,6 +63,11 @@ struct Tape {
     // added up over the whole backward pass and reduced once at its end
     double* gpart = nullptr;
     size_t gpart_n = 0;
+    // per axis, the tile partials of the adjoint sweep with the sums fused in
+    // (launch_adjoint_sums; null: that axis runs the sweep and the sums apart)
+    double* spart[3] = {nullptr, nullptr, nullptr};
+    size_t spart_n[3] = {0, 0, 0};
+    size_t stiles[3] = {0, 0, 0};
 };
 
 }  // namespace bfvm
@@ -439,12 +444,20 @@ hipError_t axis_grad(Env& e, Tape& tp, const void* y, int axis)
 
 hipError_t axis_grad_finish(Env& e, Tape& tp)
 {
+    static_assert(kChunk == bfvm::kAdjointSumsChunk, "one partial layout for both sums");
     for (int s0 = 0; s0 < e.S; s0 += kChunk) {
         const int sc = std::min(kChunk, e.S - s0);
         const double* part = tp.gpart + (size_t)(s0 / kChunk) * kRedBlocks * 2 * kChunk;
         k_axis_grad_finish<<<1, kBlock, 0, e.stream>>>(part, kRedBlocks, e.S, s0, sc, tp.gc);
         hipError_t st = hipGetLastError();
         if (st != hipSuccess) return st;
+        // the tile partials of the axes whose sums ran inside their sweeps
+        for (int axis = 0; axis < 3; ++axis) {
+            if (!tp.spart[axis]) continue;
+            const double* sp = tp.spart[axis] + (size_t)(s0 / kChunk) * tp.stiles[axis] * 2 * kChunk;
+            k_axis_grad_finish<<<1, kBlock, 0, e.stream>>>(sp, (int)tp.stiles[axis], e.S, s0, sc, tp.gc);
+            if ((st = hipGetLastError()) != hipSuccess) return st;
+        }
     }
     return hipSuccess;
 }
@@ -475,6 +488,20 @@ hipError_t sweep_adjoint(Env& e, double* lam, int axis)
     if (keep != BFVM_FP64 && keep_plan.stream >= 2) e.plan[axis] = bfvm::SweepPlan{};
     hipError_t st = on_buffer(e, lam, [&] { return bfvm::launch_sweep(e, axis); });
     e.plan[axis] = keep_plan;
+    e.elem = keep_elem;
+    e.precision = keep;
+    return st;
+}
+
+// The sweep of `axis` on the FP64 adjoint state with that axis's D/lambda
+// sums over it and the forward output y fused in (tp.spart[axis]).
+hipError_t sweep_adjoint_sums(Env& e, Tape& tp, const void* y, int axis)
+{
+    const int keep = e.precision;
+    const size_t keep_elem = e.elem;
+    e.precision = BFVM_FP64;
+    e.elem = sizeof(double);  // offsets and 4 GiB limits in FP64 values
+    const hipError_t st = bfvm::launch_adjoint_sums(e, axis, tp.lam, y, keep, tp.spart[axis]);
     e.elem = keep_elem;
     e.precision = keep;
     return st;
@@ -550,8 +577,12 @@ hipError_t adjoint_step(Env& e, Tape& tp)
     if (tp.flags & BFVM_DIFFUSION) {
         const void* ys[3] = {tp.s4, tp.s6, tp.s8};
         for (int axis = e.lod_dims - 1; axis >= 0; --axis) {
-            if ((st = sweep_adjoint(e, tp.lam, axis))) return st;
-            if ((st = axis_grad<T>(e, tp, ys[axis], axis))) return st;
+            if (tp.spart[axis]) {
+                if ((st = sweep_adjoint_sums(e, tp, ys[axis], axis))) return st;
+            } else {
+                if ((st = sweep_adjoint(e, tp.lam, axis))) return st;
+                if ((st = axis_grad<T>(e, tp, ys[axis], axis))) return st;
+            }
             if ((st = dirichlet_zero(e, tp.lam))) return st;
         }
     }
@@ -625,6 +656,7 @@ void free_tape(Env& e)
     free_ptr(tp->gc);
     free_ptr(tp->partial);
     free_ptr(tp->gpart);
+    for (double*& p : tp->spart) free_ptr(p);
     delete tp;
     e.tape = nullptr;
 }
@@ -758,6 +790,19 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
         tp->gpart_n = gpart_n;
     }
     AHIP(e, hipMemsetAsync(tp->gpart, 0, sizeof(double) * gpart_n, e.stream));
+    // the axes whose adjoint sweeps take their D/lambda sums along
+    for (int axis = 0; axis < 3; ++axis) {
+        bfvm::AdjointSumsShape sh;
+        if ((tp->flags & BFVM_DIFFUSION) && axis < e.lod_dims) sh = bfvm::adjoint_sums_shape(e, axis);
+        tp->stiles[axis] = sh.cols ? sh.tiles : 0;
+        const size_t n = tp->stiles[axis] * (size_t)((S + kChunk - 1) / kChunk) * 2 * kChunk;
+        if (tp->spart_n[axis] != n) {
+            free_ptr(tp->spart[axis]);
+            if (n) AHIP(e, hipMalloc((void**)&tp->spart[axis], sizeof(double) * n));
+            tp->spart_n[axis] = n;
+        }
+        if (n) AHIP(e, hipMemsetAsync(tp->spart[axis], 0, sizeof(double) * n, e.stream));
+    }
     const size_t na = (size_t)e.n_agents * S;
     if (na > 0) {
         free_ptr(tp->rk);

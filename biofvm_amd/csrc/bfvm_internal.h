@@ -250,6 +250,19 @@ hipError_t launch_sweep(Env& e, int axis);
 // The sweep of `axis` reading store `src` and writing store `dst` (same shape
 // and precision as the handle's; src is left unchanged).
 hipError_t launch_sweep_to(Env& e, int axis, const void* src, void* dst);
+// The adjoint's sweep of `axis` on the FP64 adjoint state lam (in place) with
+// that axis's D/lambda sums over lam and the forward sweep's output y fused
+// in, added to per-tile partial slots: per chunk of 8 substrates, tiles x 16
+// doubles (the edge sums of the chunk's substrates, then their l.y sums).
+// Call with e.precision and e.elem set to FP64 (the adjoint state's).
+constexpr int kAdjointSumsNodes = 8;
+constexpr int kAdjointSumsChunk = 8;  // substrates per chunk of the partial slots
+struct AdjointSumsShape {
+    int cols = 0;      // lines per block; 0: the fused kernel cannot run this axis
+    size_t tiles = 0;  // blocks of one launch (slots per chunk of `partial`)
+};
+AdjointSumsShape adjoint_sums_shape(const Env& e, int axis);
+hipError_t launch_adjoint_sums(Env& e, int axis, double* lam, const void* y, int y_precision, double* partial);
 // Whether this grid's x sweep can run with sources fused in (a segmented
 // one-block-per-tile shape: the autotuner's fastest, or the default); if so,
 // launch_sweep_x_fused runs it with, in its load prologue, the bulk update

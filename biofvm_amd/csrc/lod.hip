@@ -988,6 +988,104 @@ k_sweep_xt_batch(const SweepArgs* __restrict__ args, const int* __restrict__ map
     xt_tile<T, L, 0>(A, (unsigned)local, smem_xtb);
 }
 
+// The adjoint's sweep of one axis with that axis's D/lambda sums fused in
+// (k_sweep_adjoint_sums).  The sweep is the forward one (the sweep matrix A
+// is symmetric) on the FP64 adjoint state, one block per column tile; as
+// each node's final value l_i is stored, the thread adds, for its
+// substrate, l_i y_i and, over the line's edges, (l_i - l_{i+1})(y_i - y_{i+1})
+// (y = the forward sweep's output, store precision, loaded beside the
+// adjoint values).  The edge between a segment's last node and the next
+// segment's first goes through LDS.  The block's sums are added up in a
+// fixed order (segments of a column, then the columns of a substrate) onto
+// the tile's slot of `partial` (per chunk of kGradChunk substrates: tiles x
+// 2*kGradChunk doubles, edge sums then l.y sums), the same every run.
+constexpr int kGradChunk = kAdjointSumsChunk;
+
+template <typename TY, int L, int COLS>
+__global__ void __launch_bounds__(kMaxThreads, 4)
+k_sweep_adjoint_sums(SweepArgs A, const TY* __restrict__ y, double* __restrict__ partial)
+{
+    __shared__ double xa[kMaxThreads];
+    __shared__ double xb[kMaxThreads];
+    __shared__ double wfirst[COLS];
+    __shared__ double wlast[COLS];
+    __shared__ double nl[kMaxThreads];  // a segment's first node: l, then its edge sum
+    __shared__ double ny[kMaxThreads];  // a segment's first node: y, then its l.y sum
+    extern __shared__ double gtab[];
+    const unsigned ntiles = (unsigned)(A.tiles * A.planes);
+    const unsigned w = xcd_tile(blockIdx.x + blockIdx.y * (unsigned)A.tiles, ntiles, A.spread);
+    const unsigned plane = w / (unsigned)A.tiles;
+    const int tile = (int)(w - plane * (unsigned)A.tiles);
+    const SegPos P = seg_pos<double, L>(A, COLS, tile, (int)plane);
+    const double r = A.wb[P.s * 8 + 0], a = A.wb[P.s * 8 + 1];
+    double raw[L];
+    seg_load<double, L, false>(A, P, raw);
+    TY yv[L];
+    {
+        // the same nodes of the forward output (P.vo counts bytes of doubles)
+        const __amdgpu_buffer_rsrc_t ry = make_rsrc(y + P.plane0, (A.nvs - P.plane0) * sizeof(TY));
+        const unsigned vo = P.vo / (unsigned)sizeof(double) * (unsigned)sizeof(TY);
+        const unsigned step = A.line_stride * (unsigned)sizeof(TY);
+        const int last = min(L, A.n - P.p0) - 1;
+#pragma unroll
+        for (int t = 0; t < L; ++t) yv[t] = buf_load((TY*)nullptr, ry, vo + (unsigned)min(t, last) * step, 0);
+    }
+    stage_tables(A, gtab);  // while the loads are in flight
+    const int lane = threadIdx.x & (COLS - 1);
+    const int cnt = min(L, A.n - P.p0);
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc_out<double>(A, P);
+    const unsigned stepd = A.line_stride * (unsigned)sizeof(double);
+    double s_ly = 0.0, s_edge = 0.0, lnext = 0.0, lfirst = 0.0, llast = 0.0;
+    seg_core<double, L>(A, COLS, lane, P.s, P.p0, P.slot, P.last_writer, raw, r, a, xa, xb, wfirst, wlast, gtab,
+                        NoSource{}, [&](int t, double zf) {
+                            if (P.act && t < cnt) {
+                                buf_store((double*)nullptr, rs, zf, P.vo, t * stepd);
+                                s_ly += zf * (double)yv[t];
+                                if (t + 1 < cnt) s_edge += (zf - lnext) * ((double)yv[t] - (double)yv[t + 1]);
+                                lnext = zf;
+                            }
+                            if (t == L - 1) llast = zf;
+                            if (t == 0) lfirst = zf;
+                        });
+    // the edge from this segment's last node to the next segment's first
+    nl[P.slot] = lfirst;
+    ny[P.slot] = (double)yv[0];
+    lds_barrier();
+    const int g = P.p0 / L;
+    if (P.act && g + 1 < A.nseg)  // a segment with a next one holds L real nodes
+        s_edge += (llast - nl[P.slot + COLS]) * ((double)yv[L - 1] - ny[P.slot + COLS]);
+    lds_barrier();
+    const int tid = threadIdx.x;
+    nl[tid] = P.act ? s_edge : 0.0;
+    ny[tid] = P.act ? s_ly : 0.0;
+    lds_barrier();
+    // per column, its segments in order; then per substrate, its columns in order
+    if (tid < COLS) {
+        double e = 0.0, ly = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < A.nseg; ++q) {
+            e += nl[q * COLS + tid];
+            ly += ny[q * COLS + tid];
+        }
+        xa[tid] = e;
+        xb[tid] = ly;
+    }
+    lds_barrier();
+    for (int s = tid; s < A.S; s += blockDim.x) {
+        double e = 0.0, ly = 0.0;
+        // the tile's columns of substrate s: c = c0, c0 + S, ... (c0 the first)
+        const int c0 = ((s - tile * COLS) % A.S + A.S) % A.S;
+#pragma unroll 1
+        for (int c = c0; c < COLS; c += A.S) {
+            e += xa[c];
+            ly += xb[c];
+        }
+        double* slot = partial + ((size_t)(s / kGradChunk) * ntiles + w) * 2 * kGradChunk + s % kGradChunk;
+        slot[0] += e;
+        slot[kGradChunk] += ly;
+    }
+}
+
 // One thread owns one whole (line, substrate) pair and runs the reference's
 // forward elimination and back substitution along it in place
 // (BioFVM_solvers.cpp:186-201).  Used for lines too long to segment.
@@ -1914,6 +2012,64 @@ hipError_t launch_sweep_to(Env& e, int axis, const void* src, void* dst)
     e.rho = keep;
     e.out = nullptr;
     return st;
+}
+
+// The fused adjoint sweep and D/lambda sums of `axis` (k_sweep_adjoint_sums),
+// 8-node segments, the widest column tiles of up to 1024 threads: its shape
+// (lines per block, 0 when it cannot run: a line of one node, or 32-bit
+// buffer offsets short of the FP64 state) and its tiles.
+AdjointSumsShape adjoint_sums_shape(const Env& e, int axis)
+{
+    AdjointSumsShape sh;
+    const Geometry gm = geometry(e, axis);
+    constexpr int L = kAdjointSumsNodes;
+    const int nseg = (gm.n + L - 1) / L;
+    if (gm.n < 2 || e.nvs * sizeof(double) >= 0xffffffffull || e.nvox >= (1ull << 31)) return sh;
+    const size_t gbytes = sizeof(double) * (size_t)(e.ax[axis].kg0 + e.ax[axis].kg1 + 3 * kPad + 8) * e.S;
+    if (gbytes > kMaxCorrectionLds) return sh;
+    for (int cols : {64, 32, 16})
+        if (cols * nseg <= kMaxThreads) {
+            sh.cols = cols;
+            break;
+        }
+    if (sh.cols == 0) return sh;
+    const SweepArgs A = sweep_args(e, axis, sh.cols, nseg);
+    sh.tiles = (size_t)A.tiles * A.planes;
+    return sh;
+}
+
+template <typename TY, int COLS>
+hipError_t launch_adjoint_sums_t(Env& e, int axis, double* lam, const void* y, double* partial)
+{
+    constexpr int L = kAdjointSumsNodes;
+    const Geometry gm = geometry(e, axis);
+    void* keep = e.rho;
+    e.rho = lam;
+    SweepArgs A = sweep_args(e, axis, COLS, (gm.n + L - 1) / L);
+    e.rho = keep;
+    const hipError_t st = allow_table_lds((const void*)k_sweep_adjoint_sums<TY, L, COLS>, sweep_lds(A));
+    if (st != hipSuccess) return st;
+    dim3 grid(A.tiles, A.planes);
+    const int threads = (COLS * A.nseg + 63) / 64 * 64;
+    k_sweep_adjoint_sums<TY, L, COLS><<<grid, threads, sweep_lds(A), e.stream>>>(A, (const TY*)y, partial);
+    return hipGetLastError();
+}
+
+// Runs with the handle set up for the FP64 adjoint state (e.precision and
+// e.elem FP64 while the arguments are built); y in the store's precision.
+hipError_t launch_adjoint_sums(Env& e, int axis, double* lam, const void* y, int y_precision, double* partial)
+{
+    const AdjointSumsShape sh = adjoint_sums_shape(e, axis);
+    const bool f = y_precision == BFVM_FP32;
+    switch (sh.cols) {
+    case 64: return f ? launch_adjoint_sums_t<float, 64>(e, axis, lam, y, partial)
+                      : launch_adjoint_sums_t<double, 64>(e, axis, lam, y, partial);
+    case 32: return f ? launch_adjoint_sums_t<float, 32>(e, axis, lam, y, partial)
+                      : launch_adjoint_sums_t<double, 32>(e, axis, lam, y, partial);
+    case 16: return f ? launch_adjoint_sums_t<float, 16>(e, axis, lam, y, partial)
+                      : launch_adjoint_sums_t<double, 16>(e, axis, lam, y, partial);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 bool sweep_chunkable(const Env& e, int axis)
