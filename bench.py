@@ -537,16 +537,18 @@ def ensemble_bench(args):
     print(json.dumps(line), flush=True)
 
 
-def adjoint_bytes(case, w):
+def adjoint_bytes(case, w, kept=True):
     """Algorithmic HBM bytes of one backward step of the adjoint: the step
-    recomputed twice from the segment's checkpoint (3 sweeps of the store,
-    w bytes per value, read and written; the agents' FP64 constants), the
-    transposed sweeps of the FP64 adjoint state (3, read and written), the
-    D/lambda sums of each axis (adjoint state and forward state read once),
-    the agents' adjoint (their constants once more)."""
+    recomputed from the segment's checkpoint once when every step's states
+    are kept (else twice: the step starts, then the step again keeping its
+    states; 3 sweeps of the store, w bytes per value, read and written; the
+    agents' FP64 constants), the transposed sweeps of the FP64 adjoint state
+    (3, read and written) with each axis's D/lambda sums fused in (the
+    forward state read once per axis), the agents' adjoint (their constants
+    once more)."""
     nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
     agents = len(case["agent_volume"]) * (4 + 2 * case["S"] * 8)
-    return 2 * (6 * nvs * w + agents) + 6 * nvs * 8 + 3 * nvs * (8 + w) + agents
+    return (1 if kept else 2) * (6 * nvs * w + agents) + 6 * nvs * 8 + 3 * nvs * w + agents
 
 
 def differentiation_result(full):
@@ -587,15 +589,16 @@ def differentiation_result(full):
         t0 = time.perf_counter()
         g = m.tape_backward(n_agents)
         t_bwd = time.perf_counter() - t0
+        kept = m.tape_states_kept()
         m.close()
-        return g, t_fwd, t_bwd, loss
+        return g, t_fwd, t_bwd, loss, kept
 
-    g, t_fwd, t_bwd, loss = adjoint(FP32)
-    step_bytes = adjoint_bytes(case, 4)
+    g, t_fwd, t_bwd, loss, kept = adjoint(FP32)
+    step_bytes = adjoint_bytes(case, 4, kept)
     achieved = step_bytes / (t_bwd / steps) / 1e9
     out = {"value": nvs * steps / t_bwd, "unit": "voxel-substrate updates/s", "steps": steps,
            "ms_per_step": 1e3 * t_bwd / steps, "forward_s": t_fwd, "backward_s": t_bwd, "loss": loss,
-           "store_dtype": "f32",
+           "store_dtype": "f32", "states_kept": kept,
            "config": {"workload": f"differentiation_128: {case['nx']}^3 voxels x {case['S']} substrates, "
                                   f"{n_agents} agents, {steps} steps, checkpoints every 50"},
            "roofline": {"bound": "hbm", "kernel": "backward step (recomputed forward, transposed sweeps, "
@@ -607,7 +610,7 @@ def differentiation_result(full):
           "agent_uptake": "grad_uptake"}
     grads = {"fp32": g}
     if full:
-        g64, _, t_bwd64, _ = adjoint(FP64)
+        g64, _, t_bwd64, _, _ = adjoint(FP64)
         grads["fp64"] = g64
         out["fp64_backward_s"] = t_bwd64
         out["fp32_vs_fp64_adjoint"] = {

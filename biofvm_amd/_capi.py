@@ -92,6 +92,9 @@ SIGNATURES = {
     "bfvm_tape_seed_l2": (_i, [_h, _dp, _dp]),
     "bfvm_tape_backward": (_i, [_h, _dp, _dp, _dp, _dp, _dp]),
     "bfvm_tape_free": (_i, [_h]),
+    "bfvm_set_tape_keep_states": (_i, [_vp, _i]),
+    "bfvm_get_tape_keep_states": (_i, [_vp, _ip]),
+    "bfvm_tape_states_kept": (_i, [_vp, _ip]),
 }
 
 _lib = None

@@ -68,6 +68,13 @@ struct Tape {
     double* spart[3] = {nullptr, nullptr, nullptr};
     size_t spart_n[3] = {0, 0, 0};
     size_t stiles[3] = {0, 0, 0};
+    // when device memory allows: the states each ring slot's step keeps (s1
+    // if the step has a bulk update, s4, s6; s8 is the next slot, or
+    // last_s8 for a segment's last step), filled by one pass over the
+    // segment, so that each step is recomputed once rather than twice
+    std::vector<void*> kept;
+    void* last_s8 = nullptr;
+    int nkeep = 0;  // kept states per slot (0: recompute each step twice)
 };
 
 }  // namespace bfvm
@@ -657,6 +664,8 @@ void free_tape(Env& e)
     free_ptr(tp->partial);
     free_ptr(tp->gpart);
     for (double*& p : tp->spart) free_ptr(p);
+    for (void*& p : tp->kept) free_ptr(p);
+    free_ptr(tp->last_s8);
     delete tp;
     e.tape = nullptr;
 }
@@ -816,9 +825,68 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
     }
     // segments from the last
     const int K = tp->K;
+    // keep every step's states from one pass over the segment when they fit
+    // in half the free device memory, else recompute each step twice
+    {
+        const bool bulk = (tp->flags & BFVM_BULK) && e.have_bulk;
+        const int nkeep = bulk ? 3 : 2;
+        const size_t store = e.nvs * e.elem;
+        const bool ready = e.tape_keep_states && tp->nkeep == nkeep && tp->kept.size() == (size_t)K * nkeep;
+        size_t free_b = 0, total_b = 0;
+        const size_t want = ready ? 0 : (size_t)K * nkeep + (tp->last_s8 ? 0 : 1);
+        const bool fits = e.tape_keep_states && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+                          want * store <= free_b / 2;
+        if (!ready) {
+            for (void*& p : tp->kept) free_ptr(p);
+            tp->kept.clear();
+            tp->nkeep = 0;
+            if (fits) {
+                tp->kept.assign((size_t)K * nkeep, nullptr);
+                for (void*& p : tp->kept) AHIP(e, alloc_store(e, &p));
+                if (!tp->last_s8) AHIP(e, alloc_store(e, &tp->last_s8));
+                tp->nkeep = nkeep;
+            }
+        }
+    }
+    // the tape's own state buffers, pointed at again however this returns
+    struct SharedStates {
+        Tape* tp;
+        void* s[4];
+        ~SharedStates()
+        {
+            tp->s1 = s[0];
+            tp->s4 = s[1];
+            tp->s6 = s[2];
+            tp->s8 = s[3];
+        }
+    } keep_states{tp, {tp->s1, tp->s4, tp->s6, tp->s8}};
+    void* const* shared = keep_states.s;
+    // points the step's kept states (slot i of the segment [n0, n1)) at the tape
+    auto use_slot = [&](int i, int n, int n1) {
+        void** k = &tp->kept[(size_t)i * tp->nkeep];
+        const bool bulk = tp->nkeep == 3;
+        tp->s1 = bulk ? k[0] : shared[0];
+        tp->s4 = k[bulk ? 1 : 0];
+        tp->s6 = k[bulk ? 2 : 1];
+        tp->s8 = n + 1 < n1 ? tp->ring[i + 1] : tp->last_s8;
+    };
     for (int j = (int)tp->cps.size() - 1; j >= 0; --j) {
         const int n0 = j * K, n1 = std::min(n0 + K, tp->steps);
         AHIP(e, copy_store(e, tp->ring[0], tp->cps[j]));
+        if (tp->nkeep) {
+            // one pass: every step's states kept, its result the next slot
+            for (int n = n0; n < n1; ++n) {
+                use_slot(n - n0, n, n1);
+                AHIP(e, forward_step_keep(e, *tp, tp->ring[n - n0]));
+            }
+            for (int n = n1 - 1; n >= n0; --n) {
+                use_slot(n - n0, n, n1);
+                const bool bulk = tp->nkeep == 3;
+                tp->after_bulk = bulk ? tp->s1 : tp->ring[n - n0];
+                AHIP(e, e.precision == BFVM_FP64 ? adjoint_step<double>(e, *tp) : adjoint_step<float>(e, *tp));
+            }
+            continue;
+        }
         for (int n = n0; n + 1 < n1; ++n) {
             AHIP(e, copy_store(e, tp->ring[n - n0 + 1], tp->ring[n - n0]));
             AHIP(e, forward_step(e, *tp, tp->ring[n - n0 + 1]));
@@ -863,6 +931,30 @@ int bfvm_tape_free(bfvm_handle h)
     bfvm::RandGuard rand_guard;
     if (!h) return BFVM_E_ARG;
     bfvm::free_tape(*h);
+    return BFVM_OK;
+}
+
+int bfvm_set_tape_keep_states(bfvm_handle h, int on)
+{
+    bfvm::RandGuard rand_guard;
+    if (!h) return BFVM_E_ARG;
+    h->tape_keep_states = on != 0;
+    return BFVM_OK;
+}
+
+int bfvm_get_tape_keep_states(bfvm_handle h, int* on)
+{
+    bfvm::RandGuard rand_guard;
+    if (!h || !on) return BFVM_E_ARG;
+    *on = h->tape_keep_states ? 1 : 0;
+    return BFVM_OK;
+}
+
+int bfvm_tape_states_kept(bfvm_handle h, int* kept)
+{
+    bfvm::RandGuard rand_guard;
+    if (!h || !kept) return BFVM_E_ARG;
+    *kept = h->tape && h->tape->nkeep > 0 ? 1 : 0;
     return BFVM_OK;
 }
 

@@ -158,6 +158,9 @@ struct Env {
     // whether bfvm_autotune may choose the segment length (which changes the
     // solve's rounding); otherwise it keeps default_segment's
     bool tune_segments = false;
+    // the adjoint's backward pass keeps each step's states when they fit
+    // (bfvm_set_tape_keep_states)
+    bool tape_keep_states = true;
     // bumped whenever a launch shape, cache policy or XCD spread is set, so
     // that an ensemble holding this handle rebuilds its tables and graph
     unsigned plan_epoch = 0;

@@ -281,6 +281,23 @@ class DeviceMicroenvironment:
     def tape_free(self):
         self._check(self.lib.bfvm_tape_free(self.h))
 
+    @property
+    def tape_keep_states(self):
+        """The backward pass keeps each step's states where memory allows (True)."""
+        v = ctypes.c_int(0)
+        self._check(self.lib.bfvm_get_tape_keep_states(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
+    @tape_keep_states.setter
+    def tape_keep_states(self, on):
+        self._check(self.lib.bfvm_set_tape_keep_states(self.h, 1 if on else 0))
+
+    def tape_states_kept(self):
+        """Whether the last backward pass kept every step's states."""
+        v = ctypes.c_int(0)
+        self._check(self.lib.bfvm_tape_states_kept(self.h, ctypes.byref(v)))
+        return bool(v.value)
+
     def timed_run(self, dt, steps, flags=ALL, per_kernel=False):
         """Returns (ms_total, {kernel: (ms_sum, launches)} or None)."""
         ms = ctypes.c_double(0)

@@ -464,9 +464,20 @@ const char* bfvm_ensemble_last_error(bfvm_ensemble e);
  *                       NULL.  The coefficients, sources and agents must be
  *                       those the tape was recorded with.
  *   bfvm_tape_free      releases the tape's device memory
- *                       ((steps/K + K + 4) stores and an FP64 adjoint state).
+ *                       ((steps/K + K + 4) stores and an FP64 adjoint state,
+ *                       plus 2K + 1 stores, 3K + 1 with bulk fields, while
+ *                       the backward pass keeps every step's states).
  * The recomputed forward states are in the store's precision; the adjoint
  * state, all arithmetic and the gradient sums are FP64.
+ *   bfvm_set_tape_keep_states  with on = 1 (the default) the backward pass
+ *                       recomputes a segment's steps once, keeping the states
+ *                       each step's transposes read (2 or 3 stores per step
+ *                       of the segment), where they fit in half the free
+ *                       device memory; otherwise (on = 0, or too little
+ *                       memory) it recomputes each step twice (the step
+ *                       starts, then each step again).  The same gradients
+ *                       bit for bit either way.  No counterpart in the
+ *                       reference.
  */
 int bfvm_tape_record(bfvm_handle h, double dt, int steps, int flags, int checkpoint_every);
 int bfvm_tape_seed(bfvm_handle h, const double* seed);
@@ -474,6 +485,10 @@ int bfvm_tape_seed_l2(bfvm_handle h, const double* target, double* loss);
 int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double* grad_lambda,
                        double* grad_secretion, double* grad_uptake);
 int bfvm_tape_free(bfvm_handle h);
+int bfvm_set_tape_keep_states(bfvm_handle h, int on);
+int bfvm_get_tape_keep_states(bfvm_handle h, int* on);
+/* *kept = 1 when the tape's last backward pass kept every step's states. */
+int bfvm_tape_states_kept(bfvm_handle h, int* kept);
 
 #ifdef __cplusplus
 }

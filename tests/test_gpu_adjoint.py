@@ -47,11 +47,15 @@ def with_params(case, rho0=None, D=None, lam=None, sec=None, upt=None):
     return c
 
 
-def adjoint_grads(case, target, precision, checkpoint_every=2):
+def adjoint_grads(case, target, precision, checkpoint_every=2, keep_states=True):
     with setup_case(case, precision) as m:
+        m.tape_keep_states = keep_states
+        assert m.tape_keep_states == keep_states
         m.tape_record(case["dt"], case["steps"], case["phases"], checkpoint_every)
         loss = m.tape_seed_l2(target)
         g = m.tape_backward(len(case["agent_volume"]))
+        # small cases always fit, so the states are kept exactly when asked
+        assert m.tape_states_kept() == keep_states
     return loss, g
 
 
@@ -118,6 +122,20 @@ def test_checkpoint_interval_does_not_change_gradients(small):
         g = adjoint_grads(case, target, FP64, checkpoint_every=K)[1]
         for name in ref:
             np.testing.assert_array_equal(g[name], ref[name], err_msg=f"K={K} {name}")
+
+
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("fixture", ["small", "small_nodir"])
+def test_kept_states_equal_recomputed_states(fixture, precision, request):
+    # one pass keeping every step's states, or the step starts then each
+    # step again: the same forward states, the same gradients bit for bit
+    case, target = request.getfixturevalue(fixture)
+    for K in (2, case["steps"]):
+        loss_a, a = adjoint_grads(case, target, precision, checkpoint_every=K, keep_states=True)
+        loss_b, b = adjoint_grads(case, target, precision, checkpoint_every=K, keep_states=False)
+        assert loss_a == loss_b
+        for name in a:
+            np.testing.assert_array_equal(a[name], b[name], err_msg=f"K={K} {name}")
 
 
 def test_fp32_adjoint_against_fp64_gpu_differences():
