@@ -29,6 +29,7 @@
 // keeps the rounding of its early, large values (an FP32 adjoint state
 // moved the lambda gradient by 5e-4 relative after 500 steps at 128^3).  The
 // recomputed forward states stay in the store's precision.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -57,6 +58,7 @@ struct Tape {
     double* rk = nullptr;      // agents: intermediate densities (n_agents x S)
     double* gsec = nullptr;    // agents: secretion gradient (grouped order)
     double* gupt = nullptr;    // agents: uptake gradient (grouped order)
+    size_t agents_n = 0;       // values of rk, gsec, gupt
     double* gc = nullptr;      // 2*S: dJ/dc1, dJ/dc2
     double* partial = nullptr; // reduction partials
     // the D / lambda sums' block partials, per chunk of kChunk substrates,
@@ -75,6 +77,14 @@ struct Tape {
     std::vector<void*> kept;
     void* last_s8 = nullptr;
     int nkeep = 0;  // kept states per slot (0: recompute each step twice)
+    // HIP graphs of one segment's backward pass with kept states (per
+    // segment length), and what their launches have baked in
+    struct SegmentGraph {
+        hipGraphExec_t exec = nullptr;
+        int len = 0;
+        std::vector<uintptr_t> key;
+    };
+    std::vector<SegmentGraph> graphs;
 };
 
 }  // namespace bfvm
@@ -641,6 +651,74 @@ int check_unchanged(Env& e, const Tape& tp)
     return BFVM_OK;
 }
 
+// One segment's backward pass when every step's states are kept: the steps
+// again from ring[0], each keeping its states (its result the next slot),
+// then the transposed steps in reverse.  shared: the tape's own s1, s4, s6,
+// s8 (the slots' states are pointed at the tape step by step).
+hipError_t keep_pass(Env& e, Tape& tp, int len, void* const* shared)
+{
+    const bool bulk = tp.nkeep == 3;
+    auto use_slot = [&](int i) {
+        void** k = &tp.kept[(size_t)i * tp.nkeep];
+        tp.s1 = bulk ? k[0] : shared[0];
+        tp.s4 = k[bulk ? 1 : 0];
+        tp.s6 = k[bulk ? 2 : 1];
+        tp.s8 = i + 1 < len ? tp.ring[i + 1] : tp.last_s8;
+    };
+    hipError_t st;
+    for (int i = 0; i < len; ++i) {
+        use_slot(i);
+        if ((st = forward_step_keep(e, tp, tp.ring[i]))) return st;
+    }
+    for (int i = len - 1; i >= 0; --i) {
+        use_slot(i);
+        tp.after_bulk = bulk ? tp.s1 : tp.ring[i];
+        if ((st = e.precision == BFVM_FP64 ? adjoint_step<double>(e, tp) : adjoint_step<float>(e, tp))) return st;
+    }
+    return hipSuccess;
+}
+
+// What a captured keep_pass has baked in besides its length: the buffers,
+// the tables and launch shapes (through their epochs), the phases.
+std::vector<uintptr_t> segment_key(const Env& e, const Tape& tp, void* const* shared)
+{
+    std::vector<uintptr_t> k = {(uintptr_t)e.plan_epoch, (uintptr_t)e.coef_epoch, (uintptr_t)e.agent_epoch,
+                                (uintptr_t)e.source_epoch, (uintptr_t)e.xy_chunk, (uintptr_t)e.fused_sources,
+                                (uintptr_t)e.lod_dims, (uintptr_t)tp.flags, (uintptr_t)tp.nkeep,
+                                (uintptr_t)e.bulk_A, (uintptr_t)e.bulk_B, (uintptr_t)tp.lam,
+                                (uintptr_t)tp.gpart, (uintptr_t)tp.rk, (uintptr_t)tp.gsec, (uintptr_t)tp.gupt,
+                                (uintptr_t)tp.last_s8};
+    for (int i = 0; i < 4; ++i) k.push_back((uintptr_t)shared[i]);
+    for (double* p : tp.spart) k.push_back((uintptr_t)p);
+    for (void* p : tp.ring) k.push_back((uintptr_t)p);
+    for (void* p : tp.kept) k.push_back((uintptr_t)p);
+    return k;
+}
+
+// Captures keep_pass over `len` steps into a graph (its replay launches the
+// same kernels with the same arguments in the same order: the same bits).
+hipError_t capture_keep_pass(Env& e, Tape& tp, int len, void* const* shared, hipGraphExec_t* out)
+{
+    *out = nullptr;
+    hipError_t st = hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal);
+    if (st != hipSuccess) return st;
+    const hipError_t run = keep_pass(e, tp, len, shared);
+    hipGraph_t graph = nullptr;
+    st = hipStreamEndCapture(e.stream, &graph);
+    if (run != hipSuccess || st != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return run != hipSuccess ? run : st;
+    }
+    st = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (st != hipSuccess) return st;
+    if ((st = hipGraphUpload(*out, e.stream)) != hipSuccess) {
+        (void)hipGraphExecDestroy(*out);
+        *out = nullptr;
+    }
+    return st;
+}
+
 }  // namespace
 
 namespace bfvm {
@@ -666,6 +744,7 @@ void free_tape(Env& e)
     for (double*& p : tp->spart) free_ptr(p);
     for (void*& p : tp->kept) free_ptr(p);
     free_ptr(tp->last_s8);
+    for (Tape::SegmentGraph& g : tp->graphs) (void)hipGraphExecDestroy(g.exec);
     delete tp;
     e.tape = nullptr;
 }
@@ -700,9 +779,10 @@ int bfvm_tape_record(bfvm_handle h, double dt, int steps, int flags, int checkpo
     const int ncp = (steps + tp->K - 1) / tp->K;
     tp->cps.assign(ncp, nullptr);
     for (void*& p : tp->cps) AHIP(e, alloc_store(e, &p));
-    for (int n = 0; n < steps; ++n) {
-        if (n % tp->K == 0) AHIP(e, copy_store(e, tp->cps[n / tp->K], e.rho));
-        if ((rc = bfvm_step(h, dt, flags))) return rc;
+    // the steps of each segment through bfvm_run (HIP graphs of its steps)
+    for (int n = 0; n < steps; n += tp->K) {
+        AHIP(e, copy_store(e, tp->cps[n / tp->K], e.rho));
+        if ((rc = bfvm_run(h, dt, std::min(tp->K, steps - n), flags))) return rc;
     }
     return BFVM_OK;
 }
@@ -814,12 +894,16 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
     }
     const size_t na = (size_t)e.n_agents * S;
     if (na > 0) {
-        free_ptr(tp->rk);
-        free_ptr(tp->gsec);
-        free_ptr(tp->gupt);
-        AHIP(e, hipMalloc((void**)&tp->rk, sizeof(double) * na));
-        AHIP(e, hipMalloc((void**)&tp->gsec, sizeof(double) * na));
-        AHIP(e, hipMalloc((void**)&tp->gupt, sizeof(double) * na));
+        if (tp->agents_n != na) {
+            free_ptr(tp->rk);
+            free_ptr(tp->gsec);
+            free_ptr(tp->gupt);
+            tp->agents_n = 0;
+            AHIP(e, hipMalloc((void**)&tp->rk, sizeof(double) * na));
+            AHIP(e, hipMalloc((void**)&tp->gsec, sizeof(double) * na));
+            AHIP(e, hipMalloc((void**)&tp->gupt, sizeof(double) * na));
+            tp->agents_n = na;
+        }
         AHIP(e, hipMemsetAsync(tp->gsec, 0, sizeof(double) * na, e.stream));
         AHIP(e, hipMemsetAsync(tp->gupt, 0, sizeof(double) * na, e.stream));
     }
@@ -860,30 +944,39 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
             tp->s8 = s[3];
         }
     } keep_states{tp, {tp->s1, tp->s4, tp->s6, tp->s8}};
-    void* const* shared = keep_states.s;
-    // points the step's kept states (slot i of the segment [n0, n1)) at the tape
-    auto use_slot = [&](int i, int n, int n1) {
-        void** k = &tp->kept[(size_t)i * tp->nkeep];
-        const bool bulk = tp->nkeep == 3;
-        tp->s1 = bulk ? k[0] : shared[0];
-        tp->s4 = k[bulk ? 1 : 0];
-        tp->s6 = k[bulk ? 2 : 1];
-        tp->s8 = n + 1 < n1 ? tp->ring[i + 1] : tp->last_s8;
-    };
+    // one segment's pass replayed from a HIP graph (per segment length),
+    // captured from the second pass on; graphs of an earlier call holding
+    // other launches are dropped
+    const bool graphs = tp->nkeep && e.graph_steps >= 2 && !e.kernel_timing;
+    const std::vector<uintptr_t> key = graphs ? segment_key(e, *tp, keep_states.s) : std::vector<uintptr_t>();
+    for (Tape::SegmentGraph& g : tp->graphs)
+        if (!graphs || g.key != key) {
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+        }
+    tp->graphs.erase(std::remove_if(tp->graphs.begin(), tp->graphs.end(),
+                                    [](const Tape::SegmentGraph& g) { return !g.exec; }),
+                     tp->graphs.end());
+    bool warm = false;  // a pass ran eagerly: kernel attributes are set
     for (int j = (int)tp->cps.size() - 1; j >= 0; --j) {
         const int n0 = j * K, n1 = std::min(n0 + K, tp->steps);
         AHIP(e, copy_store(e, tp->ring[0], tp->cps[j]));
         if (tp->nkeep) {
-            // one pass: every step's states kept, its result the next slot
-            for (int n = n0; n < n1; ++n) {
-                use_slot(n - n0, n, n1);
-                AHIP(e, forward_step_keep(e, *tp, tp->ring[n - n0]));
+            const int len = n1 - n0;
+            Tape::SegmentGraph* g = nullptr;
+            for (Tape::SegmentGraph& c : tp->graphs)
+                if (c.len == len) g = &c;
+            if (!g && graphs && warm) {
+                hipGraphExec_t exec = nullptr;
+                AHIP(e, capture_keep_pass(e, *tp, len, keep_states.s, &exec));
+                tp->graphs.push_back({exec, len, key});
+                g = &tp->graphs.back();
             }
-            for (int n = n1 - 1; n >= n0; --n) {
-                use_slot(n - n0, n, n1);
-                const bool bulk = tp->nkeep == 3;
-                tp->after_bulk = bulk ? tp->s1 : tp->ring[n - n0];
-                AHIP(e, e.precision == BFVM_FP64 ? adjoint_step<double>(e, *tp) : adjoint_step<float>(e, *tp));
+            if (g) {
+                AHIP(e, hipGraphLaunch(g->exec, e.stream));
+            } else {
+                AHIP(e, keep_pass(e, *tp, len, keep_states.s));
+                warm = true;
             }
             continue;
         }

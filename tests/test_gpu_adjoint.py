@@ -138,6 +138,34 @@ def test_kept_states_equal_recomputed_states(fixture, precision, request):
             np.testing.assert_array_equal(a[name], b[name], err_msg=f"K={K} {name}")
 
 
+@pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
+def test_backward_graphs_equal_eager_launches(precision):
+    # segments of 3, 3, 3 and 2 steps: the short one eagerly, the first of 3
+    # steps eagerly then captured, the others replayed; a second backward on
+    # the same tape replays the kept graph.  graph_steps = 0 launches every
+    # kernel on its own (the recorded forward too): the same bits
+    case = W.adjoint_small(steps=11)
+    rng = np.random.default_rng(13)
+    target = oracle.oracle_run(case) + rng.normal(0, 0.05, case["rho0"].size)
+
+    def grads(graph_steps):
+        out = []
+        with setup_case(case, precision) as m:
+            m.graph_steps = graph_steps
+            m.tape_record(case["dt"], case["steps"], case["phases"], 3)
+            for _ in range(2):
+                loss = m.tape_seed_l2(target)
+                out.append((loss, m.tape_backward(len(case["agent_volume"]))))
+                assert m.tape_states_kept()
+        return out
+
+    eager = grads(0)
+    for loss_g, g in grads(8) + eager[1:]:
+        assert loss_g == eager[0][0]
+        for name in g:
+            np.testing.assert_array_equal(g[name], eager[0][1][name], err_msg=name)
+
+
 def test_fp32_adjoint_against_fp64_gpu_differences():
     case = W.differentiation_128(steps=40, n=32, n_agents=300)
     rng = np.random.default_rng(9)
