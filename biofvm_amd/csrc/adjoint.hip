@@ -47,7 +47,8 @@ struct Tape {
     std::vector<double> D, lambda;  // coefficients the tape was recorded with
     unsigned agent_epoch = 0, source_epoch = 0;
     std::vector<void*> cps;   // state at steps 0, K, 2K, ...
-    std::vector<void*> ring;  // starts of the steps of one segment
+    std::vector<void*> ring;  // starts of the steps of one segment (in ring_slab)
+    void* ring_slab = nullptr;
     void* s1 = nullptr;       // after bulk (the agents' input)
     const void* after_bulk = nullptr;  // s1, or the step's start when no bulk update ran
     void* s4 = nullptr;       // after the x sweep
@@ -74,7 +75,8 @@ struct Tape {
     // if the step has a bulk update, s4, s6; s8 is the next slot, or
     // last_s8 for a segment's last step), filled by one pass over the
     // segment, so that each step is recomputed once rather than twice
-    std::vector<void*> kept;
+    std::vector<void*> kept;  // in kept_slab
+    void* kept_slab = nullptr;
     void* last_s8 = nullptr;
     int nkeep = 0;  // kept states per slot (0: recompute each step twice)
     // HIP graphs of one segment's backward pass with kept states (per
@@ -634,6 +636,22 @@ hipError_t alloc_store(Env& e, void** p)
     return hipMalloc(p, e.nvs * e.elem);
 }
 
+// n stores in one allocation (one hipMalloc rather than n: the backward
+// pass's set-up), each starting on a 256-byte boundary.
+hipError_t alloc_stores(Env& e, size_t n, void** slab, std::vector<void*>& stores)
+{
+    const size_t pitch = (e.nvs * e.elem + 255) / 256 * 256;
+    stores.assign(n, nullptr);
+    const hipError_t st = hipMalloc(slab, pitch * n);
+    if (st != hipSuccess) {
+        *slab = nullptr;
+        stores.clear();
+        return st;
+    }
+    for (size_t i = 0; i < n; ++i) stores[i] = (char*)*slab + i * pitch;
+    return hipSuccess;
+}
+
 int check_tape_handle(Env& e)
 {
     if (e.ensemble_stream) return afail(e, BFVM_E_STATE, "the handle belongs to an ensemble");
@@ -729,7 +747,7 @@ void free_tape(Env& e)
     if (!tp) return;
     (void)hipStreamSynchronize(e.stream);
     for (void*& p : tp->cps) free_ptr(p);
-    for (void*& p : tp->ring) free_ptr(p);
+    free_ptr(tp->ring_slab);
     free_ptr(tp->s1);
     free_ptr(tp->s4);
     free_ptr(tp->s6);
@@ -742,7 +760,7 @@ void free_tape(Env& e)
     free_ptr(tp->partial);
     free_ptr(tp->gpart);
     for (double*& p : tp->spart) free_ptr(p);
-    for (void*& p : tp->kept) free_ptr(p);
+    free_ptr(tp->kept_slab);
     free_ptr(tp->last_s8);
     for (Tape::SegmentGraph& g : tp->graphs) (void)hipGraphExecDestroy(g.exec);
     delete tp;
@@ -863,10 +881,7 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
     }
     if (tp->flags & BFVM_BULK) AHIP(e, bfvm::ensure_bulk(e, dt));
     // work buffers
-    if ((int)tp->ring.size() != tp->K) {
-        tp->ring.assign(tp->K, nullptr);
-        for (void*& p : tp->ring) AHIP(e, alloc_store(e, &p));
-    }
+    if (!tp->ring_slab) AHIP(e, alloc_stores(e, tp->K, &tp->ring_slab, tp->ring));
     for (void** p : {&tp->s1, &tp->s4, &tp->s6, &tp->s8})
         if (!*p) AHIP(e, alloc_store(e, p));
     if (!tp->partial) AHIP(e, hipMalloc((void**)&tp->partial, sizeof(double) * kRedBlocks * 2 * kChunk));
@@ -921,15 +936,14 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
         const bool fits = e.tape_keep_states && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
                           want * store <= free_b / 2;
         if (!ready) {
-            for (void*& p : tp->kept) free_ptr(p);
+            free_ptr(tp->kept_slab);
             tp->kept.clear();
             tp->nkeep = 0;
-            if (fits) {
-                tp->kept.assign((size_t)K * nkeep, nullptr);
-                for (void*& p : tp->kept) AHIP(e, alloc_store(e, &p));
-                if (!tp->last_s8) AHIP(e, alloc_store(e, &tp->last_s8));
+            // an allocation that fails leaves the pass recomputing instead
+            if (fits && (tp->last_s8 || alloc_store(e, &tp->last_s8) == hipSuccess) &&
+                alloc_stores(e, (size_t)K * nkeep, &tp->kept_slab, tp->kept) == hipSuccess)
                 tp->nkeep = nkeep;
-            }
+            (void)hipGetLastError();
         }
     }
     // the tape's own state buffers, pointed at again however this returns
