@@ -897,7 +897,7 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
     // the axes whose adjoint sweeps take their D/lambda sums along
     for (int axis = 0; axis < 3; ++axis) {
         bfvm::AdjointSumsShape sh;
-        if ((tp->flags & BFVM_DIFFUSION) && axis < e.lod_dims) sh = bfvm::adjoint_sums_shape(e, axis);
+        if ((tp->flags & BFVM_DIFFUSION) && axis < e.lod_dims) sh = bfvm::adjoint_sums_shape(e, axis, e.precision);
         tp->stiles[axis] = sh.cols ? sh.tiles : 0;
         const size_t n = tp->stiles[axis] * (size_t)((S + kChunk - 1) / kChunk) * 2 * kChunk;
         if (tp->spart_n[axis] != n) {

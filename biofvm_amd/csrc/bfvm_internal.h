@@ -263,8 +263,9 @@ constexpr int kAdjointSumsChunk = 8;  // substrates per chunk of the partial slo
 struct AdjointSumsShape {
     int cols = 0;      // lines per block; 0: the fused kernel cannot run this axis
     size_t tiles = 0;  // blocks of one launch (slots per chunk of `partial`)
+    int xt_rows = 0;   // x: rows per block of the whole-cache-line form (0: column tiles)
 };
-AdjointSumsShape adjoint_sums_shape(const Env& e, int axis);
+AdjointSumsShape adjoint_sums_shape(const Env& e, int axis, int y_precision);
 hipError_t launch_adjoint_sums(Env& e, int axis, double* lam, const void* y, int y_precision, double* partial);
 // Whether this grid's x sweep can run with sources fused in (a segmented
 // one-block-per-tile shape: the autotuner's fastest, or the default); if so,

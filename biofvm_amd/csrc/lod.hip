@@ -874,9 +874,31 @@ __host__ __device__ __forceinline__ int xt_buffer_offset(int kg0, int kg1, int S
     return (d + 1) & ~1;
 }
 
-// One block's tile of the whole-cache-line x sweep (see above).
-template <typename T, int L, int SC>
-__device__ __forceinline__ void xt_tile(const SweepArgs& A, unsigned tile, double* smem)
+// What xt_tile tells its hook about the thread's segment: row0 = values
+// before its row, p0 its first node, s its substrate, slot its carry slot
+// (g*C + column), g its segment in the row, q its segment in the block (Q of
+// them; threads with q >= Q copy segment Q-1), act = it stores.
+struct XtPos {
+    unsigned row0;
+    int p0, s, S, slot, g, q, Q, C, nseg, n;
+    bool act;
+};
+
+// The plain sweep's hook: nothing besides the sweep.
+struct XtPlain {
+    __device__ __forceinline__ void begin(const XtPos&) {}
+    __device__ __forceinline__ void node(int, double) {}
+    __device__ __forceinline__ void end(const XtPos&, double*, double*, double*, double*) {}
+};
+
+// One block's tile of the whole-cache-line x sweep (see above).  hook.begin
+// runs once the tile's loads are issued, hook.node(t, z) receives each final
+// value (from the top of the segment down, before it is stored), hook.end
+// runs after the solve, with every thread of the block (the carry slots xa,
+// xb and the line-end values wfirst, wlast are free once it has passed a
+// block barrier).
+template <typename T, int L, int SC, class Hook>
+__device__ __forceinline__ void xt_tile(const SweepArgs& A, unsigned tile, double* smem, Hook& hook)
 {
     constexpr int G = 16 / (int)sizeof(T);  // nodes per chunk = values per lane piece
     static_assert(xt_shape_ok<T, SC>(L), "k_sweep_xt shape");
@@ -936,6 +958,9 @@ __device__ __forceinline__ void xt_tile(const SweepArgs& A, unsigned tile, doubl
             piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, kk * chunk, kLoadPolicy);
         }
     }
+    const int slot = g * C + col;
+    const XtPos pos{row0, p0, s, S, slot, g, q, Q, C, nseg, n, act && !dup};
+    hook.begin(pos);
     stage_tables(A, gtab);  // while the loads are in flight
     T raw[L];
 #pragma unroll
@@ -948,12 +973,12 @@ __device__ __forceinline__ void xt_tile(const SweepArgs& A, unsigned tile, doubl
     }
 
     const __amdgpu_buffer_rsrc_t ro = make_rsrc(A.out, A.nvs * sizeof(T));
-    const int slot = g * C + col;
     // a chunk is stored when its lane's piece lies in the row (SC: the
     // piece's node; SC = 0: the chunk's first node, chunks being whole)
     const int store_node = SC ? p0 + uj : p0;
     seg_core_at<T, L>(A, C, col, s, p0, slot, g == nseg - 1, slot, q < Q && g == 0 && !dup, raw, r, a, xa, xb,
                       wfirst, wlast, gtab, NoSource{}, [&](int t, double zf) {
+                          hook.node(t, zf);
                           mine[(t % G) * S + s] = (T)zf;
                           if (t % G == 0) {  // chunk t / G is complete
                               wave_lds_sync();
@@ -964,13 +989,15 @@ __device__ __forceinline__ void xt_tile(const SweepArgs& A, unsigned tile, doubl
                               wave_lds_sync();
                           }
                       });
+    hook.end(pos, xa, xb, wfirst, wlast);
 }
 
 template <typename T, int L, int SC>
 __global__ void __launch_bounds__(kMaxThreads, L <= 20 ? 8 : 4) k_sweep_xt(SweepArgs A)
 {
     extern __shared__ __attribute__((aligned(16))) double smem_xt[];
-    xt_tile<T, L, SC>(A, xcd_tile(blockIdx.x, gridDim.x, A.spread), smem_xt);
+    XtPlain plain;
+    xt_tile<T, L, SC>(A, xcd_tile(blockIdx.x, gridDim.x, A.spread), smem_xt, plain);
 }
 
 // Whole-cache-line x sweeps of several grids (runtime S): map[2b], map[2b+1] =
@@ -985,7 +1012,8 @@ k_sweep_xt_batch(const SweepArgs* __restrict__ args, const int* __restrict__ map
     const int member = __builtin_amdgcn_readfirstlane(map[2 * b]);
     const int local = __builtin_amdgcn_readfirstlane(map[2 * b + 1]);
     const SweepArgs A = args[member];
-    xt_tile<T, L, 0>(A, (unsigned)local, smem_xtb);
+    XtPlain plain;
+    xt_tile<T, L, 0>(A, (unsigned)local, smem_xtb, plain);
 }
 
 // The adjoint's sweep of one axis with that axis's D/lambda sums fused in
@@ -1000,6 +1028,41 @@ k_sweep_xt_batch(const SweepArgs* __restrict__ args, const int* __restrict__ map
 // the tile's slot of `partial` (per chunk of kGradChunk substrates: tiles x
 // 2*kGradChunk doubles, edge sums then l.y sums), the same every run.
 constexpr int kGradChunk = kAdjointSumsChunk;
+
+// ea += a[first + i*stride], eb += b[first + i*stride] for i = 0..n-1, in
+// that order (the same bits as a plain loop), the LDS loads issued eight at
+// a time so that their latencies overlap.
+__device__ __forceinline__ void ordered_sums(const double* a, const double* b, int first, int n, int stride,
+                                             double& ea, double& eb)
+{
+    int i = 0;
+#pragma unroll 1
+    for (; i + 8 <= n; i += 8) {
+        double ta[8], tb[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ta[k] = a[first + (i + k) * stride];
+            tb[k] = b[first + (i + k) * stride];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            ea += ta[k];
+            eb += tb[k];
+        }
+    }
+#pragma unroll 1
+    for (; i < n; ++i) {
+        ea += a[first + i * stride];
+        eb += b[first + i * stride];
+    }
+}
+
+// The tile's slot of substrate s in `partial` (edge sums at [0], l.y sums at
+// [kGradChunk]).
+__device__ __forceinline__ double* grad_slot(double* partial, int s, unsigned ntiles, unsigned tile)
+{
+    return partial + ((size_t)(s / kGradChunk) * ntiles + tile) * 2 * kGradChunk + s % kGradChunk;
+}
 
 template <typename TY, int L, int COLS>
 __global__ void __launch_bounds__(kMaxThreads, 4)
@@ -1020,6 +1083,15 @@ k_sweep_adjoint_sums(SweepArgs A, const TY* __restrict__ y, double* __restrict__
     const double r = A.wb[P.s * 8 + 0], a = A.wb[P.s * 8 + 1];
     double raw[L];
     seg_load<double, L, false>(A, P, raw);
+    // the slot's running sums, read now so the add at the end does not wait
+    // (FP32 y: the registers are there; FP64 y would lose a block per CU)
+    constexpr bool pre = sizeof(TY) == 4;
+    double old_e = 0.0, old_ly = 0.0;
+    if (pre && (int)threadIdx.x < A.S) {
+        const double* sl = grad_slot(partial, threadIdx.x, ntiles, w);
+        old_e = sl[0];
+        old_ly = sl[kGradChunk];
+    }
     TY yv[L];
     {
         // the same nodes of the forward output (P.vo counts bytes of doubles)
@@ -1062,11 +1134,7 @@ k_sweep_adjoint_sums(SweepArgs A, const TY* __restrict__ y, double* __restrict__
     // per column, its segments in order; then per substrate, its columns in order
     if (tid < COLS) {
         double e = 0.0, ly = 0.0;
-#pragma unroll 1
-        for (int q = 0; q < A.nseg; ++q) {
-            e += nl[q * COLS + tid];
-            ly += ny[q * COLS + tid];
-        }
+        ordered_sums(nl, ny, tid, A.nseg, COLS, e, ly);
         xa[tid] = e;
         xb[tid] = ly;
     }
@@ -1075,15 +1143,111 @@ k_sweep_adjoint_sums(SweepArgs A, const TY* __restrict__ y, double* __restrict__
         double e = 0.0, ly = 0.0;
         // the tile's columns of substrate s: c = c0, c0 + S, ... (c0 the first)
         const int c0 = ((s - tile * COLS) % A.S + A.S) % A.S;
-#pragma unroll 1
-        for (int c = c0; c < COLS; c += A.S) {
-            e += xa[c];
-            ly += xb[c];
-        }
-        double* slot = partial + ((size_t)(s / kGradChunk) * ntiles + w) * 2 * kGradChunk + s % kGradChunk;
-        slot[0] += e;
-        slot[kGradChunk] += ly;
+        ordered_sums(xa, xb, c0, (COLS - c0 + A.S - 1) / A.S, A.S, e, ly);
+        double* slot = grad_slot(partial, s, ntiles, w);
+        slot[0] = (pre && s == tid ? old_e : slot[0]) + e;
+        slot[kGradChunk] = (pre && s == tid ? old_ly : slot[kGradChunk]) + ly;
     }
+}
+
+// The fused adjoint sweep and sums for x in the whole-cache-line form
+// (k_sweep_xt's tile on the FP64 adjoint state, compile-time S): the
+// forward output y is loaded per thread beside the chunk loads (its nodes at
+// stride S), the sums are added as k_sweep_adjoint_sums adds them, the edge
+// to the next segment of the row goes through the carry slots, and the
+// block's sums are added up in a fixed order (segments of a column, then the
+// rows of a substrate) onto the tile's slot of `partial`, laid out as
+// k_sweep_adjoint_sums lays it out.
+template <typename TY, int L>
+struct XtAdjointSums {
+    const TY* y;
+    size_t nvs;
+    double* partial;
+    unsigned ntiles, tile;
+    static constexpr bool kPre = sizeof(TY) == 4;
+    TY yv[L];
+    int cnt;
+    double s_ly, s_edge, lnext, lfirst, llast, old_e, old_ly;
+
+    __device__ __forceinline__ void begin(const XtPos& p)
+    {
+        const __amdgpu_buffer_rsrc_t ry = make_rsrc(y, nvs * sizeof(TY));
+        cnt = min(L, p.n - p.p0);
+        const unsigned base = p.row0 + (unsigned)p.p0 * p.S + p.s;
+#pragma unroll
+        for (int t = 0; t < L; ++t)
+            yv[t] = buf_load((TY*)nullptr, ry, (base + (unsigned)(min(t, cnt - 1) * p.S)) * (unsigned)sizeof(TY), 0);
+        s_ly = s_edge = lnext = lfirst = llast = 0.0;
+        // the slot's running sums, read now so the add at the end does not wait
+        // (FP32 y: the registers are there; FP64 y would lose a block per CU)
+        old_e = old_ly = 0.0;
+        if (kPre && (int)threadIdx.x < p.S) {
+            const double* sl = grad_slot(partial, threadIdx.x, ntiles, tile);
+            old_e = sl[0];
+            old_ly = sl[kGradChunk];
+        }
+    }
+
+    __device__ __forceinline__ void node(int t, double zf)
+    {
+        if (t < cnt) {
+            s_ly += zf * (double)yv[t];
+            if (t + 1 < cnt) s_edge += (zf - lnext) * ((double)yv[t] - (double)yv[t + 1]);
+            lnext = zf;
+        }
+        if (t == L - 1) llast = zf;
+        if (t == 0) lfirst = zf;
+    }
+
+    __device__ __forceinline__ void end(const XtPos& p, double* xa, double* xb, double* wfirst, double* wlast)
+    {
+        const bool own = p.q < p.Q;  // copies of segment Q-1 write nothing
+        lds_barrier();               // every carry read is done
+        if (own) {
+            xa[p.slot] = lfirst;
+            xb[p.slot] = (double)yv[0];
+        }
+        lds_barrier();
+        // a segment with a next one in its row holds L real nodes
+        if (p.act && p.g + 1 < p.nseg)
+            s_edge += (llast - xa[p.slot + p.C]) * ((double)yv[L - 1] - xb[p.slot + p.C]);
+        lds_barrier();
+        if (own) {
+            xa[p.slot] = p.act ? s_edge : 0.0;
+            xb[p.slot] = p.act ? s_ly : 0.0;
+        }
+        lds_barrier();
+        const int tid = threadIdx.x;
+        if (tid < p.C) {  // per column, its segments in order
+            double e = 0.0, ly = 0.0;
+            ordered_sums(xa, xb, tid, p.nseg, p.C, e, ly);
+            wfirst[tid] = e;
+            wlast[tid] = ly;
+        }
+        lds_barrier();
+        for (int s = tid; s < p.S; s += blockDim.x) {  // per substrate, its rows in order
+            double e = 0.0, ly = 0.0;
+            ordered_sums(wfirst, wlast, s, p.C / p.S, p.S, e, ly);
+            double* slot = grad_slot(partial, s, ntiles, tile);
+            slot[0] = (kPre && s == tid ? old_e : slot[0]) + e;
+            slot[kGradChunk] = (kPre && s == tid ? old_ly : slot[kGradChunk]) + ly;
+        }
+    }
+};
+
+template <typename TY, int L, int SC>
+__global__ void __launch_bounds__(kMaxThreads, 4)
+k_sweep_adjoint_sums_xt(SweepArgs A, const TY* __restrict__ y, double* __restrict__ partial)
+{
+    extern __shared__ __attribute__((aligned(16))) double smem_xa[];
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x, A.spread);
+    XtAdjointSums<TY, L> hook;
+    hook.y = y;
+    hook.nvs = A.nvs;
+    hook.partial = partial;
+    hook.ntiles = gridDim.x;
+    hook.tile = tile;
+    xt_tile<double, L, SC>(A, tile, smem_xa, hook);
 }
 
 // One thread owns one whole (line, substrate) pair and runs the reference's
@@ -1572,7 +1736,8 @@ XtShape xt_shape(const Env& e, int C, int L)
     x.Q = (C / S) * ((e.nx + L - 1) / L);
     const int waves = (x.Q + segw - 1) / segw;
     x.threads = waves * 64;
-    x.pitch = xt_compile_time_s(S) ? (G + 1) * S : xt_pitch(S);
+    // compile-time forms: FP32 S = 4, 8, 16; FP64 also S = 2 (the adjoint's)
+    x.pitch = xt_compile_time_s(S) || (G == 2 && S == 2) ? (G + 1) * S : xt_pitch(S);
     x.lds = sizeof(double) * (size_t)xt_buffer_offset(e.ax[0].kg0, e.ax[0].kg1, S, x.Q, C) +
             (size_t)waves * segw * x.pitch * sizeof(T);
     return x;
@@ -2018,7 +2183,7 @@ hipError_t launch_sweep_to(Env& e, int axis, const void* src, void* dst)
 // 8-node segments, the widest column tiles of up to 1024 threads: its shape
 // (lines per block, 0 when it cannot run: a line of one node, or 32-bit
 // buffer offsets short of the FP64 state) and its tiles.
-AdjointSumsShape adjoint_sums_shape(const Env& e, int axis)
+AdjointSumsShape adjoint_sums_shape(const Env& e, int axis, int y_precision)
 {
     AdjointSumsShape sh;
     const Geometry gm = geometry(e, axis);
@@ -2027,6 +2192,23 @@ AdjointSumsShape adjoint_sums_shape(const Env& e, int axis)
     if (gm.n < 2 || e.nvs * sizeof(double) >= 0xffffffffull || e.nvox >= (1ull << 31)) return sh;
     const size_t gbytes = sizeof(double) * (size_t)(e.ax[axis].kg0 + e.ax[axis].kg1 + 3 * kPad + 8) * e.S;
     if (gbytes > kMaxCorrectionLds) return sh;
+    // x lines of 2-16 substrates (a node's FP64 values whole 16-byte pieces):
+    // the whole-cache-line form, as many rows per block as 1024 threads hold
+    // (512 with an FP64 y: its 66 VGPRs let 7 waves per SIMD run, one
+    // 1024-thread block per compute unit)
+    const int S = e.S;
+    if (axis == 0 && (S == 2 || S == 4 || S == 8 || S == 16) && e.nx >= 2) {
+        const int rows = gm.ncols / S;
+        const int most = y_precision == BFVM_FP32 ? kMaxThreads : kMaxThreads / 2;
+        const int R = std::min(rows, std::max(1, most / (nseg * S)));
+        const XtShape x = xt_shape<double>(e, R * S, L);
+        if (x.Q > 0 && x.threads <= kMaxThreads && x.lds <= kMaxCorrectionLds) {
+            sh.cols = R * S;
+            sh.xt_rows = R;
+            sh.tiles = (size_t)((rows + R - 1) / R);
+            return sh;
+        }
+    }
     for (int cols : {64, 32, 16})
         if (cols * nseg <= kMaxThreads) {
             sh.cols = cols;
@@ -2036,6 +2218,33 @@ AdjointSumsShape adjoint_sums_shape(const Env& e, int axis)
     const SweepArgs A = sweep_args(e, axis, sh.cols, nseg);
     sh.tiles = (size_t)A.tiles * A.planes;
     return sh;
+}
+
+template <typename TY, int SC>
+hipError_t launch_adjoint_sums_xt(Env& e, int cols, double* lam, const void* y, double* partial)
+{
+    constexpr int L = kAdjointSumsNodes;
+    const XtShape x = xt_shape<double>(e, cols, L);
+    void* keep = e.rho;
+    e.rho = lam;
+    const SweepArgs A = sweep_args(e, 0, cols, (e.nx + L - 1) / L);
+    e.rho = keep;
+    const hipError_t st = allow_table_lds((const void*)k_sweep_adjoint_sums_xt<TY, L, SC>, x.lds);
+    if (st != hipSuccess) return st;
+    k_sweep_adjoint_sums_xt<TY, L, SC><<<(unsigned)A.tiles, x.threads, x.lds, e.stream>>>(A, (const TY*)y, partial);
+    return hipGetLastError();
+}
+
+template <typename TY>
+hipError_t launch_adjoint_sums_xt_S(Env& e, int cols, double* lam, const void* y, double* partial)
+{
+    switch (e.S) {
+    case 2: return launch_adjoint_sums_xt<TY, 2>(e, cols, lam, y, partial);
+    case 4: return launch_adjoint_sums_xt<TY, 4>(e, cols, lam, y, partial);
+    case 8: return launch_adjoint_sums_xt<TY, 8>(e, cols, lam, y, partial);
+    case 16: return launch_adjoint_sums_xt<TY, 16>(e, cols, lam, y, partial);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <typename TY, int COLS>
@@ -2059,8 +2268,11 @@ hipError_t launch_adjoint_sums_t(Env& e, int axis, double* lam, const void* y, d
 // e.elem FP64 while the arguments are built); y in the store's precision.
 hipError_t launch_adjoint_sums(Env& e, int axis, double* lam, const void* y, int y_precision, double* partial)
 {
-    const AdjointSumsShape sh = adjoint_sums_shape(e, axis);
+    const AdjointSumsShape sh = adjoint_sums_shape(e, axis, y_precision);
     const bool f = y_precision == BFVM_FP32;
+    if (sh.xt_rows)
+        return f ? launch_adjoint_sums_xt_S<float>(e, sh.cols, lam, y, partial)
+                 : launch_adjoint_sums_xt_S<double>(e, sh.cols, lam, y, partial);
     switch (sh.cols) {
     case 64: return f ? launch_adjoint_sums_t<float, 64>(e, axis, lam, y, partial)
                       : launch_adjoint_sums_t<double, 64>(e, axis, lam, y, partial);
