@@ -560,7 +560,7 @@ def differentiation_result(full):
     FP64 differences of J (summed with math.fsum) on random directions in the
     joint parameter space (initial field, D, lambda, secretion, uptake):
     4th-order central differences (Richardson, h = 1e-4 relative,
-    truncation O(h^4)).  full: 8 directions and the FP64 adjoint beside the
+    truncation O(h^4)).  full: 32 directions and the FP64 adjoint beside the
     FP32 one; otherwise 2 directions, FP32 adjoint only."""
     import math
     from biofvm_amd import FP32, FP64, setup_case
@@ -627,7 +627,7 @@ def differentiation_result(full):
             r = mm.download() - target
         return 0.5 * math.fsum(r * r)
 
-    h, n_dir = 1e-4, (8 if full else 2)
+    h, n_dir = 1e-4, (32 if full else 2)
     worst = {k: 0.0 for k in grads}
     for _ in range(n_dir):
         ds = {k: rng.normal(size=np.shape(case[k])) * np.asarray(case[k]) for k in keys}

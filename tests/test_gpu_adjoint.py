@@ -115,6 +115,40 @@ def test_transpose_identity():
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
 
 
+@pytest.mark.parametrize("S", [1, 3, 4, 8, 16])
+def test_d_lambda_gradients_for_each_substrate_count(S):
+    # the x sums run on whole-cache-line tiles for S = 2, 4, 8, 16 (S = 2 in
+    # the cases above) and on column tiles otherwise; every substrate's D and
+    # lambda component against central differences of the oracle (FP64)
+    rng = np.random.default_rng(20 + S)
+    nx, ny, nz = 13, 6, 5
+    case = W.make_case(nx, ny, nz, S, 20.0, 10 ** rng.uniform(2, 4.5, S), 10 ** rng.uniform(-3, 0, S), 0.01, 4,
+                       DIFFUSION, rng.uniform(0, 1, nx * ny * nz * S))
+    target = oracle.oracle_run(case) + rng.normal(0, 0.05, case["rho0"].size)
+    grads = {}
+    for precision in (FP64, FP32):
+        with setup_case(case, precision) as m:
+            m.tape_record(case["dt"], case["steps"], case["phases"], 2)
+            loss = m.tape_seed_l2(target)
+            grads[precision] = m.tape_backward()
+        assert abs(loss - loss_of(oracle.oracle_run(case), target)) <= 1e-5 * loss
+    g = grads[FP64]
+    for gname in ("grad_D", "grad_lambda"):  # the FP32 store's kernels (FP32 forward states)
+        np.testing.assert_allclose(grads[FP32][gname], g[gname], rtol=1e-4, atol=1e-4 * np.abs(g[gname]).max())
+    for key, gname, ckey in (("D", "grad_D", "D"), ("lam", "grad_lambda", "lam")):
+        p = np.asarray(case[ckey], dtype=np.float64)
+        for s in range(S):
+            d = np.zeros(S)
+            d[s] = p[s]
+            jp = loss_of(oracle.oracle_run(with_params(case, **{key: p + H * d})), target)
+            jm = loss_of(oracle.oracle_run(with_params(case, **{key: p - H * d})), target)
+            fd = (jp - jm) / (2 * H)
+            ad = float(g[gname][s] * d[s])
+            # 1e-8: the differences' rounding noise (J ~ 10 to 1e-16, over 2h)
+            # is about 1e-9; a slow substrate's component can be 1e-5
+            assert abs(ad - fd) <= 1e-6 * abs(fd) + 1e-8, (key, s, ad, fd)
+
+
 def test_checkpoint_interval_does_not_change_gradients(small):
     case, target = small
     ref = adjoint_grads(case, target, FP64, checkpoint_every=1)[1]
