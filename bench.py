@@ -386,30 +386,36 @@ def ceiling_bench(args):
     """`--ceiling`: the bandwidth ceilings the sweeps are judged against,
     from libbfvm_ceiling.so (biofvm_amd/csrc/ceiling.hip): streaming passes
     over a buffer the size of the FP32 400^3 x 8 store (in place, copy, read,
-    write; the in-place pass also over a 128 MB region that stays in the
+    write, each with plain and non-temporal accesses, the faster one as
+    `gbps`; the in-place pass also over a 128 MB region that stays in the
     Infinity Cache), then per store precision the three sweeps of the
     library (autotuned, diffusion-only steps, HIP events per launch) beside
-    the same sweeps' access pattern alone (same tiles, block shapes, strides
-    and XCD numbering; loads, one barrier, stores; no solve)."""
+    the same sweeps' access pattern alone (same tiles, block shapes, strides,
+    XCD numbering and cache policy; loads, one barrier, stores; no solve)."""
     import ctypes
     from biofvm_amd import DIFFUSION, FP32, FP64, setup_case
     from biofvm_amd import build as B
     from biofvm_amd import workloads as W
     lib = ctypes.CDLL(B.CEILING_LIB)
     dpp = ctypes.POINTER(ctypes.c_double)
-    lib.bfc_stream_pass.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, dpp, dpp]
-    lib.bfc_sweep_pattern.argtypes = [ctypes.c_int] * 8 + [dpp, dpp]
+    lib.bfc_stream_pass.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int,
+                                    dpp, dpp]
+    lib.bfc_sweep_pattern.argtypes = [ctypes.c_int] * 9 + [dpp, dpp]
     lib.bfc_sweep_pattern_xt.argtypes = [ctypes.c_int] * 6 + [dpp, dpp]
     n, S = 400, 8
     store = n ** 3 * S * 4
     out = {"mode": "ceiling", "unit": "GB/s", "peak": HBM_PEAK_GBS, "reps": 10}
 
     def stream(kind, buf, region):
-        g, ms = ctypes.c_double(), ctypes.c_double()
-        rc = lib.bfc_stream_pass(kind, buf, region, 10, ctypes.byref(g), ctypes.byref(ms))
-        if rc:
-            raise RuntimeError(f"bfc_stream_pass({kind}) failed: {rc}")
-        return {"gbps": g.value, "ms": ms.value, "region_bytes": region}
+        res = {"region_bytes": region}
+        for nt, tag in ((0, "plain"), (1, "non_temporal")):
+            g, ms = ctypes.c_double(), ctypes.c_double()
+            rc = lib.bfc_stream_pass(kind, nt, buf, region, 10, ctypes.byref(g), ctypes.byref(ms))
+            if rc:
+                raise RuntimeError(f"bfc_stream_pass({kind}, {nt}) failed: {rc}")
+            res[tag] = {"gbps": g.value, "ms": ms.value}
+        res["gbps"] = max(res["plain"]["gbps"], res["non_temporal"]["gbps"])
+        return res
 
     out["streams"] = {
         "in_place_read_write": stream(0, store, store),
@@ -426,6 +432,7 @@ def ceiling_bench(args):
         m.autotune(case["dt"])
         plans = m.sweep_plans()
         spreads = [m.sweep_spread(a) for a in range(3)]
+        policies = m.sweep_cache_policies()
         m.run(case["dt"], 2, DIFFUSION)
         _, per = m.timed_run(case["dt"], 10, DIFFUSION, per_kernel=True)
         m.close()
@@ -438,11 +445,12 @@ def ceiling_bench(args):
             if streaming == 3:  # the whole-cache-line x kernel has its own pattern
                 rc = lib.bfc_sweep_pattern_xt(elem, n, cols, L, spreads[a], 10, ctypes.byref(g), ctypes.byref(pms))
             else:
-                rc = lib.bfc_sweep_pattern(elem, n, S, a, cols, L, spreads[a], 10, ctypes.byref(g),
+                rc = lib.bfc_sweep_pattern(elem, n, S, a, cols, L, spreads[a], policies[a], 10, ctypes.byref(g),
                                            ctypes.byref(pms))
             if rc:
                 raise RuntimeError(f"bfc_sweep_pattern({k}) failed: {rc}")
-            res[k] = {"plan_lines_nodes_streaming": plans[a], "spread": spreads[a], "sweep_ms": ms,
+            res[k] = {"plan_lines_nodes_streaming": plans[a], "spread": spreads[a],
+                      "non_temporal": policies[a], "sweep_ms": ms,
                       "sweep_gbps": sweep, "pattern_ms": pms.value, "pattern_gbps": g.value,
                       "sweep_over_pattern": sweep / g.value,
                       "sweep_over_in_place": sweep / out["streams"]["in_place_read_write"]["gbps"]}

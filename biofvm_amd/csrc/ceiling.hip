@@ -3,8 +3,9 @@
 // Built into its own library (libbfvm_ceiling.so), never linked into the
 // product library: these kernels only move bytes.  Two kinds:
 //  - streaming passes over one buffer (in-place read+write, copy, read only,
-//    write only), float4 per lane, grid-stride; a region smaller than the
-//    buffer, passed over again and again, stays in the 256 MB Infinity Cache;
+//    write only), one float4 per lane, plain or non-temporal; a region
+//    smaller than the buffer, passed over again and again, stays in the
+//    256 MB Infinity Cache;
 //  - the sweeps' access pattern: the same column tiles, block shapes, line
 //    and plane strides and XCD tile numbering as the segmented sweeps (C
 //    lines per block, L nodes per thread, ceil(n/L) segments per line), each
@@ -34,25 +35,27 @@ __device__ unsigned xcd_map(unsigned lin, unsigned N, int spread)
     return lin - j + (j % kXcds) * k + j / kXcds;
 }
 
-__global__ void k_pass(float4* a, float4* b, size_t n4, int kind, float s, float* sink)
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// One float4 per thread over an exact grid (every load of a wave issued at
+// once, no loop), plain or non-temporal (NT) loads and stores: the fastest
+// streaming form measured on this part (a grid-stride loop with one float4 in
+// flight per iteration reached 4.7 TB/s in place against 5.9-6.7 here).
+template <int NT>
+__global__ void __launch_bounds__(256) k_pass(f4v* a, f4v* b, size_t n4, int kind, float s, float* sink)
 {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    float acc = 0.f;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-        if (kind == 0) {  // in place
-            float4 v = a[i];
-            v.x *= s; v.y *= s; v.z *= s; v.w *= s;
-            a[i] = v;
-        } else if (kind == 1) {  // copy a -> b
-            b[i] = a[i];
-        } else if (kind == 2) {  // read
-            const float4 v = a[i];
-            acc += v.x + v.y + v.z + v.w;
-        } else {  // write
-            a[i] = make_float4(s, s, s, s);
-        }
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    f4v v = {s, s, s, s};
+    if (kind != 3) v = NT ? __builtin_nontemporal_load(a + i) : a[i];
+    if (kind == 2) {
+        if (v.x + v.y + v.z + v.w == 12345.f) sink[0] = v.x;  // keeps the load
+        return;
     }
-    if (kind == 2 && acc == 12345.f) sink[0] = acc;  // keeps the loads
+    if (kind == 0) v *= s;
+    f4v* dst = kind == 1 ? b + i : a + i;
+    if (NT) __builtin_nontemporal_store(v, dst);
+    else *dst = v;
 }
 
 // One column tile: threads (lane, g) = (t % C, t / C) load nodes g*L ..
@@ -60,7 +63,7 @@ __global__ void k_pass(float4* a, float4* b, size_t n4, int kind, float s, float
 //   axis 0: lines (k, j, s) along i, one plane of ny*nz*S columns
 //   axis 1: plane k, columns (i, s) along j
 //   axis 2: one plane of (j, i, s) columns along k
-template <typename T, int L>
+template <typename T, int L, int NT>
 __global__ void __launch_bounds__(1024) k_pattern(T* rho, int n, int C, int nseg, unsigned tiles, unsigned total,
                                                   size_t line_stride, size_t plane_stride, int axis, int S,
                                                   int nx, int spread, T scale)
@@ -78,15 +81,23 @@ __global__ void __launch_bounds__(1024) k_pattern(T* rho, int n, int C, int nseg
     const int cnt = act ? min(L, n - p0) : 0;
     T v[L];
 #pragma unroll
-    for (int t = 0; t < L; ++t) v[t] = t < cnt ? rho[base + (size_t)(p0 + t) * line_stride] : T(0);
+    for (int t = 0; t < L; ++t) {
+        T* p = rho + base + (size_t)(p0 + t) * line_stride;
+        v[t] = t < cnt ? (NT ? __builtin_nontemporal_load(p) : *p) : T(0);
+    }
     __syncthreads();
 #pragma unroll
-    for (int t = 0; t < L; ++t)
-        if (t < cnt) rho[base + (size_t)(p0 + t) * line_stride] = v[t] * scale;
+    for (int t = 0; t < L; ++t) {
+        T* p = rho + base + (size_t)(p0 + t) * line_stride;
+        if (t < cnt) {
+            if (NT) __builtin_nontemporal_store(v[t] * scale, p);
+            else *p = v[t] * scale;
+        }
+    }
 }
 
 template <typename T, int L>
-hipError_t launch_pattern(T* rho, int n, int S, int axis, int C, int spread, hipStream_t st)
+hipError_t launch_pattern(T* rho, int n, int S, int axis, int C, int spread, int nt, hipStream_t st)
 {
     const int nseg = (n + L - 1) / L;
     const size_t ncols = axis == 0 ? (size_t)n * n * S : axis == 1 ? (size_t)n * S : (size_t)n * n * S;
@@ -95,19 +106,23 @@ hipError_t launch_pattern(T* rho, int n, int S, int axis, int C, int spread, hip
     const size_t line_stride = axis == 0 ? (size_t)S : axis == 1 ? (size_t)n * S : (size_t)n * n * S;
     const size_t plane_stride = (size_t)n * n * S;
     const int threads = (C * nseg + 63) / 64 * 64;
-    k_pattern<T, L><<<tiles * planes, threads, 0, st>>>(rho, n, C, nseg, tiles, tiles * planes, line_stride,
-                                                         plane_stride, axis, S, n, spread, T(1));
+    if (nt)
+        k_pattern<T, L, 1><<<tiles * planes, threads, 0, st>>>(rho, n, C, nseg, tiles, tiles * planes, line_stride,
+                                                              plane_stride, axis, S, n, spread, T(1));
+    else
+        k_pattern<T, L, 0><<<tiles * planes, threads, 0, st>>>(rho, n, C, nseg, tiles, tiles * planes, line_stride,
+                                                              plane_stride, axis, S, n, spread, T(1));
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t pattern_dispatch(T* rho, int n, int S, int axis, int C, int L, int spread, hipStream_t st)
+hipError_t pattern_dispatch(T* rho, int n, int S, int axis, int C, int L, int spread, int nt, hipStream_t st)
 {
     switch (L) {
-    case 8: return launch_pattern<T, 8>(rho, n, S, axis, C, spread, st);
-    case 16: return launch_pattern<T, 16>(rho, n, S, axis, C, spread, st);
-    case 20: return launch_pattern<T, 20>(rho, n, S, axis, C, spread, st);
-    case 32: return launch_pattern<T, 32>(rho, n, S, axis, C, spread, st);
+    case 8: return launch_pattern<T, 8>(rho, n, S, axis, C, spread, nt, st);
+    case 16: return launch_pattern<T, 16>(rho, n, S, axis, C, spread, nt, st);
+    case 20: return launch_pattern<T, 20>(rho, n, S, axis, C, spread, nt, st);
+    case 32: return launch_pattern<T, 32>(rho, n, S, axis, C, spread, nt, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -213,11 +228,13 @@ extern "C" {
 
 // A streaming pass of `kind` (0 in-place read+write, 1 copy, 2 read, 3 write)
 // over `region_bytes` of a `buffer_bytes` buffer (a multiple of 16 bytes),
-// `reps` times after one untimed pass.  *gbps = bytes moved per pass (2x the
-// region for 0 and 1) / average pass time.  Returns 0 on success.
-int bfc_stream_pass(int kind, size_t buffer_bytes, size_t region_bytes, int reps, double* gbps, double* ms)
+// with plain (nt = 0) or non-temporal (nt = 1) loads and stores, `reps` times
+// after one untimed pass.  *gbps = bytes moved per pass (2x the region for 0
+// and 1) / average pass time.  Returns 0 on success.
+int bfc_stream_pass(int kind, int nt, size_t buffer_bytes, size_t region_bytes, int reps, double* gbps, double* ms)
 {
-    if (kind < 0 || kind > 3 || region_bytes > buffer_bytes || region_bytes % 16 || reps < 1) return 1;
+    if (kind < 0 || kind > 3 || nt < 0 || nt > 1 || region_bytes > buffer_bytes || region_bytes % 16 || reps < 1)
+        return 1;
     float *a = nullptr, *b = nullptr, *sink = nullptr;
     if (hipMalloc((void**)&a, buffer_bytes) != hipSuccess) return 3;
     if (kind == 1 && hipMalloc((void**)&b, buffer_bytes) != hipSuccess) {
@@ -227,9 +244,11 @@ int bfc_stream_pass(int kind, size_t buffer_bytes, size_t region_bytes, int reps
     (void)hipMalloc((void**)&sink, 16);
     (void)hipMemset(a, 0, buffer_bytes);
     const size_t n4 = region_bytes / 16;
+    const unsigned blocks = (unsigned)((n4 + 255) / 256);
     const int rc = time_reps(
         [&] {
-            k_pass<<<8192, 256>>>((float4*)a, (float4*)b, n4, kind, 1.0f, sink);
+            if (nt) k_pass<1><<<blocks, 256>>>((f4v*)a, (f4v*)b, n4, kind, 1.0f, sink);
+            else k_pass<0><<<blocks, 256>>>((f4v*)a, (f4v*)b, n4, kind, 1.0f, sink);
             return hipGetLastError();
         },
         reps, ms);
@@ -244,12 +263,14 @@ int bfc_stream_pass(int kind, size_t buffer_bytes, size_t region_bytes, int reps
 
 // The access pattern of one sweep on an n^3 grid of S substrates (elem = 4
 // for an FP32 store, 8 for FP64): C lines per block, L in {8, 16, 20, 32} nodes
-// per thread, XCD spread as the sweeps take it.  *gbps = 2 x the store's
-// bytes / average launch time.
-int bfc_sweep_pattern(int elem, int n, int S, int axis, int C, int L, int spread, int reps, double* gbps,
+// per thread, XCD spread and cache policy (nt = 1: non-temporal loads and
+// stores) as the sweeps take them.  *gbps = 2 x the store's bytes / average
+// launch time.
+int bfc_sweep_pattern(int elem, int n, int S, int axis, int C, int L, int spread, int nt, int reps, double* gbps,
                       double* ms)
 {
-    if ((elem != 4 && elem != 8) || axis < 0 || axis > 2 || C <= 0 || C * ((n + L - 1) / L) > 1024 || reps < 1)
+    if ((elem != 4 && elem != 8) || axis < 0 || axis > 2 || C <= 0 || C * ((n + L - 1) / L) > 1024 || reps < 1 ||
+        nt < 0 || nt > 1)
         return 1;
     const size_t ncols = axis == 1 ? (size_t)n * S : (size_t)n * n * S;
     if (ncols % (size_t)C) return 1;
@@ -259,8 +280,8 @@ int bfc_sweep_pattern(int elem, int n, int S, int axis, int C, int L, int spread
     (void)hipMemset(rho, 0, bytes);
     const int rc = time_reps(
         [&] {
-            return elem == 4 ? pattern_dispatch<float>((float*)rho, n, S, axis, C, L, spread, nullptr)
-                             : pattern_dispatch<double>((double*)rho, n, S, axis, C, L, spread, nullptr);
+            return elem == 4 ? pattern_dispatch<float>((float*)rho, n, S, axis, C, L, spread, nt, nullptr)
+                             : pattern_dispatch<double>((double*)rho, n, S, axis, C, L, spread, nt, nullptr);
         },
         reps, ms);
     (void)hipDeviceSynchronize();
