@@ -54,6 +54,10 @@ struct Tape {
     void* s4 = nullptr;       // after the x sweep
     void* s6 = nullptr;       // after the y sweep
     void* s8 = nullptr;       // after the z sweep (the step's result)
+    // the agent voxels' values of a step's start (n_agent_voxels x S, store
+    // precision) while the agents run in place on it (forward_step_keep)
+    void* asave = nullptr;
+    size_t asave_bytes = 0;
     double* lam = nullptr;    // adjoint state (FP64)
     bool seeded = false;
     double* rk = nullptr;      // agents: intermediate densities (n_agents x S)
@@ -549,25 +553,40 @@ hipError_t forward_step(Env& e, const Tape& tp, void* buf)
 
 // The step from `start` again, keeping s1 (after bulk), s4, s6, s8 (after
 // each sweep; without diffusion all three hold the state after the agents).
+// `start` is left as it was on entry.
 hipError_t forward_step_keep(Env& e, Tape& tp, const void* start)
 {
     hipError_t st;
-    // without a bulk update the state after it is the start itself (the
-    // ring slot stays untouched until the step's transposes have run)
-    if ((tp.flags & BFVM_BULK) && e.have_bulk) {
-        if ((st = copy_store(e, tp.s1, start))) return st;
-        if ((st = on_buffer(e, tp.s1, [&] { return bfvm::launch_bulk(e); }))) return st;
-        tp.after_bulk = tp.s1;
-    } else {
-        tp.after_bulk = start;
-    }
-    if ((st = copy_store(e, tp.s4, tp.after_bulk))) return st;
-    if ((tp.flags & BFVM_CELLS) && e.n_agents > 0 &&
-        (st = on_buffer(e, tp.s4, [&] { return bfvm::launch_agents(e); })))
-        return st;
+    const bool bulk = (tp.flags & BFVM_BULK) && e.have_bulk;
+    const bool cells = (tp.flags & BFVM_CELLS) && e.n_agents > 0;
     const bool diff = tp.flags & BFVM_DIFFUSION;
+    int axis = 0;
+    if (!bulk && diff && e.lod_dims > 0 && e.n_dir == 0 && (!cells || tp.asave)) {
+        // the agents run on the start itself, their voxels' values saved
+        // first and written back after the x sweep, which reads the start
+        // and writes s4: no copy of the store (the start is the previous
+        // step's result, and its agent voxels the agents' adjoint input)
+        tp.after_bulk = start;
+        void* s0 = const_cast<void*>(start);
+        if (cells && (st = on_buffer(e, s0, [&] { return bfvm::launch_agents_saving(e, tp.asave); }))) return st;
+        if ((st = bfvm::launch_sweep_to(e, 0, start, tp.s4))) return st;
+        if (cells && (st = on_buffer(e, s0, [&] { return bfvm::launch_agents_restore(e, tp.asave); }))) return st;
+        axis = 1;
+    } else {
+        // without a bulk update the state after it is the start itself (the
+        // ring slot stays untouched until the step's transposes have run)
+        if (bulk) {
+            if ((st = copy_store(e, tp.s1, start))) return st;
+            if ((st = on_buffer(e, tp.s1, [&] { return bfvm::launch_bulk(e); }))) return st;
+            tp.after_bulk = tp.s1;
+        } else {
+            tp.after_bulk = start;
+        }
+        if ((st = copy_store(e, tp.s4, tp.after_bulk))) return st;
+        if (cells && (st = on_buffer(e, tp.s4, [&] { return bfvm::launch_agents(e); }))) return st;
+    }
     void* bufs[3] = {tp.s4, tp.s6, tp.s8};
-    for (int axis = 0; axis < 3; ++axis) {
+    for (; axis < 3; ++axis) {
         const bool sweep = diff && axis < e.lod_dims;
         if (axis > 0 && sweep && e.n_dir == 0) {
             // the y and z sweeps read the kept state and write the next one
@@ -705,7 +724,7 @@ std::vector<uintptr_t> segment_key(const Env& e, const Tape& tp, void* const* sh
                                 (uintptr_t)e.lod_dims, (uintptr_t)tp.flags, (uintptr_t)tp.nkeep,
                                 (uintptr_t)e.bulk_A, (uintptr_t)e.bulk_B, (uintptr_t)tp.lam,
                                 (uintptr_t)tp.gpart, (uintptr_t)tp.rk, (uintptr_t)tp.gsec, (uintptr_t)tp.gupt,
-                                (uintptr_t)tp.last_s8};
+                                (uintptr_t)tp.last_s8, (uintptr_t)tp.asave};
     for (int i = 0; i < 4; ++i) k.push_back((uintptr_t)shared[i]);
     for (double* p : tp.spart) k.push_back((uintptr_t)p);
     for (void* p : tp.ring) k.push_back((uintptr_t)p);
@@ -752,6 +771,7 @@ void free_tape(Env& e)
     free_ptr(tp->s4);
     free_ptr(tp->s6);
     free_ptr(tp->s8);
+    free_ptr(tp->asave);
     free_ptr(tp->lam);
     free_ptr(tp->rk);
     free_ptr(tp->gsec);
@@ -921,6 +941,14 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
         }
         AHIP(e, hipMemsetAsync(tp->gsec, 0, sizeof(double) * na, e.stream));
         AHIP(e, hipMemsetAsync(tp->gupt, 0, sizeof(double) * na, e.stream));
+    }
+    // the agent voxels' saved values (forward_step_keep)
+    const size_t asave_bytes = (tp->flags & BFVM_CELLS) ? (size_t)e.n_agent_voxels * S * e.elem : 0;
+    if (tp->asave_bytes != asave_bytes) {
+        free_ptr(tp->asave);
+        tp->asave_bytes = 0;
+        if (asave_bytes) AHIP(e, hipMalloc(&tp->asave, asave_bytes));
+        tp->asave_bytes = asave_bytes;
     }
     // segments from the last
     const int K = tp->K;

@@ -247,6 +247,11 @@ hipError_t launch_bulk(Env& e);
 // with_bulk: the bulk update of each agent voxel first (the fused x sweep
 // then applies it to the voxels without agents only)
 hipError_t launch_agents(Env& e, bool with_bulk = false);
+// launch_agents (no bulk update) with each agent voxel's values as they were
+// before it copied to save (n_agent_voxels * S values of the store's type),
+// and the copy written back to the store.
+hipError_t launch_agents_saving(Env& e, void* save);
+hipError_t launch_agents_restore(Env& e, const void* save);
 // Gradient vectors of the store into out (nvs*3 values, store precision).
 hipError_t launch_gradient(Env& e, void* out);
 hipError_t launch_sweep(Env& e, int axis);

@@ -65,11 +65,13 @@ __global__ void k_bulk(T* __restrict__ rho, const double* __restrict__ A,
 // applies the group's agents in list order, rho = (rho + t1)/t2, so the
 // result is the reference's single-threaded result, with no atomics.  The
 // constants are FP64 and the chain runs in FP64 for either store precision.
-template <typename T>
+// SAVE: the value each thread reads first goes to save[t] as well (the
+// backward pass keeps a step's start this way while the step runs in place).
+template <typename T, bool SAVE = false>
 __global__ void k_agents(T* __restrict__ rho, int S, int n_groups, const int* __restrict__ gvox,
                          const int* __restrict__ gstart, const double* __restrict__ t1,
                          const double* __restrict__ t2, const double* __restrict__ bulk_a,
-                         const double* __restrict__ bulk_b)
+                         const double* __restrict__ bulk_b, T* __restrict__ save = nullptr)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)n_groups * S) return;
@@ -77,7 +79,9 @@ __global__ void k_agents(T* __restrict__ rho, int S, int n_groups, const int* __
     int s = (int)(t - (size_t)g * S);
     const size_t i = (size_t)gvox[g] * S + s;
     T* r = rho + i;
-    double v = (double)*r;
+    const T r0 = *r;
+    if (SAVE) save[t] = r0;
+    double v = (double)r0;
     // the bulk update of this voxel first, as k_bulk would have left it (the
     // x sweep applies it to the voxels without agents)
     if (bulk_a) v = (double)(T)((v + bulk_a[i]) / bulk_b[i]);
@@ -87,6 +91,19 @@ __global__ void k_agents(T* __restrict__ rho, int S, int n_groups, const int* __
         v = (v + t1[q]) / t2[q];
     }
     *r = (T)v;
+}
+
+// Writes back what k_agents<T, true> saved: rho at every agent voxel as it
+// was before the agents ran.
+template <typename T>
+__global__ void k_agents_restore(T* __restrict__ rho, int S, int n_groups, const int* __restrict__ gvox,
+                                 const T* __restrict__ save)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n_groups * S) return;
+    const int g = (int)(t / S);
+    const int s = (int)(t - (size_t)g * S);
+    rho[(size_t)gvox[g] * S + s] = save[t];
 }
 
 // k_agents for several grids: thread t belongs to the member whose range
@@ -241,6 +258,34 @@ hipError_t launch_agents(Env& e, bool with_bulk)
         k_agents<float><<<grid_for(n), kBlock, 0, e.stream>>>(
             (float*)e.rho, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
             (const double*)e.agent_t1, (const double*)e.agent_t2, ba, bb);
+    return hipGetLastError();
+}
+
+hipError_t launch_agents_saving(Env& e, void* save)
+{
+    if (e.n_agent_voxels == 0) return hipSuccess;
+    const size_t n = (size_t)e.n_agent_voxels * e.S;
+    if (e.precision == BFVM_FP64)
+        k_agents<double, true><<<grid_for(n), kBlock, 0, e.stream>>>(
+            (double*)e.rho, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
+            (const double*)e.agent_t1, (const double*)e.agent_t2, nullptr, nullptr, (double*)save);
+    else
+        k_agents<float, true><<<grid_for(n), kBlock, 0, e.stream>>>(
+            (float*)e.rho, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
+            (const double*)e.agent_t1, (const double*)e.agent_t2, nullptr, nullptr, (float*)save);
+    return hipGetLastError();
+}
+
+hipError_t launch_agents_restore(Env& e, const void* save)
+{
+    if (e.n_agent_voxels == 0) return hipSuccess;
+    const size_t n = (size_t)e.n_agent_voxels * e.S;
+    if (e.precision == BFVM_FP64)
+        k_agents_restore<double><<<grid_for(n), kBlock, 0, e.stream>>>(
+            (double*)e.rho, e.S, e.n_agent_voxels, e.agent_seg_voxel, (const double*)save);
+    else
+        k_agents_restore<float><<<grid_for(n), kBlock, 0, e.stream>>>(
+            (float*)e.rho, e.S, e.n_agent_voxels, e.agent_seg_voxel, (const float*)save);
     return hipGetLastError();
 }
 
