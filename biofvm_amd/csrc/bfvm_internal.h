@@ -110,7 +110,7 @@ struct SweepPlan {
                      // next tile while solving one, 2 x sweep on LDS-staged row tiles,
                      // 3 x sweep with whole-cache-line loads and stores (k_sweep_xt)
     int nt = 0;      // 1: non-temporal loads and stores (y and z sweeps whose tiles'
-                     // row pieces are whole 128-byte cache lines)
+                     // row pieces are whole 128-byte cache lines, kernel 3 x sweeps)
 };
 
 constexpr int kDefaultSpread = -1;

@@ -951,6 +951,33 @@ int bfvm_autotune(bfvm_handle h, double dt)
         }
         e.spread[axis] = best_spread;
     }
+    // the whole-cache-line x sweep's cache policy, timed with the y sweep
+    // that follows it: with ordinary stores the x sweep leaves its lines in
+    // the caches, which slows the next kernel more than it slows itself
+    if (st == hipSuccess && e.plan[0].stream == 3 && bfvm::sweep_nt_allowed(e, 0, e.plan[0])) {
+        int best_nt = e.plan[0].nt;
+        float best_ms = 1e30f;
+        for (int nt : {0, 1}) {
+            e.plan[0].nt = nt;
+            auto pair = [&] {
+                hipError_t q = bfvm::launch_sweep(e, 0);
+                if (q == hipSuccess && e.lod_dims >= 2) q = bfvm::launch_sweep(e, 1);
+                return q;
+            };
+            if ((st = pair()) != hipSuccess) break;  // warm-up
+            (void)hipEventRecord(t0, e.stream);
+            for (int rep = 0; rep < 3 && st == hipSuccess; ++rep) st = pair();
+            (void)hipEventRecord(t1, e.stream);
+            if (st != hipSuccess || (st = hipEventSynchronize(t1)) != hipSuccess) break;
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_nt = nt;
+            }
+        }
+        e.plan[0].nt = best_nt;
+    }
     // the x sweep with the sources fused in loads the bulk fields beside the
     // store, so its best shape is timed on its own (on the scratch store,
     // with the handle's bulk fields, agents and Dirichlet nodes)
@@ -1094,7 +1121,7 @@ int bfvm_set_sweep_cache_policy(bfvm_handle h, int axis, int non_temporal)
     if (non_temporal && (p.cols == 0 || !bfvm::sweep_nt_allowed(*h, axis, p)))
         return fail(h, BFVM_E_ARG,
                     "non-temporal loads and stores are for y and z sweeps whose tiles' row pieces are "
-                    "whole 128-byte lines");
+                    "whole 128-byte lines and for whole-cache-line x sweeps");
     if (p.cols == 0) return BFVM_OK;  // the one-thread-per-line fallback has no policy
     p.nt = non_temporal ? 1 : 0;
     h->plan[axis] = p;

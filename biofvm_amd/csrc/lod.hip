@@ -897,7 +897,7 @@ struct XtPlain {
 // runs after the solve, with every thread of the block (the carry slots xa,
 // xb and the line-end values wfirst, wlast are free once it has passed a
 // block barrier).
-template <typename T, int L, int SC, class Hook>
+template <typename T, int L, int SC, class Hook, int POL = kLoadPolicy>
 __device__ __forceinline__ void xt_tile(const SweepArgs& A, unsigned tile, double* smem, Hook& hook)
 {
     constexpr int G = 16 / (int)sizeof(T);  // nodes per chunk = values per lane piece
@@ -942,20 +942,20 @@ __device__ __forceinline__ void xt_tile(const SweepArgs& A, unsigned tile, doubl
     if (p0 + L <= n) {
 #pragma unroll
         for (int k = 0; k < L / G; ++k)
-            piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, k * chunk, kLoadPolicy);
+            piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, k * chunk, POL);
     } else if (SC) {  // the segment holding node n-1: nodes past it read node n-1
         const int oj = G * s - uj * S;
 #pragma unroll
         for (int k = 0; k < L / G; ++k) {
             const unsigned node = (unsigned)min(p0 + G * k + uj, n - 1);
             piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, (row0 + node * S + oj) * (unsigned)sizeof(T), 0,
-                                                             kLoadPolicy);
+                                                             POL);
         }
     } else {  // rows of whole chunks: chunks past node n-1 read the row's last chunk
 #pragma unroll
         for (int k = 0; k < L / G; ++k) {
             const int kk = min(k, (n - p0) / G - 1);
-            piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, kk * chunk, kLoadPolicy);
+            piece[k] = __builtin_amdgcn_raw_buffer_load_b128(rs, vo, kk * chunk, POL);
         }
     }
     const int slot = g * C + col;
@@ -985,19 +985,20 @@ __device__ __forceinline__ void xt_tile(const SweepArgs& A, unsigned tile, doubl
                               const u32x4 out = *(const u32x4*)(mine + wj);
                               if (act && !dup && store_node + t < n)
                                   __builtin_amdgcn_raw_buffer_store_b128(out, ro, vo, (t / G) * chunk,
-                                                                         kStorePolicy);
+                                                                         POL);
                               wave_lds_sync();
                           }
                       });
     hook.end(pos, xa, xb, wfirst, wlast);
 }
 
-template <typename T, int L, int SC>
+// POL: the cache policy of the store's loads and stores (SweepPlan::nt).
+template <typename T, int L, int SC, int POL>
 __global__ void __launch_bounds__(kMaxThreads, L <= 20 ? 8 : 4) k_sweep_xt(SweepArgs A)
 {
     extern __shared__ __attribute__((aligned(16))) double smem_xt[];
     XtPlain plain;
-    xt_tile<T, L, SC>(A, xcd_tile(blockIdx.x, gridDim.x, A.spread), smem_xt, plain);
+    xt_tile<T, L, SC, XtPlain, POL>(A, xcd_tile(blockIdx.x, gridDim.x, A.spread), smem_xt, plain);
 }
 
 // Whole-cache-line x sweeps of several grids (runtime S): map[2b], map[2b+1] =
@@ -1488,7 +1489,12 @@ hipError_t launch_seg_pol(Env& e, int axis, int nseg, int k0, int k1)
 
 // Non-temporal loads and stores (plan.nt, see kStreamingPolicy) are offered
 // for y and z sweeps whose tiles' row pieces are whole 128-byte lines (with
-// 64-byte pieces they moved 20-25 % more bytes than the sweep's own).
+// 64-byte pieces they moved 20-25 % more bytes than the sweep's own), and
+// for the whole-cache-line x kernel (each lane moves 16-byte pieces of
+// chunks that no other load reads again): at 400^3 x 8 FP32 with 1e6
+// agents, in full steps, x 0.711 -> 0.692 ms, and the kernels after it no
+// longer meet its lines in the caches: y 0.776 -> 0.759 ms, the agent
+// kernel of the next step 0.084 -> 0.065 ms.
 template <typename T, int COLS>
 constexpr bool streams_whole_lines()
 {
@@ -1497,7 +1503,8 @@ constexpr bool streams_whole_lines()
 
 bool nt_shape(const Env& e, int axis, int cols, int stream)
 {
-    return axis != 0 && stream != 2 && (size_t)cols * e.elem >= 128;
+    if (axis == 0) return stream == 3;
+    return stream != 2 && (size_t)cols * e.elem >= 128;
 }
 
 template <typename T, int L, int COLS>
@@ -1760,9 +1767,15 @@ hipError_t launch_xt(Env& e, const SweepPlan& p, int k0, int k1)
         const XtShape x = xt_shape<T>(e, p.cols, L);
         SweepArgs A = sweep_args(e, 0, p.cols, (e.nx + L - 1) / L, k0, k1);
         A.xt_pitch = x.pitch;
-        const hipError_t st = allow_table_lds((const void*)k_sweep_xt<T, L, SC>, x.lds);
+        const bool nt = p.nt != 0 && nt_shape(e, 0, p.cols, p.stream);
+        const void* k = nt ? (const void*)k_sweep_xt<T, L, SC, kStreamingPolicy>
+                           : (const void*)k_sweep_xt<T, L, SC, kLoadPolicy>;
+        const hipError_t st = allow_table_lds(k, x.lds);
         if (st != hipSuccess) return st;
-        k_sweep_xt<T, L, SC><<<(unsigned)A.tiles, x.threads, x.lds, e.stream>>>(A);
+        if (nt)
+            k_sweep_xt<T, L, SC, kStreamingPolicy><<<(unsigned)A.tiles, x.threads, x.lds, e.stream>>>(A);
+        else
+            k_sweep_xt<T, L, SC, kLoadPolicy><<<(unsigned)A.tiles, x.threads, x.lds, e.stream>>>(A);
         return hipGetLastError();
     }
 }
@@ -1973,7 +1986,10 @@ std::vector<SweepPlan> sweep_candidates(const Env& e, int axis, bool any_length)
             for (int L : kXtNodes) {
                 const bool ok = e.precision == BFVM_FP64 ? xt_plan_valid<double>(e, R * e.S, L)
                                                          : xt_plan_valid<float>(e, R * e.S, L);
-                if ((L <= 8 || L < 2 * n) && (Lfix == 0 || L == Lfix) && ok) out.push_back(SweepPlan{R * e.S, L, 3});
+                if ((L <= 8 || L < 2 * n) && (Lfix == 0 || L == Lfix) && ok) {
+                    out.push_back(SweepPlan{R * e.S, L, 3});
+                    out.push_back(SweepPlan{R * e.S, L, 3, 1});
+                }
             }
     return out;
 }

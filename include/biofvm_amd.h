@@ -242,9 +242,12 @@ int bfvm_set_sweep_plan(bfvm_handle h, int axis, int lines_per_block, int nodes_
 /* Cache policy of one axis's sweep: non_temporal = 1 when its loads and
  * stores of the store carry the non-temporal (streaming) hint, 0 for the
  * default.  Offered for the y and z sweeps whose tiles' row pieces are whole
- * 128-byte lines (lines per block x value size >= 128, column kernels):
- * each value is read and written once per sweep, and the hint keeps the
- * sweep from evicting the other kernels' data; bfvm_autotune times both.
+ * 128-byte lines (lines per block x value size >= 128, column kernels) and
+ * for x sweeps on the whole-cache-line kernel (streaming = 3): each value is
+ * read and written once per sweep, and the hint keeps the sweep from
+ * evicting the other kernels' data and from leaving its own lines for the
+ * next kernel to meet; bfvm_autotune times both (for the x sweep, together
+ * with the y sweep after it).
  * Setting 1 elsewhere returns BFVM_E_ARG; setting a launch shape by hand
  * (bfvm_set_sweep_plan) resets the axis to the default policy.  Results do
  * not depend on it.  No counterpart in the reference. */

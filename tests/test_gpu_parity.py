@@ -381,15 +381,18 @@ def test_cache_policy_never_changes_results(precision):
     outs = []
     for nt in (0, 1):
         with setup_case(case, precision) as m:
+            m.set_sweep_plan(0, 4 * S, 8, 3)  # the whole-cache-line x kernel, 4 rows per block
             for axis in (1, 2):
                 m.set_sweep_plan(axis, 32, 16)  # row pieces of 128 or 256 bytes
+            for axis in (0, 1, 2):
                 m.set_sweep_cache_policy(axis, nt)
-            assert m.sweep_cache_policies() == [0, nt, nt]
+            assert m.sweep_cache_policies() == [nt, nt, nt]
             m.run(case["dt"], case["steps"], case["phases"])
             outs.append(m.download())
         with pytest.raises(BfvmError):
             with setup_case(case, precision) as m:
-                m.set_sweep_cache_policy(0, 1)  # x keeps the default policy
+                m.set_sweep_plan(0, 32, 8)  # column tiles: x keeps the default policy
+                m.set_sweep_cache_policy(0, 1)
     np.testing.assert_array_equal(outs[1], outs[0])
     assert rel_err(outs[0], oracle.oracle_run(case)) <= TOL[precision]
 
@@ -420,7 +423,20 @@ def test_cache_policy_of_every_shape_never_changes_results(precision):
                 np.testing.assert_array_equal(outs[1], outs[0], err_msg=str((axis, cols, L, streaming)))
                 tried += 1
             m.set_sweep_plan(axis, 0, 0)
-    assert tried >= 20
+        # the whole-cache-line x kernel, 1-8 rows per block
+        for rows in (1, 2, 3, 5, 8):
+            for L in (8, 16):
+                outs = []
+                for nt in (0, 1):
+                    m.upload(case["rho0"])
+                    m.set_sweep_plan(0, rows * S, L, 3)
+                    m.set_sweep_cache_policy(0, nt)
+                    assert m.sweep_cache_policies()[0] == nt
+                    m.run(case["dt"], case["steps"], case["phases"])
+                    outs.append(m.download())
+                np.testing.assert_array_equal(outs[1], outs[0], err_msg=str((0, rows, L)))
+                tried += 1
+    assert tried >= 30
 
 
 @pytest.mark.parametrize("precision", [FP64, FP32], ids=["fp64", "fp32"])
