@@ -757,15 +757,17 @@ def gpu_main(args, cpu):
         tname = "float" if main_prec == FP32 else "double"
         # the row-tile kernel's template has no lines-per-block parameter, the
         # whole-cache-line x kernel's has the substrate count instead; the
-        # column kernels carry a cache policy (non-temporal, 2, for y and z)
+        # column and whole-cache-line kernels carry a cache policy (2:
+        # non-temporal)
+        pol = 2 if policies["xyz".index(dom[-1])] else 0
         if kname == "k_sweep_rows":
             targs = f"{tname}, {L}"
         elif kname == "k_sweep_xt":
-            targs = f"{tname}, {L}, {case['S']}"
+            targs = f"{tname}, {L}, {case['S']}, {pol}"
         elif kname == "k_sweep_x_src":
             targs = f"{tname}, {L}, {cols}"
         else:
-            targs = f"{tname}, {L}, {cols}, {2 if policies['xyz'.index(dom[-1])] else 0}"
+            targs = f"{tname}, {L}, {cols}, {pol}"
         traffic = measured_traffic(args.workload, f"{kname}<{targs}>")
     step_bytes = algorithmic_bytes(case, w_main)
 
