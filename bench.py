@@ -20,7 +20,8 @@ would slow them: 1.5 % at 400^3 x 8).  ``cpu_baseline`` times BioFVM itself (com
 source, oracle/_ref) on the whole workload on this host's cores (a child process set up while
 the GPU timings run, its steps timed after them; see cpu_child), ``cpu_baseline_1core`` the
 same on one thread.  The line also carries compact ``ensemble_64`` and
-``differentiation_128`` objects (ensemble_result, differentiation_result).
+``differentiation_128`` objects (ensemble_result, differentiation_result) and
+compact ``lod_50`` and ``drop_in_200`` objects (config_result).
 
 ``--workload ensemble_64`` times the batching form instead: 64 independent
 uneven grids (FP32 stores) advanced together, one step = one HIP graph replay
@@ -149,6 +150,48 @@ def time_precision(case, precision, steps, warmup, fused=True, plan_x=""):
     m.sync()
     total_steps = warmup + steps + prof_steps
     return m, wall, ev_ms, per, total_steps
+
+
+def sources_fused(case, per):
+    """Whether the bulk update and the first two Dirichlet overwrites ran
+    inside the x sweep: no bulk launch, one Dirichlet launch per step (before z)."""
+    ph = case["phases"]
+    has_bulk = bool(ph & 1 and case.get("bulk_supply") is not None)
+    has_dir = bool(ph & 4 and case.get("dir_idx") is not None and len(case["dir_idx"]) > 0)
+    return bool((has_bulk or has_dir) and per.get("bulk", (0, 0))[1] == 0
+                and (not has_dir or per.get("dirichlet", (0, 0))[1] < 2))
+
+
+def config_result(workload, steps, warmup):
+    """lod_50 or drop_in_200 (FP64 stores), compact, for the default line: the
+    C ABI path timed as the main workload is (autotuned, graph replays), the
+    step's HBM fraction and the dominant kernel's roofline (its own
+    --workload gives the full line, with the facade and the CPU baseline)."""
+    from biofvm_amd import FP64
+    from biofvm_amd import workloads as W
+    case = getattr(W, workload)()
+    nvs = case["nx"] * case["ny"] * case["nz"] * case["S"]
+    m, wall, ev_ms, per, _ = time_precision(case, FP64, steps, warmup)
+    plans = m.sweep_plans()
+    m.close()
+    fused = sources_fused(case, per)
+    dom = max(per, key=lambda k: per[k][0])
+    dom_ms, dom_n = per[dom]
+    dom_bytes = kernel_bytes(case, dom, 8, fused)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    n_agents = 0 if case.get("agent_volume") is None else len(case["agent_volume"])
+    n_dir = 0 if case.get("dir_idx") is None else len(case["dir_idx"])
+    return {"value": nvs * steps / wall, "unit": "voxel-substrate updates/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": 1e3 * wall / steps, "events_ms_per_step": ev_ms / steps, "store_dtype": "f64",
+            "config": {"workload": f"{workload}: {case['nx']}^3 voxels x {case['S']} substrates, "
+                                   f"{n_agents} agents, {n_dir} Dirichlet nodes, phases={case['phases']}"},
+            "hbm_frac_step": algorithmic_bytes(case, 8) / (wall / steps) / 1e9 / HBM_PEAK_GBS,
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_step": dom_bytes,
+                         "ms_per_step": dom_ms, "launches_per_step": dom_n},
+            "sources_fused_in_x_sweep": fused,
+            "kernels_ms_per_step": {k: v[0] for k, v in per.items()},
+            "sweep_plans_lines_nodes_streaming": dict(zip("xyz", plans))}
 
 
 def error_curve(case, total, every):
@@ -673,7 +716,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true",
-                    help="leave the compact ensemble_64 and differentiation_128 objects out of the default line")
+                    help="leave the compact ensemble_64, differentiation_128, lod_50 and drop_in_200 objects "
+                         "out of the default line")
     ap.add_argument("--plan-x", default="", metavar="LINES,NODES,KERNEL",
                     help="after autotuning, run the FP32 x sweep with this shape (profiling aid)")
     ap.add_argument("--error-steps", type=int, default=1000,
@@ -736,13 +780,7 @@ def gpu_main(args, cpu):
     spreads = [m32.sweep_spread(ax) for ax in range(3)]
     value = nvs * steps / wall
     step_ms = 1e3 * wall / steps
-    ph = case["phases"]
-    has_bulk = bool(ph & 1 and case.get("bulk_supply") is not None)
-    has_dir = bool(ph & 4 and case.get("dir_idx") is not None and len(case["dir_idx"]) > 0)
-    # the bulk update and the first two Dirichlet overwrites ran inside the x
-    # sweep: no bulk launch, one Dirichlet launch per step (before z)
-    fused = bool((has_bulk or has_dir) and per.get("bulk", (0, 0))[1] == 0
-                 and (not has_dir or per.get("dirichlet", (0, 0))[1] < 2))
+    fused = sources_fused(case, per)
     dom = max(per, key=lambda k: per[k][0])
     dom_ms, dom_n = per[dom]  # per step
     dom_avg = dom_ms / dom_n  # per launch
@@ -853,6 +891,11 @@ def gpu_main(args, cpu):
         line["ensemble_64"] = ensemble_result(20, 5, False)
         print("[bench] differentiation_128", file=sys.stderr, flush=True)
         line["differentiation_128"] = differentiation_result(False)
+        # the FP64 configurations (the CPU-runnable grid and the drop-in step)
+        print("[bench] lod_50", file=sys.stderr, flush=True)
+        line["lod_50"] = config_result("lod_50", 100, 10)
+        print("[bench] drop_in_200", file=sys.stderr, flush=True)
+        line["drop_in_200"] = config_result("drop_in_200", 50, 5)
     if cpu is not None:
         line["cpu_baseline"], one_core = cpu.result(case)
         if one_core:
