@@ -315,9 +315,12 @@ struct AgentBatch {
     const double* t1;
     const double* t2;
     int S, n_groups;
-    long long first;  // first global thread of this member
+    long long first;  // first global thread of this member (a multiple of kAgentBatchBlock)
 };
-hipError_t launch_agents_batch(int precision, const AgentBatch* d_tab, int nm, long long threads,
+// Block size of k_agents_batch: each member's threads start on a block
+// boundary, so block b works for member block_member[b] alone.
+constexpr int kAgentBatchBlock = 256;
+hipError_t launch_agents_batch(int precision, const AgentBatch* d_tab, const int* d_block_member, int blocks,
                                hipStream_t st);
 
 // Thomas/Woodbury tables for dt (rebuilt when dt, D or lambda changed);

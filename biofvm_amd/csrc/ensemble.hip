@@ -59,8 +59,9 @@ struct bfvm_ensemble_t {
     size_t xtlds = 0;
     int tab_xt_x = -1;
     bfvm::AgentBatch* d_agents = nullptr;
+    int* d_agent_blocks = nullptr;  // the member of each block of k_agents_batch
     int n_agent_members = 0;
-    long long agent_threads = 0;
+    int agent_blocks = 0;
 
     // graphs of one step and of graph_steps steps (replays of the latter
     // carry runs of that many steps: one launch gap instead of one per step)
@@ -116,6 +117,8 @@ void free_tables(bfvm_ensemble_t& E)
     }
     if (E.d_agents) (void)hipFree(E.d_agents);
     E.d_agents = nullptr;
+    if (E.d_agent_blocks) (void)hipFree(E.d_agent_blocks);
+    E.d_agent_blocks = nullptr;
     if (E.d_rows) (void)hipFree(E.d_rows);
     if (E.d_rmap) (void)hipFree(E.d_rmap);
     E.d_rows = nullptr;
@@ -297,8 +300,9 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
             EHIP(&E, hipMemcpy(E.d_first[axis], map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice));
         }
     }
+    // each member's (voxel group, substrate) threads start on a block boundary
     std::vector<bfvm::AgentBatch> ag;
-    long long threads = 0;
+    std::vector<int> agent_blocks;
     for (Env* pm : E.m) {
         if (pm->n_agent_voxels == 0) continue;
         bfvm::AgentBatch b;
@@ -309,15 +313,22 @@ int prepare(bfvm_ensemble_t& E, double dt, int flags)
         b.t2 = (const double*)pm->agent_t2;
         b.S = pm->S;
         b.n_groups = pm->n_agent_voxels;
-        b.first = threads;
-        threads += (long long)pm->n_agent_voxels * pm->S;
+        b.first = (long long)agent_blocks.size() * bfvm::kAgentBatchBlock;
+        const long long items = (long long)pm->n_agent_voxels * pm->S;
+        const long long nb = (items + bfvm::kAgentBatchBlock - 1) / bfvm::kAgentBatchBlock;
+        if ((long long)agent_blocks.size() + nb > 0x7fffffffll)
+            return efail(&E, BFVM_E_ARG, "too many agent voxels in the ensemble");
+        agent_blocks.insert(agent_blocks.end(), (size_t)nb, (int)ag.size());
         ag.push_back(b);
     }
     E.n_agent_members = (int)ag.size();
-    E.agent_threads = threads;
+    E.agent_blocks = (int)agent_blocks.size();
     if (!ag.empty()) {
         EHIP(&E, hipMalloc((void**)&E.d_agents, sizeof(bfvm::AgentBatch) * ag.size()));
         EHIP(&E, hipMemcpy(E.d_agents, ag.data(), sizeof(bfvm::AgentBatch) * ag.size(),
+                           hipMemcpyHostToDevice));
+        EHIP(&E, hipMalloc((void**)&E.d_agent_blocks, sizeof(int) * agent_blocks.size()));
+        EHIP(&E, hipMemcpy(E.d_agent_blocks, agent_blocks.data(), sizeof(int) * agent_blocks.size(),
                            hipMemcpyHostToDevice));
     }
     for (size_t i = 0; i < E.m.size(); ++i) {
@@ -342,9 +353,9 @@ hipError_t enqueue(bfvm_ensemble_t& E, int flags)
         for (Env* pm : E.m)
             if (pm->have_bulk && (st = on_stream(*pm, E.stream, [&] { return bfvm::launch_bulk(*pm); })))
                 return st;
-    if ((flags & BFVM_CELLS) && E.agent_threads > 0)
-        if ((st = bfvm::launch_agents_batch(E.precision, E.d_agents, E.n_agent_members,
-                                            E.agent_threads, E.stream)))
+    if ((flags & BFVM_CELLS) && E.agent_blocks > 0)
+        if ((st = bfvm::launch_agents_batch(E.precision, E.d_agents, E.d_agent_blocks, E.agent_blocks,
+                                            E.stream)))
             return st;
     if (flags & BFVM_DIFFUSION) {
         for (int axis = 0; axis < 3; ++axis) {
