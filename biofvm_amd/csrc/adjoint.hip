@@ -55,7 +55,8 @@ struct Tape {
     void* s6 = nullptr;       // after the y sweep
     void* s8 = nullptr;       // after the z sweep (the step's result)
     // the agent voxels' values of a step's start (n_agent_voxels x S, store
-    // precision) while the agents run in place on it (forward_step_keep)
+    // precision) while the agents run in place on it (forward_step_keep):
+    // two halves of asave_bytes / 2, for a step and the one after it
     void* asave = nullptr;
     size_t asave_bytes = 0;
     double* lam = nullptr;    // adjoint state (FP64)
@@ -551,28 +552,63 @@ hipError_t forward_step(Env& e, const Tape& tp, void* buf)
     });
 }
 
+// Where the agent voxels' values of a step's start are saved (half k % 2).
+void* asave_half(const Tape& tp, int k)
+{
+    return (char*)tp.asave + (size_t)(k & 1) * (tp.asave_bytes / 2);
+}
+
+// A step's start changed at its agent voxels (the agents ran on it in place),
+// to be written back from `save` before anything reads the start again.
+struct PendingRestore {
+    void* start = nullptr;
+    const void* save = nullptr;
+};
+
+hipError_t restore_start(Env& e, const PendingRestore& p)
+{
+    if (!p.start) return hipSuccess;
+    return on_buffer(e, p.start, [&] { return bfvm::launch_agents_restore(e, p.save); });
+}
+
 // The step from `start` again, keeping s1 (after bulk), s4, s6, s8 (after
 // each sweep; without diffusion all three hold the state after the agents).
-// `start` is left as it was on entry.
-hipError_t forward_step_keep(Env& e, Tape& tp, const void* start)
+// Without a bulk update or Dirichlet nodes the agents run on `start` itself,
+// their voxels' values saved to `save` first, and the x sweep reads `start`
+// and writes s4 (no copy of the store); `start` is then written back at once,
+// or, with `defer`, left to the caller (*pending): the next step's agent
+// kernel writes it back (`prev`) in the same launch.  `prev` is always
+// written back here.
+hipError_t forward_step_keep(Env& e, Tape& tp, const void* start, void* save, const PendingRestore& prev,
+                             bool defer, PendingRestore* pending)
 {
     hipError_t st;
     const bool bulk = (tp.flags & BFVM_BULK) && e.have_bulk;
     const bool cells = (tp.flags & BFVM_CELLS) && e.n_agents > 0;
     const bool diff = tp.flags & BFVM_DIFFUSION;
     int axis = 0;
+    if (pending) *pending = PendingRestore{};
     if (!bulk && diff && e.lod_dims > 0 && e.n_dir == 0 && (!cells || tp.asave)) {
-        // the agents run on the start itself, their voxels' values saved
-        // first and written back after the x sweep, which reads the start
-        // and writes s4: no copy of the store (the start is the previous
-        // step's result, and its agent voxels the agents' adjoint input)
         tp.after_bulk = start;
         void* s0 = const_cast<void*>(start);
-        if (cells && (st = on_buffer(e, s0, [&] { return bfvm::launch_agents_saving(e, tp.asave); }))) return st;
+        if (cells) {
+            st = on_buffer(e, s0, [&] {
+                return prev.start ? bfvm::launch_agents_restore_then_save(e, save, prev.start, prev.save)
+                                  : bfvm::launch_agents_saving(e, save);
+            });
+            if (st) return st;
+        } else if ((st = restore_start(e, prev))) {
+            return st;
+        }
         if ((st = bfvm::launch_sweep_to(e, 0, start, tp.s4))) return st;
-        if (cells && (st = on_buffer(e, s0, [&] { return bfvm::launch_agents_restore(e, tp.asave); }))) return st;
+        if (cells) {
+            const PendingRestore mine{s0, save};
+            if (defer && pending) *pending = mine;
+            else if ((st = restore_start(e, mine))) return st;
+        }
         axis = 1;
     } else {
+        if ((st = restore_start(e, prev))) return st;
         // without a bulk update the state after it is the start itself (the
         // ring slot stays untouched until the step's transposes have run)
         if (bulk) {
@@ -703,10 +739,14 @@ hipError_t keep_pass(Env& e, Tape& tp, int len, void* const* shared)
         tp.s8 = i + 1 < len ? tp.ring[i + 1] : tp.last_s8;
     };
     hipError_t st;
+    PendingRestore pend;
     for (int i = 0; i < len; ++i) {
         use_slot(i);
-        if ((st = forward_step_keep(e, tp, tp.ring[i]))) return st;
+        PendingRestore next;
+        if ((st = forward_step_keep(e, tp, tp.ring[i], asave_half(tp, i), pend, true, &next))) return st;
+        pend = next;
     }
+    if ((st = restore_start(e, pend))) return st;
     for (int i = len - 1; i >= 0; --i) {
         use_slot(i);
         tp.after_bulk = bulk ? tp.s1 : tp.ring[i];
@@ -943,7 +983,7 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
         AHIP(e, hipMemsetAsync(tp->gupt, 0, sizeof(double) * na, e.stream));
     }
     // the agent voxels' saved values (forward_step_keep)
-    const size_t asave_bytes = (tp->flags & BFVM_CELLS) ? (size_t)e.n_agent_voxels * S * e.elem : 0;
+    const size_t asave_bytes = (tp->flags & BFVM_CELLS) ? 2 * (size_t)e.n_agent_voxels * S * e.elem : 0;
     if (tp->asave_bytes != asave_bytes) {
         free_ptr(tp->asave);
         tp->asave_bytes = 0;
@@ -1027,7 +1067,8 @@ int bfvm_tape_backward(bfvm_handle h, double* grad_rho0, double* grad_D, double*
             AHIP(e, forward_step(e, *tp, tp->ring[n - n0 + 1]));
         }
         for (int n = n1 - 1; n >= n0; --n) {
-            AHIP(e, forward_step_keep(e, *tp, tp->ring[n - n0]));
+            AHIP(e, forward_step_keep(e, *tp, tp->ring[n - n0], asave_half(*tp, 0), PendingRestore{}, false,
+                                      nullptr));
             AHIP(e, e.precision == BFVM_FP64 ? adjoint_step<double>(e, *tp) : adjoint_step<float>(e, *tp));
         }
     }

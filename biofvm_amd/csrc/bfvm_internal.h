@@ -252,6 +252,9 @@ hipError_t launch_agents(Env& e, bool with_bulk = false);
 // and the copy written back to the store.
 hipError_t launch_agents_saving(Env& e, void* save);
 hipError_t launch_agents_restore(Env& e, const void* save);
+// launch_agents_restore on prev_rho (prev_save) and launch_agents_saving on
+// e.rho (save) in one launch; prev_rho is another store than e.rho.
+hipError_t launch_agents_restore_then_save(Env& e, void* save, void* prev_rho, const void* prev_save);
 // Gradient vectors of the store into out (nvs*3 values, store precision).
 hipError_t launch_gradient(Env& e, void* out);
 hipError_t launch_sweep(Env& e, int axis);

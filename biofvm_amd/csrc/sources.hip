@@ -97,6 +97,34 @@ __global__ void k_agents(T* __restrict__ rho, int S, int n_groups, const int* __
     __builtin_nontemporal_store((T)v, r);
 }
 
+// k_agents<T, true> on rho, after writing prev_save back into prev_rho at the
+// same agent voxels (the previous step's start, which its own saving agent
+// kernel changed): one launch for the restore of one step and the agents of
+// the next.  prev_rho and rho are different stores.
+template <typename T>
+__global__ void k_agents_restore_then_save(T* __restrict__ rho, T* __restrict__ save, T* __restrict__ prev_rho,
+                                           const T* __restrict__ prev_save, int S, int n_groups,
+                                           const int* __restrict__ gvox, const int* __restrict__ gstart,
+                                           const double* __restrict__ t1, const double* __restrict__ t2)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n_groups * S) return;
+    const int g = (int)(t / S);
+    const int s = (int)(t - (size_t)g * S);
+    const size_t i = (size_t)gvox[g] * S + s;
+    prev_rho[i] = prev_save[t];
+    T* r = rho + i;
+    const T r0 = __builtin_nontemporal_load(r);
+    save[t] = r0;
+    double v = (double)r0;
+    const int a1 = gstart[g + 1];
+    for (int a = gstart[g]; a < a1; ++a) {
+        const size_t q = (size_t)a * S + s;
+        v = (v + t1[q]) / t2[q];
+    }
+    __builtin_nontemporal_store((T)v, r);
+}
+
 // Writes back what k_agents<T, true> saved: rho at every agent voxel as it
 // was before the agents ran.
 template <typename T>
@@ -274,6 +302,21 @@ hipError_t launch_agents_saving(Env& e, void* save)
         k_agents<float, true><<<grid_for(n), kBlock, 0, e.stream>>>(
             (float*)e.rho, e.S, e.n_agent_voxels, e.agent_seg_voxel, e.agent_seg_start,
             (const double*)e.agent_t1, (const double*)e.agent_t2, nullptr, nullptr, (float*)save);
+    return hipGetLastError();
+}
+
+hipError_t launch_agents_restore_then_save(Env& e, void* save, void* prev_rho, const void* prev_save)
+{
+    if (e.n_agent_voxels == 0) return hipSuccess;
+    const size_t n = (size_t)e.n_agent_voxels * e.S;
+    if (e.precision == BFVM_FP64)
+        k_agents_restore_then_save<double><<<grid_for(n), kBlock, 0, e.stream>>>(
+            (double*)e.rho, (double*)save, (double*)prev_rho, (const double*)prev_save, e.S, e.n_agent_voxels,
+            e.agent_seg_voxel, e.agent_seg_start, (const double*)e.agent_t1, (const double*)e.agent_t2);
+    else
+        k_agents_restore_then_save<float><<<grid_for(n), kBlock, 0, e.stream>>>(
+            (float*)e.rho, (float*)save, (float*)prev_rho, (const float*)prev_save, e.S, e.n_agent_voxels,
+            e.agent_seg_voxel, e.agent_seg_start, (const double*)e.agent_t1, (const double*)e.agent_t2);
     return hipGetLastError();
 }
 
